@@ -1,0 +1,198 @@
+// gc_matchfind.cpp -- plain (per-chunk) match-finding entry points.
+//
+// Kernel-level forms mirror the reference kernel ABI (one launch over the
+// sub-blocks of one buffer); host-level forms mirror the two Rust host
+// functions: copy in, find, copy out.
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/gcompress.h"
+#include "gc_runtime.h"
+
+using namespace gc;
+
+namespace {
+
+bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
+
+uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Items for a buffer tiled in sub-blocks of `sub` bytes from its start.
+void lzma2_items(std::vector<gc_lzma2_item>& v, uint64_t total, uint32_t sub, uint32_t nsub) {
+    v.resize(nsub);
+    for (uint32_t s = 0; s < nsub; s++) {
+        uint64_t start = (uint64_t)s * sub;
+        gc_lzma2_item& it = v[s];
+        it.src_off = start;
+        it.avail = total - start;
+        it.out_pos = start;
+        it.n = (uint32_t)((total - start) < sub ? (total - start) : sub);
+        it.out_span = sub;
+    }
+}
+
+void zstd_items(std::vector<gc_zstd_item>& v, uint64_t total, uint32_t chunk, uint32_t nsub, uint32_t max_seq) {
+    v.resize(nsub);
+    for (uint32_t s = 0; s < nsub; s++) {
+        uint64_t start = (uint64_t)s * chunk;
+        gc_zstd_item& it = v[s];
+        it.src_off = start;
+        it.avail = total - start;
+        it.scratch_pos = start;
+        it.seq_pos = (uint64_t)s * max_seq;
+        it.lit_pos = start;
+        it.count_pos = s;
+        it.n = (uint32_t)((total - start) < chunk ? (total - start) : chunk);
+    }
+}
+
+int check_sub(uint32_t sub) {
+    if (sub == 0 || sub > GC_SUB_BLOCK)
+        return fail(GC_EINVAL, "sub-block size must be 1..65536 (positions are kept in 16 bits)");
+    return GC_OK;
+}
+
+int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t* d_packed,
+              gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth) {
+    std::vector<gc_lzma2_item> items;
+    lzma2_items(items, total, sub, nsub);
+    int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_lzma2_item));
+    if (rc) return rc;
+    return launch_lzma2(ctx, d_in, (const gc_lzma2_item*)ctx->items.p, nsub, nullptr, nsub, d_packed, d_m, d_c,
+                        max_matches, depth);
+}
+
+int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, uint32_t nsub, void* d_seq,
+             uint32_t fmt, uint8_t* d_lit, uint32_t* d_sc, uint32_t* d_lc, uint32_t depth, uint32_t max_seq) {
+    std::vector<gc_zstd_item> items;
+    zstd_items(items, total, chunk, nsub, max_seq);
+    int rc = ctx->info.ensure((size_t)total * 4 + 256);
+    if (rc) return rc;
+    rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_zstd_item));
+    if (rc) return rc;
+    return launch_zstd(ctx, d_in, (const gc_zstd_item*)ctx->items.p, nsub, nullptr, nullptr, nsub,
+                       (uint32_t)div_up(nsub, 4), (uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, depth,
+                       max_seq);
+}
+
+// Device copy of a host buffer, padded so that aligned reads near the end
+// stay inside the allocation.
+int stage_input(gc_ctx* ctx, const uint8_t* chunk, size_t n) {
+    int rc = ctx->input.ensure(n + 64);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(ctx->input.p, chunk, n, hipMemcpyHostToDevice, ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "hipMemcpyAsync(input)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gc_zstd_match_find_dev(gc_ctx* ctx, const uint8_t* d_input, void* d_sequences, uint32_t seq_format,
+                           uint8_t* d_literals, uint32_t* d_seq_counts, uint32_t* d_lit_counts, uint32_t chunk_size,
+                           uint32_t num_chunks, uint64_t total_input_size, uint32_t search_depth,
+                           uint32_t max_sequences_per_chunk) {
+    if (!ctx || !d_input || !d_sequences || !d_seq_counts) return fail(GC_EINVAL, "null argument");
+    if (int rc = check_sub(chunk_size)) return rc;
+    if (!aligned4(d_input)) return fail(GC_EINVAL, "d_input must be 4-byte aligned");
+    if (seq_format > GC_SEQ_FORMAT_REF) return fail(GC_EINVAL, "unknown seq_format");
+    if ((uint64_t)num_chunks > div_up(total_input_size, chunk_size))
+        return fail(GC_EINVAL, "num_chunks exceeds the sub-chunks of total_input_size");
+    if (num_chunks == 0) return GC_OK;
+    return run_zstd(ctx, d_input, total_input_size, chunk_size, num_chunks, d_sequences, seq_format, d_literals,
+                    d_seq_counts, d_lit_counts, search_depth, max_sequences_per_chunk);
+}
+
+int gc_lzma2_match_find_dev(gc_ctx* ctx, const uint8_t* d_input, gc_lzma_match* d_matches, uint32_t* d_match_counts,
+                            uint32_t sub_block_size, uint32_t num_sub_blocks, uint64_t total_input_size,
+                            uint32_t max_matches_per_pos, uint32_t search_depth) {
+    if (!ctx || !d_input || !d_matches || !d_match_counts) return fail(GC_EINVAL, "null argument");
+    if (int rc = check_sub(sub_block_size)) return rc;
+    if (!aligned4(d_input)) return fail(GC_EINVAL, "d_input must be 4-byte aligned");
+    if (max_matches_per_pos == 0) return fail(GC_EINVAL, "max_matches_per_pos must be >= 1");
+    if ((uint64_t)num_sub_blocks > div_up(total_input_size, sub_block_size))
+        return fail(GC_EINVAL, "num_sub_blocks exceeds the sub-blocks of total_input_size");
+    if (num_sub_blocks == 0) return GC_OK;
+    return run_lzma2(ctx, d_input, total_input_size, sub_block_size, num_sub_blocks, nullptr, d_matches,
+                     d_match_counts, max_matches_per_pos, search_depth);
+}
+
+int gc_lzma2_match_find_packed_dev(gc_ctx* ctx, const uint8_t* d_input, uint32_t* d_packed, uint32_t sub_block_size,
+                                   uint64_t total_input_size, uint32_t search_depth) {
+    if (!ctx || !d_input || !d_packed) return fail(GC_EINVAL, "null argument");
+    if (int rc = check_sub(sub_block_size)) return rc;
+    if (!aligned4(d_input)) return fail(GC_EINVAL, "d_input must be 4-byte aligned");
+    uint64_t nsub = div_up(total_input_size, sub_block_size);
+    if (nsub == 0) return GC_OK;
+    if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
+    return run_lzma2(ctx, d_input, total_input_size, sub_block_size, (uint32_t)nsub, d_packed, nullptr, nullptr, 1,
+                     search_depth);
+}
+
+int gc_lzma2_find_matches_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t* packed) {
+    if (!ctx || (!chunk && n) || (!packed && n)) return fail(GC_EINVAL, "null argument");
+    if (n == 0) return GC_OK;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    if (!rc) rc = ctx->out_a.ensure(n * 4);
+    if (!rc)
+        rc = gc_lzma2_match_find_packed_dev(ctx, (const uint8_t*)ctx->input.p, (uint32_t*)ctx->out_a.p, GC_SUB_BLOCK,
+                                            n, GC_LZMA2_DEPTH);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(packed, ctx->out_a.p, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "gc_lzma2_find_matches_packed: copy out");
+    }
+    return rc;
+}
+
+int gc_gpu_find_matches(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t* match_distances,
+                        uint32_t* match_lengths, uint32_t* match_counts) {
+    if (!ctx || !match_distances || !match_lengths || !match_counts) return fail(GC_EINVAL, "null argument");
+    const size_t nsub = (size_t)div_up(n, GC_SUB_BLOCK);
+    const size_t slots = nsub * GC_SUB_BLOCK;
+    std::vector<uint32_t> packed(n);
+    int rc = gc_lzma2_find_matches_packed(ctx, chunk, n, packed.data());
+    if (rc) return rc;
+    // GpuMatchData layout: slot k of position q at q*8 + k, unused slots 0.
+    memset(match_distances, 0, slots * GC_LZMA2_MAX_MATCHES * sizeof(uint32_t));
+    memset(match_lengths, 0, slots * GC_LZMA2_MAX_MATCHES * sizeof(uint32_t));
+    memset(match_counts, 0, slots * sizeof(uint32_t));
+    for (size_t q = 0; q < n; q++) {
+        uint32_t v = packed[q];
+        if (!v) continue;
+        match_distances[q * GC_LZMA2_MAX_MATCHES] = v >> 16;
+        match_lengths[q * GC_LZMA2_MAX_MATCHES] = v & 0xFFFFu;
+        match_counts[q] = 1;
+    }
+    return GC_OK;
+}
+
+int gc_zstd_find_sequences(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, gc_zstd_sequence* seqs,
+                           uint32_t* seq_counts) {
+    if (!ctx || (!chunk && n) || !seqs || !seq_counts) return fail(GC_EINVAL, "null argument");
+    if (n == 0) return GC_OK;
+    const uint32_t max_seq = GC_SUB_BLOCK / 3;  // compress_zstd_custom.rs:55
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    const size_t seq_bytes = (size_t)nsub * max_seq * sizeof(gc_zstd_sequence);
+    if (!rc) rc = ctx->out_a.ensure(seq_bytes);
+    if (!rc) rc = ctx->out_b.ensure(nsub * 4);
+    if (!rc)
+        rc = run_zstd(ctx, (const uint8_t*)ctx->input.p, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p,
+                      GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)ctx->out_b.p, nullptr, zstd_depth_for_level(level),
+                      max_seq);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(seq_counts, ctx->out_b.p, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(seqs, ctx->out_a.p, seq_bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "gc_zstd_find_sequences: copy out");
+    }
+    return rc;
+}
+
+}  // extern "C"

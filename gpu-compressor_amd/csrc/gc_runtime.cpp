@@ -1,0 +1,212 @@
+// gc_runtime.cpp -- context, embedded code object, device memory, launches.
+#include "gc_runtime.h"
+
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/gcompress.h"
+
+// The gfx950 code object built from kernels.hip is linked into this library
+// and loaded with hipModuleLoadData (no files, no JIT at run time).
+#ifndef GC_CO_PATH
+#error "GC_CO_PATH must name the code object to embed"
+#endif
+__asm__(".section .rodata\n"
+        ".balign 4096\n"
+        ".globl gc_code_object\n"
+        "gc_code_object:\n"
+        ".incbin \"" GC_CO_PATH "\"\n"
+        ".byte 0\n"
+        ".previous\n");
+extern "C" const unsigned char gc_code_object[];
+
+namespace gc {
+
+static thread_local std::string t_error;
+
+void set_error(const std::string& msg) { t_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    t_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+    t_error = buf;
+    return GC_EHIP;
+}
+
+int DevBuf::ensure(size_t bytes) {
+    if (bytes <= size && p) return GC_OK;
+    release();
+    size_t want = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    size = want;
+    return GC_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    size = 0;
+}
+
+int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes) {
+    int rc = buf.ensure(bytes);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(items)");
+    e = hipStreamSynchronize(ctx->stream);  // src is pageable host memory owned by the caller
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return GC_OK;
+}
+
+static int launch(gc_ctx* ctx, hipFunction_t f, uint32_t grid, uint32_t block, void** args, const char* name) {
+    if (grid == 0) return GC_OK;
+    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr);
+    if (e != hipSuccess) return hip_fail(e, name);
+    return GC_OK;
+}
+
+int launch_lzma2(gc_ctx* ctx, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
+                 uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
+                 uint32_t max_matches, uint32_t depth) {
+    void* args[] = {&d_arena, &d_items, &n_items, &d_queue, &d_packed, &d_ref_m, &d_ref_c, &max_matches, &depth};
+    return launch(ctx, ctx->k.lzma2, grid, 256, args, "launch gc_lzma2_kernel");
+}
+
+int launch_zstd(gc_ctx* ctx, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
+                uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
+                uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
+                uint32_t* d_lit_counts, uint32_t depth, uint32_t max_seq) {
+    void* a1[] = {&d_arena, &d_items, &n_items, &d_queue_cand, &d_info};
+    int rc = launch(ctx, ctx->k.zstd_cand, grid_cand, 256, a1, "launch gc_zstd_cand_kernel");
+    if (rc) return rc;
+    const uint32_t* info = d_info;
+    void* a2[] = {&d_arena, &d_items,        &n_items,       &d_queue_parse, &info,  &d_seq,
+                  &seq_fmt, &d_lit,          &d_seq_counts,  &d_lit_counts,  &depth, &max_seq};
+    return launch(ctx, ctx->k.zstd_parse, grid_parse, 256, a2, "launch gc_zstd_parse_kernel");
+}
+
+int launch_fill(gc_ctx* ctx, uint32_t* p, uint32_t v, uint64_t n) {
+    uint64_t blocks = (n + 255) / 256;
+    uint32_t grid = (uint32_t)(blocks < 4096 ? (blocks ? blocks : 1) : 4096);
+    void* args[] = {&p, &v, &n};
+    return launch(ctx, ctx->k.fill_u32, grid, 256, args, "launch gc_fill_u32");
+}
+
+uint32_t zstd_depth_for_level(uint32_t level) {
+    // compress_zstd_custom.rs:57-61
+    if (level == 1) return GC_ZSTD_DEPTH_LAZY;
+    if (level == 2) return 64u;
+    return 1u;
+}
+
+}  // namespace gc
+
+using namespace gc;
+
+extern "C" {
+
+const char* gc_last_error(void) { return t_error.c_str(); }
+
+int gc_device_count(void) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    return n;
+}
+
+gc_ctx* gc_open(int device) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipSetDevice");
+        return nullptr;
+    }
+    gc_ctx* ctx = new gc_ctx();
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    if (ctx->cu_count <= 0) ctx->cu_count = 256;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module, gc_code_object);
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2, ctx->k.module, "gc_lzma2_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_cand, ctx->k.module, "gc_zstd_cand_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
+    if (e != hipSuccess) {
+        hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
+        gc_close(ctx);
+        return nullptr;
+    }
+    return ctx;
+}
+
+void gc_close(gc_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->items.release();
+    ctx->info.release();
+    ctx->input.release();
+    ctx->out_a.release();
+    ctx->out_b.release();
+    ctx->out_c.release();
+    if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void* gc_dev_alloc(gc_ctx* ctx, size_t bytes) {
+    if (!ctx) return nullptr;
+    void* p = nullptr;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        hip_fail(e, "hipMalloc");
+        return nullptr;
+    }
+    return p;
+}
+
+int gc_dev_free(gc_ctx* ctx, void* p) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    hipError_t e = hipFree(p);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "hipFree");
+}
+
+int gc_htod(gc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "gc_htod");
+}
+
+int gc_dtoh(gc_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "gc_dtoh");
+}
+
+int gc_dev_zero(gc_ctx* ctx, void* p, size_t bytes) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    hipError_t e = hipMemsetAsync(p, 0, bytes, ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "hipMemsetAsync");
+}
+
+int gc_sync(gc_ctx* ctx) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+void* gc_stream(gc_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// gc_runtime.h -- internal host-side declarations of libgcompress.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gc_device.h"
+
+namespace gc {
+
+// Thread-local error message behind gc_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+// Grow-only device buffer owned by a context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t size = 0;
+    int ensure(size_t bytes);
+    void release();
+};
+
+// Loaded kernels.
+struct Kernels {
+    hipModule_t module = nullptr;
+    hipFunction_t lzma2 = nullptr;
+    hipFunction_t zstd_cand = nullptr;
+    hipFunction_t zstd_parse = nullptr;
+    hipFunction_t fill_u32 = nullptr;
+};
+
+}  // namespace gc
+
+struct gc_ctx {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    gc::Kernels k;
+    // Scratch reused across calls.
+    gc::DevBuf items;      // work items
+    gc::DevBuf info;       // zstd per-position candidate words
+    gc::DevBuf input;      // staging for host-pointer entry points
+    gc::DevBuf out_a, out_b, out_c;
+    std::mutex mu;
+};
+
+namespace gc {
+
+// Launch helpers (all on ctx->stream).
+int launch_lzma2(gc_ctx* ctx, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
+                 uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
+                 uint32_t max_matches, uint32_t depth);
+int launch_zstd(gc_ctx* ctx, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
+                uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
+                uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
+                uint32_t* d_lit_counts, uint32_t depth, uint32_t max_seq);
+int launch_fill(gc_ctx* ctx, uint32_t* p, uint32_t v, uint64_t n);
+int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
+
+uint32_t zstd_depth_for_level(uint32_t level);
+
+}  // namespace gc

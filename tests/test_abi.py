@@ -1,0 +1,32 @@
+"""The C-ABI library loads on the CPU and exports every symbol the header declares."""
+import os
+import re
+
+import gcompress
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "gcompress.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = re.findall(r"\b(gc_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_library_exports_header():
+    L = gcompress.lib()
+    names = header_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in gcompress.h but not exported"
+
+
+def test_python_signatures_cover_header():
+    declared = set(header_functions())
+    bound = {n for n, _, _ in gcompress.SIGNATURES}
+    assert declared == bound, (declared - bound, bound - declared)
+
+
+def test_last_error_is_a_string():
+    assert isinstance(gcompress.lib().gc_last_error(), bytes)
