@@ -53,11 +53,11 @@
 __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint64_t avail) {
     const uint8_t* p = base + off;
     if ((uint64_t)off + 8 <= avail) {
-        uintptr_t a = (uintptr_t)p;
-        const uint32_t* q = (const uint32_t*)(a & ~(uintptr_t)3);
+        const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+        const uint32_t* q = (const uint32_t*)(p - sh);  // derived from base: stays a global access
         uint32_t lo = q[0];
         uint32_t hi = q[1];
-        return __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(a & 3));
+        return __builtin_amdgcn_alignbyte(hi, lo, sh);
     }
     uint32_t v = 0;
     for (uint32_t i = 0; i < 4; i++)
@@ -139,12 +139,18 @@ __device__ __forceinline__ uint32_t cand_tile(CandLds& s, const uint8_t* src, ui
     // reads another lane's id shares the slice with it and marks it; both are
     // then resolved exactly below.  Lanes that share a whole hash always
     // share a slice, so none of them escapes the exact path.
+    // Other lanes write the same bytes between this lane's store and load,
+    // which the compiler cannot see: the empty asm with a memory clobber
+    // keeps it from forwarding the store.  LDS operations of one wavefront
+    // execute in program order, so the load sees every lane's store.
     const uint32_t slot = h & (GC_DUP_SIZE - 1);
     s.flag[wave][lane] = 0;
     if (valid) s.dup[wave][slot] = (uint8_t)lane;
+    __asm__ volatile("" ::: "memory");
     uint32_t back = valid ? s.dup[wave][slot] : lane;
     bool loser = valid && back != lane;
     if (loser) s.flag[wave][back & (WAVE - 1)] = 1;
+    __asm__ volatile("" ::: "memory");
     bool dup = loser || (valid && s.flag[wave][lane] != 0);
 
     // Exact groups among the duplicate lanes (one step per distinct hash).
@@ -189,7 +195,7 @@ __device__ __forceinline__ void table_clear(CandLds& s) {
 // Output per position: packed (distance << 16 | length), 0 = no match, and
 // optionally the reference layout (slot 0 of LzmaMatch[max] and counts).
 // ---------------------------------------------------------------------------
-__device__ void lzma2_subblock(CandLds& s, const gc_lzma2_item& it, const uint8_t* arena, uint32_t* packed,
+__device__ __forceinline__ void lzma2_subblock(CandLds& s, const gc_lzma2_item it, const uint8_t* arena, uint32_t* packed,
                                gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches, uint32_t depth) {
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
@@ -265,7 +271,7 @@ gc_lzma2_kernel(const uint8_t* arena, const gc_lzma2_item* items, uint32_t n_ite
 // in the reference) and whether its first 3 bytes verify, with the common
 // length up to GC_ZSTD_SHORT.  Stored as (len << 16 | cand); len 0 = no.
 // ---------------------------------------------------------------------------
-__device__ void zstd_cand_subblock(CandLds& s, const gc_zstd_item& it, const uint8_t* arena, uint32_t* info) {
+__device__ __forceinline__ void zstd_cand_subblock(CandLds& s, const gc_zstd_item it, const uint8_t* arena, uint32_t* info) {
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
@@ -315,7 +321,7 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
 }
 
-__device__ void zstd_parse_subblock(const gc_zstd_item& it, const uint8_t* arena, const uint32_t* info,
+__device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const uint8_t* arena, const uint32_t* info,
                                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts,
                                     uint32_t* lit_counts, uint32_t depth, uint32_t max_seq) {
     const uint8_t* src = arena + it.src_off;
