@@ -1,0 +1,256 @@
+"""Match-find throughput on one MI355X: batched (persistent kernels + hipGraph) vs plain per-chunk.
+
+Default workload: the four-kind synthetic mix at 1 GiB in 16 MiB chunks, match-found at Zstd L1
+(lazy, depth 16) and L2 (depth 64) on one stream.  A step is one pass over the whole GiB at both
+levels (2 GiB of match-find input).  Inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--size-mib 1024] [--workload zstd|lzma2]
+
+Prints one JSON line (see the keys below); --gpus N > 1 exits with a message: this path runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-compressor_amd"))
+
+import gcompress  # noqa: E402
+
+MiB = 1 << 20
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """HIP events on an arbitrary stream (the library's own), via the runtime's C API."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [C.c_void_p]
+        self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.L.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.L.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.L.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def cpu_baseline_zstd(data: np.ndarray, levels, sample_bytes: int):
+    """The oracle (CPU restatement of the reference kernel) on a bounded sample, all host cores."""
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    sample = data[:sample_bytes]
+    t = time.perf_counter()
+    for lv in levels:
+        oracle.zstd_match_find(sample, depth={1: 16, 2: 64}.get(lv, 1), threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": round(sample.size * len(levels) / dt / 1e9, 4), "unit": "GB/s", "cores": cores,
+            "kind": "port",
+            "sample": f"first {sample.size // MiB} MiB of the corpus at levels {list(levels)} "
+                      f"({sample.size * len(levels) // MiB} MiB of match-find input), oracle on {cores} threads"}
+
+
+def cpu_baseline_lzma2(data: np.ndarray, sample_bytes: int):
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    cores = min(os.cpu_count() or 1, 16)
+    sample = data[:sample_bytes]
+    t = time.perf_counter()
+    oracle.lzma2_match_find(sample, 65536, 1, 32, threads=cores)
+    dt = time.perf_counter() - t
+    return {"value": round(sample.size / dt / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"first {sample.size // MiB} MiB of the corpus, oracle on {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size-mib", type=int, default=1024)
+    ap.add_argument("--chunk-mib", type=int, default=16)
+    ap.add_argument("--workload", choices=["zstd", "lzma2"], default="zstd")
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
+              "(sub-blocks shard with no exchange, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    hip = Hip()
+    ctx = gcompress.context(0)
+    L = gcompress.lib()
+    stream = L.gc_stream(ctx.h)
+    n = a.size_mib * MiB
+    chunk = a.chunk_mib * MiB
+    t0 = time.perf_counter()
+    seed = 2 if a.workload == "zstd" else 3
+    if a.workload == "zstd":
+        data = gcompress.corpus(n, seed, (1, 1, 1, 1), 65536, True)
+        levels = [1, 2]
+        algo = gcompress.ALGO_ZSTD
+    else:
+        # disk-image-like mix: 40% structured, 25% text, 20% back-references, 15% random, 1 MiB
+        # segments, 8 MiB zero gaps every 96 MiB
+        data = gcompress.corpus(n, seed, (15, 25, 20, 40), MiB, True, 96 * MiB, 8 * MiB)
+        levels = [1]
+        algo = gcompress.ALGO_LZMA2
+    gen_s = time.perf_counter() - t0
+    d_in = ctx.alloc(n + 256)
+    ctx.htod(d_in, data)
+    chunks = [(o, min(chunk, n - o)) for o in range(0, n, chunk)]
+    inputs = [(o, s, algo, lv) for lv in levels for (o, s) in chunks]
+    batch = gcompress.Batch(ctx, d_in, n + 256 - 256, inputs)
+    work_bytes = n * len(levels)
+
+    # ---- plain per-chunk form: one launch sequence per chunk, synchronised per chunk ----
+    nsub_chunk = -(-chunk // 65536)
+    max_seq = 65536 // 3
+    if algo == gcompress.ALGO_ZSTD:
+        d_seq = ctx.alloc(nsub_chunk * max_seq * 16)
+        d_cnt = ctx.alloc(nsub_chunk * 4)
+    else:
+        d_pk = ctx.alloc(chunk * 4)
+
+    def plain_pass():
+        for lv in levels:
+            for (o, s) in chunks:
+                if algo == gcompress.ALGO_ZSTD:
+                    rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_seq, 0, None, d_cnt, None, 65536,
+                                                  -(-s // 65536), s, {1: 16, 2: 64}.get(lv, 1), max_seq)
+                else:
+                    rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_pk, 65536, s, 32)
+                assert rc == 0, L.gc_last_error()
+                ctx.sync()
+
+    plain_pass()  # warm-up
+    t = time.perf_counter()
+    for _ in range(a.plain_steps):
+        plain_pass()
+    plain_s = (time.perf_counter() - t) / a.plain_steps
+    plain_gbs = work_bytes / plain_s / 1e9
+
+    # ---- batched form ----
+    tune = batch.tune()
+    tuned_ms = None
+    if not a.no_autotune:
+        tune, tuned_ms = batch.autotune(reps=1)
+    for _ in range(a.warmup):
+        batch.run()
+    ctx.sync()
+    e0, e1 = hip.event(), hip.event()
+    ctx.sync()
+    t = time.perf_counter()
+    hip.record(e0, stream)
+    for _ in range(a.steps):
+        batch.run()
+    hip.record(e1, stream)
+    ctx.sync()
+    wall_s = time.perf_counter() - t
+    ev_ms = hip.elapsed_ms(e0, e1)
+    ms_step = ev_ms / a.steps
+    value = work_bytes / (ms_step / 1e3) / 1e9
+
+    # per-kernel durations (kernel-by-kernel launch with events on the library stream)
+    kt = [batch.run_timed() for _ in range(3)]
+    k_ms = [float(np.mean([k[i] for k in kt])) for i in range(3)]  # lzma2, zstd cand, zstd parse
+
+    # algorithmic bytes and results summary (outside the timed region)
+    n_sub = 0
+    n_seq = 0
+    cover = 0
+    for i in range(len(inputs)):
+        if algo == gcompress.ALGO_ZSTD:
+            seqs, counts = batch.zstd(i, with_seqs=(i % 16 == 0))
+            n_sub += counts.size
+            n_seq += int(counts.sum())
+            if seqs is not None:
+                for s in range(counts.size):
+                    cover += int(seqs[s, : counts[s], 2].sum())
+        else:
+            n_sub += batch.n_sub(i)
+    if algo == gcompress.ALGO_ZSTD:
+        alg_bytes = work_bytes + 16 * n_seq + 4 * n_sub
+        names = ["gc_zstd_cand_kernel", "gc_zstd_parse_kernel"]
+        durs = [k_ms[1], k_ms[2]]
+        dom = int(np.argmax(durs))
+        # the candidate pass reads the input; the parse reads it again and writes the sequences
+        dom_bytes = work_bytes if dom == 0 else alg_bytes
+        dom_ms = durs[dom]
+        dom_name = names[dom]
+    else:
+        alg_bytes = 5 * work_bytes
+        dom_name, dom_ms, dom_bytes = "gc_lzma2_kernel", k_ms[0], alg_bytes
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+
+    cpu = None
+    if not a.no_cpu:
+        if algo == gcompress.ALGO_ZSTD:
+            cpu = cpu_baseline_zstd(data, levels, a.cpu_sample_mib * MiB)
+        else:
+            cpu = cpu_baseline_lzma2(data, min(a.cpu_sample_mib, 256) * MiB)
+
+    out = {
+        "metric": "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded four-kind mix: random, word text, back-references, structured binary)",
+        "config": {
+            "workload": (f"drop-in Zstd L1+L2 match-find, {a.size_mib} MiB mix in {a.chunk_mib} MiB chunks"
+                         if algo == gcompress.ALGO_ZSTD else
+                         f"LZMA2 L1 match-find, {a.size_mib} MiB disk-image-like mix"),
+            "form": "batched: persistent kernels over a device work queue, one hipGraph launch per step",
+            "input_bytes_per_step": work_bytes,
+            "sub_block": 65536,
+            "levels": levels,
+            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "lzma2_wg_per_cu": tune[2]},
+        },
+        "plain_per_chunk_gbs": round(plain_gbs, 3),
+        "batched_vs_plain": round(value / plain_gbs, 2),
+        "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
+        "kernel_ms": {"gc_lzma2_kernel": round(k_ms[0], 3), "gc_zstd_cand_kernel": round(k_ms[1], 3),
+                      "gc_zstd_parse_kernel": round(k_ms[2], 3)},
+        "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq,
+                    "match_coverage_sampled": round(cover / (work_bytes / 16), 4) if cover else None},
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "algorithmic_bytes": dom_bytes},
+        "cpu_baseline": cpu,
+        "corpus_gen_s": round(gen_s, 2),
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -29,6 +29,9 @@ ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 HOST_SOURCES = [
     "gc_runtime.cpp",
     "gc_matchfind.cpp",
+    "gc_batch.cpp",
+    "gc_corpus.cpp",
+    "gc_codec.cpp",
 ]
 KERNEL_SOURCES = ["kernels.hip"]
 

@@ -18,6 +18,8 @@ typedef struct {
     uint64_t out_pos;  // first output position
     uint32_t n;        // sub-block length, 1..65536
     uint32_t out_span; // positions covered in the reference-layout counts
+    uint32_t depth;    // probe depth (0 = no hash probe), lzma2_match_find.cu:63
+    uint32_t pad;
 } gc_lzma2_item;
 
 // One Zstd sub-block of work.
@@ -29,4 +31,6 @@ typedef struct {
     uint64_t lit_pos;     // first literal byte (sub-block index * chunk size)
     uint32_t count_pos;   // index into seq_counts / lit_counts
     uint32_t n;           // sub-block length, 1..65536
+    uint32_t depth;       // search depth (> 1 enables the lazy step), zstd_compress.cu:188
+    uint32_t pad;
 } gc_zstd_item;

@@ -19,7 +19,7 @@ bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Items for a buffer tiled in sub-blocks of `sub` bytes from its start.
-void lzma2_items(std::vector<gc_lzma2_item>& v, uint64_t total, uint32_t sub, uint32_t nsub) {
+void lzma2_items(std::vector<gc_lzma2_item>& v, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t depth) {
     v.resize(nsub);
     for (uint32_t s = 0; s < nsub; s++) {
         uint64_t start = (uint64_t)s * sub;
@@ -29,10 +29,13 @@ void lzma2_items(std::vector<gc_lzma2_item>& v, uint64_t total, uint32_t sub, ui
         it.out_pos = start;
         it.n = (uint32_t)((total - start) < sub ? (total - start) : sub);
         it.out_span = sub;
+        it.depth = depth;
+        it.pad = 0;
     }
 }
 
-void zstd_items(std::vector<gc_zstd_item>& v, uint64_t total, uint32_t chunk, uint32_t nsub, uint32_t max_seq) {
+void zstd_items(std::vector<gc_zstd_item>& v, uint64_t total, uint32_t chunk, uint32_t nsub, uint32_t max_seq,
+                uint32_t depth) {
     v.resize(nsub);
     for (uint32_t s = 0; s < nsub; s++) {
         uint64_t start = (uint64_t)s * chunk;
@@ -44,6 +47,8 @@ void zstd_items(std::vector<gc_zstd_item>& v, uint64_t total, uint32_t chunk, ui
         it.lit_pos = start;
         it.count_pos = s;
         it.n = (uint32_t)((total - start) < chunk ? (total - start) : chunk);
+        it.depth = depth;
+        it.pad = 0;
     }
 }
 
@@ -56,24 +61,23 @@ int check_sub(uint32_t sub) {
 int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t* d_packed,
               gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth) {
     std::vector<gc_lzma2_item> items;
-    lzma2_items(items, total, sub, nsub);
+    lzma2_items(items, total, sub, nsub, depth);
     int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_lzma2_item));
     if (rc) return rc;
-    return launch_lzma2(ctx, d_in, (const gc_lzma2_item*)ctx->items.p, nsub, nullptr, nsub, d_packed, d_m, d_c,
-                        max_matches, depth);
+    return launch_lzma2(ctx, ctx->stream, d_in, (const gc_lzma2_item*)ctx->items.p, nsub, nullptr, nsub, d_packed, d_m, d_c,
+                        max_matches);
 }
 
 int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, uint32_t nsub, void* d_seq,
              uint32_t fmt, uint8_t* d_lit, uint32_t* d_sc, uint32_t* d_lc, uint32_t depth, uint32_t max_seq) {
     std::vector<gc_zstd_item> items;
-    zstd_items(items, total, chunk, nsub, max_seq);
+    zstd_items(items, total, chunk, nsub, max_seq, depth);
     int rc = ctx->info.ensure((size_t)total * 4 + 256);
     if (rc) return rc;
     rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_zstd_item));
     if (rc) return rc;
-    return launch_zstd(ctx, d_in, (const gc_zstd_item*)ctx->items.p, nsub, nullptr, nullptr, nsub,
-                       (uint32_t)div_up(nsub, 4), (uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, depth,
-                       max_seq);
+    return launch_zstd(ctx, ctx->stream, d_in, (const gc_zstd_item*)ctx->items.p, nsub, nullptr, nullptr, nsub,
+                       (uint32_t)div_up(nsub, 4), (uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq);
 }
 
 // Device copy of a host buffer, padded so that aligned reads near the end
@@ -193,6 +197,48 @@ int gc_zstd_find_sequences(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t
         if (e != hipSuccess) rc = hip_fail(e, "gc_zstd_find_sequences: copy out");
     }
     return rc;
+}
+
+int gc_zstd_find_sequences_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, gc_zstd_sequence* seqs,
+                                  size_t seq_capacity, uint32_t* seq_counts, size_t* total) {
+    if (!ctx || (!chunk && n) || !seq_counts || !total) return fail(GC_EINVAL, "null argument");
+    *total = 0;
+    if (n == 0) return GC_OK;
+    const uint32_t max_seq = GC_SUB_BLOCK / 3;  // compress_zstd_custom.rs:55
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    const size_t seq_bytes = (size_t)nsub * max_seq * sizeof(gc_zstd_sequence);
+    if (!rc) rc = ctx->out_a.ensure(seq_bytes);
+    if (!rc) rc = ctx->out_b.ensure(nsub * 4);
+    if (!rc)
+        rc = run_zstd(ctx, (const uint8_t*)ctx->input.p, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p,
+                      GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)ctx->out_b.p, nullptr, zstd_depth_for_level(level),
+                      max_seq);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(seq_counts, ctx->out_b.p, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_zstd_find_sequences_packed: counts");
+    std::vector<uint64_t> offs(nsub);
+    uint64_t t = 0;
+    for (uint64_t s = 0; s < nsub; s++) {
+        offs[s] = t;
+        t += seq_counts[s];
+    }
+    *total = (size_t)t;
+    if (!seqs) return GC_OK;
+    if (t > seq_capacity) return fail(GC_ESIZE, "sequence capacity below the number of sequences found");
+    if (t == 0) return GC_OK;
+    rc = upload(ctx, ctx->out_c, offs.data(), nsub * 8);
+    if (!rc) rc = ctx->out_d.ensure(t * sizeof(gc_zstd_sequence));
+    if (!rc)
+        rc = launch_gather(ctx, ctx->stream, ctx->out_a.p, (const uint32_t*)ctx->out_b.p, (const uint64_t*)ctx->out_c.p,
+                           ctx->out_d.p, (uint32_t)nsub, max_seq);
+    if (rc) return rc;
+    e = hipMemcpyAsync(seqs, ctx->out_d.p, t * sizeof(gc_zstd_sequence), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "gc_zstd_find_sequences_packed: sequences");
 }
 
 }  // extern "C"

@@ -67,38 +67,44 @@ int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes) {
     return GC_OK;
 }
 
-static int launch(gc_ctx* ctx, hipFunction_t f, uint32_t grid, uint32_t block, void** args, const char* name) {
+static int launch(hipStream_t stream, hipFunction_t f, uint32_t grid, uint32_t block, void** args, const char* name) {
     if (grid == 0) return GC_OK;
-    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr);
+    hipError_t e = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, stream, args, nullptr);
     if (e != hipSuccess) return hip_fail(e, name);
     return GC_OK;
 }
 
-int launch_lzma2(gc_ctx* ctx, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
+int launch_lzma2(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
                  uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
-                 uint32_t max_matches, uint32_t depth) {
-    void* args[] = {&d_arena, &d_items, &n_items, &d_queue, &d_packed, &d_ref_m, &d_ref_c, &max_matches, &depth};
-    return launch(ctx, ctx->k.lzma2, grid, 256, args, "launch gc_lzma2_kernel");
+                 uint32_t max_matches) {
+    void* args[] = {&d_arena, &d_items, &n_items, &d_queue, &d_packed, &d_ref_m, &d_ref_c, &max_matches};
+    return launch(stream, ctx->k.lzma2, grid, 256, args, "launch gc_lzma2_kernel");
 }
 
-int launch_zstd(gc_ctx* ctx, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
+int launch_zstd(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
                 uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
                 uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
-                uint32_t* d_lit_counts, uint32_t depth, uint32_t max_seq) {
+                uint32_t* d_lit_counts, uint32_t max_seq) {
     void* a1[] = {&d_arena, &d_items, &n_items, &d_queue_cand, &d_info};
-    int rc = launch(ctx, ctx->k.zstd_cand, grid_cand, 256, a1, "launch gc_zstd_cand_kernel");
+    int rc = launch(stream, ctx->k.zstd_cand, grid_cand, 256, a1, "launch gc_zstd_cand_kernel");
     if (rc) return rc;
     const uint32_t* info = d_info;
     void* a2[] = {&d_arena, &d_items,        &n_items,       &d_queue_parse, &info,  &d_seq,
-                  &seq_fmt, &d_lit,          &d_seq_counts,  &d_lit_counts,  &depth, &max_seq};
-    return launch(ctx, ctx->k.zstd_parse, grid_parse, 256, a2, "launch gc_zstd_parse_kernel");
+                  &seq_fmt, &d_lit,          &d_seq_counts,  &d_lit_counts,  &max_seq};
+    return launch(stream, ctx->k.zstd_parse, grid_parse, 256, a2, "launch gc_zstd_parse_kernel");
 }
 
-int launch_fill(gc_ctx* ctx, uint32_t* p, uint32_t v, uint64_t n) {
+int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n) {
     uint64_t blocks = (n + 255) / 256;
     uint32_t grid = (uint32_t)(blocks < 4096 ? (blocks ? blocks : 1) : 4096);
     void* args[] = {&p, &v, &n};
-    return launch(ctx, ctx->k.fill_u32, grid, 256, args, "launch gc_fill_u32");
+    return launch(stream, ctx->k.fill_u32, grid, 256, args, "launch gc_fill_u32");
+}
+
+int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
+                  void* d_out, uint32_t nsub, uint32_t max_seq) {
+    void* args[] = {&d_in, &d_counts, &d_offsets, &d_out, &max_seq};
+    return launch(stream, ctx->k.gather_seqs, nsub, 256, args, "launch gc_gather_seqs");
 }
 
 uint32_t zstd_depth_for_level(uint32_t level) {
@@ -135,11 +141,13 @@ gc_ctx* gc_open(int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
     if (ctx->cu_count <= 0) ctx->cu_count = 256;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module, gc_code_object);
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2, ctx->k.module, "gc_lzma2_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_cand, ctx->k.module, "gc_zstd_cand_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);
@@ -158,8 +166,11 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_a.release();
     ctx->out_b.release();
     ctx->out_c.release();
+    ctx->out_d.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     delete ctx;
 }
 

@@ -32,6 +32,7 @@ struct Kernels {
     hipFunction_t zstd_cand = nullptr;
     hipFunction_t zstd_parse = nullptr;
     hipFunction_t fill_u32 = nullptr;
+    hipFunction_t gather_seqs = nullptr;
 };
 
 }  // namespace gc
@@ -39,27 +40,30 @@ struct Kernels {
 struct gc_ctx {
     int device = 0;
     int cu_count = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // main stream (default priority)
+    hipStream_t stream2 = nullptr;  // second stream for independent work (default priority)
     gc::Kernels k;
     // Scratch reused across calls.
     gc::DevBuf items;      // work items
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
-    gc::DevBuf out_a, out_b, out_c;
+    gc::DevBuf out_a, out_b, out_c, out_d;
     std::mutex mu;
 };
 
 namespace gc {
 
 // Launch helpers (all on ctx->stream).
-int launch_lzma2(gc_ctx* ctx, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
+int launch_lzma2(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
                  uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
-                 uint32_t max_matches, uint32_t depth);
-int launch_zstd(gc_ctx* ctx, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
+                 uint32_t max_matches);
+int launch_zstd(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
                 uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
                 uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
-                uint32_t* d_lit_counts, uint32_t depth, uint32_t max_seq);
-int launch_fill(gc_ctx* ctx, uint32_t* p, uint32_t v, uint64_t n);
+                uint32_t* d_lit_counts, uint32_t max_seq);
+int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
+int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
+                  void* d_out, uint32_t nsub, uint32_t max_seq);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
 
 uint32_t zstd_depth_for_level(uint32_t level);

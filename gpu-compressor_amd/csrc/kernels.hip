@@ -196,7 +196,8 @@ __device__ __forceinline__ void table_clear(CandLds& s) {
 // optionally the reference layout (slot 0 of LzmaMatch[max] and counts).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void lzma2_subblock(CandLds& s, const gc_lzma2_item it, const uint8_t* arena, uint32_t* packed,
-                               gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches, uint32_t depth) {
+                               gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
+    const uint32_t depth = it.depth;
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
@@ -248,12 +249,12 @@ __device__ __forceinline__ void lzma2_subblock(CandLds& s, const gc_lzma2_item i
 
 extern "C" __global__ void __launch_bounds__(CAND_THREADS)
 gc_lzma2_kernel(const uint8_t* arena, const gc_lzma2_item* items, uint32_t n_items, uint32_t* queue,
-                uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches, uint32_t depth) {
+                uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
     __shared__ CandLds s;
     __shared__ uint32_t next;
     if (!queue) {
         for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x)
-            lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches, depth);
+            lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches);
         return;
     }
     for (;;) {
@@ -262,7 +263,7 @@ gc_lzma2_kernel(const uint8_t* arena, const gc_lzma2_item* items, uint32_t n_ite
         uint32_t i = next;
         __syncthreads();
         if (i >= n_items) break;
-        lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches, depth);
+        lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches);
     }
 }
 
@@ -323,7 +324,8 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
 
 __device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const uint8_t* arena, const uint32_t* info,
                                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts,
-                                    uint32_t* lit_counts, uint32_t depth, uint32_t max_seq) {
+                                    uint32_t* lit_counts, uint32_t max_seq) {
+    const uint32_t depth = it.depth;
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
@@ -422,12 +424,11 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const
 extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
 gc_zstd_parse_kernel(const uint8_t* arena, const gc_zstd_item* items, uint32_t n_items, uint32_t* queue,
                      const uint32_t* info, void* seq_out, uint32_t seq_fmt, uint8_t* literals,
-                     uint32_t* seq_counts, uint32_t* lit_counts, uint32_t depth, uint32_t max_seq) {
+                     uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
     const uint32_t wave = threadIdx.x / WAVE;
     if (!queue) {
         for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES)
-            zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, depth,
-                                max_seq);
+            zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
         return;
     }
     for (;;) {
@@ -435,8 +436,7 @@ gc_zstd_parse_kernel(const uint8_t* arena, const gc_zstd_item* items, uint32_t n
         if (lane_id() == 0) i = atomicAdd(queue, 1u);
         i = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
         if (i >= n_items) break;
-        zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, depth,
-                            max_seq);
+        zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
     }
 }
 
@@ -444,4 +444,15 @@ gc_zstd_parse_kernel(const uint8_t* arena, const gc_zstd_item* items, uint32_t n
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// Packs the used sequence slots of each sub-block back to back:
+// out[offsets[s] + i] = in[s * max_seq + i] for i < counts[s].
+extern "C" __global__ void __launch_bounds__(256)
+gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets, uint4* out, uint32_t max_seq) {
+    const uint32_t s = blockIdx.x;
+    const uint32_t c = counts[s];
+    const uint4* src = in + (uint64_t)s * max_seq;
+    uint4* dst = out + offsets[s];
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
 }

@@ -57,7 +57,47 @@ SIGNATURES = [
     ("gc_gpu_find_matches", C.c_int, [_vp, _vp, _sz, _vp, _vp, _vp]),
     ("gc_lzma2_find_matches_packed", C.c_int, [_vp, _vp, _sz, _vp]),
     ("gc_zstd_find_sequences", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _vp]),
+    ("gc_zstd_find_sequences_packed", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp, _vp]),
+    ("gc_zstd_custom_bound", _sz, [_sz]),
+    ("gc_zstd_encode_sequences", C.c_int, [_vp, _sz, _vp, _vp, _vp, _sz, _vp, _vp, C.c_int]),
+    ("gc_compress_chunk_zstd_custom", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp, _vp]),
+    ("gc_zstd_frame_decode", C.c_int, [_vp, _sz, _vp, _sz, _vp]),
+    ("gc_lzma2_raw_encode", C.c_int, [_vp, _sz, C.c_uint32, _vp, _sz, _vp]),
+    ("gc_lzma2_raw_decode", C.c_int, [_vp, _sz, _vp, _sz, _vp]),
+    ("gc_encode_single_sub_block", C.c_int, [_vp, _sz, _vp, _vp, _sz, _vp]),
+    ("gc_container_bound", _sz, [_sz, C.c_uint32]),
+    ("gc_nvzs_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
+    ("gc_nvlz_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
+    ("gc_nvlz_l0_bound", _sz, [_sz, C.c_uint32]),
+    ("gc_nvlz_compress_l0", C.c_int, [_vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
+    ("gc_container_info", C.c_int, [_vp, _sz, _vp, _vp, _vp, _vp]),
+    ("gc_decompress", C.c_int, [_vp, _sz, _vp, _sz, _vp]),
+    ("gc_batch_create", _vp, [_vp, _vp, C.c_uint64, _vp, C.c_uint32]),
+    ("gc_batch_destroy", C.c_int, [_vp]),
+    ("gc_batch_run", C.c_int, [_vp]),
+    ("gc_batch_run_timed", C.c_int, [_vp, _vp]),
+    ("gc_batch_set_tune", C.c_int, [_vp, _vp]),
+    ("gc_batch_get_tune", C.c_int, [_vp, _vp]),
+    ("gc_batch_autotune", C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    ("gc_batch_info", C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp]),
+    ("gc_batch_device_outputs", C.c_int, [_vp, _vp, _vp, _vp]),
+    ("gc_batch_copy_zstd", C.c_int, [_vp, C.c_uint32, _vp, _vp]),
+    ("gc_batch_copy_lzma2", C.c_int, [_vp, C.c_uint32, _vp]),
+    ("gc_corpus_generate", C.c_int, [_vp, C.c_uint64, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint64,
+                                     C.c_uint64]),
 ]
+
+ALGO_ZSTD = 0
+ALGO_LZMA2 = 1
+
+
+class BatchInput(C.Structure):
+    _fields_ = [("offset", C.c_uint64), ("size", C.c_uint64), ("algo", C.c_uint32), ("level", C.c_uint32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("cand_wg_per_cu", C.c_uint32), ("parse_wg_per_cu", C.c_uint32),
+                ("lzma2_wg_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class GcError(RuntimeError):
@@ -195,3 +235,188 @@ def gpu_find_matches(chunk_data, device_id: int = 0) -> GpuMatchData:
     ctx = context(device_id)
     _check(lib().gc_gpu_find_matches(ctx.h, _ptr(src), src.size, _ptr(d), _ptr(l), _ptr(c)), "gc_gpu_find_matches")
     return GpuMatchData(d, l, c, nsub, SUB_BLOCK, LZMA2_MAX_MATCHES)
+
+
+def corpus(n: int, seed: int, weights=(1, 1, 1, 1), segment: int = 65536, exact_shares: bool = True,
+           gap_every: int = 0, gap_len: int = 0) -> np.ndarray:
+    """Deterministic four-kind synthetic corpus (gc_corpus_generate)."""
+    out = np.empty(n, dtype=np.uint8)
+    w = np.asarray(weights, dtype=np.float64)
+    _check(lib().gc_corpus_generate(_ptr(out), n, seed, _ptr(w), segment, int(exact_shares), gap_every, gap_len),
+           "gc_corpus_generate")
+    return out
+
+
+class Batch:
+    """Many inputs in one persistent, graph-launched pass (gc_batch_*).
+
+    `inputs` is a list of (offset, size, algo, level) over a device arena."""
+
+    def __init__(self, ctx: Context, d_arena: int, arena_size: int, inputs):
+        self.ctx = ctx
+        arr = (BatchInput * len(inputs))(*[BatchInput(*t) for t in inputs])
+        self.inputs = list(inputs)
+        h = lib().gc_batch_create(ctx.h, d_arena, arena_size, C.cast(arr, _vp), len(inputs))
+        if not h:
+            raise GcError(f"gc_batch_create: {lib().gc_last_error().decode()}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().gc_batch_destroy(self.h)
+            self.h = None
+
+    def run(self):
+        _check(lib().gc_batch_run(self.h), "gc_batch_run")
+
+    def run_timed(self):
+        ms = (C.c_float * 3)()
+        _check(lib().gc_batch_run_timed(self.h, C.cast(ms, _vp)), "gc_batch_run_timed")
+        return list(ms)
+
+    def set_tune(self, cand, parse, lzma2):
+        t = Tune(cand, parse, lzma2, 0)
+        _check(lib().gc_batch_set_tune(self.h, C.byref(t)), "gc_batch_set_tune")
+
+    def tune(self):
+        t = Tune()
+        _check(lib().gc_batch_get_tune(self.h, C.byref(t)), "gc_batch_get_tune")
+        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.lzma2_wg_per_cu)
+
+    def autotune(self, reps=2):
+        t = Tune()
+        ms = C.c_float()
+        _check(lib().gc_batch_autotune(self.h, reps, C.byref(t), C.byref(ms)), "gc_batch_autotune")
+        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.lzma2_wg_per_cu), ms.value
+
+    def n_sub(self, i):
+        ns = C.c_uint32()
+        _check(lib().gc_batch_info(self.h, i, C.byref(ns), None, None), "gc_batch_info")
+        return ns.value
+
+    def zstd(self, i, with_seqs=True):
+        ns = self.n_sub(i)
+        counts = np.zeros(ns, np.uint32)
+        seqs = np.zeros((ns, MAX_SEQ, 4), np.uint32) if with_seqs else None
+        _check(lib().gc_batch_copy_zstd(self.h, i, _ptr(seqs) if with_seqs else None, _ptr(counts)),
+               "gc_batch_copy_zstd")
+        return seqs, counts
+
+    def lzma2(self, i):
+        out = np.zeros(self.inputs[i][1], np.uint32)
+        _check(lib().gc_batch_copy_lzma2(self.h, i, _ptr(out)), "gc_batch_copy_lzma2")
+        return out
+
+
+# ---- consumers and containers ------------------------------------------------
+
+def compress_chunk_zstd_custom(chunk_data, device_id: int = 0, level: int = 1):
+    """compress_zstd_custom.rs:47-203 -> (list of frames, list of sizes)."""
+    src = _as_u8(chunk_data)
+    nsub = -(-src.size // SUB_BLOCK)
+    cap = lib().gc_zstd_custom_bound(src.size)
+    out = np.empty(max(cap, 1), np.uint8)
+    sizes = np.zeros(max(nsub, 1), np.uint64)
+    n = C.c_size_t()
+    _check(lib().gc_compress_chunk_zstd_custom(context(device_id).h, _ptr(src), src.size, level, _ptr(out), cap,
+                                               _ptr(sizes), C.byref(n)), "gc_compress_chunk_zstd_custom")
+    frames, o = [], 0
+    for s in sizes[:nsub]:
+        frames.append(out[o:o + int(s)].tobytes())
+        o += int(s)
+    return frames, [int(s) for s in sizes[:nsub]]
+
+
+def encode_single_sub_block(sub_data, match_counts) -> bytes:
+    """compress_lzma2_custom.rs:570-597."""
+    src = _as_u8(sub_data)
+    mc = np.ascontiguousarray(match_counts, dtype=np.uint32)
+    cap = src.size + src.size // 8 + 1024
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_encode_single_sub_block(_ptr(src), src.size, _ptr(mc), _ptr(out), cap, C.byref(n)),
+           "gc_encode_single_sub_block")
+    return out[: n.value].tobytes()
+
+
+def nvzs_compress(data, level: int = 1, device_id: int = 0, threads: int = 1) -> bytes:
+    src = _as_u8(data)
+    cap = lib().gc_container_bound(src.size, ALGO_ZSTD)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvzs_compress(context(device_id).h, _ptr(src), src.size, level, threads, _ptr(out), cap,
+                                  C.byref(n)), "gc_nvzs_compress")
+    return out[: n.value].tobytes()
+
+
+def nvlz_compress(data, level: int = 1, dict_mb: int = 128, device_id: int = 0, threads: int = 0) -> bytes:
+    src = _as_u8(data)
+    cap = lib().gc_container_bound(src.size, ALGO_LZMA2)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvlz_compress(context(device_id).h, _ptr(src), src.size, level, dict_mb, threads, _ptr(out),
+                                  cap, C.byref(n)), "gc_nvlz_compress")
+    return out[: n.value].tobytes()
+
+
+def nvlz_compress_l0(data, dict_mb: int = 128, preset: int = 0, threads: int = 0) -> bytes:
+    src = _as_u8(data)
+    cap = lib().gc_nvlz_l0_bound(src.size, dict_mb)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvlz_compress_l0(_ptr(src), src.size, dict_mb, preset, threads, _ptr(out), cap, C.byref(n)),
+           "gc_nvlz_compress_l0")
+    return out[: n.value].tobytes()
+
+
+def container_info(blob: bytes):
+    b = _as_u8(blob)
+    algo, orig, ch, nch = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+    _check(lib().gc_container_info(_ptr(b), b.size, C.byref(algo), C.byref(orig), C.byref(ch), C.byref(nch)),
+           "gc_container_info")
+    return algo.value, orig.value, ch.value, nch.value
+
+
+def decompress(blob: bytes) -> bytes:
+    """decompress.rs:9-379 (NVZS via libzstd, NVLZ via liblzma)."""
+    b = _as_u8(blob)
+    _, orig, _, _ = container_info(b)
+    out = np.empty(max(orig, 1), np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_decompress(_ptr(b), b.size, _ptr(out), orig, C.byref(n)), "gc_decompress")
+    return out[: n.value].tobytes()
+
+
+def zstd_encode_sequences(src, seqs: np.ndarray, counts: np.ndarray, threads: int = 1):
+    """CPU consumer alone: packed ZSTD_Sequence rows + per-sub-chunk counts -> (frames, sizes)."""
+    s = _as_u8(src)
+    seqs = np.ascontiguousarray(seqs, dtype=np.uint32).reshape(-1, 4)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    cap = lib().gc_zstd_custom_bound(s.size)
+    out = np.empty(max(cap, 1), np.uint8)
+    nsub = -(-s.size // SUB_BLOCK)
+    sizes = np.zeros(max(nsub, 1), np.uint64)
+    n = C.c_size_t()
+    _check(lib().gc_zstd_encode_sequences(_ptr(s), s.size, _ptr(seqs), _ptr(counts), _ptr(out), cap, _ptr(sizes),
+                                          C.byref(n), threads), "gc_zstd_encode_sequences")
+    frames, o = [], 0
+    for z in sizes[:nsub]:
+        frames.append(out[o:o + int(z)].tobytes())
+        o += int(z)
+    return frames, [int(z) for z in sizes[:nsub]]
+
+
+def zstd_frame_decode(frame: bytes, expected: int) -> bytes:
+    f = _as_u8(frame)
+    out = np.empty(max(expected, 1), np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_zstd_frame_decode(_ptr(f), f.size, _ptr(out), expected, C.byref(n)), "gc_zstd_frame_decode")
+    return out[: n.value].tobytes()
+
+
+def lzma2_raw_decode(blob: bytes, expected: int) -> bytes:
+    f = _as_u8(blob)
+    out = np.empty(max(expected, 1), np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_lzma2_raw_decode(_ptr(f), f.size, _ptr(out), expected, C.byref(n)), "gc_lzma2_raw_decode")
+    return out[: n.value].tobytes()

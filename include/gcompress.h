@@ -140,6 +140,139 @@ int gc_lzma2_find_matches_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, ui
 int gc_zstd_find_sequences(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, gc_zstd_sequence* seqs,
                            uint32_t* seq_counts);
 
+/* Sequences of every 64 KiB sub-chunk packed back to back (sub-chunk s
+ * starts at sum(seq_counts[0..s))): what ZSTD_compressSequences consumes.
+ * total receives the number of sequences; with seqs == NULL only the counts
+ * are returned; GC_ESIZE if seq_capacity < total. */
+int gc_zstd_find_sequences_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, gc_zstd_sequence* seqs,
+                                  size_t seq_capacity, uint32_t* seq_counts, size_t* total);
+
+/* ---- consumers and containers (CPU; libzstd / liblzma) ----------------- */
+
+/* Frames produced by gc_compress_chunk_zstd_custom for an n-byte chunk fit in
+ * this many bytes (n_sub * ZSTD_compressBound(65536)); 0 if libzstd is missing. */
+size_t gc_zstd_custom_bound(size_t n);
+
+/* compress_zstd_custom.rs:120-199: one frame per 64 KiB sub-chunk from the
+ * packed sequences (ZSTD_compressSequences, default CCtx level, reset per
+ * sub-chunk; ZSTD_compress level 1 when libzstd rejects the sequences).
+ * Frames are written back to back; sizes[s] receives each frame's size.
+ * threads = 1 is the reference's shape; more threads give the same bytes. */
+int gc_zstd_encode_sequences(const uint8_t* src, size_t n, const gc_zstd_sequence* seqs, const uint32_t* seq_counts,
+                             uint8_t* out, size_t cap, uint64_t* sizes, size_t* out_len, int threads);
+
+/* compress_chunk_zstd_custom (compress_zstd_custom.rs:47-203): GPU match
+ * finding + libzstd; (Vec<Vec<u8>>, Vec<usize>) becomes frames back to back
+ * in out plus sizes[n_sub]. */
+int gc_compress_chunk_zstd_custom(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out,
+                                  size_t cap, uint64_t* sizes, size_t* out_len);
+
+/* One Zstd frame -> exactly `expected` bytes (replaces the nvCOMP decode of
+ * decompress.rs:146-365 with libzstd ZSTD_decompress). */
+int gc_zstd_frame_decode(const uint8_t* src, size_t n, uint8_t* out, size_t expected, size_t* out_len);
+
+/* compress_lzma2.rs:89-133: liblzma raw LZMA2 encoder at preset (<= 9). */
+int gc_lzma2_raw_encode(const uint8_t* src, size_t n, uint32_t preset, uint8_t* out, size_t cap, size_t* out_len);
+
+/* compress_lzma2.rs:135-172: raw LZMA2 decoder, 64 MiB dictionary. */
+int gc_lzma2_raw_decode(const uint8_t* src, size_t n, uint8_t* out, size_t expected, size_t* out_len);
+
+/* encode_single_sub_block (compress_lzma2_custom.rs:570-597): liblzma
+ * preset 9 if any match_counts[i] > 0, else raw LZMA2 chunks; both end with
+ * the 0x00 end marker.  cap >= n + n/8 + 1024 always suffices. */
+int gc_encode_single_sub_block(const uint8_t* sub, size_t n, const uint32_t* match_counts, uint8_t* out, size_t cap,
+                               size_t* out_len);
+
+/* Largest NVZS (algo GC_ALGO_ZSTD) / NVLZ level >= 1 container for n bytes. */
+size_t gc_container_bound(size_t n, uint32_t algo);
+
+/* pipeline.rs:19-282 at level > 0: NVZS container of the whole input
+ * (chunk size 65536, one frame per sub-chunk). */
+int gc_nvzs_compress(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, int threads, uint8_t* out,
+                     size_t cap, size_t* out_len);
+
+/* pipeline_lzma2.rs:150-292 at level > 0: dictionary-size chunks (dict_mb,
+ * 0 = 128) match-found on the GPU, sub-blocks encoded by a pool of threads
+ * (<= 0: min(cores, 8)), NVLZ with chunk size 65536. */
+int gc_nvlz_compress(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint32_t dict_mb, int threads,
+                     uint8_t* out, size_t cap, size_t* out_len);
+
+/* pipeline_lzma2.rs:293-333, level 0 (CPU only, the reference's baseline):
+ * liblzma on whole dictionary-size chunks at `preset`. */
+size_t gc_nvlz_l0_bound(size_t n, uint32_t dict_mb);
+int gc_nvlz_compress_l0(const uint8_t* src, size_t n, uint32_t dict_mb, uint32_t preset, int threads, uint8_t* out,
+                        size_t cap, size_t* out_len);
+
+/* decompress.rs:35-143: header of an NVZS / NVLZ container. */
+int gc_container_info(const uint8_t* in, size_t n, uint32_t* algo, uint64_t* original_size, uint64_t* chunk_size,
+                      uint64_t* num_chunks);
+
+/* decompress.rs:9-379: whole container -> original bytes. */
+int gc_decompress(const uint8_t* in, size_t n, uint8_t* out, size_t cap, size_t* out_len);
+
+/* ---- batched form ----------------------------------------------------- */
+/* No single reference counterpart: the reference calls the two host
+ * functions once per chunk (pipeline.rs:145, pipeline_lzma2.rs:190).  A batch
+ * puts every 64 KiB sub-block of many inputs into one device work queue,
+ * drained by persistent kernels launched from one captured hipGraph.  Per
+ * input, results equal gc_zstd_find_sequences / gc_lzma2_find_matches_packed
+ * on that input alone. */
+
+#define GC_ALGO_ZSTD 0u
+#define GC_ALGO_LZMA2 1u
+
+typedef struct {
+    uint64_t offset; /* byte offset of the input in the device arena (multiple of 4) */
+    uint64_t size;   /* input bytes */
+    uint32_t algo;   /* GC_ALGO_ZSTD or GC_ALGO_LZMA2 */
+    uint32_t level;  /* Zstd: 1 -> depth 16, 2 -> depth 64, else 1; LZMA2: depth 32 */
+} gc_batch_input;
+
+/* Output-invariant launch knobs (persistent workgroups per CU). */
+typedef struct {
+    uint32_t cand_wg_per_cu;  /* candidate pass, 1..4 (LDS bound) */
+    uint32_t parse_wg_per_cu; /* Zstd parse, 1..16 */
+    uint32_t lzma2_wg_per_cu; /* LZMA2, 1..4 (LDS bound) */
+    uint32_t reserved;
+} gc_tune;
+
+typedef struct gc_batch gc_batch;
+
+/* Inputs must lie inside d_arena[0, arena_size); the arena stays owned by the
+ * caller and must outlive the batch.  Output buffers are owned by the batch:
+ * Zstd sequences (21845 slots of gc_zstd_sequence per sub-block) and
+ * per-sub-block counts, LZMA2 packed matches (one u32 per position). */
+gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_size, const gc_batch_input* inputs,
+                          uint32_t n_inputs);
+int gc_batch_destroy(gc_batch* b);
+/* Enqueue one pass on the context stream (graph launch; asynchronous). */
+int gc_batch_run(gc_batch* b);
+/* One pass launched kernel by kernel with events, synchronous:
+ * ms[0] LZMA2 kernel, ms[1] Zstd candidate kernel, ms[2] Zstd parse kernel. */
+int gc_batch_run_timed(gc_batch* b, float* ms);
+int gc_batch_set_tune(gc_batch* b, const gc_tune* t);
+int gc_batch_get_tune(gc_batch* b, gc_tune* t);
+/* Try the knob values on this batch (reps timed passes each) and keep the fastest. */
+int gc_batch_autotune(gc_batch* b, uint32_t reps, gc_tune* best, float* best_ms);
+/* Where input i's results live: n_sub sub-blocks; Zstd: sequences from slot
+ * out_base, counts from count_base; LZMA2: packed from position out_base. */
+int gc_batch_info(gc_batch* b, uint32_t i, uint32_t* n_sub, uint64_t* out_base, uint64_t* count_base);
+int gc_batch_device_outputs(gc_batch* b, gc_zstd_sequence** d_seqs, uint32_t** d_counts, uint32_t** d_packed);
+/* Copy input i's results to the host (synchronous).  Zstd: counts[n_sub] and
+ * seqs[n_sub * 21845] (only the used slots are written; seqs may be NULL). */
+int gc_batch_copy_zstd(gc_batch* b, uint32_t i, gc_zstd_sequence* seqs, uint32_t* counts);
+int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed);
+
+/* ---- synthetic corpora -------------------------------------------------- */
+/* Deterministic four-kind mix (uniform random, word text, back-references,
+ * structured binary) in segments of `segment` bytes; weights[4] are relative
+ * shares.  exact_shares: each group of 20 segments holds round(20*w) of each
+ * kind, shuffled; otherwise kinds are drawn independently.  gap_every /
+ * gap_len: a zero gap of gap_len bytes after every gap_every bytes of content
+ * (0 = none).  No reference counterpart (workload generator). */
+int gc_corpus_generate(uint8_t* out, uint64_t n, uint64_t seed, const double* weights, uint64_t segment,
+                       uint32_t exact_shares, uint64_t gap_every, uint64_t gap_len);
+
 #ifdef __cplusplus
 }
 #endif

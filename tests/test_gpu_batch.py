@@ -1,0 +1,66 @@
+"""The batched form (persistent kernels, device queue, hipGraph) equals the plain form per input."""
+import numpy as np
+import pytest
+
+import gcompress
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _arena(ctx, inputs_data, align=64):
+    offs, total = [], 0
+    for d in inputs_data:
+        offs.append(total)
+        total += -(-max(d.size, 1) // align) * align
+    buf = np.zeros(total + 256, np.uint8)
+    for o, d in zip(offs, inputs_data):
+        buf[o:o + d.size] = d
+    p = ctx.alloc(buf.size)
+    ctx.htod(p, buf)
+    return p, buf.size, offs
+
+
+def test_batch_mixed_equals_plain(gctx):
+    rng = np.random.default_rng(5)
+    sizes = [1, 3, 4, 100, 65535, 65536, 65537, 200000, 1 << 20, 3 * 65536 + 7, 777777, 50000]
+    datas = [gcompress.corpus(s, 100 + i, rng.dirichlet([1, 1, 1, 1])) for i, s in enumerate(sizes)]
+    algos = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)]
+    p, size, offs = _arena(gctx, datas)
+    inputs = [(offs[i], sizes[i], *algos[i % 3]) for i in range(len(sizes))]
+    b = gcompress.Batch(gctx, p, size, inputs)
+    try:
+        for rep in range(3):  # replays of one graph give the same results
+            print(f"[batch] replay {rep}", flush=True)
+            b.run()
+            gctx.sync()
+            for i, (o, s, algo, lv) in enumerate(inputs):
+                if algo == gcompress.ALGO_LZMA2:
+                    got = b.lzma2(i)
+                    np.testing.assert_array_equal(got, oracle.lzma2_packed(datas[i]))
+                else:
+                    seqs, counts = b.zstd(i)
+                    oseq, osc, _, _ = oracle.zstd_match_find(datas[i], depth={1: 16, 2: 64}[lv])
+                    np.testing.assert_array_equal(counts, osc)
+                    for k, z in enumerate(oracle.zstd_as_zstd_sequences(oseq, osc)):
+                        np.testing.assert_array_equal(seqs[k, : counts[k]], z)
+        ms = b.run_timed()
+        assert len(ms) == 3 and all(m >= 0 for m in ms)
+        best, _ = b.autotune(reps=1)
+        b.run()
+        gctx.sync()
+        np.testing.assert_array_equal(b.lzma2(2), oracle.lzma2_packed(datas[2]))
+    finally:
+        b.close()
+        gctx.free(p)
+
+
+def test_batch_rejects_out_of_arena(gctx):
+    p = gctx.alloc(4096)
+    try:
+        with pytest.raises(gcompress.GcError):
+            gcompress.Batch(gctx, p, 4096, [(0, 8192, gcompress.ALGO_ZSTD, 1)])
+        with pytest.raises(gcompress.GcError):
+            gcompress.Batch(gctx, p, 4096, [(2, 100, gcompress.ALGO_ZSTD, 1)])
+    finally:
+        gctx.free(p)
