@@ -1,0 +1,71 @@
+"""End to end on the GPU: match finding -> libzstd / liblzma -> NVZS / NVLZ -> byte-exact round trip."""
+import numpy as np
+import pytest
+
+import datagen
+import gcompress
+import oracle
+
+pytestmark = pytest.mark.gpu
+SUB = 65536
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_compress_chunk_zstd_custom_round_trip(level):
+    data = datagen.mix(6 * SUB + 999, 21)
+    print(f"[e2e zstd] level={level} n={data.size}", flush=True)
+    frames, sizes = gcompress.compress_chunk_zstd_custom(data, 0, level)
+    assert len(frames) == 7 and sizes == [len(f) for f in frames]
+    for s, f in enumerate(frames):
+        want = data[s * SUB:(s + 1) * SUB].tobytes()
+        assert gcompress.zstd_frame_decode(f, len(want)) == want
+    # the frames are exactly those of the oracle's sequences through the same consumer
+    seq, sc, _, _ = oracle.zstd_match_find(data, depth={1: 16, 2: 64}[level])
+    rows = np.concatenate(oracle.zstd_as_zstd_sequences(seq, sc))
+    ofr, _ = gcompress.zstd_encode_sequences(data, rows, sc, 1)
+    assert ofr == frames
+
+
+def test_packed_sequences_match_full_layout(gctx):
+    data = datagen.mix(4 * SUB + 17, 22)
+    seqs, counts = gctx.zstd_sequences(data, 1)
+    nsub = counts.size
+    packed = np.zeros((nsub * gcompress.MAX_SEQ, 4), np.uint32)
+    cnt = np.zeros(nsub, np.uint32)
+    import ctypes as C
+    total = C.c_size_t()
+    rc = gcompress.lib().gc_zstd_find_sequences_packed(gctx.h, data.ctypes.data, data.size, 1, packed.ctypes.data,
+                                                        packed.shape[0], cnt.ctypes.data, C.byref(total))
+    assert rc == 0
+    np.testing.assert_array_equal(cnt, counts)
+    np.testing.assert_array_equal(packed[: total.value], np.concatenate([seqs[s, : counts[s]] for s in range(nsub)]))
+
+
+@pytest.mark.parametrize("n", [0, 1, 100, SUB, 3 * SUB + 5])
+def test_nvzs_round_trip(n):
+    data = datagen.mix(n, 30) if n else np.zeros(0, np.uint8)
+    print(f"[e2e nvzs] n={n}", flush=True)
+    blob = gcompress.nvzs_compress(data, 1)
+    algo, orig, ch, nch = gcompress.container_info(blob)
+    assert (algo, orig, ch, nch) == (gcompress.ALGO_ZSTD, n, SUB, -(-n // SUB))
+    assert gcompress.decompress(blob) == data.tobytes()
+
+
+@pytest.mark.parametrize("n", [0, 5, SUB, 5 * SUB + 3])
+def test_nvlz_round_trip_and_triage(n):
+    data = gcompress.corpus(n, 31, (15, 25, 20, 40), 4096) if n else np.zeros(0, np.uint8)
+    print(f"[e2e nvlz] n={n}", flush=True)
+    blob = gcompress.nvlz_compress(data, 1, dict_mb=1, threads=4)
+    algo, orig, ch, nch = gcompress.container_info(blob)
+    assert (algo, orig, ch, nch) == (gcompress.ALGO_LZMA2, n, SUB, -(-n // SUB))
+    assert gcompress.decompress(blob) == data.tobytes()
+    # triage: an entry is raw exactly when the oracle finds no match in that sub-block
+    if n:
+        _, c = oracle.lzma2_match_find(data, SUB, 8, 32)
+        sizes = np.frombuffer(blob[28:28 + 8 * nch], "<u8")
+        pos = 28 + 8 * nch
+        for s in range(nch):
+            entry = blob[pos:pos + sizes[s]]
+            raw = not c[s * SUB:(s + 1) * SUB].any()
+            assert (entry[0] == 0x01 and len(entry) == min(SUB, n - s * SUB) + 4) == raw
+            pos += sizes[s]
