@@ -1,4 +1,4 @@
-"""Match-find throughput on one MI355X: batched (persistent kernels + hipGraph) vs plain per-chunk.
+"""Match-find throughput on one MI355X: batched (one launch per kernel over all chunks) vs plain per-chunk.
 
 Default workload: the four-kind synthetic mix at 1 GiB in 16 MiB chunks, match-found at Zstd L1
 (lazy, depth 16) and L2 (depth 64) on one stream.  A step is one pass over the whole GiB at both
@@ -229,7 +229,7 @@ def main():
             "workload": (f"drop-in Zstd L1+L2 match-find, {a.size_mib} MiB mix in {a.chunk_mib} MiB chunks"
                          if algo == gcompress.ALGO_ZSTD else
                          f"LZMA2 L1 match-find, {a.size_mib} MiB disk-image-like mix"),
-            "form": "batched: persistent kernels over a device work queue, one hipGraph launch per step",
+            "form": "batched: every sub-block of every chunk and level in one launch per kernel (grid-stride items)",
             "input_bytes_per_step": work_bytes,
             "sub_block": 65536,
             "levels": levels,

@@ -1,6 +1,5 @@
 // gc_batch.cpp -- batched form: many inputs of uneven size, every 64 KiB
-// sub-block of every input in one device work queue, drained by persistent
-// kernels; the launch sequence is captured once in a hipGraph and replayed.
+// sub-block of every input in one launch per kernel.
 //
 // Per input the results are exactly those of the plain per-chunk form:
 // sub-blocks are tiled from each input's own start and are independent.
@@ -24,24 +23,14 @@ struct gc_batch {
     std::vector<uint64_t> cnt_base;  // zstd: first count index
     uint32_t n_z = 0, n_l = 0;       // items
     uint64_t z_bytes = 0, l_bytes = 0;
-    DevBuf zitems, litems, info, seq, counts, packed, queue;
+    DevBuf zitems, litems, info, seq, counts, packed;
     gc_tune tune{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     hipEvent_t ev[6] = {};
-    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
 
 const uint32_t kMaxSeq = GC_SUB_BLOCK / 3;
-
-void drop_graph(gc_batch* b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-}
 
 uint32_t grid_for(gc_batch* b, uint32_t per_cu, uint32_t items) {
     uint64_t g = (uint64_t)per_cu * (uint64_t)b->ctx->cu_count;
@@ -49,73 +38,38 @@ uint32_t grid_for(gc_batch* b, uint32_t per_cu, uint32_t items) {
     return (uint32_t)g;
 }
 
-// Enqueue one full pass: reset the queues, then the Zstd chain on the main
-// stream and LZMA2 on the second stream, joined back into the main stream.
+// Enqueue one full pass on the main stream.  Items of all inputs go to
+// each kernel in one launch; every workgroup takes items blockIdx.x,
+// blockIdx.x + gridDim.x, ... (static grid-stride assignment).
+//
+// The other form built here, a device work queue drained by persistent
+// workgroups and replayed from a captured hipGraph with the LZMA2 kernel
+// on a forked stream, hung on its first replay on MI355X and its cause has
+// not been found; it is switched off (gc_batch_set_mode refuses it).
 int enqueue(gc_batch* b, bool timed) {
     gc_ctx* ctx = b->ctx;
-    uint32_t* q = (uint32_t*)b->queue.p;
-    int rc = launch_fill(ctx, ctx->stream, q, 0, 4);
-    if (rc) return rc;
     if (timed) (void)hipEventRecord(b->ev[0], ctx->stream);
-    const bool both = b->n_z && b->n_l;
-    hipStream_t ls = ctx->stream;
-    if (both && !timed) {
-        hipError_t e = hipEventRecord(b->fork, ctx->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream2, b->fork, 0);
-        if (e != hipSuccess) return hip_fail(e, "batch fork");
-        ls = ctx->stream2;
-    }
+    int rc;
     if (b->n_l) {
-        rc = launch_lzma2(ctx, ls, b->arena, (const gc_lzma2_item*)b->litems.p, b->n_l, q + 2,
+        rc = launch_lzma2(ctx, ctx->stream, b->arena, (const gc_lzma2_item*)b->litems.p, b->n_l, nullptr,
                           grid_for(b, b->tune.lzma2_wg_per_cu, b->n_l), (uint32_t*)b->packed.p, nullptr, nullptr, 1);
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[1], ctx->stream);
     if (b->n_z) {
-        // cand and parse as two launches so that each can be timed
-        rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, q, q + 1,
-                         grid_for(b, b->tune.cand_wg_per_cu, b->n_z),
-                         timed ? 0u : grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
-                         (uint32_t*)b->info.p, b->seq.p, GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr,
-                         kMaxSeq);
+        rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, nullptr, nullptr,
+                         grid_for(b, b->tune.cand_wg_per_cu, b->n_z), 0, (uint32_t*)b->info.p, b->seq.p,
+                         GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr, kMaxSeq);
         if (rc) return rc;
-        if (timed) {
-            (void)hipEventRecord(b->ev[2], ctx->stream);
-            rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, q, q + 1, 0,
-                             grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4), (uint32_t*)b->info.p, b->seq.p,
-                             GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr, kMaxSeq);
-            if (rc) return rc;
-        }
-    } else if (timed) {
-        (void)hipEventRecord(b->ev[2], ctx->stream);
+    }
+    if (timed) (void)hipEventRecord(b->ev[2], ctx->stream);
+    if (b->n_z) {
+        rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, nullptr, nullptr, 0,
+                         grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4), (uint32_t*)b->info.p, b->seq.p,
+                         GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr, kMaxSeq);
+        if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[3], ctx->stream);
-    if (both && !timed) {
-        hipError_t e = hipEventRecord(b->join, ctx->stream2);
-        if (e == hipSuccess) e = hipStreamWaitEvent(ctx->stream, b->join, 0);
-        if (e != hipSuccess) return hip_fail(e, "batch join");
-    }
-    return GC_OK;
-}
-
-int build_graph(gc_batch* b) {
-    gc_ctx* ctx = b->ctx;
-    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    int rc = enqueue(b, false);
-    hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(ctx->stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    b->graph = g;
-    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        drop_graph(b);
-        return hip_fail(e, "hipGraphInstantiate");
-    }
     return GC_OK;
 }
 
@@ -140,7 +94,7 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     b->arena = d_arena;
     b->arena_size = arena_size;
     b->inputs.assign(inputs, inputs + n_inputs);
-    b->tune = gc_tune{4, 8, 4, 0};
+    b->tune = gc_tune{64, 64, 64, 0};  // grid = items unless tuned down
     std::vector<gc_zstd_item> zi;
     std::vector<gc_lzma2_item> li;
     uint64_t zseq = 0, zcnt = 0, zinfo = 0, lpos = 0;
@@ -198,8 +152,8 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     std::stable_sort(li.begin(), li.end(), lcost);
     b->n_z = (uint32_t)zi.size();
     b->n_l = (uint32_t)li.size();
-    int rc = b->queue.ensure(16);
-    if (!rc && b->n_z) rc = upload(ctx, b->zitems, zi.data(), zi.size() * sizeof(gc_zstd_item));
+    int rc = GC_OK;
+    if (b->n_z) rc = upload(ctx, b->zitems, zi.data(), zi.size() * sizeof(gc_zstd_item));
     if (!rc && b->n_l) rc = upload(ctx, b->litems, li.data(), li.size() * sizeof(gc_lzma2_item));
     if (!rc && b->n_z) rc = b->info.ensure(zinfo * 4);
     if (!rc && b->n_z) rc = b->seq.ensure(zseq * sizeof(gc_zstd_sequence));
@@ -208,11 +162,6 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     for (int k = 0; !rc && k < 6; k++) {
         hipError_t e = hipEventCreate(&b->ev[k]);
         if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate");
-    }
-    if (!rc) {
-        hipError_t e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
-        if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate(fork/join)");
     }
     if (rc) {
         std::string keep = gc_last_error();
@@ -227,30 +176,24 @@ int gc_batch_destroy(gc_batch* b) {
     if (!b) return GC_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    (void)hipStreamSynchronize(b->ctx->stream2);
-    drop_graph(b);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    if (b->fork) (void)hipEventDestroy(b->fork);
-    if (b->join) (void)hipEventDestroy(b->join);
     b->zitems.release();
     b->litems.release();
     b->info.release();
     b->seq.release();
     b->counts.release();
     b->packed.release();
-    b->queue.release();
     delete b;
     return GC_OK;
 }
 
 int gc_batch_set_tune(gc_batch* b, const gc_tune* t) {
     if (!b || !t) return fail(GC_EINVAL, "null argument");
-    if (t->cand_wg_per_cu < 1 || t->cand_wg_per_cu > 4 || t->lzma2_wg_per_cu < 1 || t->lzma2_wg_per_cu > 4 ||
-        t->parse_wg_per_cu < 1 || t->parse_wg_per_cu > 16)
-        return fail(GC_EINVAL, "tune out of range (cand/lzma2 1..4, parse 1..16 workgroups per CU)");
+    if (t->cand_wg_per_cu < 1 || t->cand_wg_per_cu > 64 || t->lzma2_wg_per_cu < 1 || t->lzma2_wg_per_cu > 64 ||
+        t->parse_wg_per_cu < 1 || t->parse_wg_per_cu > 64)
+        return fail(GC_EINVAL, "tune out of range (1..64 workgroups per CU)");
     b->tune = *t;
-    drop_graph(b);  // grid sizes are baked into the graph
     return GC_OK;
 }
 
@@ -264,12 +207,17 @@ int gc_batch_run(gc_batch* b) {
     if (!b) return fail(GC_EINVAL, "null batch");
     std::lock_guard<std::mutex> g(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
-    if (!b->exec) {
-        int rc = build_graph(b);
-        if (rc) return rc;
-    }
-    hipError_t e = hipGraphLaunch(b->exec, b->ctx->stream);
-    return e == hipSuccess ? GC_OK : hip_fail(e, "hipGraphLaunch");
+    return enqueue(b, false);
+}
+
+int gc_batch_set_mode(gc_batch* b, uint32_t mode) {
+    if (!b) return fail(GC_EINVAL, "null batch");
+    if (mode == GC_BATCH_STATIC) return GC_OK;
+    if (mode == GC_BATCH_QUEUE_GRAPH)
+        return fail(GC_EINVAL,
+                    "the device-queue + hipGraph form is switched off: it hung on its first replay on MI355X "
+                    "and the cause has not been found");
+    return fail(GC_EINVAL, "unknown batch mode");
 }
 
 int gc_batch_run_timed(gc_batch* b, float* ms) {
@@ -333,11 +281,12 @@ int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed) {
 
 int gc_batch_autotune(gc_batch* b, uint32_t reps, gc_tune* best, float* best_ms) {
     if (!b || !best) return fail(GC_EINVAL, "null argument");
-    // Output-invariant knobs only: persistent grid sizes per kernel.  The
-    // hash table size and every other parameter of the algorithm are fixed.
-    static const uint32_t cand_opts[] = {2, 3, 4};
-    static const uint32_t parse_opts[] = {4, 8, 12, 16};
-    static const uint32_t lz_opts[] = {2, 3, 4};
+    // Output-invariant knobs only: workgroups per CU of each kernel (the grid
+    // is min(knob * CUs, items), items assigned grid-stride).  The hash table
+    // size and every other parameter of the algorithm are fixed.
+    static const uint32_t cand_opts[] = {1, 2, 4, 64};
+    static const uint32_t parse_opts[] = {2, 4, 8, 16};
+    static const uint32_t lz_opts[] = {1, 2, 4, 64};
     if (reps == 0) reps = 1;
     gc_tune keep = b->tune, top = b->tune;
     float top_ms[3] = {1e30f, 1e30f, 1e30f};

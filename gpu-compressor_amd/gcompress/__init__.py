@@ -75,6 +75,7 @@ SIGNATURES = [
     ("gc_batch_create", _vp, [_vp, _vp, C.c_uint64, _vp, C.c_uint32]),
     ("gc_batch_destroy", C.c_int, [_vp]),
     ("gc_batch_run", C.c_int, [_vp]),
+    ("gc_batch_set_mode", C.c_int, [_vp, C.c_uint32]),
     ("gc_batch_run_timed", C.c_int, [_vp, _vp]),
     ("gc_batch_set_tune", C.c_int, [_vp, _vp]),
     ("gc_batch_get_tune", C.c_int, [_vp, _vp]),
@@ -89,6 +90,8 @@ SIGNATURES = [
 
 ALGO_ZSTD = 0
 ALGO_LZMA2 = 1
+BATCH_STATIC = 0
+BATCH_QUEUE_GRAPH = 1
 
 
 class BatchInput(C.Structure):
@@ -248,7 +251,7 @@ def corpus(n: int, seed: int, weights=(1, 1, 1, 1), segment: int = 65536, exact_
 
 
 class Batch:
-    """Many inputs in one persistent, graph-launched pass (gc_batch_*).
+    """Many inputs in one pass, one launch per kernel (gc_batch_*).
 
     `inputs` is a list of (offset, size, algo, level) over a device arena."""
 
@@ -268,6 +271,9 @@ class Batch:
 
     def run(self):
         _check(lib().gc_batch_run(self.h), "gc_batch_run")
+
+    def set_mode(self, mode):
+        _check(lib().gc_batch_set_mode(self.h, mode), "gc_batch_set_mode")
 
     def run_timed(self):
         ms = (C.c_float * 3)()

@@ -213,10 +213,9 @@ int gc_decompress(const uint8_t* in, size_t n, uint8_t* out, size_t cap, size_t*
 /* ---- batched form ----------------------------------------------------- */
 /* No single reference counterpart: the reference calls the two host
  * functions once per chunk (pipeline.rs:145, pipeline_lzma2.rs:190).  A batch
- * puts every 64 KiB sub-block of many inputs into one device work queue,
- * drained by persistent kernels launched from one captured hipGraph.  Per
- * input, results equal gc_zstd_find_sequences / gc_lzma2_find_matches_packed
- * on that input alone. */
+ * runs every 64 KiB sub-block of many inputs in one launch per kernel (work
+ * items assigned grid-stride to the workgroups).  Per input, results equal
+ * gc_zstd_find_sequences / gc_lzma2_find_matches_packed on that input alone. */
 
 #define GC_ALGO_ZSTD 0u
 #define GC_ALGO_LZMA2 1u
@@ -228,13 +227,21 @@ typedef struct {
     uint32_t level;  /* Zstd: 1 -> depth 16, 2 -> depth 64, else 1; LZMA2: depth 32 */
 } gc_batch_input;
 
-/* Output-invariant launch knobs (persistent workgroups per CU). */
+/* Output-invariant launch knobs: workgroups per CU of each kernel (grid =
+ * min(knob * CUs, items)), 1..64. */
 typedef struct {
-    uint32_t cand_wg_per_cu;  /* candidate pass, 1..4 (LDS bound) */
-    uint32_t parse_wg_per_cu; /* Zstd parse, 1..16 */
-    uint32_t lzma2_wg_per_cu; /* LZMA2, 1..4 (LDS bound) */
+    uint32_t cand_wg_per_cu;  /* Zstd candidate pass */
+    uint32_t parse_wg_per_cu; /* Zstd parse (4 sub-blocks per workgroup) */
+    uint32_t lzma2_wg_per_cu; /* LZMA2 */
     uint32_t reserved;
 } gc_tune;
+
+/* Batch execution modes.  GC_BATCH_QUEUE_GRAPH (a device work queue drained
+ * by persistent workgroups, replayed from a captured hipGraph) is switched
+ * off: it hung on its first replay on MI355X and the cause is not found;
+ * gc_batch_set_mode refuses it with GC_EINVAL. */
+#define GC_BATCH_STATIC 0u
+#define GC_BATCH_QUEUE_GRAPH 1u
 
 typedef struct gc_batch gc_batch;
 
@@ -245,8 +252,9 @@ typedef struct gc_batch gc_batch;
 gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_size, const gc_batch_input* inputs,
                           uint32_t n_inputs);
 int gc_batch_destroy(gc_batch* b);
-/* Enqueue one pass on the context stream (graph launch; asynchronous). */
+/* Enqueue one pass on the context stream (asynchronous). */
 int gc_batch_run(gc_batch* b);
+int gc_batch_set_mode(gc_batch* b, uint32_t mode);
 /* One pass launched kernel by kernel with events, synchronous:
  * ms[0] LZMA2 kernel, ms[1] Zstd candidate kernel, ms[2] Zstd parse kernel. */
 int gc_batch_run_timed(gc_batch* b, float* ms);
