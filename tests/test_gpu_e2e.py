@@ -59,13 +59,13 @@ def test_nvlz_round_trip_and_triage(n):
     algo, orig, ch, nch = gcompress.container_info(blob)
     assert (algo, orig, ch, nch) == (gcompress.ALGO_LZMA2, n, SUB, -(-n // SUB))
     assert gcompress.decompress(blob) == data.tobytes()
-    # triage: an entry is raw exactly when the oracle finds no match in that sub-block
+    # each entry is exactly what the reference consumer makes from the oracle's match counts
     if n:
         _, c = oracle.lzma2_match_find(data, SUB, 8, 32)
         sizes = np.frombuffer(blob[28:28 + 8 * nch], "<u8")
         pos = 28 + 8 * nch
         for s in range(nch):
             entry = blob[pos:pos + sizes[s]]
-            raw = not c[s * SUB:(s + 1) * SUB].any()
-            assert (entry[0] == 0x01 and len(entry) == min(SUB, n - s * SUB) + 4) == raw
+            sub = data[s * SUB:(s + 1) * SUB]
+            assert entry == gcompress.encode_single_sub_block(sub, c[s * SUB:s * SUB + sub.size])
             pos += sizes[s]
