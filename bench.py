@@ -175,7 +175,8 @@ def main():
 
     # per-kernel durations (kernel-by-kernel launch with events on the library stream)
     kt = [batch.run_timed() for _ in range(3)]
-    k_ms = [float(np.mean([k[i] for k in kt])) for i in range(3)]  # lzma2, zstd cand, zstd parse
+    k_ms = [float(np.mean([k[i] for k in kt])) for i in range(4)]  # cand, lzma2 len, zstd info, zstd parse
+    knames = ["gc_cand_kernel", "gc_lzma2_len_kernel", "gc_zstd_info_kernel", "gc_zstd_parse_kernel"]
 
     # algorithmic bytes and results summary (outside the timed region)
     n_sub = 0
@@ -191,18 +192,17 @@ def main():
                     cover += int(seqs[s, : counts[s], 2].sum())
         else:
             n_sub += batch.n_sub(i)
+    # Algorithmic bytes (what the path must move): Zstd N_in + 16 n_seq + 4 n_sub, LZMA2 5 N_in.
+    # Per kernel, the share it is responsible for: every pass reads the input once; the parse
+    # writes the sequences and counts, the LZMA2 length pass the 4-byte match per position.
     if algo == gcompress.ALGO_ZSTD:
         alg_bytes = work_bytes + 16 * n_seq + 4 * n_sub
-        names = ["gc_zstd_cand_kernel", "gc_zstd_parse_kernel"]
-        durs = [k_ms[1], k_ms[2]]
-        dom = int(np.argmax(durs))
-        # the candidate pass reads the input; the parse reads it again and writes the sequences
-        dom_bytes = work_bytes if dom == 0 else alg_bytes
-        dom_ms = durs[dom]
-        dom_name = names[dom]
+        share = {0: work_bytes, 2: work_bytes, 3: alg_bytes}
     else:
         alg_bytes = 5 * work_bytes
-        dom_name, dom_ms, dom_bytes = "gc_lzma2_kernel", k_ms[0], alg_bytes
+        share = {0: work_bytes, 1: alg_bytes}
+    dom = max(share, key=lambda i: k_ms[i])
+    dom_name, dom_ms, dom_bytes = knames[dom], k_ms[dom], share[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
 
     cpu = None
@@ -233,13 +233,12 @@ def main():
             "input_bytes_per_step": work_bytes,
             "sub_block": 65536,
             "levels": levels,
-            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "lzma2_wg_per_cu": tune[2]},
+            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "elem_wg_per_cu": tune[2]},
         },
         "plain_per_chunk_gbs": round(plain_gbs, 3),
         "batched_vs_plain": round(value / plain_gbs, 2),
         "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
-        "kernel_ms": {"gc_lzma2_kernel": round(k_ms[0], 3), "gc_zstd_cand_kernel": round(k_ms[1], 3),
-                      "gc_zstd_parse_kernel": round(k_ms[2], 3)},
+        "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)},
         "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq,
                     "match_coverage_sampled": round(cover / (work_bytes / 16), 4) if cover else None},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,

@@ -21,9 +21,9 @@ struct gc_batch {
     std::vector<uint32_t> nsub;      // sub-blocks per input
     std::vector<uint64_t> out_base;  // zstd: first sequence slot; lzma2: first packed position
     std::vector<uint64_t> cnt_base;  // zstd: first count index
-    uint32_t n_z = 0, n_l = 0;       // items
+    uint32_t n_z = 0, n_l = 0;       // items: LZMA2 at [0, n_l), Zstd at [n_l, n_l + n_z)
     uint64_t z_bytes = 0, l_bytes = 0;
-    DevBuf zitems, litems, info, seq, counts, packed;
+    DevBuf items, cand, info, seq, counts, packed;
     gc_tune tune{};
     hipEvent_t ev[6] = {};
 };
@@ -48,28 +48,35 @@ uint32_t grid_for(gc_batch* b, uint32_t per_cu, uint32_t items) {
 // not been found; it is switched off (gc_batch_set_mode refuses it).
 int enqueue(gc_batch* b, bool timed) {
     gc_ctx* ctx = b->ctx;
-    if (timed) (void)hipEventRecord(b->ev[0], ctx->stream);
-    int rc;
+    hipStream_t st = ctx->stream;
+    const gc_item* all = (const gc_item*)b->items.p;
+    const gc_item* zit = all + b->n_l;
+    const uint32_t n_all = b->n_l + b->n_z;
+    const uint16_t* cand = (const uint16_t*)b->cand.p;
+    if (timed) (void)hipEventRecord(b->ev[0], st);
+    int rc = launch_cand(ctx, st, b->arena, all, n_all, grid_for(b, b->tune.cand_wg_per_cu, n_all),
+                         (uint16_t*)b->cand.p);
+    if (rc) return rc;
+    if (timed) (void)hipEventRecord(b->ev[1], st);
     if (b->n_l) {
-        rc = launch_lzma2(ctx, ctx->stream, b->arena, (const gc_lzma2_item*)b->litems.p, b->n_l, nullptr,
-                          grid_for(b, b->tune.lzma2_wg_per_cu, b->n_l), (uint32_t*)b->packed.p, nullptr, nullptr, 1);
+        rc = launch_lzma2_len(ctx, st, b->arena, all, b->n_l, grid_for(b, b->tune.elem_wg_per_cu, b->n_l), cand,
+                              (uint32_t*)b->packed.p, nullptr, nullptr, 1);
         if (rc) return rc;
     }
-    if (timed) (void)hipEventRecord(b->ev[1], ctx->stream);
+    if (timed) (void)hipEventRecord(b->ev[2], st);
     if (b->n_z) {
-        rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, nullptr, nullptr,
-                         grid_for(b, b->tune.cand_wg_per_cu, b->n_z), 0, (uint32_t*)b->info.p, b->seq.p,
-                         GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr, kMaxSeq);
+        rc = launch_zstd_info(ctx, st, b->arena, zit, b->n_z, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
+                              (uint32_t*)b->info.p);
         if (rc) return rc;
     }
-    if (timed) (void)hipEventRecord(b->ev[2], ctx->stream);
+    if (timed) (void)hipEventRecord(b->ev[3], st);
     if (b->n_z) {
-        rc = launch_zstd(ctx, ctx->stream, b->arena, (const gc_zstd_item*)b->zitems.p, b->n_z, nullptr, nullptr, 0,
-                         grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4), (uint32_t*)b->info.p, b->seq.p,
-                         GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)b->counts.p, nullptr, kMaxSeq);
+        rc = launch_zstd_parse(ctx, st, b->arena, zit, b->n_z, grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
+                               (const uint32_t*)b->info.p, b->seq.p, GC_SEQ_FORMAT_ZSTD, nullptr,
+                               (uint32_t*)b->counts.p, nullptr, kMaxSeq);
         if (rc) return rc;
     }
-    if (timed) (void)hipEventRecord(b->ev[3], ctx->stream);
+    if (timed) (void)hipEventRecord(b->ev[4], st);
     return GC_OK;
 }
 
@@ -95,9 +102,8 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     b->arena_size = arena_size;
     b->inputs.assign(inputs, inputs + n_inputs);
     b->tune = gc_tune{64, 64, 64, 0};  // grid = items unless tuned down
-    std::vector<gc_zstd_item> zi;
-    std::vector<gc_lzma2_item> li;
-    uint64_t zseq = 0, zcnt = 0, zinfo = 0, lpos = 0;
+    std::vector<gc_item> zi, li;
+    uint64_t zseq = 0, zcnt = 0, zpos = 0, lpos = 0;
     for (uint32_t i = 0; i < n_inputs; i++) {
         const gc_batch_input& in = inputs[i];
         if (in.algo > GC_ALGO_LZMA2 || (in.offset & 3) || in.offset > arena_size || in.size > arena_size - in.offset) {
@@ -105,57 +111,53 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
             delete b;
             return nullptr;
         }
-        uint32_t ns = (uint32_t)((in.size + GC_SUB_BLOCK - 1) / GC_SUB_BLOCK);
+        const uint32_t ns = (uint32_t)((in.size + GC_SUB_BLOCK - 1) / GC_SUB_BLOCK);
+        const bool z = in.algo == GC_ALGO_ZSTD;
         b->nsub.push_back(ns);
-        if (in.algo == GC_ALGO_ZSTD) {
-            b->out_base.push_back(zseq);
-            b->cnt_base.push_back(zcnt);
-            for (uint32_t s = 0; s < ns; s++) {
-                uint64_t start = (uint64_t)s * GC_SUB_BLOCK;
-                gc_zstd_item it{};
-                it.src_off = in.offset + start;
-                it.avail = arena_size - it.src_off;
-                it.scratch_pos = zinfo + start;
+        b->out_base.push_back(z ? zseq : lpos);
+        b->cnt_base.push_back(z ? zcnt : 0);
+        for (uint32_t s = 0; s < ns; s++) {
+            const uint64_t start = (uint64_t)s * GC_SUB_BLOCK;
+            gc_item it{};
+            it.src_off = in.offset + start;
+            it.avail = arena_size - it.src_off;
+            it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, in.size - start);
+            it.out_span = GC_SUB_BLOCK;
+            if (z) {
+                it.opos = zpos + start;
                 it.seq_pos = zseq + (uint64_t)s * kMaxSeq;
-                it.lit_pos = 0;
                 it.count_pos = (uint32_t)(zcnt + s);
-                it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, in.size - start);
                 it.depth = zstd_depth_for_level(in.level);
                 zi.push_back(it);
-            }
-            zseq += (uint64_t)ns * kMaxSeq;
-            zcnt += ns;
-            zinfo += (uint64_t)ns * GC_SUB_BLOCK;
-            b->z_bytes += in.size;
-        } else {
-            b->out_base.push_back(lpos);
-            b->cnt_base.push_back(0);
-            for (uint32_t s = 0; s < ns; s++) {
-                uint64_t start = (uint64_t)s * GC_SUB_BLOCK;
-                gc_lzma2_item it{};
-                it.src_off = in.offset + start;
-                it.avail = arena_size - it.src_off;
-                it.out_pos = lpos + start;
-                it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, in.size - start);
-                it.out_span = GC_SUB_BLOCK;
+            } else {
+                it.opos = lpos + start;
                 it.depth = GC_LZMA2_DEPTH;
                 li.push_back(it);
             }
+        }
+        if (z) {
+            zseq += (uint64_t)ns * kMaxSeq;
+            zcnt += ns;
+            zpos += (uint64_t)ns * GC_SUB_BLOCK;
+            b->z_bytes += in.size;
+        } else {
             lpos += (uint64_t)ns * GC_SUB_BLOCK;
             b->l_bytes += in.size;
         }
     }
-    // Longest work first: a full sub-block before a ragged tail.
-    auto zcost = [](const gc_zstd_item& a, const gc_zstd_item& c) { return a.n > c.n; };
-    std::stable_sort(zi.begin(), zi.end(), zcost);
-    auto lcost = [](const gc_lzma2_item& a, const gc_lzma2_item& c) { return a.n > c.n; };
-    std::stable_sort(li.begin(), li.end(), lcost);
+    // Longest work first: full sub-blocks before ragged tails.
+    auto longer = [](const gc_item& a, const gc_item& c) { return a.n > c.n; };
+    std::stable_sort(zi.begin(), zi.end(), longer);
+    std::stable_sort(li.begin(), li.end(), longer);
     b->n_z = (uint32_t)zi.size();
     b->n_l = (uint32_t)li.size();
+    std::vector<gc_item> all(li);
+    all.insert(all.end(), zi.begin(), zi.end());
+    for (size_t k = 0; k < all.size(); k++) all[k].cpos = (uint64_t)k * GC_SUB_BLOCK;  // candidates: one slot per item
     int rc = GC_OK;
-    if (b->n_z) rc = upload(ctx, b->zitems, zi.data(), zi.size() * sizeof(gc_zstd_item));
-    if (!rc && b->n_l) rc = upload(ctx, b->litems, li.data(), li.size() * sizeof(gc_lzma2_item));
-    if (!rc && b->n_z) rc = b->info.ensure(zinfo * 4);
+    if (!all.empty()) rc = upload(ctx, b->items, all.data(), all.size() * sizeof(gc_item));
+    if (!rc && !all.empty()) rc = b->cand.ensure(all.size() * (size_t)GC_SUB_BLOCK * 2);
+    if (!rc && b->n_z) rc = b->info.ensure(zpos * 4);
     if (!rc && b->n_z) rc = b->seq.ensure(zseq * sizeof(gc_zstd_sequence));
     if (!rc && b->n_z) rc = b->counts.ensure(zcnt * 4);
     if (!rc && b->n_l) rc = b->packed.ensure(lpos * 4);
@@ -178,8 +180,8 @@ int gc_batch_destroy(gc_batch* b) {
     (void)hipStreamSynchronize(b->ctx->stream);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
-    b->zitems.release();
-    b->litems.release();
+    b->items.release();
+    b->cand.release();
     b->info.release();
     b->seq.release();
     b->counts.release();
@@ -190,7 +192,7 @@ int gc_batch_destroy(gc_batch* b) {
 
 int gc_batch_set_tune(gc_batch* b, const gc_tune* t) {
     if (!b || !t) return fail(GC_EINVAL, "null argument");
-    if (t->cand_wg_per_cu < 1 || t->cand_wg_per_cu > 64 || t->lzma2_wg_per_cu < 1 || t->lzma2_wg_per_cu > 64 ||
+    if (t->cand_wg_per_cu < 1 || t->cand_wg_per_cu > 64 || t->elem_wg_per_cu < 1 || t->elem_wg_per_cu > 64 ||
         t->parse_wg_per_cu < 1 || t->parse_wg_per_cu > 64)
         return fail(GC_EINVAL, "tune out of range (1..64 workgroups per CU)");
     b->tune = *t;
@@ -226,12 +228,10 @@ int gc_batch_run_timed(gc_batch* b, float* ms) {
     (void)hipSetDevice(b->ctx->device);
     int rc = enqueue(b, true);
     if (rc) return rc;
-    hipError_t e = hipEventSynchronize(b->ev[3]);
+    hipError_t e = hipEventSynchronize(b->ev[4]);
     if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
-    // [0] lzma2, [1] zstd candidate pass, [2] zstd parse
-    (void)hipEventElapsedTime(&ms[0], b->ev[0], b->ev[1]);
-    (void)hipEventElapsedTime(&ms[1], b->ev[1], b->ev[2]);
-    (void)hipEventElapsedTime(&ms[2], b->ev[2], b->ev[3]);
+    // [0] candidates, [1] LZMA2 lengths, [2] Zstd info, [3] Zstd parse
+    for (int k = 0; k < 4; k++) (void)hipEventElapsedTime(&ms[k], b->ev[k], b->ev[k + 1]);
     return GC_OK;
 }
 
@@ -284,61 +284,46 @@ int gc_batch_autotune(gc_batch* b, uint32_t reps, gc_tune* best, float* best_ms)
     // Output-invariant knobs only: workgroups per CU of each kernel (the grid
     // is min(knob * CUs, items), items assigned grid-stride).  The hash table
     // size and every other parameter of the algorithm are fixed.
-    static const uint32_t cand_opts[] = {1, 2, 4, 64};
-    static const uint32_t parse_opts[] = {2, 4, 8, 16};
-    static const uint32_t lz_opts[] = {1, 2, 4, 64};
+    static const uint32_t opts[] = {1, 2, 4, 8, 64};
     if (reps == 0) reps = 1;
-    gc_tune keep = b->tune, top = b->tune;
-    float top_ms[3] = {1e30f, 1e30f, 1e30f};
-    auto measure = [&](gc_tune t, float out[3]) -> int {
+    gc_tune top = b->tune;
+    auto measure = [&](const gc_tune& t, float out[4]) -> int {
         int rc = gc_batch_set_tune(b, &t);
         if (rc) return rc;
-        float acc[3] = {0, 0, 0}, ms[3];
+        float acc[4] = {0, 0, 0, 0}, ms[4];
         for (uint32_t r = 0; r < reps + 1; r++) {
             rc = gc_batch_run_timed(b, ms);
             if (rc) return rc;
             if (r == 0) continue;  // warm-up
-            for (int k = 0; k < 3; k++) acc[k] += ms[k];
+            for (int k = 0; k < 4; k++) acc[k] += ms[k];
         }
-        for (int k = 0; k < 3; k++) out[k] = acc[k] / reps;
+        for (int k = 0; k < 4; k++) out[k] = acc[k] / reps;
         return GC_OK;
     };
-    // The three kernels are independent: tune each knob on its own kernel.
-    for (uint32_t c : cand_opts) {
+    // The kernels are independent: tune each knob on the kernels it drives.
+    float best_cand = 1e30f, best_elem = 1e30f, best_parse = 1e30f, ms[4];
+    for (uint32_t v : opts) {
         gc_tune t = top;
-        t.cand_wg_per_cu = c;
-        float ms[3];
+        t.cand_wg_per_cu = v;
         if (int rc = measure(t, ms)) return rc;
-        if (ms[1] < top_ms[1]) {
-            top_ms[1] = ms[1];
-            top.cand_wg_per_cu = c;
-        }
+        if (ms[0] < best_cand) best_cand = ms[0], top.cand_wg_per_cu = v;
     }
-    for (uint32_t p : parse_opts) {
+    for (uint32_t v : opts) {
         gc_tune t = top;
-        t.parse_wg_per_cu = p;
-        float ms[3];
+        t.elem_wg_per_cu = v;
         if (int rc = measure(t, ms)) return rc;
-        if (ms[2] < top_ms[2]) {
-            top_ms[2] = ms[2];
-            top.parse_wg_per_cu = p;
-        }
+        if (ms[1] + ms[2] < best_elem) best_elem = ms[1] + ms[2], top.elem_wg_per_cu = v;
     }
-    for (uint32_t l : lz_opts) {
+    for (uint32_t v : opts) {
         gc_tune t = top;
-        t.lzma2_wg_per_cu = l;
-        float ms[3];
+        t.parse_wg_per_cu = v;
         if (int rc = measure(t, ms)) return rc;
-        if (ms[0] < top_ms[0]) {
-            top_ms[0] = ms[0];
-            top.lzma2_wg_per_cu = l;
-        }
+        if (ms[3] < best_parse) best_parse = ms[3], top.parse_wg_per_cu = v;
     }
-    (void)keep;
     int rc = gc_batch_set_tune(b, &top);
     if (rc) return rc;
     *best = top;
-    if (best_ms) *best_ms = (b->n_l ? top_ms[0] : 0) + (b->n_z ? top_ms[1] + top_ms[2] : 0);
+    if (best_ms) *best_ms = best_cand + best_elem + best_parse;
     return GC_OK;
 }
 

@@ -19,36 +19,22 @@ bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
 // Items for a buffer tiled in sub-blocks of `sub` bytes from its start.
-void lzma2_items(std::vector<gc_lzma2_item>& v, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t depth) {
-    v.resize(nsub);
-    for (uint32_t s = 0; s < nsub; s++) {
-        uint64_t start = (uint64_t)s * sub;
-        gc_lzma2_item& it = v[s];
-        it.src_off = start;
-        it.avail = total - start;
-        it.out_pos = start;
-        it.n = (uint32_t)((total - start) < sub ? (total - start) : sub);
-        it.out_span = sub;
-        it.depth = depth;
-        it.pad = 0;
-    }
-}
-
-void zstd_items(std::vector<gc_zstd_item>& v, uint64_t total, uint32_t chunk, uint32_t nsub, uint32_t max_seq,
+void make_items(std::vector<gc_item>& v, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t max_seq,
                 uint32_t depth) {
     v.resize(nsub);
     for (uint32_t s = 0; s < nsub; s++) {
-        uint64_t start = (uint64_t)s * chunk;
-        gc_zstd_item& it = v[s];
+        uint64_t start = (uint64_t)s * sub;
+        gc_item& it = v[s];
         it.src_off = start;
         it.avail = total - start;
-        it.scratch_pos = start;
+        it.cpos = start;
+        it.opos = start;
         it.seq_pos = (uint64_t)s * max_seq;
         it.lit_pos = start;
         it.count_pos = s;
-        it.n = (uint32_t)((total - start) < chunk ? (total - start) : chunk);
+        it.n = (uint32_t)((total - start) < sub ? (total - start) : sub);
         it.depth = depth;
-        it.pad = 0;
+        it.out_span = sub;
     }
 }
 
@@ -60,24 +46,34 @@ int check_sub(uint32_t sub) {
 
 int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t* d_packed,
               gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth) {
-    std::vector<gc_lzma2_item> items;
-    lzma2_items(items, total, sub, nsub, depth);
-    int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_lzma2_item));
+    std::vector<gc_item> items;
+    make_items(items, total, sub, nsub, 0, depth);
+    int rc = ctx->cand.ensure((size_t)nsub * sub * 2 + 256);
+    if (!rc) rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
     if (rc) return rc;
-    return launch_lzma2(ctx, ctx->stream, d_in, (const gc_lzma2_item*)ctx->items.p, nsub, nullptr, nsub, d_packed, d_m, d_c,
-                        max_matches);
+    const gc_item* d_items = (const gc_item*)ctx->items.p;
+    rc = launch_cand(ctx, ctx->stream, d_in, d_items, nsub, nsub, (uint16_t*)ctx->cand.p);
+    if (rc) return rc;
+    return launch_lzma2_len(ctx, ctx->stream, d_in, d_items, nsub, nsub, (const uint16_t*)ctx->cand.p, d_packed, d_m,
+                            d_c, max_matches);
 }
 
 int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, uint32_t nsub, void* d_seq,
              uint32_t fmt, uint8_t* d_lit, uint32_t* d_sc, uint32_t* d_lc, uint32_t depth, uint32_t max_seq) {
-    std::vector<gc_zstd_item> items;
-    zstd_items(items, total, chunk, nsub, max_seq, depth);
-    int rc = ctx->info.ensure((size_t)total * 4 + 256);
+    std::vector<gc_item> items;
+    make_items(items, total, chunk, nsub, max_seq, depth);
+    int rc = ctx->cand.ensure((size_t)nsub * chunk * 2 + 256);
+    if (!rc) rc = ctx->info.ensure((size_t)nsub * chunk * 4 + 256);
+    if (!rc) rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
     if (rc) return rc;
-    rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_zstd_item));
+    const gc_item* d_items = (const gc_item*)ctx->items.p;
+    rc = launch_cand(ctx, ctx->stream, d_in, d_items, nsub, nsub, (uint16_t*)ctx->cand.p);
+    if (!rc)
+        rc = launch_zstd_info(ctx, ctx->stream, d_in, d_items, nsub, nsub, (const uint16_t*)ctx->cand.p,
+                              (uint32_t*)ctx->info.p);
     if (rc) return rc;
-    return launch_zstd(ctx, ctx->stream, d_in, (const gc_zstd_item*)ctx->items.p, nsub, nullptr, nullptr, nsub,
-                       (uint32_t)div_up(nsub, 4), (uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq);
+    return launch_zstd_parse(ctx, ctx->stream, d_in, d_items, nsub, (uint32_t)div_up(nsub, 4),
+                             (const uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq);
 }
 
 // Device copy of a host buffer, padded so that aligned reads near the end

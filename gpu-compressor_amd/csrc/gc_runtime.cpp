@@ -74,24 +74,30 @@ static int launch(hipStream_t stream, hipFunction_t f, uint32_t grid, uint32_t b
     return GC_OK;
 }
 
-int launch_lzma2(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
-                 uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
-                 uint32_t max_matches) {
-    void* args[] = {&d_arena, &d_items, &n_items, &d_queue, &d_packed, &d_ref_m, &d_ref_c, &max_matches};
-    return launch(stream, ctx->k.lzma2, grid, 256, args, "launch gc_lzma2_kernel");
+int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                uint32_t grid, uint16_t* cand16) {
+    void* args[] = {&arena, &items, &n_items, &cand16};
+    return launch(stream, ctx->k.cand, grid, 256, args, "launch gc_cand_kernel");
 }
 
-int launch_zstd(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
-                uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
-                uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
-                uint32_t* d_lit_counts, uint32_t max_seq) {
-    void* a1[] = {&d_arena, &d_items, &n_items, &d_queue_cand, &d_info};
-    int rc = launch(stream, ctx->k.zstd_cand, grid_cand, 256, a1, "launch gc_zstd_cand_kernel");
-    if (rc) return rc;
-    const uint32_t* info = d_info;
-    void* a2[] = {&d_arena, &d_items,        &n_items,       &d_queue_parse, &info,  &d_seq,
-                  &seq_fmt, &d_lit,          &d_seq_counts,  &d_lit_counts,  &max_seq};
-    return launch(stream, ctx->k.zstd_parse, grid_parse, 256, a2, "launch gc_zstd_parse_kernel");
+int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                     uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
+                     uint32_t max_matches) {
+    void* args[] = {&arena, &items, &n_items, &cand16, &packed, &ref_m, &ref_c, &max_matches};
+    return launch(stream, ctx->k.lzma2_len, grid, 256, args, "launch gc_lzma2_len_kernel");
+}
+
+int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info) {
+    void* args[] = {&arena, &items, &n_items, &cand16, &info};
+    return launch(stream, ctx->k.zstd_info, grid, 256, args, "launch gc_zstd_info_kernel");
+}
+
+int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                      uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
+                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
+    void* args[] = {&arena, &items, &n_items, &info, &seq, &seq_fmt, &lit, &seq_counts, &lit_counts, &max_seq};
+    return launch(stream, ctx->k.zstd_parse, grid, 256, args, "launch gc_zstd_parse_kernel");
 }
 
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n) {
@@ -143,8 +149,9 @@ gc_ctx* gc_open(int device) {
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module, gc_code_object);
-    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2, ctx->k.module, "gc_lzma2_kernel");
-    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_cand, ctx->k.module, "gc_zstd_cand_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand, ctx->k.module, "gc_cand_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len, ctx->k.module, "gc_lzma2_len_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info, ctx->k.module, "gc_zstd_info_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
@@ -161,6 +168,7 @@ void gc_close(gc_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->items.release();
+    ctx->cand.release();
     ctx->info.release();
     ctx->input.release();
     ctx->out_a.release();

@@ -28,8 +28,9 @@ struct DevBuf {
 // Loaded kernels.
 struct Kernels {
     hipModule_t module = nullptr;
-    hipFunction_t lzma2 = nullptr;
-    hipFunction_t zstd_cand = nullptr;
+    hipFunction_t cand = nullptr;
+    hipFunction_t lzma2_len = nullptr;
+    hipFunction_t zstd_info = nullptr;
     hipFunction_t zstd_parse = nullptr;
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
@@ -45,6 +46,7 @@ struct gc_ctx {
     gc::Kernels k;
     // Scratch reused across calls.
     gc::DevBuf items;      // work items
+    gc::DevBuf cand;       // per-position candidates (u16)
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
     gc::DevBuf out_a, out_b, out_c, out_d;
@@ -53,14 +55,17 @@ struct gc_ctx {
 
 namespace gc {
 
-// Launch helpers (all on ctx->stream).
-int launch_lzma2(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_lzma2_item* d_items, uint32_t n_items,
-                 uint32_t* d_queue, uint32_t grid, uint32_t* d_packed, gc_lzma_match* d_ref_m, uint32_t* d_ref_c,
-                 uint32_t max_matches);
-int launch_zstd(gc_ctx* ctx, hipStream_t stream, const uint8_t* d_arena, const gc_zstd_item* d_items, uint32_t n_items,
-                uint32_t* d_queue_cand, uint32_t* d_queue_parse, uint32_t grid_cand, uint32_t grid_parse,
-                uint32_t* d_info, void* d_seq, uint32_t seq_fmt, uint8_t* d_lit, uint32_t* d_seq_counts,
-                uint32_t* d_lit_counts, uint32_t max_seq);
+// Launch helpers.  grid = workgroups; items are taken grid-stride.
+int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                uint32_t grid, uint16_t* cand16);
+int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                     uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
+                     uint32_t max_matches);
+int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info);
+int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                      uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
+                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq);
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);

@@ -11,22 +11,22 @@
 //      cand(p) = max{ q < p : q + 3 < n, H(q) = H(p) }   (or "none")
 //    because both reference kernels insert every position q with q + 3 < n
 //    into the 14-bit table exactly once, in increasing order, before any
-//    later lookup.  A 256-lane workgroup computes it tile by tile with the
-//    reference's own table (16384 x u16 in LDS); inside a wavefront the
-//    lanes that share a hash are found with ballots, so only the first lane
-//    of each group reads the table and only the last one writes it.
-//  * LZMA2: the whole per-position output is a function of cand(p) and the
-//    bytes, so the candidate tile is followed directly by the length
-//    compares (fused; no scratch).
-//  * Zstd: the parse (heads, repeat offsets, the lazy step) is serial per
-//    sub-block.  A first kernel stores, per position, the candidate and a
-//    short verified length; a second kernel walks the sub-block with one
-//    wavefront, testing 64 positions per step against the current repeat
-//    offsets and jumping straight to the first one that matches.
+//    later lookup.  gc_cand_kernel computes it tile by tile with the
+//    reference's own table (16384 x u16 in LDS); inside a wavefront the lanes
+//    that share a hash are grouped with ballots, so only the first lane of a
+//    group reads the table and only the last one writes it.  The kernel does
+//    nothing but LDS work and prefetched, coalesced input loads.
+//  * Everything else per position is elementwise given cand(p): the LZMA2
+//    match (gc_lzma2_len_kernel) and the Zstd verify + short length
+//    (gc_zstd_info_kernel).  These kernels use no LDS and run at full
+//    occupancy, which hides the latency of their gathers.
+//  * The Zstd parse (heads, repeat offsets, the lazy step) is serial per
+//    sub-block: gc_zstd_parse_kernel walks each sub-block with one wavefront,
+//    testing 128 positions per step against the current repeat offsets and
+//    jumping straight to the first one that matches.
 //
-// Every kernel reads its sub-blocks from a list of work items so that the
-// same code serves one chunk (plain form) and a batch of many inputs
-// (persistent form, items taken from a device queue).
+// Work arrives as a list of sub-blocks (gc_item), so one launch serves one
+// chunk (plain form) or every sub-block of many inputs (batched form).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -38,18 +38,22 @@
 #define GC_DUP_BITS 10
 #define GC_DUP_SIZE (1u << GC_DUP_BITS)
 #define GC_LZMA_MAX 273u
-#define GC_ZSTD_SHORT 16u  // lengths >= this are re-measured by the parse
+#define GC_ZSTD_SHORT 64u  // Zstd lengths >= this are re-measured by the parse
 
 #define WAVE 64
 #define CAND_THREADS 256
 #define CAND_WAVES (CAND_THREADS / WAVE)
+#define ELEM_THREADS 256
+#define PARSE_WAVES 4
 
 // ---------------------------------------------------------------------------
-// Byte access.  Inputs are 4-byte aligned; a 4-byte value at any offset is
-// built from the two aligned dwords around it.  `avail` is the number of
+// Helpers
+// ---------------------------------------------------------------------------
+
+// 4 bytes at base[off], little endian.  Inputs are 4-byte aligned; the value
+// is built from the two aligned dwords around it.  `avail` is the number of
 // readable bytes from `base`: near the end of the buffer the value is built
 // byte by byte so that nothing past the end is touched.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint64_t avail) {
     const uint8_t* p = base + off;
     if ((uint64_t)off + 8 <= avail) {
@@ -73,11 +77,23 @@ __device__ __forceinline__ uint32_t lane_id() {
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
 
-// Common-prefix length of src[a..] and src[b..], at most `limit` bytes,
-// one lane.
+__device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// Barrier for LDS traffic only: outstanding global loads (prefetches) stay
+// in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Common-prefix length of src[a..] and src[b..] from `start` (bytes before
+// it known equal), at most `limit`; one lane.
 __device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
-                                             uint32_t limit) {
-    uint32_t len = 0;
+                                             uint32_t start, uint32_t limit) {
+    uint32_t len = start;
     while (len < limit) {
         uint32_t x = ld32(src, a + len, avail) ^ ld32(src, b + len, avail);
         if (x) {
@@ -90,7 +106,7 @@ __device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail,
 }
 
 // Common-prefix length, whole wavefront cooperating (256 bytes per step).
-// a, b, limit must be uniform.
+// a, b, start and limit must be uniform.
 __device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
                                              uint32_t limit, uint32_t start) {
     const uint32_t lane = lane_id();
@@ -102,8 +118,7 @@ __device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail,
         uint64_t mm = ballot(x != 0);
         if (mm) {
             uint32_t k = (uint32_t)__builtin_ctzll(mm);
-            uint32_t xk = (uint32_t)__builtin_amdgcn_readlane((int)x, (int)k);
-            len += 4 * k + ((uint32_t)__builtin_ctz(xk) >> 3);
+            len += 4 * k + ((uint32_t)__builtin_ctz(bcast(x, k)) >> 3);
             break;
         }
         len += 4 * WAVE;
@@ -112,26 +127,23 @@ __device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail,
 }
 
 // ---------------------------------------------------------------------------
-// Candidate tile.  All 256 threads of the workgroup call this for the tile
-// starting at t0; on return `cand` holds cand(p) for p = t0 + threadIdx.x
-// (GC_NONE when there is none or p + 3 >= n) and the table has every
-// position of the tile inserted.
+// Candidate kernel: cand16[it.cpos + p] = cand(p) (GC_NONE when there is none
+// or p + 3 >= n) for every position of every item.
 // ---------------------------------------------------------------------------
 struct CandLds {
-    uint16_t table[GC_TABLE];            // the reference's 14-bit table
-    uint8_t dup[CAND_WAVES][GC_DUP_SIZE]; // write/read-back duplicate test
-    uint8_t flag[CAND_WAVES][WAVE];      // lanes that won a duplicate test
+    uint16_t table[GC_TABLE];              // the reference's 14-bit table
+    uint8_t dup[CAND_WAVES][GC_DUP_SIZE];  // write/read-back duplicate test
+    uint8_t flag[CAND_WAVES][WAVE];        // lanes that won a duplicate test
 };
 
-__device__ __forceinline__ uint32_t cand_tile(CandLds& s, const uint8_t* src, uint64_t avail, uint32_t n,
-                                              uint32_t t0, uint32_t& word) {
+// One tile of 256 positions starting at t0; `word` is this thread's 4 bytes
+// (p = t0 + threadIdx.x), valid when p + 3 < n.
+__device__ __forceinline__ uint32_t cand_tile(CandLds& s, uint32_t word, uint32_t n, uint32_t t0) {
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid / WAVE;
     const uint32_t lane = tid % WAVE;
     const uint32_t p = t0 + tid;
     const bool valid = p + 3 < n;
-
-    word = valid ? ld32(src, p, avail) : 0;
     const uint32_t h = hash4(word);
 
     // Duplicate test: every valid lane writes its id under a 10-bit slice of
@@ -159,203 +171,186 @@ __device__ __forceinline__ uint32_t cand_tile(CandLds& s, const uint8_t* src, ui
     uint64_t rest = ballot(dup);
     while (rest) {
         uint32_t lead = (uint32_t)__builtin_ctzll(rest);
-        uint32_t hl = (uint32_t)__builtin_amdgcn_readlane((int)h, (int)lead);
+        uint32_t hl = bcast(h, lead);
         uint64_t g = ballot(dup && h == hl);
         if (dup && h == hl) group = g;
         rest &= ~g;
     }
     const uint64_t below = group & (lane_bit - 1);
     const uint64_t above = group & ~(lane_bit | (lane_bit - 1));
-    const uint32_t base = t0 + wave * WAVE;
 
     uint32_t cand = GC_NONE;
-    if (below) cand = base + 63u - (uint32_t)__builtin_clzll(below);
+    if (below) cand = t0 + wave * WAVE + 63u - (uint32_t)__builtin_clzll(below);
 
-    // Table rounds: wavefront r reads and writes the table after every
-    // lower wavefront of the tile has written it.
+    // Table rounds: wavefront r reads and writes the table after every lower
+    // wavefront of the tile has written it.
 #pragma unroll
     for (uint32_t r = 0; r < CAND_WAVES; r++) {
         if (wave == r && valid) {
             if (!below) cand = s.table[h];
             if (!above) s.table[h] = (uint16_t)p;
         }
-        __syncthreads();
+        lds_barrier();
     }
-    return cand;
+    return valid ? cand : GC_NONE;
 }
 
-__device__ __forceinline__ void table_clear(CandLds& s) {
-    uint32_t* t = (uint32_t*)s.table;
-    for (uint32_t i = threadIdx.x; i < GC_TABLE / 2; i += CAND_THREADS) t[i] = 0xFFFFFFFFu;
-    __syncthreads();
+extern "C" __global__ void __launch_bounds__(CAND_THREADS)
+gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uint16_t* cand16) {
+    __shared__ CandLds s;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const gc_item it = items[i];
+        const uint8_t* src = arena + it.src_off;
+        const uint32_t n = it.n;
+        uint32_t* t = (uint32_t*)s.table;
+        for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
+        lds_barrier();
+        uint32_t next = (tid + 3 < n) ? ld32(src, tid, it.avail) : 0;
+        for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
+            const uint32_t word = next;
+            const uint32_t q = t0 + CAND_THREADS + tid;  // next tile, loaded while this one runs
+            next = (q + 3 < n) ? ld32(src, q, it.avail) : 0;
+            const uint32_t c = cand_tile(s, word, n, t0);
+            if (t0 + tid < n) cand16[it.cpos + t0 + tid] = (uint16_t)c;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// LZMA2: one sub-block, fused candidate + lengths.
-// Output per position: packed (distance << 16 | length), 0 = no match, and
-// optionally the reference layout (slot 0 of LzmaMatch[max] and counts).
+// LZMA2 (lzma2_match_find.cu:96-185) per position, given cand(p):
+//   p + 1 < n, and either the hash candidate shares >= 2 bytes (distance
+//   p - cand, length = common run capped at min(n - p, 273)), or no such
+//   match and src[p-1] == src[p] == src[p+1] (distance 1, run capped alike).
+// Output: packed[it.opos + p] = distance << 16 | length (0 = none) and/or the
+// reference layout (slot 0 of LzmaMatch[max_matches] and counts).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void lzma2_subblock(CandLds& s, const gc_lzma2_item it, const uint8_t* arena, uint32_t* packed,
-                               gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
-    const uint32_t depth = it.depth;
-    const uint8_t* src = arena + it.src_off;
-    const uint64_t avail = it.avail;
-    const uint32_t n = it.n;
-    table_clear(s);
-    for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
-        uint32_t word;
-        uint32_t c = cand_tile(s, src, avail, n, t0, word);
-        const uint32_t p = t0 + threadIdx.x;
-        if (p >= n) continue;
-        uint32_t out = 0;
-        if (p + 1 < n) {
-            uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
-            uint32_t a = GC_NONE, dist = 0;
-            if (c != GC_NONE && depth > 0) {
-                // the probe reports a match only from 2 common bytes on
-                if (((ld32(src, c, avail) ^ word) & 0xFFFFu) == 0) {
-                    a = c;
+extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
+                    uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const gc_item it = items[i];
+        const uint8_t* src = arena + it.src_off;
+        const uint64_t avail = it.avail;
+        const uint32_t n = it.n;
+        const uint32_t depth = it.depth;
+        for (uint32_t p = threadIdx.x; p < n; p += ELEM_THREADS) {
+            uint32_t out = 0;
+            if (p + 1 < n) {
+                const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
+                const uint32_t cur = ld32(src, p, avail);
+                const uint32_t c = (p + 3 < n) ? cand16[it.cpos + p] : GC_NONE;
+                uint32_t a = GC_NONE, dist = 0;
+                if (c != GC_NONE && depth > 0 && ((ld32(src, c, avail) ^ cur) & 0xFFFFu) == 0) {
+                    a = c;  // the probe reports a match only from 2 common bytes on
                     dist = p - c;
                 }
-            }
-            if (a == GC_NONE && p > 0) {
-                // distance-1 run: src[p-1] == src[p] == src[p+1]
-                uint32_t prev = ld32(src, p - 1, avail);
-                uint32_t cur = (p + 3 < n) ? word : ld32(src, p, avail);
-                if (((prev ^ cur) & 0xFFFFu) == 0) {
-                    a = p - 1;
+                if (a == GC_NONE && p > 0 && ((ld32(src, p - 1, avail) ^ cur) & 0xFFFFu) == 0) {
+                    a = p - 1;  // distance-1 run: src[p-1] == src[p] == src[p+1]
                     dist = 1;
                 }
+                if (a != GC_NONE) out = (dist << 16) | lcp_lane(src, avail, a, p, 2, maxlen);
             }
-            if (a != GC_NONE) out = (dist << 16) | lcp_lane(src, avail, a, p, maxlen);
-        }
-        const uint64_t q = it.out_pos + p;
-        if (packed) packed[q] = out;
-        if (ref_c) {
-            ref_c[q] = out ? 1u : 0u;
-            if (out) {
-                gc_lzma_match m;
-                m.distance = out >> 16;
-                m.length = out & 0xFFFFu;
-                ref_m[q * max_matches] = m;
+            const uint64_t q = it.opos + p;
+            if (packed) packed[q] = out;
+            if (ref_c) {
+                ref_c[q] = out ? 1u : 0u;
+                if (out) {
+                    gc_lzma_match m;
+                    m.distance = out >> 16;
+                    m.length = out & 0xFFFFu;
+                    ref_m[q * max_matches] = m;
+                }
             }
         }
-    }
-    // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
-    if (ref_c)
-        for (uint32_t p = n + threadIdx.x; p < it.out_span; p += CAND_THREADS) ref_c[it.out_pos + p] = 0;
-    __syncthreads();
-}
-
-extern "C" __global__ void __launch_bounds__(CAND_THREADS)
-gc_lzma2_kernel(const uint8_t* arena, const gc_lzma2_item* items, uint32_t n_items, uint32_t* queue,
-                uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
-    __shared__ CandLds s;
-    __shared__ uint32_t next;
-    if (!queue) {
-        for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x)
-            lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches);
-        return;
-    }
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
-        __syncthreads();
-        uint32_t i = next;
-        __syncthreads();
-        if (i >= n_items) break;
-        lzma2_subblock(s, items[i], arena, packed, ref_m, ref_c, max_matches);
+        // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
+        if (ref_c)
+            for (uint32_t p = n + threadIdx.x; p < it.out_span; p += ELEM_THREADS) ref_c[it.opos + p] = 0;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Zstd, pass 1: per position, the candidate (empty table slots read as 0, as
-// in the reference) and whether its first 3 bytes verify, with the common
-// length up to GC_ZSTD_SHORT.  Stored as (len << 16 | cand); len 0 = no.
+// Zstd per position, given cand(p): the candidate as the reference's table
+// returns it (an empty slot reads as position 0, zstd_compress.cu:198-200),
+// and whether it verifies 3 bytes (:241-245) with the common length capped
+// at GC_ZSTD_SHORT: info = len << 16 | cand, len 0 = no.  A length equal to
+// the cap is re-measured by the parse.  Written for p + 3 < n.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void zstd_cand_subblock(CandLds& s, const gc_zstd_item it, const uint8_t* arena, uint32_t* info) {
-    const uint8_t* src = arena + it.src_off;
-    const uint64_t avail = it.avail;
-    const uint32_t n = it.n;
-    table_clear(s);
-    for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
-        uint32_t word;
-        uint32_t c = cand_tile(s, src, avail, n, t0, word);
-        const uint32_t p = t0 + threadIdx.x;
-        if (p + 3 >= n) continue;  // never a head (the parse stops at n - 3)
-        if (c == GC_NONE) c = 0;   // empty slot = position 0 (zstd_compress.cu:198-200)
-        uint32_t len = 0;
-        if (p > 0 && ((ld32(src, c, avail) ^ word) & 0xFFFFFFu) == 0) {
-            uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
-            len = lcp_lane(src, avail, c, p, lim);
-        }
-        info[it.scratch_pos + p] = (len << 16) | c;
-    }
-    __syncthreads();
-}
-
-extern "C" __global__ void __launch_bounds__(CAND_THREADS)
-gc_zstd_cand_kernel(const uint8_t* arena, const gc_zstd_item* items, uint32_t n_items, uint32_t* queue,
+extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
                     uint32_t* info) {
-    __shared__ CandLds s;
-    __shared__ uint32_t next;
-    if (!queue) {
-        for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) zstd_cand_subblock(s, items[i], arena, info);
-        return;
-    }
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(queue, 1u);
-        __syncthreads();
-        uint32_t i = next;
-        __syncthreads();
-        if (i >= n_items) break;
-        zstd_cand_subblock(s, items[i], arena, info);
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const gc_item it = items[i];
+        const uint8_t* src = arena + it.src_off;
+        const uint64_t avail = it.avail;
+        const uint32_t n = it.n;
+        for (uint32_t p = threadIdx.x; p + 3 < n; p += ELEM_THREADS) {
+            uint32_t c = cand16[it.cpos + p];
+            if (c == GC_NONE) c = 0;
+            uint32_t len = 0;
+            if (p > 0 && ((ld32(src, c, avail) ^ ld32(src, p, avail)) & 0xFFFFFFu) == 0) {
+                const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
+                len = lcp_lane(src, avail, c, p, 3, lim);
+            }
+            info[it.opos + p] = (len << 16) | c;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Zstd, pass 2: the parse of zstd_compress.cu:230-338 for one sub-block, one
-// wavefront.  Each step tests positions p .. p+63 with the current repeat
-// offsets; every position before the first that matches is a literal, so
-// the walk jumps there and decides the sequence exactly as the reference.
+// Zstd parse (zstd_compress.cu:230-338), one wavefront per sub-block.  Each
+// step tests positions pos .. pos+127 (two per lane) with the current repeat
+// offsets; every position before the first one that matches is a literal,
+// so the walk jumps there and decides the sequence exactly as the reference.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t len) {
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
 }
 
-__device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const uint8_t* arena, const uint32_t* info,
-                                    void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts,
-                                    uint32_t* lit_counts, uint32_t max_seq) {
-    const uint32_t depth = it.depth;
+struct Probe {
+    uint32_t e;  // info word (0 when the position cannot be a head)
+    bool r0, r1, r2;
+};
+
+__device__ __forceinline__ Probe probe(const uint8_t* src, uint64_t avail, const uint32_t* inf, uint32_t x,
+                                       uint32_t n, uint32_t rep0, uint32_t rep1, uint32_t rep2) {
+    Probe pr{0, false, false, false};
+    if (x + 3 < n) {
+        pr.e = inf[x];
+        const uint32_t w = ld32(src, x, avail) & 0xFFFFFFu;
+        if (rep0 <= x) pr.r0 = (ld32(src, x - rep0, avail) & 0xFFFFFFu) == w;
+        if (rep1 <= x) pr.r1 = (ld32(src, x - rep1, avail) & 0xFFFFFFu) == w;
+        if (rep2 <= x) pr.r2 = (ld32(src, x - rep2, avail) & 0xFFFFFFu) == w;
+    }
+    return pr;
+}
+
+__device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint8_t* arena, const uint32_t* info,
+                                                    void* seq_out, uint32_t seq_fmt, uint8_t* literals,
+                                                    uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
+    const uint32_t depth = it.depth;
     const uint32_t lane = lane_id();
-    const uint32_t* inf = info + it.scratch_pos;
-    uint32_t rep0 = 1, rep1 = 4, rep2 = 8;
+    const uint32_t* inf = info + it.opos;
+    uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
     uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0;
     uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
 
     while (pos + 3 < n && nseq < max_seq) {
-        const uint32_t x = pos + lane;
-        const bool act = x + 3 < n;
-        uint32_t w = 0, e = 0;
-        bool r0 = false, r1 = false, r2 = false;
-        if (act) {
-            e = inf[x];
-            w = ld32(src, x, avail) & 0xFFFFFFu;
-            if (rep0 <= x) r0 = (ld32(src, x - rep0, avail) & 0xFFFFFFu) == w;
-            if (rep1 <= x) r1 = (ld32(src, x - rep1, avail) & 0xFFFFFFu) == w;
-            if (rep2 <= x) r2 = (ld32(src, x - rep2, avail) & 0xFFFFFFu) == w;
-        }
-        const bool hm = (e >> 16) != 0;
-        const uint64_t m = ballot(hm || r0 || r1 || r2);
-        if (!m) {
-            pos += WAVE;
+        const Probe a = probe(src, avail, inf, pos + lane, n, rep0, rep1, rep2);
+        const Probe b = probe(src, avail, inf, pos + WAVE + lane, n, rep0, rep1, rep2);
+        const uint64_t ma = ballot((a.e >> 16) != 0 || a.r0 || a.r1 || a.r2);
+        const uint64_t mb = ballot((b.e >> 16) != 0 || b.r0 || b.r1 || b.r2);
+        if (!ma && !mb) {
+            pos += 2 * WAVE;
             continue;
         }
-        const uint32_t k = (uint32_t)__builtin_ctzll(m);
-        uint32_t head = pos + k;
-        const uint32_t ek = (uint32_t)__builtin_amdgcn_readlane((int)e, (int)k);
+        const bool in_a = ma != 0;
+        const uint32_t k = in_a ? (uint32_t)__builtin_ctzll(ma) : (uint32_t)__builtin_ctzll(mb);
+        uint32_t head = pos + (in_a ? k : WAVE + k);
+        const uint32_t ek = bcast(in_a ? a.e : b.e, k);
         uint32_t len, off;
         if (ek >> 16) {
             // hash candidate verified
@@ -364,14 +359,24 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const
             len = ek >> 16;
             if (len >= GC_ZSTD_SHORT) len = lcp_wave(src, avail, c, head, n - head, GC_ZSTD_SHORT);
         } else {
-            // first repeat offset that verifies (rep0, rep1, rep2 in order)
-            const bool k0 = ballot(r0) >> k & 1, k1 = ballot(r1) >> k & 1;
+            // the first repeat offset that verifies (rep0, rep1, rep2 in order)
+            const bool k0 = (ballot(in_a ? a.r0 : b.r0) >> k) & 1;
+            const bool k1 = (ballot(in_a ? a.r1 : b.r1) >> k) & 1;
             off = k0 ? rep0 : (k1 ? rep1 : rep2);
             len = lcp_wave(src, avail, head - off, head, n - head, 3);
         }
-        // one lazy step at head + 1 (zstd_compress.cu:278-307)
+        // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
+        // is already in a register of the next lane
         if (depth > 1 && head + 4 < n) {
-            const uint32_t e2 = inf[head + 1];
+            uint32_t e2;
+            if (in_a && k < WAVE - 1)
+                e2 = bcast(a.e, k + 1);
+            else if (in_a)
+                e2 = bcast(b.e, 0);
+            else if (k < WAVE - 1)
+                e2 = bcast(b.e, k + 1);
+            else
+                e2 = inf[head + 1];
             if (e2 >> 16) {
                 const uint32_t c2 = e2 & 0xFFFFu;
                 uint32_t len2 = e2 >> 16;
@@ -420,27 +425,16 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_zstd_item it, const
     }
 }
 
-#define PARSE_WAVES 4
 extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
-gc_zstd_parse_kernel(const uint8_t* arena, const gc_zstd_item* items, uint32_t n_items, uint32_t* queue,
-                     const uint32_t* info, void* seq_out, uint32_t seq_fmt, uint8_t* literals,
-                     uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
+gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint32_t* info,
+                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
+                     uint32_t max_seq) {
     const uint32_t wave = threadIdx.x / WAVE;
-    if (!queue) {
-        for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES)
-            zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
-        return;
-    }
-    for (;;) {
-        uint32_t i = 0;
-        if (lane_id() == 0) i = atomicAdd(queue, 1u);
-        i = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
-        if (i >= n_items) break;
+    for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES)
         zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
-    }
 }
 
-// Small fill kernel (used instead of memset nodes inside captured graphs).
+// Small fill kernel.
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;

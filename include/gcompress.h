@@ -230,9 +230,9 @@ typedef struct {
 /* Output-invariant launch knobs: workgroups per CU of each kernel (grid =
  * min(knob * CUs, items)), 1..64. */
 typedef struct {
-    uint32_t cand_wg_per_cu;  /* Zstd candidate pass */
+    uint32_t cand_wg_per_cu;  /* candidate pass (LDS table), both finders */
     uint32_t parse_wg_per_cu; /* Zstd parse (4 sub-blocks per workgroup) */
-    uint32_t lzma2_wg_per_cu; /* LZMA2 */
+    uint32_t elem_wg_per_cu;  /* per-position passes: LZMA2 lengths, Zstd verify */
     uint32_t reserved;
 } gc_tune;
 
@@ -255,8 +255,9 @@ int gc_batch_destroy(gc_batch* b);
 /* Enqueue one pass on the context stream (asynchronous). */
 int gc_batch_run(gc_batch* b);
 int gc_batch_set_mode(gc_batch* b, uint32_t mode);
-/* One pass launched kernel by kernel with events, synchronous:
- * ms[0] LZMA2 kernel, ms[1] Zstd candidate kernel, ms[2] Zstd parse kernel. */
+/* One pass with events between the kernels, synchronous: ms[0] candidate
+ * kernel, ms[1] LZMA2 length kernel, ms[2] Zstd verify kernel, ms[3] Zstd
+ * parse kernel. */
 int gc_batch_run_timed(gc_batch* b, float* ms);
 int gc_batch_set_tune(gc_batch* b, const gc_tune* t);
 int gc_batch_get_tune(gc_batch* b, gc_tune* t);
