@@ -218,6 +218,38 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uin
 }
 
 // ---------------------------------------------------------------------------
+// Per-position passes.  Each thread takes a strip of STRIP consecutive
+// positions and walks it backwards: where position p+1 matches along the same
+// diagonal (its target is p's target + 1), the common length at p is one more
+// than at p+1, capped (the caps of p and p+1 differ by at most one byte), so
+// a long match costs one direct measurement per strip instead of one per
+// position.
+// ---------------------------------------------------------------------------
+#define STRIP 8
+
+// The 16 bytes src[p0 .. p0+15] as four dwords; bytes past `avail` read as 0.
+// Sub-blocks of a size that is not a multiple of 4 start unaligned: then
+// each dword is assembled by ld32.
+__device__ __forceinline__ void load_strip(const uint8_t* src, uint32_t p0, uint64_t avail, uint32_t d[4]) {
+    if ((((uintptr_t)(src + p0)) & 3) == 0 && (uint64_t)p0 + 16 <= avail) {
+        const uint32_t* q = (const uint32_t*)(src + p0);
+        d[0] = q[0];
+        d[1] = q[1];
+        d[2] = q[2];
+        d[3] = q[3];
+        return;
+    }
+    for (int k = 0; k < 4; k++) d[k] = ld32(src, p0 + 4 * k, avail);
+}
+
+// The 4 bytes at p0 + j from the strip's dwords.
+__device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) {
+    const uint32_t k = j >> 2;
+    const uint32_t hi = k + 1 < 4 ? d[k + 1] : 0;
+    return __builtin_amdgcn_alignbyte(hi, d[k], j & 3);
+}
+
+// ---------------------------------------------------------------------------
 // LZMA2 (lzma2_match_find.cu:96-185) per position, given cand(p):
 //   p + 1 < n, and either the hash candidate shares >= 2 bytes (distance
 //   p - cand, length = common run capped at min(n - p, 273)), or no such
@@ -234,32 +266,60 @@ gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items
         const uint64_t avail = it.avail;
         const uint32_t n = it.n;
         const uint32_t depth = it.depth;
-        for (uint32_t p = threadIdx.x; p < n; p += ELEM_THREADS) {
-            uint32_t out = 0;
-            if (p + 1 < n) {
-                const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
-                const uint32_t cur = ld32(src, p, avail);
-                const uint32_t c = (p + 3 < n) ? cand16[it.cpos + p] : GC_NONE;
-                uint32_t a = GC_NONE, dist = 0;
-                if (c != GC_NONE && depth > 0 && ((ld32(src, c, avail) ^ cur) & 0xFFFFu) == 0) {
-                    a = c;  // the probe reports a match only from 2 common bytes on
-                    dist = p - c;
-                }
-                if (a == GC_NONE && p > 0 && ((ld32(src, p - 1, avail) ^ cur) & 0xFFFFu) == 0) {
-                    a = p - 1;  // distance-1 run: src[p-1] == src[p] == src[p+1]
-                    dist = 1;
-                }
-                if (a != GC_NONE) out = (dist << 16) | lcp_lane(src, avail, a, p, 2, maxlen);
+        for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += ELEM_THREADS * STRIP) {
+            uint32_t d[4];
+            load_strip(src, p0, avail, d);
+            const uint32_t prev0 = p0 > 0 ? ld32(src, p0 - 1, avail) : 0;
+            uint32_t tgt[STRIP], vc[STRIP];
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) {
+                const uint32_t p = p0 + j;
+                tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
             }
-            const uint64_t q = it.opos + p;
-            if (packed) packed[q] = out;
-            if (ref_c) {
-                ref_c[q] = out ? 1u : 0u;
-                if (out) {
-                    gc_lzma_match m;
-                    m.distance = out >> 16;
-                    m.length = out & 0xFFFFu;
-                    ref_m[q * max_matches] = m;
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32(src, tgt[j], avail) : 0;
+            uint32_t out[STRIP];
+            uint32_t next_t = GC_NONE, next_len = 0;
+#pragma unroll
+            for (int j = STRIP - 1; j >= 0; j--) {
+                const uint32_t p = p0 + j;
+                uint32_t o = 0, t = GC_NONE;
+                if (p + 1 < n) {
+                    const uint32_t cur = strip_word(d, j);
+                    const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
+                    if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
+                        t = tgt[j];  // the probe reports a match only from 2 common bytes on
+                    } else if (p > 0) {
+                        const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
+                        if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
+                    }
+                    if (t != GC_NONE) {
+                        uint32_t len;
+                        if (next_t != GC_NONE && next_t == t + 1)
+                            len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
+                        else
+                            len = lcp_lane(src, avail, t, p, 2, maxlen);
+                        o = ((p - t) << 16) | len;
+                        next_len = len;
+                    }
+                }
+                next_t = t;
+                out[j] = o;
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) {
+                const uint32_t p = p0 + j;
+                if (p >= n) break;
+                const uint64_t q = it.opos + p;
+                if (packed) packed[q] = out[j];
+                if (ref_c) {
+                    ref_c[q] = out[j] ? 1u : 0u;
+                    if (out[j]) {
+                        gc_lzma_match m;
+                        m.distance = out[j] >> 16;
+                        m.length = out[j] & 0xFFFFu;
+                        ref_m[q * max_matches] = m;
+                    }
                 }
             }
         }
@@ -284,45 +344,63 @@ gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items
         const uint8_t* src = arena + it.src_off;
         const uint64_t avail = it.avail;
         const uint32_t n = it.n;
-        for (uint32_t p = threadIdx.x; p + 3 < n; p += ELEM_THREADS) {
-            uint32_t c = cand16[it.cpos + p];
-            if (c == GC_NONE) c = 0;
-            uint32_t len = 0;
-            if (p > 0 && ((ld32(src, c, avail) ^ ld32(src, p, avail)) & 0xFFFFFFu) == 0) {
-                const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
-                len = lcp_lane(src, avail, c, p, 3, lim);
+        for (uint32_t p0 = threadIdx.x * STRIP; p0 + 3 < n; p0 += ELEM_THREADS * STRIP) {
+            uint32_t d[4];
+            load_strip(src, p0, avail, d);
+            uint32_t c[STRIP], vc[STRIP];
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) {
+                const uint32_t p = p0 + j;
+                uint32_t cj = p + 3 < n ? cand16[it.cpos + p] : 0;
+                c[j] = cj == GC_NONE ? 0 : cj;
             }
-            info[it.opos + p] = (len << 16) | c;
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) vc[j] = p0 + j + 3 < n ? ld32(src, c[j], avail) : 0;
+            uint32_t out[STRIP];
+            uint32_t next_len = 0;
+#pragma unroll
+            for (int j = STRIP - 1; j >= 0; j--) {
+                const uint32_t p = p0 + j;
+                uint32_t len = 0;
+                if (p + 3 < n && p > 0 && ((vc[j] ^ strip_word(d, j)) & 0xFFFFFFu) == 0) {
+                    const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
+                    if (j + 1 < STRIP && next_len && c[j + 1] == c[j] + 1)
+                        len = next_len + 1 < lim ? next_len + 1 : lim;
+                    else
+                        len = lcp_lane(src, avail, c[j], p, 3, lim);
+                }
+                next_len = len;
+                out[j] = (len << 16) | c[j];
+            }
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++)
+                if (p0 + j + 3 < n) info[it.opos + p0 + j] = out[j];
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Zstd parse (zstd_compress.cu:230-338), one wavefront per sub-block.  Each
-// step tests positions pos .. pos+127 (two per lane) with the current repeat
-// offsets; every position before the first one that matches is a literal,
-// so the walk jumps there and decides the sequence exactly as the reference.
+// Zstd parse (zstd_compress.cu:230-338), one wavefront per sub-block.
+//
+// The wavefront holds a window of 64 positions x = pos + lane: the info word
+// of x and, for each current repeat offset r, whether the 3 bytes at x - r
+// equal those at x.  The first lane whose position matches (hash candidate
+// verified, or a repeat offset) is the next head: every position before it
+// is a literal.  After a sequence the window slides by the bytes consumed;
+// lanes that stay keep their data (the repeat offsets after a sequence are
+// the new offset and the two most recent old ones, whose flags are kept), so
+// a step loads one gather for the new offset plus the positions that enter.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t len) {
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
 }
 
-struct Probe {
-    uint32_t e;  // info word (0 when the position cannot be a head)
-    bool r0, r1, r2;
-};
+__device__ __forceinline__ uint32_t shfl(uint32_t v, uint32_t src_lane) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+}
 
-__device__ __forceinline__ Probe probe(const uint8_t* src, uint64_t avail, const uint32_t* inf, uint32_t x,
-                                       uint32_t n, uint32_t rep0, uint32_t rep1, uint32_t rep2) {
-    Probe pr{0, false, false, false};
-    if (x + 3 < n) {
-        pr.e = inf[x];
-        const uint32_t w = ld32(src, x, avail) & 0xFFFFFFu;
-        if (rep0 <= x) pr.r0 = (ld32(src, x - rep0, avail) & 0xFFFFFFu) == w;
-        if (rep1 <= x) pr.r1 = (ld32(src, x - rep1, avail) & 0xFFFFFFu) == w;
-        if (rep2 <= x) pr.r2 = (ld32(src, x - rep2, avail) & 0xFFFFFFu) == w;
-    }
-    return pr;
+__device__ __forceinline__ bool rep_eq(const uint8_t* src, uint64_t avail, uint32_t x, uint32_t r, uint32_t w) {
+    return r <= x && (ld32(src, x - r, avail) & 0xFFFFFFu) == w;
 }
 
 __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint8_t* arena, const uint32_t* info,
@@ -338,19 +416,31 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
     uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0;
     uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
 
+    // window registers of this lane: info word, 3 bytes at x, repeat flags
+    uint32_t e = 0, w = 0, f = 0;
+    auto fresh = [&](uint32_t x) {
+        e = 0;
+        w = 0;
+        f = 0;
+        if (x + 3 < n) {
+            e = inf[x];
+            w = ld32(src, x, avail) & 0xFFFFFFu;
+            f = (rep_eq(src, avail, x, rep0, w) ? 1u : 0u) | (rep_eq(src, avail, x, rep1, w) ? 2u : 0u) |
+                (rep_eq(src, avail, x, rep2, w) ? 4u : 0u);
+        }
+    };
+    fresh(lane);
+
     while (pos + 3 < n && nseq < max_seq) {
-        const Probe a = probe(src, avail, inf, pos + lane, n, rep0, rep1, rep2);
-        const Probe b = probe(src, avail, inf, pos + WAVE + lane, n, rep0, rep1, rep2);
-        const uint64_t ma = ballot((a.e >> 16) != 0 || a.r0 || a.r1 || a.r2);
-        const uint64_t mb = ballot((b.e >> 16) != 0 || b.r0 || b.r1 || b.r2);
-        if (!ma && !mb) {
-            pos += 2 * WAVE;
+        const uint64_t m = ballot((e >> 16) != 0 || f != 0);
+        if (!m) {
+            pos += WAVE;
+            fresh(pos + lane);
             continue;
         }
-        const bool in_a = ma != 0;
-        const uint32_t k = in_a ? (uint32_t)__builtin_ctzll(ma) : (uint32_t)__builtin_ctzll(mb);
-        uint32_t head = pos + (in_a ? k : WAVE + k);
-        const uint32_t ek = bcast(in_a ? a.e : b.e, k);
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        uint32_t head = pos + k;
+        const uint32_t ek = bcast(e, k);
         uint32_t len, off;
         if (ek >> 16) {
             // hash candidate verified
@@ -360,23 +450,14 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
             if (len >= GC_ZSTD_SHORT) len = lcp_wave(src, avail, c, head, n - head, GC_ZSTD_SHORT);
         } else {
             // the first repeat offset that verifies (rep0, rep1, rep2 in order)
-            const bool k0 = (ballot(in_a ? a.r0 : b.r0) >> k) & 1;
-            const bool k1 = (ballot(in_a ? a.r1 : b.r1) >> k) & 1;
-            off = k0 ? rep0 : (k1 ? rep1 : rep2);
+            const uint32_t fk = bcast(f, k);
+            off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
             len = lcp_wave(src, avail, head - off, head, n - head, 3);
         }
         // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
-        // is already in a register of the next lane
+        // is in the next lane unless head is the window's last position
         if (depth > 1 && head + 4 < n) {
-            uint32_t e2;
-            if (in_a && k < WAVE - 1)
-                e2 = bcast(a.e, k + 1);
-            else if (in_a)
-                e2 = bcast(b.e, 0);
-            else if (k < WAVE - 1)
-                e2 = bcast(b.e, k + 1);
-            else
-                e2 = inf[head + 1];
+            const uint32_t e2 = k + 1 < WAVE ? bcast(e, k + 1) : inf[head + 1];
             if (e2 >> 16) {
                 const uint32_t c2 = e2 & 0xFFFFu;
                 uint32_t len2 = e2 >> 16;
@@ -408,11 +489,39 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
             }
         }
         nseq++;
+        const uint32_t o0 = rep0, o1 = rep1, o2 = rep2;
         rep2 = rep1;
         rep1 = rep0;
         rep0 = off;
+        const uint32_t shift = head + len - pos;
         pos = head + len;
         lit_start = pos;
+        if (shift >= WAVE) {
+            fresh(pos + lane);
+            continue;
+        }
+        // slide the window by `shift`; lanes that stay keep their data
+        const uint32_t from = lane + shift;
+        const uint32_t se = shfl(e, from & (WAVE - 1));
+        const uint32_t sw = shfl(w, from & (WAVE - 1));
+        const uint32_t sf = shfl(f, from & (WAVE - 1));
+        if (from < WAVE) {
+            const uint32_t x = pos + lane;
+            e = se;
+            w = sw;
+            uint32_t b0;
+            if (off == o0)
+                b0 = sf & 1;
+            else if (off == o1)
+                b0 = (sf >> 1) & 1;
+            else if (off == o2)
+                b0 = (sf >> 2) & 1;
+            else
+                b0 = (x + 3 < n && rep_eq(src, avail, x, off, sw)) ? 1u : 0u;
+            f = b0 | ((sf & 3u) << 1);  // flags now for (off, o0, o1)
+        } else {
+            fresh(pos + lane);
+        }
     }
     // trailing literals (zstd_compress.cu:341-344)
     if (n > lit_start) {
