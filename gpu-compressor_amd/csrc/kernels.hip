@@ -52,11 +52,13 @@
 
 // 4 bytes at base[off], little endian.  Inputs are 4-byte aligned; the value
 // is built from the two aligned dwords around it.  `avail` is the number of
-// readable bytes from `base`: near the end of the buffer the value is built
-// byte by byte so that nothing past the end is touched.
+// readable bytes from `base`.  SAFE callers guarantee avail >= n + 16 for a
+// sub-block of n bytes, which covers every read below; otherwise a read that
+// would pass the end is built byte by byte from the bytes that exist.
+template <bool SAFE>
 __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint64_t avail) {
     const uint8_t* p = base + off;
-    if ((uint64_t)off + 8 <= avail) {
+    if (SAFE || (uint64_t)off + 8 <= avail) {
         const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
         const uint32_t* q = (const uint32_t*)(p - sh);  // derived from base: stays a global access
         uint32_t lo = q[0];
@@ -68,6 +70,10 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint
         if ((uint64_t)off + i < avail) v |= (uint32_t)p[i] << (8 * i);
     return v;
 }
+
+// Per-item reads may skip bounds checks when the arena extends 16 bytes past
+// the sub-block (every sub-block but the last of an arena).
+__device__ __forceinline__ bool item_safe(uint64_t avail, uint32_t n) { return avail >= (uint64_t)n + 16; }
 
 __device__ __forceinline__ uint32_t hash4(uint32_t v) { return (v * 2654435761u) >> (32 - GC_HASH_BITS); }
 
@@ -91,11 +97,12 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Common-prefix length of src[a..] and src[b..] from `start` (bytes before
 // it known equal), at most `limit`; one lane.
+template <bool SAFE>
 __device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
                                              uint32_t start, uint32_t limit) {
     uint32_t len = start;
     while (len < limit) {
-        uint32_t x = ld32(src, a + len, avail) ^ ld32(src, b + len, avail);
+        uint32_t x = ld32<SAFE>(src, a + len, avail) ^ ld32<SAFE>(src, b + len, avail);
         if (x) {
             len += (uint32_t)__builtin_ctz(x) >> 3;
             break;
@@ -107,6 +114,7 @@ __device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail,
 
 // Common-prefix length, whole wavefront cooperating (256 bytes per step).
 // a, b, start and limit must be uniform.
+template <bool SAFE>
 __device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
                                              uint32_t limit, uint32_t start) {
     const uint32_t lane = lane_id();
@@ -114,7 +122,7 @@ __device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail,
     while (len < limit) {
         uint32_t o = len + 4 * lane;
         uint32_t x = 0;
-        if (o < limit) x = ld32(src, a + o, avail) ^ ld32(src, b + o, avail);
+        if (o < limit) x = ld32<SAFE>(src, a + o, avail) ^ ld32<SAFE>(src, b + o, avail);
         uint64_t mm = ballot(x != 0);
         if (mm) {
             uint32_t k = (uint32_t)__builtin_ctzll(mm);
@@ -195,25 +203,33 @@ __device__ __forceinline__ uint32_t cand_tile(CandLds& s, uint32_t word, uint32_
     return valid ? cand : GC_NONE;
 }
 
+template <bool SAFE>
+__device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const uint8_t* arena, uint16_t* cand16) {
+    const uint32_t tid = threadIdx.x;
+    const uint8_t* src = arena + it.src_off;
+    const uint32_t n = it.n;
+    uint32_t* t = (uint32_t*)s.table;
+    for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
+    lds_barrier();
+    uint32_t next = (tid + 3 < n) ? ld32<SAFE>(src, tid, it.avail) : 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
+        const uint32_t word = next;
+        const uint32_t q = t0 + CAND_THREADS + tid;  // next tile, loaded while this one runs
+        next = (q + 3 < n) ? ld32<SAFE>(src, q, it.avail) : 0;
+        const uint32_t c = cand_tile(s, word, n, t0);
+        if (t0 + tid < n) cand16[it.cpos + t0 + tid] = (uint16_t)c;
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(CAND_THREADS)
 gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uint16_t* cand16) {
     __shared__ CandLds s;
-    const uint32_t tid = threadIdx.x;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = items[i];
-        const uint8_t* src = arena + it.src_off;
-        const uint32_t n = it.n;
-        uint32_t* t = (uint32_t*)s.table;
-        for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
-        lds_barrier();
-        uint32_t next = (tid + 3 < n) ? ld32(src, tid, it.avail) : 0;
-        for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
-            const uint32_t word = next;
-            const uint32_t q = t0 + CAND_THREADS + tid;  // next tile, loaded while this one runs
-            next = (q + 3 < n) ? ld32(src, q, it.avail) : 0;
-            const uint32_t c = cand_tile(s, word, n, t0);
-            if (t0 + tid < n) cand16[it.cpos + t0 + tid] = (uint16_t)c;
-        }
+        if (item_safe(it.avail, it.n))
+            cand_item<true>(s, it, arena, cand16);
+        else
+            cand_item<false>(s, it, arena, cand16);
     }
 }
 
@@ -230,8 +246,9 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uin
 // The 16 bytes src[p0 .. p0+15] as four dwords; bytes past `avail` read as 0.
 // Sub-blocks of a size that is not a multiple of 4 start unaligned: then
 // each dword is assembled by ld32.
+template <bool SAFE>
 __device__ __forceinline__ void load_strip(const uint8_t* src, uint32_t p0, uint64_t avail, uint32_t d[4]) {
-    if ((((uintptr_t)(src + p0)) & 3) == 0 && (uint64_t)p0 + 16 <= avail) {
+    if ((((uintptr_t)(src + p0)) & 3) == 0 && (SAFE || (uint64_t)p0 + 16 <= avail)) {
         const uint32_t* q = (const uint32_t*)(src + p0);
         d[0] = q[0];
         d[1] = q[1];
@@ -239,7 +256,7 @@ __device__ __forceinline__ void load_strip(const uint8_t* src, uint32_t p0, uint
         d[3] = q[3];
         return;
     }
-    for (int k = 0; k < 4; k++) d[k] = ld32(src, p0 + 4 * k, avail);
+    for (int k = 0; k < 4; k++) d[k] = ld32<SAFE>(src, p0 + 4 * k, avail);
 }
 
 // The 4 bytes at p0 + j from the strip's dwords.
@@ -257,75 +274,85 @@ __device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) 
 // Output: packed[it.opos + p] = distance << 16 | length (0 = none) and/or the
 // reference layout (slot 0 of LzmaMatch[max_matches] and counts).
 // ---------------------------------------------------------------------------
+template <bool SAFE>
+__device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
+                                           uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
+                                           uint32_t max_matches) {
+    const uint8_t* src = arena + it.src_off;
+    const uint64_t avail = it.avail;
+    const uint32_t n = it.n;
+    const uint32_t depth = it.depth;
+    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += ELEM_THREADS * STRIP) {
+        uint32_t d[4];
+        load_strip<SAFE>(src, p0, avail, d);
+        const uint32_t prev0 = p0 > 0 ? ld32<SAFE>(src, p0 - 1, avail) : 0;
+        uint32_t tgt[STRIP], vc[STRIP];
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++) {
+            const uint32_t p = p0 + j;
+            tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32<SAFE>(src, tgt[j], avail) : 0;
+        uint32_t out[STRIP];
+        uint32_t next_t = GC_NONE, next_len = 0;
+#pragma unroll
+        for (int j = STRIP - 1; j >= 0; j--) {
+            const uint32_t p = p0 + j;
+            uint32_t o = 0, t = GC_NONE;
+            if (p + 1 < n) {
+                const uint32_t cur = strip_word(d, j);
+                const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
+                if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
+                    t = tgt[j];  // the probe reports a match only from 2 common bytes on
+                } else if (p > 0) {
+                    const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
+                    if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
+                }
+                if (t != GC_NONE) {
+                    uint32_t len;
+                    if (next_t != GC_NONE && next_t == t + 1)
+                        len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
+                    else
+                        len = lcp_lane<SAFE>(src, avail, t, p, 2, maxlen);
+                    o = ((p - t) << 16) | len;
+                    next_len = len;
+                }
+            }
+            next_t = t;
+            out[j] = o;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++) {
+            const uint32_t p = p0 + j;
+            if (p >= n) break;
+            const uint64_t q = it.opos + p;
+            if (packed) packed[q] = out[j];
+            if (ref_c) {
+                ref_c[q] = out[j] ? 1u : 0u;
+                if (out[j]) {
+                    gc_lzma_match m;
+                    m.distance = out[j] >> 16;
+                    m.length = out[j] & 0xFFFFu;
+                    ref_m[q * max_matches] = m;
+                }
+            }
+        }
+    }
+    // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
+    if (ref_c)
+        for (uint32_t p = n + threadIdx.x; p < it.out_span; p += ELEM_THREADS) ref_c[it.opos + p] = 0;
+}
+
 extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
 gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
                     uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = items[i];
-        const uint8_t* src = arena + it.src_off;
-        const uint64_t avail = it.avail;
-        const uint32_t n = it.n;
-        const uint32_t depth = it.depth;
-        for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += ELEM_THREADS * STRIP) {
-            uint32_t d[4];
-            load_strip(src, p0, avail, d);
-            const uint32_t prev0 = p0 > 0 ? ld32(src, p0 - 1, avail) : 0;
-            uint32_t tgt[STRIP], vc[STRIP];
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++) {
-                const uint32_t p = p0 + j;
-                tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32(src, tgt[j], avail) : 0;
-            uint32_t out[STRIP];
-            uint32_t next_t = GC_NONE, next_len = 0;
-#pragma unroll
-            for (int j = STRIP - 1; j >= 0; j--) {
-                const uint32_t p = p0 + j;
-                uint32_t o = 0, t = GC_NONE;
-                if (p + 1 < n) {
-                    const uint32_t cur = strip_word(d, j);
-                    const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
-                    if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
-                        t = tgt[j];  // the probe reports a match only from 2 common bytes on
-                    } else if (p > 0) {
-                        const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
-                        if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
-                    }
-                    if (t != GC_NONE) {
-                        uint32_t len;
-                        if (next_t != GC_NONE && next_t == t + 1)
-                            len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
-                        else
-                            len = lcp_lane(src, avail, t, p, 2, maxlen);
-                        o = ((p - t) << 16) | len;
-                        next_len = len;
-                    }
-                }
-                next_t = t;
-                out[j] = o;
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++) {
-                const uint32_t p = p0 + j;
-                if (p >= n) break;
-                const uint64_t q = it.opos + p;
-                if (packed) packed[q] = out[j];
-                if (ref_c) {
-                    ref_c[q] = out[j] ? 1u : 0u;
-                    if (out[j]) {
-                        gc_lzma_match m;
-                        m.distance = out[j] >> 16;
-                        m.length = out[j] & 0xFFFFu;
-                        ref_m[q * max_matches] = m;
-                    }
-                }
-            }
-        }
-        // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
-        if (ref_c)
-            for (uint32_t p = n + threadIdx.x; p < it.out_span; p += ELEM_THREADS) ref_c[it.opos + p] = 0;
+        if (item_safe(it.avail, it.n))
+            lzma2_item<true>(it, arena, cand16, packed, ref_m, ref_c, max_matches);
+        else
+            lzma2_item<false>(it, arena, cand16, packed, ref_m, ref_c, max_matches);
     }
 }
 
@@ -336,46 +363,55 @@ gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items
 // at GC_ZSTD_SHORT: info = len << 16 | cand, len 0 = no.  A length equal to
 // the cap is re-measured by the parse.  Written for p + 3 < n.
 // ---------------------------------------------------------------------------
+template <bool SAFE>
+__device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
+                                               uint32_t* info) {
+    const uint8_t* src = arena + it.src_off;
+    const uint64_t avail = it.avail;
+    const uint32_t n = it.n;
+    for (uint32_t p0 = threadIdx.x * STRIP; p0 + 3 < n; p0 += ELEM_THREADS * STRIP) {
+        uint32_t d[4];
+        load_strip<SAFE>(src, p0, avail, d);
+        uint32_t c[STRIP], vc[STRIP];
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++) {
+            const uint32_t p = p0 + j;
+            uint32_t cj = p + 3 < n ? cand16[it.cpos + p] : 0;
+            c[j] = cj == GC_NONE ? 0 : cj;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++) vc[j] = ld32<SAFE>(src, c[j], avail);
+        uint32_t out[STRIP];
+        uint32_t next_len = 0;
+#pragma unroll
+        for (int j = STRIP - 1; j >= 0; j--) {
+            const uint32_t p = p0 + j;
+            uint32_t len = 0;
+            if (p + 3 < n && p > 0 && ((vc[j] ^ strip_word(d, j)) & 0xFFFFFFu) == 0) {
+                const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
+                if (j + 1 < STRIP && next_len && c[j + 1] == c[j] + 1)
+                    len = next_len + 1 < lim ? next_len + 1 : lim;
+                else
+                    len = lcp_lane<SAFE>(src, avail, c[j], p, 3, lim);
+            }
+            next_len = len;
+            out[j] = (len << 16) | c[j];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < STRIP; j++)
+            if (p0 + j + 3 < n) info[it.opos + p0 + j] = out[j];
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
 gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
                     uint32_t* info) {
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = items[i];
-        const uint8_t* src = arena + it.src_off;
-        const uint64_t avail = it.avail;
-        const uint32_t n = it.n;
-        for (uint32_t p0 = threadIdx.x * STRIP; p0 + 3 < n; p0 += ELEM_THREADS * STRIP) {
-            uint32_t d[4];
-            load_strip(src, p0, avail, d);
-            uint32_t c[STRIP], vc[STRIP];
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++) {
-                const uint32_t p = p0 + j;
-                uint32_t cj = p + 3 < n ? cand16[it.cpos + p] : 0;
-                c[j] = cj == GC_NONE ? 0 : cj;
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++) vc[j] = p0 + j + 3 < n ? ld32(src, c[j], avail) : 0;
-            uint32_t out[STRIP];
-            uint32_t next_len = 0;
-#pragma unroll
-            for (int j = STRIP - 1; j >= 0; j--) {
-                const uint32_t p = p0 + j;
-                uint32_t len = 0;
-                if (p + 3 < n && p > 0 && ((vc[j] ^ strip_word(d, j)) & 0xFFFFFFu) == 0) {
-                    const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
-                    if (j + 1 < STRIP && next_len && c[j + 1] == c[j] + 1)
-                        len = next_len + 1 < lim ? next_len + 1 : lim;
-                    else
-                        len = lcp_lane(src, avail, c[j], p, 3, lim);
-                }
-                next_len = len;
-                out[j] = (len << 16) | c[j];
-            }
-#pragma unroll
-            for (uint32_t j = 0; j < STRIP; j++)
-                if (p0 + j + 3 < n) info[it.opos + p0 + j] = out[j];
-        }
+        if (item_safe(it.avail, it.n))
+            zstd_info_item<true>(it, arena, cand16, info);
+        else
+            zstd_info_item<false>(it, arena, cand16, info);
     }
 }
 
@@ -399,13 +435,19 @@ __device__ __forceinline__ uint32_t shfl(uint32_t v, uint32_t src_lane) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
 }
 
+// 3 bytes at x - r equal w (the 3 bytes at x), for a head candidate x
+// (x + 3 < n); the read is clamped so that it is always in the sub-block.
+template <bool SAFE>
 __device__ __forceinline__ bool rep_eq(const uint8_t* src, uint64_t avail, uint32_t x, uint32_t r, uint32_t w) {
-    return r <= x && (ld32(src, x - r, avail) & 0xFFFFFFu) == w;
+    const bool ok = r <= x;
+    const uint32_t v = ld32<SAFE>(src, ok ? x - r : 0, avail) & 0xFFFFFFu;
+    return ok && v == w;
 }
 
-__device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint8_t* arena, const uint32_t* info,
-                                                    void* seq_out, uint32_t seq_fmt, uint8_t* literals,
-                                                    uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
+template <bool SAFE>
+__device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t* arena, const uint32_t* info,
+                                                void* seq_out, uint32_t seq_fmt, uint8_t* literals,
+                                                uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
@@ -419,23 +461,24 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
     // window registers of this lane: info word, 3 bytes at x, repeat flags
     uint32_t e = 0, w = 0, f = 0;
     auto fresh = [&](uint32_t x) {
-        e = 0;
-        w = 0;
-        f = 0;
-        if (x + 3 < n) {
-            e = inf[x];
-            w = ld32(src, x, avail) & 0xFFFFFFu;
-            f = (rep_eq(src, avail, x, rep0, w) ? 1u : 0u) | (rep_eq(src, avail, x, rep1, w) ? 2u : 0u) |
-                (rep_eq(src, avail, x, rep2, w) ? 4u : 0u);
-        }
+        const bool act = x + 3 < n;
+        const uint32_t xc = act ? x : 0;  // reads stay in the sub-block
+        const uint32_t ev = inf[xc];
+        const uint32_t wv = ld32<SAFE>(src, xc, avail) & 0xFFFFFFu;
+        const uint32_t fv = (rep_eq<SAFE>(src, avail, xc, rep0, wv) ? 1u : 0u) |
+                            (rep_eq<SAFE>(src, avail, xc, rep1, wv) ? 2u : 0u) |
+                            (rep_eq<SAFE>(src, avail, xc, rep2, wv) ? 4u : 0u);
+        e = act ? ev : 0;
+        w = act ? wv : 0;
+        f = act ? fv : 0;
     };
-    fresh(lane);
+    if (n > 3) fresh(lane);
 
     while (pos + 3 < n && nseq < max_seq) {
         const uint64_t m = ballot((e >> 16) != 0 || f != 0);
         if (!m) {
             pos += WAVE;
-            fresh(pos + lane);
+            if (pos + 3 < n) fresh(pos + lane);
             continue;
         }
         const uint32_t k = (uint32_t)__builtin_ctzll(m);
@@ -447,12 +490,12 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
             const uint32_t c = ek & 0xFFFFu;
             off = head - c;
             len = ek >> 16;
-            if (len >= GC_ZSTD_SHORT) len = lcp_wave(src, avail, c, head, n - head, GC_ZSTD_SHORT);
+            if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, avail, c, head, n - head, GC_ZSTD_SHORT);
         } else {
             // the first repeat offset that verifies (rep0, rep1, rep2 in order)
             const uint32_t fk = bcast(f, k);
             off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
-            len = lcp_wave(src, avail, head - off, head, n - head, 3);
+            len = lcp_wave<SAFE>(src, avail, head - off, head, n - head, 3);
         }
         // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
         // is in the next lane unless head is the window's last position
@@ -461,7 +504,8 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
             if (e2 >> 16) {
                 const uint32_t c2 = e2 & 0xFFFFu;
                 uint32_t len2 = e2 >> 16;
-                if (len2 >= GC_ZSTD_SHORT) len2 = lcp_wave(src, avail, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
+                if (len2 >= GC_ZSTD_SHORT)
+                    len2 = lcp_wave<SAFE>(src, avail, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
                 if (len2 > len + 1) {
                     head += 1;
                     len = len2;
@@ -496,6 +540,7 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
         const uint32_t shift = head + len - pos;
         pos = head + len;
         lit_start = pos;
+        if (pos + 3 >= n) break;
         if (shift >= WAVE) {
             fresh(pos + lane);
             continue;
@@ -507,8 +552,6 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
         const uint32_t sf = shfl(f, from & (WAVE - 1));
         if (from < WAVE) {
             const uint32_t x = pos + lane;
-            e = se;
-            w = sw;
             uint32_t b0;
             if (off == o0)
                 b0 = sf & 1;
@@ -517,7 +560,9 @@ __device__ __forceinline__ void zstd_parse_subblock(const gc_item it, const uint
             else if (off == o2)
                 b0 = (sf >> 2) & 1;
             else
-                b0 = (x + 3 < n && rep_eq(src, avail, x, off, sw)) ? 1u : 0u;
+                b0 = (x + 3 < n && rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, sw)) ? 1u : 0u;
+            e = se;
+            w = sw;
             f = b0 | ((sf & 3u) << 1);  // flags now for (off, o0, o1)
         } else {
             fresh(pos + lane);
@@ -538,9 +583,15 @@ extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
 gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint32_t* info,
                      void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
                      uint32_t max_seq) {
-    const uint32_t wave = threadIdx.x / WAVE;
-    for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES)
-        zstd_parse_subblock(items[i], arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
+    // the wavefront index is uniform: items are read with scalar loads
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES) {
+        const gc_item it = items[i];
+        if (item_safe(it.avail, it.n))
+            zstd_parse_item<true>(it, arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
+        else
+            zstd_parse_item<false>(it, arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
+    }
 }
 
 // Small fill kernel.
