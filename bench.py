@@ -235,7 +235,8 @@ def main():
             "input_bytes_per_step": work_bytes,
             "sub_block": 65536,
             "levels": levels,
-            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "elem_wg_per_cu": tune[2]},
+            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "elem_wg_per_cu": tune[2],
+                     "info_strip": tune[3]},
         },
         "plain_per_chunk_gbs": round(plain_gbs, 3) if plain_gbs else None,
         "batched_vs_plain": round(value / plain_gbs, 2) if plain_gbs else None,

@@ -66,7 +66,7 @@ int enqueue(gc_batch* b, bool timed) {
     if (timed) (void)hipEventRecord(b->ev[2], st);
     if (b->n_z) {
         rc = launch_zstd_info(ctx, st, b->arena, zit, b->n_z, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
-                              (uint32_t*)b->info.p);
+                              (uint32_t*)b->info.p, b->tune.info_strip);
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[3], st);
@@ -101,7 +101,7 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     b->arena = d_arena;
     b->arena_size = arena_size;
     b->inputs.assign(inputs, inputs + n_inputs);
-    b->tune = gc_tune{64, 64, 64, 0};  // grid = items unless tuned down
+    b->tune = gc_tune{64, 64, 64, 8};  // grid = items unless tuned down
     std::vector<gc_item> zi, li;
     uint64_t zseq = 0, zcnt = 0, zpos = 0, lpos = 0;
     for (uint32_t i = 0; i < n_inputs; i++) {
@@ -195,6 +195,7 @@ int gc_batch_set_tune(gc_batch* b, const gc_tune* t) {
     if (t->cand_wg_per_cu < 1 || t->cand_wg_per_cu > 64 || t->elem_wg_per_cu < 1 || t->elem_wg_per_cu > 64 ||
         t->parse_wg_per_cu < 1 || t->parse_wg_per_cu > 64)
         return fail(GC_EINVAL, "tune out of range (1..64 workgroups per CU)");
+    if (t->info_strip != 4 && t->info_strip != 8) return fail(GC_EINVAL, "info_strip must be 4 or 8");
     b->tune = *t;
     return GC_OK;
 }
@@ -301,7 +302,7 @@ int gc_batch_autotune(gc_batch* b, uint32_t reps, gc_tune* best, float* best_ms)
         return GC_OK;
     };
     // The kernels are independent: tune each knob on the kernels it drives.
-    float best_cand = 1e30f, best_elem = 1e30f, best_parse = 1e30f, ms[4];
+    float best_cand = 1e30f, best_elem = 1e30f, best_elem_z = 1e30f, best_parse = 1e30f, ms[4];
     for (uint32_t v : opts) {
         gc_tune t = top;
         t.cand_wg_per_cu = v;
@@ -313,6 +314,12 @@ int gc_batch_autotune(gc_batch* b, uint32_t reps, gc_tune* best, float* best_ms)
         t.elem_wg_per_cu = v;
         if (int rc = measure(t, ms)) return rc;
         if (ms[1] + ms[2] < best_elem) best_elem = ms[1] + ms[2], top.elem_wg_per_cu = v;
+    }
+    for (uint32_t v : {4u, 8u}) {
+        gc_tune t = top;
+        t.info_strip = v;
+        if (int rc = measure(t, ms)) return rc;
+        if (ms[2] < best_elem_z) best_elem_z = ms[2], top.info_strip = v;
     }
     for (uint32_t v : opts) {
         gc_tune t = top;

@@ -18,17 +18,23 @@ bool aligned4(const void* p) { return ((uintptr_t)p & 3u) == 0; }
 
 uint64_t div_up(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Scratch positions per item: the sub-block size rounded up to 8, so that
+// per-position passes can move whole 8-position strips.
+uint64_t pitch(uint32_t sub) { return ((uint64_t)sub + 7) & ~(uint64_t)7; }
+
 // Items for a buffer tiled in sub-blocks of `sub` bytes from its start.
+// internal_out: the per-position output is scratch (Zstd info) and takes the
+// padded pitch; otherwise it is the caller's array indexed by position.
 void make_items(std::vector<gc_item>& v, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t max_seq,
-                uint32_t depth) {
+                uint32_t depth, bool internal_out) {
     v.resize(nsub);
     for (uint32_t s = 0; s < nsub; s++) {
         uint64_t start = (uint64_t)s * sub;
         gc_item& it = v[s];
         it.src_off = start;
         it.avail = total - start;
-        it.cpos = start;
-        it.opos = start;
+        it.cpos = (uint64_t)s * pitch(sub);
+        it.opos = internal_out ? (uint64_t)s * pitch(sub) : start;
         it.seq_pos = (uint64_t)s * max_seq;
         it.lit_pos = start;
         it.count_pos = s;
@@ -47,8 +53,8 @@ int check_sub(uint32_t sub) {
 int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t* d_packed,
               gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth) {
     std::vector<gc_item> items;
-    make_items(items, total, sub, nsub, 0, depth);
-    int rc = ctx->cand.ensure((size_t)nsub * sub * 2 + 256);
+    make_items(items, total, sub, nsub, 0, depth, false);
+    int rc = ctx->cand.ensure((size_t)nsub * pitch(sub) * 2 + 256);
     if (!rc) rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
     if (rc) return rc;
     const gc_item* d_items = (const gc_item*)ctx->items.p;
@@ -61,9 +67,9 @@ int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, ui
 int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, uint32_t nsub, void* d_seq,
              uint32_t fmt, uint8_t* d_lit, uint32_t* d_sc, uint32_t* d_lc, uint32_t depth, uint32_t max_seq) {
     std::vector<gc_item> items;
-    make_items(items, total, chunk, nsub, max_seq, depth);
-    int rc = ctx->cand.ensure((size_t)nsub * chunk * 2 + 256);
-    if (!rc) rc = ctx->info.ensure((size_t)nsub * chunk * 4 + 256);
+    make_items(items, total, chunk, nsub, max_seq, depth, true);
+    int rc = ctx->cand.ensure((size_t)nsub * pitch(chunk) * 2 + 256);
+    if (!rc) rc = ctx->info.ensure((size_t)nsub * pitch(chunk) * 4 + 256);
     if (!rc) rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
     if (rc) return rc;
     const gc_item* d_items = (const gc_item*)ctx->items.p;

@@ -88,8 +88,9 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
 }
 
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                     uint32_t grid, const uint16_t* cand16, uint32_t* info) {
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip) {
     void* args[] = {&arena, &items, &n_items, &cand16, &info};
+    if (strip == 4) return launch(stream, ctx->k.zstd_info4, grid, 256, args, "launch gc_zstd_info4_kernel");
     return launch(stream, ctx->k.zstd_info, grid, 256, args, "launch gc_zstd_info_kernel");
 }
 
@@ -152,6 +153,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand, ctx->k.module, "gc_cand_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len, ctx->k.module, "gc_lzma2_len_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info, ctx->k.module, "gc_zstd_info_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info4, ctx->k.module, "gc_zstd_info4_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");

@@ -31,6 +31,7 @@ struct Kernels {
     hipFunction_t cand = nullptr;
     hipFunction_t lzma2_len = nullptr;
     hipFunction_t zstd_info = nullptr;
+    hipFunction_t zstd_info4 = nullptr;
     hipFunction_t zstd_parse = nullptr;
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
@@ -62,7 +63,7 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
                      uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                      uint32_t max_matches);
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                     uint32_t grid, const uint16_t* cand16, uint32_t* info);
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip = 8);
 int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                       uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
                       uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq);

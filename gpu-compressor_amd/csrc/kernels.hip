@@ -55,6 +55,13 @@
 // readable bytes from `base`.  SAFE callers guarantee avail >= n + 16 for a
 // sub-block of n bytes, which covers every read below; otherwise a read that
 // would pass the end is built byte by byte from the bytes that exist.
+__device__ __attribute__((noinline)) uint32_t ld32_tail(const uint8_t* p, uint32_t off, uint64_t avail) {
+    uint32_t v = 0;
+    for (uint32_t i = 0; i < 4; i++)
+        if ((uint64_t)off + i < avail) v |= (uint32_t)p[i] << (8 * i);
+    return v;
+}
+
 template <bool SAFE>
 __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint64_t avail) {
     const uint8_t* p = base + off;
@@ -65,10 +72,7 @@ __device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint
         uint32_t hi = q[1];
         return __builtin_amdgcn_alignbyte(hi, lo, sh);
     }
-    uint32_t v = 0;
-    for (uint32_t i = 0; i < 4; i++)
-        if ((uint64_t)off + i < avail) v |= (uint32_t)p[i] << (8 * i);
-    return v;
+    return ld32_tail(p, off, avail);  // the last bytes of the arena, byte by byte
 }
 
 // Per-item reads may skip bounds checks when the arena extends 16 bytes past
@@ -363,33 +367,43 @@ gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items
 // at GC_ZSTD_SHORT: info = len << 16 | cand, len 0 = no.  A length equal to
 // the cap is re-measured by the parse.  Written for p + 3 < n.
 // ---------------------------------------------------------------------------
-template <bool SAFE>
+template <bool SAFE, uint32_t W>
 __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
                                                uint32_t* info) {
+    // cpos and opos are multiples of 8 and each item owns its positions
+    // rounded up to 8: whole strips of W positions are read and written with
+    // vector accesses
     const uint8_t* src = arena + it.src_off;
     const uint64_t avail = it.avail;
     const uint32_t n = it.n;
-    for (uint32_t p0 = threadIdx.x * STRIP; p0 + 3 < n; p0 += ELEM_THREADS * STRIP) {
+    for (uint32_t p0 = threadIdx.x * W; p0 + 3 < n; p0 += ELEM_THREADS * W) {
         uint32_t d[4];
         load_strip<SAFE>(src, p0, avail, d);
-        uint32_t c[STRIP], vc[STRIP];
+        uint32_t cv[W / 2];
+        if constexpr (W == 8) {
+            const uint4 cw = *(const uint4*)(cand16 + it.cpos + p0);
+            cv[0] = cw.x, cv[1] = cw.y, cv[2] = cw.z, cv[3] = cw.w;
+        } else {
+            const uint2 cw = *(const uint2*)(cand16 + it.cpos + p0);
+            cv[0] = cw.x, cv[1] = cw.y;
+        }
+        uint32_t c[W], vc[W];
 #pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++) {
-            const uint32_t p = p0 + j;
-            uint32_t cj = p + 3 < n ? cand16[it.cpos + p] : 0;
-            c[j] = cj == GC_NONE ? 0 : cj;
+        for (uint32_t j = 0; j < W; j++) {
+            const uint32_t cj = (cv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            c[j] = (cj == GC_NONE || p0 + j + 3 >= n) ? 0 : cj;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++) vc[j] = ld32<SAFE>(src, c[j], avail);
-        uint32_t out[STRIP];
+        for (uint32_t j = 0; j < W; j++) vc[j] = ld32<SAFE>(src, c[j], avail);
+        uint32_t out[W];
         uint32_t next_len = 0;
 #pragma unroll
-        for (int j = STRIP - 1; j >= 0; j--) {
+        for (int j = W - 1; j >= 0; j--) {
             const uint32_t p = p0 + j;
             uint32_t len = 0;
             if (p + 3 < n && p > 0 && ((vc[j] ^ strip_word(d, j)) & 0xFFFFFFu) == 0) {
                 const uint32_t lim = n - p < GC_ZSTD_SHORT ? n - p : GC_ZSTD_SHORT;
-                if (j + 1 < STRIP && next_len && c[j + 1] == c[j] + 1)
+                if (j + 1 < (int)W && next_len && c[j + 1] == c[j] + 1)
                     len = next_len + 1 < lim ? next_len + 1 : lim;
                 else
                     len = lcp_lane<SAFE>(src, avail, c[j], p, 3, lim);
@@ -397,22 +411,35 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
             next_len = len;
             out[j] = (len << 16) | c[j];
         }
-#pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++)
-            if (p0 + j + 3 < n) info[it.opos + p0 + j] = out[j];
+        uint4* o = (uint4*)(info + it.opos + p0);
+        o[0] = make_uint4(out[0], out[1], out[2], out[3]);
+        if constexpr (W == 8) o[1] = make_uint4(out[4], out[5], out[6], out[7]);
     }
 }
 
-extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
-gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
-                    uint32_t* info) {
+template <uint32_t W>
+__device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                                                const uint16_t* cand16, uint32_t* info) {
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = items[i];
         if (item_safe(it.avail, it.n))
-            zstd_info_item<true>(it, arena, cand16, info);
+            zstd_info_item<true, W>(it, arena, cand16, info);
         else
-            zstd_info_item<false>(it, arena, cand16, info);
+            zstd_info_item<false, W>(it, arena, cand16, info);
     }
+}
+
+// Strips of 8 or 4 positions: the same words, different register use.
+extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
+                    uint32_t* info) {
+    zstd_info_items<8>(arena, items, n_items, cand16, info);
+}
+
+extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
+                     uint32_t* info) {
+    zstd_info_items<4>(arena, items, n_items, cand16, info);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,27 +485,42 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0;
     uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
 
-    // window registers of this lane: info word, 3 bytes at x, repeat flags
-    uint32_t e = 0, w = 0, f = 0;
-    auto fresh = [&](uint32_t x) {
+    // Registers of this lane: the window position x = pos + lane (info word,
+    // 3 bytes at x, repeat flags) and the position 64 ahead (info word and
+    // 3 bytes), loaded one step before it enters the window.
+    uint32_t e = 0, w = 0, f = 0, ae = 0, aw = 0;
+    auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& wv) {
         const bool act = x + 3 < n;
         const uint32_t xc = act ? x : 0;  // reads stay in the sub-block
-        const uint32_t ev = inf[xc];
-        const uint32_t wv = ld32<SAFE>(src, xc, avail) & 0xFFFFFFu;
+        const uint32_t i0 = inf[xc];
+        const uint32_t w0 = ld32<SAFE>(src, xc, avail) & 0xFFFFFFu;
+        ev = act ? i0 : 0;
+        wv = act ? w0 : 0;
+    };
+    auto flags = [&](uint32_t x, uint32_t wv) -> uint32_t {
+        const bool act = x + 3 < n;
+        const uint32_t xc = act ? x : 0;
         const uint32_t fv = (rep_eq<SAFE>(src, avail, xc, rep0, wv) ? 1u : 0u) |
                             (rep_eq<SAFE>(src, avail, xc, rep1, wv) ? 2u : 0u) |
                             (rep_eq<SAFE>(src, avail, xc, rep2, wv) ? 4u : 0u);
-        e = act ? ev : 0;
-        w = act ? wv : 0;
-        f = act ? fv : 0;
+        return act ? fv : 0;
     };
-    if (n > 3) fresh(lane);
+    if (n > 3) {
+        peek(lane, e, w);
+        f = flags(lane, w);
+        peek(WAVE + lane, ae, aw);
+    }
 
     while (pos + 3 < n && nseq < max_seq) {
         const uint64_t m = ballot((e >> 16) != 0 || f != 0);
         if (!m) {
+            // no head in the window: the ahead registers become the window
             pos += WAVE;
-            if (pos + 3 < n) fresh(pos + lane);
+            e = ae;
+            w = aw;
+            f = flags(pos + lane, w);
+            __asm__ volatile("" ::: "memory");  // keep the prefetch behind the gathers
+            peek(pos + WAVE + lane, ae, aw);
             continue;
         }
         const uint32_t k = (uint32_t)__builtin_ctzll(m);
@@ -498,9 +540,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             len = lcp_wave<SAFE>(src, avail, head - off, head, n - head, 3);
         }
         // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
-        // is in the next lane unless head is the window's last position
+        // is in the next lane (or the first ahead lane)
         if (depth > 1 && head + 4 < n) {
-            const uint32_t e2 = k + 1 < WAVE ? bcast(e, k + 1) : inf[head + 1];
+            const uint32_t e2 = k + 1 < WAVE ? bcast(e, k + 1) : bcast(ae, 0);
             if (e2 >> 16) {
                 const uint32_t c2 = e2 & 0xFFFFu;
                 uint32_t len2 = e2 >> 16;
@@ -541,17 +583,27 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         pos = head + len;
         lit_start = pos;
         if (pos + 3 >= n) break;
-        if (shift >= WAVE) {
-            fresh(pos + lane);
+        const uint32_t x = pos + lane;
+        if (shift >= 2 * WAVE) {
+            peek(x, e, w);
+            f = flags(x, w);
+            __asm__ volatile("" ::: "memory");
+            peek(x + WAVE, ae, aw);
             continue;
         }
-        // slide the window by `shift`; lanes that stay keep their data
-        const uint32_t from = lane + shift;
-        const uint32_t se = shfl(e, from & (WAVE - 1));
-        const uint32_t sw = shfl(w, from & (WAVE - 1));
-        const uint32_t sf = shfl(f, from & (WAVE - 1));
-        if (from < WAVE) {
-            const uint32_t x = pos + lane;
+        // Slide both windows by `shift`.  Window lane i takes position
+        // pos + i from the old window (lane i + shift) or the old ahead
+        // registers (lane i + shift - 64); lanes that stay keep their repeat
+        // flags, others compute them.
+        const uint32_t from = lane + shift;  // < 128
+        const uint32_t fl = from & (WAVE - 1);
+        const uint32_t se = shfl(e, fl), sw = shfl(w, fl), sf = shfl(f, fl);
+        const uint32_t sae = shfl(ae, fl), saw = shfl(aw, fl);
+        const bool stay = from < WAVE;
+        e = stay ? se : sae;
+        w = stay ? sw : saw;
+        uint32_t nf;
+        if (stay) {
             uint32_t b0;
             if (off == o0)
                 b0 = sf & 1;
@@ -560,12 +612,20 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             else if (off == o2)
                 b0 = (sf >> 2) & 1;
             else
-                b0 = (x + 3 < n && rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, sw)) ? 1u : 0u;
-            e = se;
-            w = sw;
-            f = b0 | ((sf & 3u) << 1);  // flags now for (off, o0, o1)
+                b0 = rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, w) && x + 3 < n ? 1u : 0u;
+            nf = b0 | ((sf & 3u) << 1);  // flags now for (off, o0, o1)
         } else {
-            fresh(pos + lane);
+            nf = flags(x, w);
+        }
+        f = nf;
+        // ahead lane i takes position pos + 64 + i from the old ahead lane
+        // i + shift, or loads it
+        __asm__ volatile("" ::: "memory");  // keep the prefetch behind the gathers
+        if (shift < WAVE && stay) {
+            ae = sae;
+            aw = saw;
+        } else {
+            peek(x + WAVE, ae, aw);
         }
     }
     // trailing literals (zstd_compress.cu:341-344)

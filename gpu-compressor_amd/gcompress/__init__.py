@@ -100,7 +100,7 @@ class BatchInput(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("cand_wg_per_cu", C.c_uint32), ("parse_wg_per_cu", C.c_uint32),
-                ("elem_wg_per_cu", C.c_uint32), ("reserved", C.c_uint32)]
+                ("elem_wg_per_cu", C.c_uint32), ("info_strip", C.c_uint32)]
 
 
 class GcError(RuntimeError):
@@ -281,20 +281,20 @@ class Batch:
         _check(lib().gc_batch_run_timed(self.h, C.cast(ms, _vp)), "gc_batch_run_timed")
         return list(ms)
 
-    def set_tune(self, cand, parse, elem):
-        t = Tune(cand, parse, elem, 0)
+    def set_tune(self, cand, parse, elem, info_strip=8):
+        t = Tune(cand, parse, elem, info_strip)
         _check(lib().gc_batch_set_tune(self.h, C.byref(t)), "gc_batch_set_tune")
 
     def tune(self):
         t = Tune()
         _check(lib().gc_batch_get_tune(self.h, C.byref(t)), "gc_batch_get_tune")
-        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.elem_wg_per_cu)
+        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.elem_wg_per_cu, t.info_strip)
 
     def autotune(self, reps=2):
         t = Tune()
         ms = C.c_float()
         _check(lib().gc_batch_autotune(self.h, reps, C.byref(t), C.byref(ms)), "gc_batch_autotune")
-        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.elem_wg_per_cu), ms.value
+        return (t.cand_wg_per_cu, t.parse_wg_per_cu, t.elem_wg_per_cu, t.info_strip), ms.value
 
     def n_sub(self, i):
         ns = C.c_uint32()

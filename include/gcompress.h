@@ -228,12 +228,12 @@ typedef struct {
 } gc_batch_input;
 
 /* Output-invariant launch knobs: workgroups per CU of each kernel (grid =
- * min(knob * CUs, items)), 1..64. */
+ * min(knob * CUs, items)), 1..64, and the Zstd verify pass's strip width. */
 typedef struct {
     uint32_t cand_wg_per_cu;  /* candidate pass (LDS table), both finders */
     uint32_t parse_wg_per_cu; /* Zstd parse (4 sub-blocks per workgroup) */
     uint32_t elem_wg_per_cu;  /* per-position passes: LZMA2 lengths, Zstd verify */
-    uint32_t reserved;
+    uint32_t info_strip;      /* positions per thread in the Zstd verify pass: 4 or 8 */
 } gc_tune;
 
 /* Batch execution modes.  GC_BATCH_QUEUE_GRAPH (a device work queue drained

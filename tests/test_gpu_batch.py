@@ -78,7 +78,7 @@ def _check_batch(gctx, b, datas, inputs, p):
                         np.testing.assert_array_equal(seqs[k, : counts[k]], z)
         ms = b.run_timed()
         assert len(ms) == 4 and all(m >= 0 for m in ms)
-        b.set_tune(1, 2, 1)  # fewer workgroups than items: grid-stride assignment
+        b.set_tune(1, 2, 1, 4)  # fewer workgroups than items: grid-stride assignment; narrow strips
         b.run()
         gctx.sync()
         np.testing.assert_array_equal(b.lzma2(2), oracle.lzma2_packed(datas[2]))
