@@ -592,16 +592,23 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             continue;
         }
         // Slide both windows by `shift`.  Window lane i takes position
-        // pos + i from the old window (lane i + shift) or the old ahead
-        // registers (lane i + shift - 64); lanes that stay keep their repeat
-        // flags, others compute them.
-        const uint32_t from = lane + shift;  // < 128
+        // pos + i from the old window (lane i + shift), the old ahead
+        // registers (lane i + shift - 64) or, past those, memory; lanes that
+        // stay keep their repeat flags, others compute them.
+        const uint32_t from = lane + shift;  // < 192
         const uint32_t fl = from & (WAVE - 1);
         const uint32_t se = shfl(e, fl), sw = shfl(w, fl), sf = shfl(f, fl);
         const uint32_t sae = shfl(ae, fl), saw = shfl(aw, fl);
         const bool stay = from < WAVE;
-        e = stay ? se : sae;
-        w = stay ? sw : saw;
+        if (stay) {
+            e = se;
+            w = sw;
+        } else if (from < 2 * WAVE) {
+            e = sae;
+            w = saw;
+        } else {
+            peek(x, e, w);
+        }
         uint32_t nf;
         if (stay) {
             uint32_t b0;

@@ -20,6 +20,16 @@ def cases():
     out.append(("backrefs", datagen.backrefs(4 * 65536 + 3, 2)))
     out.append(("structured", datagen.structured(4 * 65536 + 5, 3)))
     out.append(("mix", datagen.mix(8 * 65536 + 99, 4)))
+    # copies of 64..160 bytes: the parse window slides past its ahead registers
+    rng = np.random.default_rng(99)
+    parts, total = [rng.integers(0, 256, 4096, dtype=np.uint8)], 4096
+    while total < 3 * 65536:
+        ln = int(rng.integers(64, 161))
+        src = int(rng.integers(0, total - ln))
+        parts.append(np.concatenate(parts)[src:src + ln])
+        parts.append(rng.integers(0, 256, int(rng.integers(0, 40)), dtype=np.uint8))
+        total = sum(p.size for p in parts)
+    out.append(("longcopies", np.concatenate(parts)))
     for k in (1, 2, 3, 4, 7, 8, 9, 16, 63, 64, 65, 200):
         out.append((f"period{k}", datagen.periodic(65536 + 17, k, k)))
     return out
