@@ -445,21 +445,18 @@ gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_item
 // ---------------------------------------------------------------------------
 // Zstd parse (zstd_compress.cu:230-338), one wavefront per sub-block.
 //
-// The wavefront holds a window of 64 positions x = pos + lane: the info word
-// of x and, for each current repeat offset r, whether the 3 bytes at x - r
-// equal those at x.  The first lane whose position matches (hash candidate
+// The wavefront holds an aligned window of 64 positions x = wbase + lane
+// (wbase a multiple of 64): the info word of x, the 3 bytes at x and, for
+// each current repeat offset r, whether the 3 bytes at x - r are the same.
+// The next window's info words and bytes are loaded one window early.  The
+// first lane at or after pos whose position matches (hash candidate
 // verified, or a repeat offset) is the next head: every position before it
-// is a literal.  After a sequence the window slides by the bytes consumed;
-// lanes that stay keep their data (the repeat offsets after a sequence are
-// the new offset and the two most recent old ones, whose flags are kept), so
-// a step loads one gather for the new offset plus the positions that enter.
+// is a literal.  After a sequence the repeat offsets become (new, old0,
+// old1): each lane renames its two kept flags and gathers the new one, so no
+// data moves between lanes until the walk leaves the window.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t len) {
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
-}
-
-__device__ __forceinline__ uint32_t shfl(uint32_t v, uint32_t src_lane) {
-    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
 }
 
 // 3 bytes at x - r equal w (the 3 bytes at x), for a head candidate x
@@ -482,12 +479,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     const uint32_t lane = lane_id();
     const uint32_t* inf = info + it.opos;
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
-    uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0;
+    uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0, wbase = 0;
     uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
 
-    // Registers of this lane: the window position x = pos + lane (info word,
-    // 3 bytes at x, repeat flags) and the position 64 ahead (info word and
-    // 3 bytes), loaded one step before it enters the window.
     uint32_t e = 0, w = 0, f = 0, ae = 0, aw = 0;
     auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& wv) {
         const bool act = x + 3 < n;
@@ -508,23 +502,26 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     if (n > 3) {
         peek(lane, e, w);
         f = flags(lane, w);
+        __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
         peek(WAVE + lane, ae, aw);
     }
 
     while (pos + 3 < n && nseq < max_seq) {
-        const uint64_t m = ballot((e >> 16) != 0 || f != 0);
+        const uint32_t lo = pos - wbase;
+        const uint64_t m = ballot(lane >= lo && ((e >> 16) != 0 || f != 0));
         if (!m) {
-            // no head in the window: the ahead registers become the window
-            pos += WAVE;
+            // no head left in the window: every position up to its end is a literal
+            wbase += WAVE;
+            pos = wbase;
             e = ae;
             w = aw;
-            f = flags(pos + lane, w);
-            __asm__ volatile("" ::: "memory");  // keep the prefetch behind the gathers
-            peek(pos + WAVE + lane, ae, aw);
+            f = flags(wbase + lane, w);
+            __asm__ volatile("" ::: "memory");
+            peek(wbase + WAVE + lane, ae, aw);
             continue;
         }
         const uint32_t k = (uint32_t)__builtin_ctzll(m);
-        uint32_t head = pos + k;
+        uint32_t head = wbase + k;
         const uint32_t ek = bcast(e, k);
         uint32_t len, off;
         if (ek >> 16) {
@@ -540,7 +537,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             len = lcp_wave<SAFE>(src, avail, head - off, head, n - head, 3);
         }
         // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
-        // is in the next lane (or the first ahead lane)
+        // is in the next lane, or the first ahead lane
         if (depth > 1 && head + 4 < n) {
             const uint32_t e2 = k + 1 < WAVE ? bcast(e, k + 1) : bcast(ae, 0);
             if (e2 >> 16) {
@@ -579,60 +576,35 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         rep2 = rep1;
         rep1 = rep0;
         rep0 = off;
-        const uint32_t shift = head + len - pos;
         pos = head + len;
         lit_start = pos;
         if (pos + 3 >= n) break;
-        const uint32_t x = pos + lane;
-        if (shift >= 2 * WAVE) {
-            peek(x, e, w);
-            f = flags(x, w);
+        if (pos >= wbase + WAVE) {
+            // the walk left the window: move to the window holding pos
+            const uint32_t nb = pos & ~(WAVE - 1);
+            if (nb == wbase + WAVE) {
+                e = ae;
+                w = aw;
+            } else {
+                peek(nb + lane, e, w);
+            }
+            wbase = nb;
+            f = flags(wbase + lane, w);
             __asm__ volatile("" ::: "memory");
-            peek(x + WAVE, ae, aw);
-            continue;
-        }
-        // Slide both windows by `shift`.  Window lane i takes position
-        // pos + i from the old window (lane i + shift), the old ahead
-        // registers (lane i + shift - 64) or, past those, memory; lanes that
-        // stay keep their repeat flags, others compute them.
-        const uint32_t from = lane + shift;  // < 192
-        const uint32_t fl = from & (WAVE - 1);
-        const uint32_t se = shfl(e, fl), sw = shfl(w, fl), sf = shfl(f, fl);
-        const uint32_t sae = shfl(ae, fl), saw = shfl(aw, fl);
-        const bool stay = from < WAVE;
-        if (stay) {
-            e = se;
-            w = sw;
-        } else if (from < 2 * WAVE) {
-            e = sae;
-            w = saw;
+            peek(wbase + WAVE + lane, ae, aw);
         } else {
-            peek(x, e, w);
-        }
-        uint32_t nf;
-        if (stay) {
+            // flags now for (off, o0, o1)
+            const uint32_t x = wbase + lane;
             uint32_t b0;
             if (off == o0)
-                b0 = sf & 1;
+                b0 = f & 1;
             else if (off == o1)
-                b0 = (sf >> 1) & 1;
+                b0 = (f >> 1) & 1;
             else if (off == o2)
-                b0 = (sf >> 2) & 1;
+                b0 = (f >> 2) & 1;
             else
-                b0 = rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, w) && x + 3 < n ? 1u : 0u;
-            nf = b0 | ((sf & 3u) << 1);  // flags now for (off, o0, o1)
-        } else {
-            nf = flags(x, w);
-        }
-        f = nf;
-        // ahead lane i takes position pos + 64 + i from the old ahead lane
-        // i + shift, or loads it
-        __asm__ volatile("" ::: "memory");  // keep the prefetch behind the gathers
-        if (shift < WAVE && stay) {
-            ae = sae;
-            aw = saw;
-        } else {
-            peek(x + WAVE, ae, aw);
+                b0 = (x + 3 < n && rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, w)) ? 1u : 0u;
+            f = b0 | ((f & 3u) << 1);
         }
     }
     // trailing literals (zstd_compress.cu:341-344)

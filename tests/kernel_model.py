@@ -182,3 +182,96 @@ def zstd_parse(d, info, depth=16, max_seq=None):
                 nae[i], naw[i] = peek(pos + WAVE + i)
         e, w, f, ae, aw = ne, nw, nf, nae, naw
     return seqs
+
+
+def zstd_parse_fixed(d, info, depth=16, max_seq=None):
+    """gc_zstd_parse_kernel, aligned-window form: lanes hold positions wbase + i (wbase a multiple
+    of 64) and the next window ahead; lanes before pos are masked out of the search."""
+    n = len(d)
+    if max_seq is None:
+        max_seq = 65536 // 3
+    rep = [1, 4, 8]
+    pos = lit_start = 0
+    seqs = []
+
+    def peek(x):
+        if x + 3 < n:
+            return info[x], v3(d, x)
+        return 0, (0, 0, 0)
+
+    def eq(x, r, w):
+        return x + 3 < n and r <= x and v3(d, x - r) == w
+
+    def flags(x, w):
+        return (1 if eq(x, rep[0], w) else 0) | (2 if eq(x, rep[1], w) else 0) | (4 if eq(x, rep[2], w) else 0)
+
+    wbase = 0
+    e, w = zip(*[peek(i) for i in range(WAVE)]) if n > 3 else ([0] * WAVE, [(0, 0, 0)] * WAVE)
+    e, w = list(e), list(w)
+    f = [flags(i, w[i]) for i in range(WAVE)]
+    ae, aw = map(list, zip(*[peek(WAVE + i) for i in range(WAVE)]))
+    while pos + 3 < n and len(seqs) < max_seq:
+        lo = pos - wbase
+        m = [i for i in range(lo, WAVE) if (e[i] >> 16) or f[i]]
+        if not m:
+            wbase += WAVE
+            pos = wbase
+            e, w = ae, aw
+            f = [flags(wbase + i, w[i]) for i in range(WAVE)]
+            ae, aw = map(list, zip(*[peek(wbase + WAVE + i) for i in range(WAVE)]))
+            continue
+        k = m[0]
+        head = wbase + k
+        ek = e[k]
+        if ek >> 16:
+            c = ek & 0xFFFF
+            off = head - c
+            ln = ek >> 16
+            if ln >= SHORT:
+                ln = common(d, c, head, SHORT, n - head)
+        else:
+            fk = f[k]
+            off = rep[0] if fk & 1 else (rep[1] if fk & 2 else rep[2])
+            ln = common(d, head - off, head, 3, n - head)
+        if depth > 1 and head + 4 < n:
+            e2 = e[k + 1] if k + 1 < WAVE else ae[0]
+            if e2 >> 16:
+                c2 = e2 & 0xFFFF
+                l2 = e2 >> 16
+                if l2 >= SHORT:
+                    l2 = common(d, c2, head + 1, SHORT, n - head - 1)
+                if l2 > ln + 1:
+                    head += 1
+                    ln = l2
+                    off = head - c2
+        seqs.append((head - lit_start, ln - 3, off))
+        o0, o1, o2 = rep
+        rep = [off, o0, o1]
+        pos = head + ln
+        lit_start = pos
+        if pos + 3 >= n:
+            break
+        if pos >= wbase + WAVE:
+            nb = pos & ~(WAVE - 1)
+            if nb == wbase + WAVE:
+                e, w = ae, aw
+            else:
+                e, w = map(list, zip(*[peek(nb + i) for i in range(WAVE)]))
+            wbase = nb
+            f = [flags(wbase + i, w[i]) for i in range(WAVE)]
+            ae, aw = map(list, zip(*[peek(wbase + WAVE + i) for i in range(WAVE)]))
+        else:
+            nf = []
+            for i in range(WAVE):
+                sf = f[i]
+                if off == o0:
+                    b0 = sf & 1
+                elif off == o1:
+                    b0 = (sf >> 1) & 1
+                elif off == o2:
+                    b0 = (sf >> 2) & 1
+                else:
+                    b0 = 1 if eq(wbase + i, off, w[i]) else 0
+                nf.append(b0 | ((sf & 3) << 1))
+            f = nf
+    return seqs

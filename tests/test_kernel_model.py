@@ -44,8 +44,10 @@ def test_info_strips_equal_direct(name, data, strip):
 
 @pytest.mark.parametrize("name,data", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("depth", [1, 16])
-def test_parse_window_equals_oracle(name, data, depth):
+@pytest.mark.parametrize("form", ["sliding", "aligned"])
+def test_parse_window_equals_oracle(name, data, depth, form):
     d = data.tolist()
-    got = km.zstd_parse(d, km.zstd_info(d), depth)
+    parse = km.zstd_parse if form == "sliding" else km.zstd_parse_fixed
+    got = parse(d, km.zstd_info(d), depth)
     seq, sc, _, _ = oracle.zstd_match_find(data, depth=depth)
     assert got == [tuple(x) for x in seq[0, : sc[0]].tolist()]
