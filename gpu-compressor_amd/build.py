@@ -32,6 +32,7 @@ HOST_SOURCES = [
     "gc_batch.cpp",
     "gc_corpus.cpp",
     "gc_codec.cpp",
+    "gc_serve.cpp",
 ]
 KERNEL_SOURCES = ["kernels.hip"]
 

@@ -84,6 +84,14 @@ SIGNATURES = [
     ("gc_batch_device_outputs", C.c_int, [_vp, _vp, _vp, _vp]),
     ("gc_batch_copy_zstd", C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     ("gc_batch_copy_lzma2", C.c_int, [_vp, C.c_uint32, _vp]),
+    ("gc_server_start", _vp, [_vp, C.c_uint64, C.c_uint32]),
+    ("gc_server_submit", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("gc_server_wait", C.c_int, [_vp, C.c_uint64, C.c_int, _vp]),
+    ("gc_server_result_zstd", C.c_int, [_vp, C.c_uint64, _vp, C.c_uint64, _vp]),
+    ("gc_server_result_lzma2", C.c_int, [_vp, C.c_uint64, _vp]),
+    ("gc_server_release", C.c_int, [_vp, C.c_uint64]),
+    ("gc_server_get_stats", C.c_int, [_vp, _vp]),
+    ("gc_server_stop", C.c_int, [_vp]),
     ("gc_corpus_generate", C.c_int, [_vp, C.c_uint64, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint64,
                                      C.c_uint64]),
 ]
@@ -101,6 +109,16 @@ class BatchInput(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("cand_wg_per_cu", C.c_uint32), ("parse_wg_per_cu", C.c_uint32),
                 ("elem_wg_per_cu", C.c_uint32), ("info_strip", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("size", C.c_uint64), ("algo", C.c_uint32), ("n_sub", C.c_uint32), ("n_sequences", C.c_uint64),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("batches", C.c_uint64), ("items", C.c_uint64), ("completed", C.c_uint64), ("bytes", C.c_uint64),
+                ("hbm_in_use", C.c_uint64), ("pending", C.c_uint64)]
 
 
 class GcError(RuntimeError):
@@ -427,3 +445,57 @@ def lzma2_raw_decode(blob: bytes, expected: int) -> bytes:
     n = C.c_size_t()
     _check(lib().gc_lzma2_raw_decode(_ptr(f), f.size, _ptr(out), expected, C.byref(n)), "gc_lzma2_raw_decode")
     return out[: n.value].tobytes()
+
+
+class Server:
+    """Served form (gc_server_*): continuous batching of match-finding requests."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_batch_items: int = 0):
+        self.ctx = ctx
+        h = lib().gc_server_start(ctx.h, hbm_budget, max_batch_items)
+        if not h:
+            raise GcError(f"gc_server_start: {lib().gc_last_error().decode()}")
+        self.h = h
+        self._keep = {}
+
+    def submit(self, data: np.ndarray, algo: int, level: int = 1) -> int:
+        """Submit a request; `data` is kept referenced until release()."""
+        rid = lib().gc_server_submit(self.h, _ptr(data), data.size, algo, level)
+        if rid < 0:
+            _check(int(rid), "gc_server_submit")
+        self._keep[rid] = data
+        return rid
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        info = RequestInfo()
+        rc = lib().gc_server_wait(self.h, rid, timeout_ms, C.byref(info))
+        if rc == 1:
+            return None
+        _check(rc, "gc_server_wait")
+        return info
+
+    def result_zstd(self, rid: int, n_sub: int, n_seq: int):
+        seqs = np.zeros((max(n_seq, 1), 4), np.uint32)
+        counts = np.zeros(max(n_sub, 1), np.uint32)
+        _check(lib().gc_server_result_zstd(self.h, rid, _ptr(seqs), seqs.shape[0], _ptr(counts)),
+               "gc_server_result_zstd")
+        return seqs[:n_seq], counts[:n_sub]
+
+    def result_lzma2(self, rid: int, n: int):
+        out = np.zeros(max(n, 1), np.uint32)
+        _check(lib().gc_server_result_lzma2(self.h, rid, _ptr(out)), "gc_server_result_lzma2")
+        return out[:n]
+
+    def release(self, rid: int):
+        _check(lib().gc_server_release(self.h, rid), "gc_server_release")
+        self._keep.pop(rid, None)
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().gc_server_get_stats(self.h, C.byref(st)), "gc_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def stop(self):
+        if self.h:
+            lib().gc_server_stop(self.h)
+            self.h = None

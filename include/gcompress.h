@@ -272,6 +272,46 @@ int gc_batch_device_outputs(gc_batch* b, gc_zstd_sequence** d_seqs, uint32_t** d
 int gc_batch_copy_zstd(gc_batch* b, uint32_t i, gc_zstd_sequence* seqs, uint32_t* counts);
 int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed);
 
+/* ---- served form ------------------------------------------------------- */
+/* No reference counterpart (the reference compresses one chunk at a time,
+ * pipeline.rs:138-175).  A server thread admits requests in arrival order
+ * while their device memory fits hbm_budget bytes, copies them to the device
+ * as they are admitted, and whenever the GPU is free launches every request
+ * whose input has arrived as one batch; requests arriving meanwhile join the
+ * next batch.  A request's results equal the plain form's for its input
+ * alone.  Request memory (caller's `data`) must stay valid until the request
+ * is done. */
+typedef struct gc_server gc_server;
+
+typedef struct {
+    uint64_t size;
+    uint32_t algo;
+    uint32_t n_sub;
+    uint64_t n_sequences; /* Zstd: sequences found */
+    double t_submit, t_admit, t_start, t_done; /* seconds, steady clock */
+} gc_request_info;
+
+typedef struct {
+    uint64_t batches, items, completed, bytes;
+    uint64_t hbm_in_use, pending;
+} gc_server_stats;
+
+/* max_batch_items: sub-blocks per batch (0 = 65536). */
+gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_items);
+/* Returns the request id (> 0), or a negative GC_E* code (GC_EBUDGET: the
+ * request alone needs more than the budget). */
+int64_t gc_server_submit(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level);
+/* 0 done, 1 still running after timeout_ms (< 0 waits), < 0 failed. */
+int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info);
+/* Results of a finished request: Zstd packed sequences (cap records) and per
+ * sub-block counts; LZMA2 packed matches, one u32 per input byte. */
+int gc_server_result_zstd(gc_server* s, uint64_t id, gc_zstd_sequence* seqs, uint64_t cap, uint32_t* seq_counts);
+int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed);
+int gc_server_release(gc_server* s, uint64_t id);
+int gc_server_get_stats(gc_server* s, gc_server_stats* out);
+/* Finishes every submitted request, then stops. */
+int gc_server_stop(gc_server* s);
+
 /* ---- synthetic corpora -------------------------------------------------- */
 /* Deterministic four-kind mix (uniform random, word text, back-references,
  * structured binary) in segments of `segment` bytes; weights[4] are relative

@@ -1,0 +1,75 @@
+"""Served form: each request's result is the same alone and in a concurrent mix, equal to the
+oracle; admission under a tight HBM budget; oversized requests are rejected."""
+import numpy as np
+import pytest
+
+import gcompress
+import oracle
+
+pytestmark = pytest.mark.gpu
+SUB = 65536
+
+
+def _requests(seed, count, max_size=600000):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(1024), np.log(max_size))))
+        data = gcompress.corpus(n, 1000 + seed * 100 + i, rng.dirichlet([1, 1, 1, 1]))
+        algo, level = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2),
+                       (gcompress.ALGO_LZMA2, 1)][i % 4]
+        out.append((data, algo, level))
+    return out
+
+
+def _check(srv, rid, data, algo, level):
+    info = srv.wait(rid)
+    assert info.size == data.size
+    if algo == gcompress.ALGO_LZMA2:
+        got = srv.result_lzma2(rid, data.size)
+        np.testing.assert_array_equal(got, oracle.lzma2_packed(data))
+    else:
+        seqs, counts = srv.result_zstd(rid, info.n_sub, info.n_sequences)
+        oseq, osc, _, _ = oracle.zstd_match_find(data, depth={1: 16, 2: 64}[level])
+        np.testing.assert_array_equal(counts, osc)
+        want = oracle.zstd_as_zstd_sequences(oseq, osc)
+        np.testing.assert_array_equal(seqs, np.concatenate(want) if want else np.zeros((0, 4), np.uint32))
+    assert info.t_submit <= info.t_admit <= info.t_start <= info.t_done
+    srv.release(rid)
+
+
+def test_alone_and_in_mix(gctx):
+    reqs = _requests(1, 24)
+    srv = gcompress.Server(gctx, 2 << 30)
+    try:
+        print("[serve] alone", flush=True)
+        for data, algo, level in reqs[:6]:
+            rid = srv.submit(data, algo, level)
+            _check(srv, rid, data, algo, level)
+        print("[serve] mix", flush=True)
+        ids = [srv.submit(data, algo, level) for data, algo, level in reqs]
+        for rid, (data, algo, level) in zip(ids, reqs):
+            _check(srv, rid, data, algo, level)
+        st = srv.stats()
+        assert st["completed"] == 30 and st["hbm_in_use"] == 0
+    finally:
+        srv.stop()
+
+
+def test_tight_budget_and_rejection(gctx):
+    reqs = _requests(2, 12, 300000)
+    srv = gcompress.Server(gctx, 24 << 20)  # a few requests at a time
+    try:
+        print("[serve] tight budget", flush=True)
+        ids = [srv.submit(data, algo, level) for data, algo, level in reqs]
+        for rid, (data, algo, level) in zip(ids, reqs):
+            _check(srv, rid, data, algo, level)
+        assert srv.stats()["batches"] >= 2
+        big = np.zeros(32 << 20, np.uint8)
+        with pytest.raises(gcompress.GcError, match="budget"):
+            srv.submit(big, gcompress.ALGO_ZSTD, 1)
+        rid = srv.submit(np.zeros(0, np.uint8), gcompress.ALGO_LZMA2, 1)
+        assert srv.wait(rid).size == 0
+        srv.release(rid)
+    finally:
+        srv.stop()
