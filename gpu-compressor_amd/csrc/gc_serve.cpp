@@ -8,8 +8,15 @@
 //   3. whenever no batch is running, launches every request whose input has
 //      arrived as one batch (one launch per kernel over all their sub-blocks),
 //   4. when a batch ends, starts the next one at once and meanwhile brings the
-//      finished requests' results back on the copy stream (sequences packed by
-//      gc_gather_seqs first), frees their memory and marks them done.
+//      finished requests' results back on a second copy stream (sequences
+//      packed by gc_gather_seqs first), straight into the caller's buffers
+//      when it gave some (gc_server_submit_into), frees their memory and marks
+//      them done.
+// Per-batch sequence counts come back on the compute stream right after the
+// parse (into pinned memory), so the host never waits on a synchronous copy
+// between batches.  The server thread only polls events: a batch that ends
+// while results of an earlier one are still being copied waits in `finished`
+// and the device goes on with the next batch.
 // So requests that arrive while others run are admitted as memory frees up
 // and join the next batch; a result does not depend on what it was batched
 // with (sub-blocks are independent).  No reference counterpart: the
@@ -95,6 +102,9 @@ struct Req {
     uint64_t dev_off = 0, dev_len = 0;            // the request's region of the pool
     uint64_t in_off = 0, cand_off = 0, info_off = 0, out_off = 0;  // byte offsets in the region
     uint32_t item0 = 0;        // first Zstd count index of the request in its batch
+    void* dst = nullptr;       // caller's result buffer (gc_server_submit_into), else results kept here
+    uint64_t dst_cap = 0;      // its size in bytes
+    uint32_t* dst_counts = nullptr;
     hipEvent_t copied = nullptr;
     State state = PENDING;
     int error = 0;
@@ -143,8 +153,14 @@ struct gc_server {
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     // batch state (server thread only)
     std::vector<uint64_t> running, fetching;
-    hipEvent_t batch_done = nullptr, fetch_done = nullptr;
+    std::deque<std::vector<uint64_t>> finished;  // batches done on the device, results not yet copied
+    hipStream_t fetch_stream = nullptr;  // results back to the host (the context's stream2 brings inputs in)
+    hipEvent_t batch_start = nullptr, batch_done = nullptr, fetch_start = nullptr, fetch_done = nullptr;
     DevBuf items, counts, meta;   // per-batch sequence counts; counts + offsets for the gathers
+    uint32_t* counts_host = nullptr;  // pinned copy of `counts`
+    size_t counts_host_cap = 0;       // in u32
+    gc_item* items_host = nullptr;    // pinned staging of `items`
+    size_t items_host_cap = 0;        // in items
     uint8_t* meta_host = nullptr; // pinned staging of `meta`
     size_t meta_cap = 0;
     std::vector<gc_item> host_items;
@@ -154,7 +170,8 @@ struct gc_server {
     void loop();
     void admit_locked();
     bool launch_batch();
-    void finish_batch();
+    void batch_ended();
+    void finish_batch(std::vector<uint64_t>& done);
     void complete_fetch();
 };
 
@@ -239,8 +256,34 @@ bool gc_server::launch_batch() {
     std::lock_guard<std::mutex> g(ctx->mu);
     int rc = items.ensure(n_all * sizeof(gc_item));
     if (!rc) rc = counts.ensure((size_t)n_z * 4 + 4);
+    // pinned staging (no batch runs now, so the last batch's copies are done)
+    if (!rc && n_z > counts_host_cap) {
+        (void)hipHostFree(counts_host);
+        counts_host_cap = 0;
+        if (hipHostMalloc((void**)&counts_host, (size_t)n_z * 4, 0) != hipSuccess) {
+            counts_host = nullptr;
+            rc = fail(GC_ENOMEM, "gc_server: pinned counts buffer");
+        } else {
+            counts_host_cap = n_z;
+        }
+    }
+    if (!rc && n_all > items_host_cap) {
+        (void)hipHostFree(items_host);
+        items_host_cap = 0;
+        if (hipHostMalloc((void**)&items_host, (size_t)n_all * sizeof(gc_item), 0) != hipSuccess) {
+            items_host = nullptr;
+            rc = fail(GC_ENOMEM, "gc_server: pinned item buffer");
+        } else {
+            items_host_cap = n_all;
+        }
+    }
     hipError_t e = hipSuccess;
-    if (!rc) e = hipMemcpyAsync(items.p, host_items.data(), n_all * sizeof(gc_item), hipMemcpyHostToDevice, ctx->stream);
+    if (!rc) {
+        memcpy(items_host, host_items.data(), (size_t)n_all * sizeof(gc_item));
+        e = hipEventRecord(batch_start, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(items.p, items_host, n_all * sizeof(gc_item), hipMemcpyHostToDevice, ctx->stream);
+    }
     const gc_item* all = (const gc_item*)items.p;
     const double t = now_s();
     for (Req* r : batch) {
@@ -258,7 +301,9 @@ bool gc_server::launch_batch() {
         if (!rc && n_z)
             rc = launch_zstd_parse(ctx, ctx->stream, pool, all + n_l, n_z, (n_z + 3) / 4, (const uint32_t*)pool,
                                    pool, GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)counts.p, nullptr, kMaxSeq);
-        if (!rc) e = hipEventRecord(batch_done, ctx->stream);
+        if (!rc && n_z)
+            e = hipMemcpyAsync(counts_host, counts.p, (size_t)n_z * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (!rc && e == hipSuccess) e = hipEventRecord(batch_done, ctx->stream);
     }
     running.clear();
     for (Req* r : batch) running.push_back(r->id);
@@ -269,6 +314,8 @@ bool gc_server::launch_batch() {
             r->error = rc ? rc : GC_EHIP;
             r->t_done = now_s();
             alloc.release(r->dev_off, r->dev_len);
+            if (r->copied) (void)hipEventDestroy(r->copied);
+            r->copied = nullptr;
         }
         running.clear();
         cv_done.notify_all();
@@ -279,20 +326,31 @@ bool gc_server::launch_batch() {
     return true;
 }
 
-// The batch ended (before the next one starts): bring its results back on
-// the copy stream.  Zstd: the counts come back first; each request's used
-// sequence slots are then packed by gc_gather_seqs (counts and offsets staged
-// in `meta`) and copied.
-void gc_server::finish_batch() {
-    fetching.swap(running);
-    running.clear();
+// The running batch ended on the device: take its sequence counts (already
+// in pinned memory) and queue it for the copy back.
+void gc_server::batch_ended() {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, batch_start, batch_done) == hipSuccess) stats.t_batches += ms * 1e-3;
+    for (uint64_t id : running) {
+        Req& r = *reqs[id];
+        if (r.algo == GC_ALGO_ZSTD) r.counts.assign(counts_host + r.item0, counts_host + r.item0 + r.nsub);
+    }
+    finished.emplace_back();
+    finished.back().swap(running);
+}
+
+// Bring a finished batch's results back on the fetch stream (no other fetch
+// is in flight).  Zstd: each request's used sequence slots are packed by
+// gc_gather_seqs (counts and offsets staged in `meta`) and copied.
+void gc_server::finish_batch(std::vector<uint64_t>& done) {
+    fetching.swap(done);
+    done.clear();
     std::lock_guard<std::mutex> g(ctx->mu);
-    hipStream_t cs = ctx->stream2;
+    hipStream_t cs = fetch_stream;
+    (void)hipEventRecord(fetch_start, cs);
     uint32_t n_z = 0;
     for (uint64_t id : fetching)
         if (reqs[id]->algo == GC_ALGO_ZSTD) n_z += reqs[id]->nsub;
-    std::vector<uint32_t> host_counts(n_z);
-    if (n_z) (void)hipMemcpy(host_counts.data(), counts.p, (size_t)n_z * 4, hipMemcpyDeviceToHost);
     // meta: counts (u32) for every Zstd item, then offsets (u64)
     const size_t off_bytes = up((size_t)n_z * 4, 8);
     if (off_bytes + (size_t)n_z * 8 > meta_cap) {  // a batch larger than any before
@@ -310,7 +368,6 @@ void gc_server::finish_batch() {
         Req& r = *reqs[id];
         r.state = FETCHING;
         if (r.algo != GC_ALGO_ZSTD || !meta_host) continue;
-        r.counts.assign(host_counts.begin() + r.item0, host_counts.begin() + r.item0 + r.nsub);
         uint64_t tot = 0;
         for (uint32_t s2 = 0; s2 < r.nsub; s2++) {
             mc[r.item0 + s2] = r.counts[s2];
@@ -318,6 +375,10 @@ void gc_server::finish_batch() {
             tot += r.counts[s2];
         }
         r.fetch_total = tot;
+        if (r.dst) {
+            memcpy(r.dst_counts, r.counts.data(), (size_t)r.nsub * 4);
+            if (tot * sizeof(gc_zstd_sequence) > r.dst_cap) r.error = GC_ESIZE;
+        }
     }
     if (n_z && meta_host) (void)hipMemcpyAsync(meta.p, meta_host, off_bytes + (size_t)n_z * 8, hipMemcpyHostToDevice, cs);
     for (uint64_t id : fetching) {
@@ -325,22 +386,26 @@ void gc_server::finish_batch() {
         if (r.error) continue;
         uint8_t* base = pool + r.dev_off;
         if (r.algo == GC_ALGO_ZSTD) {
-            r.seqs.resize(r.fetch_total);
+            if (!r.dst) r.seqs.resize(r.fetch_total);
             if (!r.fetch_total) continue;
+            void* to = r.dst ? r.dst : (void*)r.seqs.data();
             (void)launch_gather(ctx, cs, base + r.out_off, (const uint32_t*)meta.p + r.item0,
                                 (const uint64_t*)((uint8_t*)meta.p + off_bytes) + r.item0, base + r.cand_off, r.nsub,
                                 kMaxSeq);
-            (void)hipMemcpyAsync(r.seqs.data(), base + r.cand_off, r.fetch_total * sizeof(gc_zstd_sequence),
+            (void)hipMemcpyAsync(to, base + r.cand_off, r.fetch_total * sizeof(gc_zstd_sequence),
                                  hipMemcpyDeviceToHost, cs);
         } else {
-            r.packed.resize(r.n);
-            (void)hipMemcpyAsync(r.packed.data(), base + r.out_off, r.n * 4, hipMemcpyDeviceToHost, cs);
+            if (!r.dst) r.packed.resize(r.n);
+            void* to = r.dst ? r.dst : (void*)r.packed.data();
+            (void)hipMemcpyAsync(to, base + r.out_off, r.n * 4, hipMemcpyDeviceToHost, cs);
         }
     }
     (void)hipEventRecord(fetch_done, cs);
 }
 
 void gc_server::complete_fetch() {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, fetch_start, fetch_done) == hipSuccess) stats.t_fetch += ms * 1e-3;
     std::lock_guard<std::mutex> g(mu);
     const double t = now_s();
     for (uint64_t id : fetching) {
@@ -363,7 +428,7 @@ void gc_server::loop() {
         {
             std::unique_lock<std::mutex> lk(mu);
             admit_locked();
-            bool idle = running.empty() && fetching.empty() && pending.empty();
+            bool idle = running.empty() && fetching.empty() && finished.empty() && pending.empty();
             bool any_copying = false;
             for (auto& kv : reqs)
                 if (kv.second->state == COPYING || kv.second->state == READY) any_copying = true;
@@ -374,20 +439,25 @@ void gc_server::loop() {
             }
         }
         bool progressed = false;
+        double t0 = now_s();
         if (!running.empty() && hipEventQuery(batch_done) == hipSuccess) {
-            if (!fetching.empty()) {
-                (void)hipEventSynchronize(fetch_done);
-                complete_fetch();
-            }
-            finish_batch();
+            batch_ended();
             progressed = true;
         }
-        if (running.empty()) progressed |= launch_batch();
+        if (running.empty()) progressed |= launch_batch();  // the device goes on at once
         if (!fetching.empty() && hipEventQuery(fetch_done) == hipSuccess) {
             complete_fetch();
             progressed = true;
         }
-        if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (fetching.empty() && !finished.empty()) {
+            finish_batch(finished.front());
+            finished.pop_front();
+            progressed = true;
+        }
+        if (progressed)
+            stats.t_host += now_s() - t0;
+        else
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -404,8 +474,11 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
     s->budget = up(hbm_budget, kAlign);
     s->max_batch_items = max_batch_items ? max_batch_items : 65536;
     hipError_t e = hipMalloc((void**)&s->pool, s->budget + 64);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->batch_done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->fetch_done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&s->batch_start);
+    if (e == hipSuccess) e = hipEventCreate(&s->batch_done);
+    if (e == hipSuccess) e = hipEventCreate(&s->fetch_start);
+    if (e == hipSuccess) e = hipEventCreate(&s->fetch_done);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking);
     const size_t meta_bytes = (size_t)s->max_batch_items * 12 + 64;
     s->meta_cap = meta_bytes;
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, meta_bytes, 0);
@@ -413,6 +486,7 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
     if (e != hipSuccess) {
         hip_fail(e, "gc_server_start");
         if (s->pool) (void)hipFree(s->pool);
+        if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
         delete s;
         return nullptr;
     }
@@ -422,8 +496,20 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
 }
 
 int64_t gc_server_submit(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level) {
+    return gc_server_submit_into(s, data, n, algo, level, nullptr, 0, nullptr);
+}
+
+int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level, void* out,
+                              uint64_t out_bytes, uint32_t* seq_counts) {
     if (!s || (!data && n) || algo > GC_ALGO_LZMA2) return fail(GC_EINVAL, "gc_server_submit: bad argument");
+    if (out && algo == GC_ALGO_ZSTD && !seq_counts)
+        return fail(GC_EINVAL, "gc_server_submit_into: Zstd results need seq_counts");
+    if (out && algo == GC_ALGO_LZMA2 && out_bytes < n * 4)
+        return fail(GC_ESIZE, "gc_server_submit_into: LZMA2 results need 4 bytes per input byte");
     auto r = std::make_unique<Req>();
+    r->dst = out;
+    r->dst_cap = out_bytes;
+    r->dst_counts = seq_counts;
     r->data = data;
     r->n = n;
     r->algo = algo;
@@ -471,6 +557,8 @@ int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* i
         info->t_start = r.t_start;
         info->t_done = r.t_done;
     }
+    if (r.state == FAILED && r.error == GC_ESIZE)
+        return fail(GC_ESIZE, "request failed: its sequences exceed the result buffer given at submit");
     if (r.state == FAILED) return fail(r.error ? r.error : GC_EHIP, "request failed");
     return GC_OK;
 }
@@ -482,6 +570,7 @@ int gc_server_result_zstd(gc_server* s, uint64_t id, gc_zstd_sequence* seqs, uin
     if (it == s->reqs.end() || it->second->state != DONE || it->second->algo != GC_ALGO_ZSTD)
         return fail(GC_EINVAL, "no finished Zstd request with that id");
     Req& r = *it->second;
+    if (r.dst) return fail(GC_EINVAL, "the request's results went to the buffers given at submit");
     if (r.seqs.size() > cap) return fail(GC_ESIZE, "sequence capacity below the request's sequence count");
     memcpy(seq_counts, r.counts.data(), r.counts.size() * 4);
     if (!r.seqs.empty()) memcpy(seqs, r.seqs.data(), r.seqs.size() * sizeof(gc_zstd_sequence));
@@ -494,6 +583,7 @@ int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed) {
     auto it = s->reqs.find(id);
     if (it == s->reqs.end() || it->second->state != DONE || it->second->algo != GC_ALGO_LZMA2)
         return fail(GC_EINVAL, "no finished LZMA2 request with that id");
+    if (it->second->dst) return fail(GC_EINVAL, "the request's results went to the buffers given at submit");
     memcpy(packed, it->second->packed.data(), it->second->n * 4);
     return GC_OK;
 }
@@ -527,6 +617,12 @@ int gc_server_stop(gc_server* s) {
     if (s->th.joinable()) s->th.join();
     (void)hipStreamSynchronize(s->ctx->stream);
     (void)hipStreamSynchronize(s->ctx->stream2);
+    (void)hipStreamSynchronize(s->fetch_stream);
+    (void)hipStreamDestroy(s->fetch_stream);
+    if (s->counts_host) (void)hipHostFree(s->counts_host);
+    if (s->items_host) (void)hipHostFree(s->items_host);
+    if (s->batch_start) (void)hipEventDestroy(s->batch_start);
+    if (s->fetch_start) (void)hipEventDestroy(s->fetch_start);
     for (auto& kv : s->reqs)
         if (kv.second->copied) (void)hipEventDestroy(kv.second->copied);
     if (s->batch_done) (void)hipEventDestroy(s->batch_done);

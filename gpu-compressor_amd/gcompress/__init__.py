@@ -86,6 +86,7 @@ SIGNATURES = [
     ("gc_batch_copy_lzma2", C.c_int, [_vp, C.c_uint32, _vp]),
     ("gc_server_start", _vp, [_vp, C.c_uint64, C.c_uint32]),
     ("gc_server_submit", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("gc_server_submit_into", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp, C.c_uint64, _vp]),
     ("gc_server_wait", C.c_int, [_vp, C.c_uint64, C.c_int, _vp]),
     ("gc_server_result_zstd", C.c_int, [_vp, C.c_uint64, _vp, C.c_uint64, _vp]),
     ("gc_server_result_lzma2", C.c_int, [_vp, C.c_uint64, _vp]),
@@ -118,7 +119,8 @@ class RequestInfo(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("items", C.c_uint64), ("completed", C.c_uint64), ("bytes", C.c_uint64),
-                ("hbm_in_use", C.c_uint64), ("pending", C.c_uint64)]
+                ("hbm_in_use", C.c_uint64), ("pending", C.c_uint64), ("t_batches", C.c_double),
+                ("t_fetch", C.c_double), ("t_host", C.c_double)]
 
 
 class GcError(RuntimeError):
@@ -464,6 +466,16 @@ class Server:
         if rid < 0:
             _check(int(rid), "gc_server_submit")
         self._keep[rid] = data
+        return rid
+
+    def submit_into(self, data: np.ndarray, algo: int, level: int, out: np.ndarray, counts=None) -> int:
+        """Submit with the results written straight into `out` (LZMA2: u32 per byte; Zstd: (k, 4) u32
+        sequence records) and, for Zstd, `counts` (u32 per sub-block)."""
+        cp = _ptr(counts) if counts is not None else None
+        rid = lib().gc_server_submit_into(self.h, _ptr(data), data.size, algo, level, _ptr(out), out.nbytes, cp)
+        if rid < 0:
+            _check(int(rid), "gc_server_submit_into")
+        self._keep[rid] = (data, out, counts)
         return rid
 
     def wait(self, rid: int, timeout_ms: int = -1):

@@ -294,6 +294,9 @@ typedef struct {
 typedef struct {
     uint64_t batches, items, completed, bytes;
     uint64_t hbm_in_use, pending;
+    double t_batches;    /* device seconds of the batches (events around each) */
+    double t_fetch;      /* seconds of result copies on the copy-back stream (events around each batch's) */
+    double t_host;       /* server-thread seconds spent building batches and staging result copies */
 } gc_server_stats;
 
 /* max_batch_items: sub-blocks per batch (0 = 65536). */
@@ -301,6 +304,16 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
 /* Returns the request id (> 0), or a negative GC_E* code (GC_EBUDGET: the
  * request alone needs more than the budget). */
 int64_t gc_server_submit(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level);
+/* As gc_server_submit, with the results copied by DMA straight into the
+ * caller's memory (pinned memory from hipHostMalloc gives full PCIe rate):
+ * LZMA2 `out` holds n u32 (out_bytes >= 4n, else GC_ESIZE at once); Zstd
+ * `out` holds packed gc_zstd_sequence records and `seq_counts` one u32 per
+ * sub-block.  A Zstd request whose sequences exceed out_bytes fails with
+ * GC_ESIZE (gc_request_info.n_sequences says how many it found).  Both
+ * buffers must stay valid until the request is done; gc_server_result_* do
+ * not apply to such requests. */
+int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level, void* out,
+                              uint64_t out_bytes, uint32_t* seq_counts);
 /* 0 done, 1 still running after timeout_ms (< 0 waits), < 0 failed. */
 int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info);
 /* Results of a finished request: Zstd packed sequences (cap records) and per

@@ -73,3 +73,52 @@ def test_tight_budget_and_rejection(gctx):
         srv.release(rid)
     finally:
         srv.stop()
+
+
+def test_submit_into_caller_buffers(gctx):
+    """Results written straight into the caller's buffers equal the oracle; a Zstd buffer too small
+    for the sequences fails that request alone with GC_ESIZE."""
+    reqs = _requests(3, 16, 400000)
+    srv = gcompress.Server(gctx, 1 << 30)
+    try:
+        print("[serve] submit_into", flush=True)
+        outs = []
+        for data, algo, level in reqs:
+            nsub = -(-data.size // SUB)
+            if algo == gcompress.ALGO_LZMA2:
+                out, counts = np.full(data.size, 0xDEADBEEF, np.uint32), None
+            else:
+                out, counts = np.zeros((data.size // 3 + 64, 4), np.uint32), np.zeros(nsub, np.uint32)
+            outs.append((srv.submit_into(data, algo, level, out, counts), out, counts))
+        for (rid, out, counts), (data, algo, level) in zip(outs, reqs):
+            info = srv.wait(rid)
+            if algo == gcompress.ALGO_LZMA2:
+                np.testing.assert_array_equal(out, oracle.lzma2_packed(data))
+            else:
+                oseq, osc, _, _ = oracle.zstd_match_find(data, depth={1: 16, 2: 64}[level])
+                np.testing.assert_array_equal(counts, osc)
+                want = oracle.zstd_as_zstd_sequences(oseq, osc)
+                want = np.concatenate(want) if want else np.zeros((0, 4), np.uint32)
+                assert info.n_sequences == want.shape[0]
+                np.testing.assert_array_equal(out[:want.shape[0]], want)
+            with pytest.raises(gcompress.GcError, match="buffers given"):
+                if algo == gcompress.ALGO_LZMA2:
+                    srv.result_lzma2(rid, data.size)
+                else:
+                    srv.result_zstd(rid, info.n_sub, info.n_sequences)
+            srv.release(rid)
+        print("[serve] submit_into too small", flush=True)
+        text = gcompress.corpus(200000, 77, [0, 1, 0, 0])
+        small = np.zeros((4, 4), np.uint32)
+        cnt = np.zeros(4, np.uint32)
+        rid = srv.submit_into(text, gcompress.ALGO_ZSTD, 1, small, cnt)
+        ok = srv.submit_into(text, gcompress.ALGO_LZMA2, 1, np.zeros(text.size, np.uint32))
+        with pytest.raises(gcompress.GcError, match="exceed the result buffer"):
+            srv.wait(rid)
+        srv.wait(ok)
+        srv.release(rid)
+        srv.release(ok)
+        with pytest.raises(gcompress.GcError, match="4 bytes per input byte"):
+            srv.submit_into(text, gcompress.ALGO_LZMA2, 1, np.zeros(10, np.uint32))
+    finally:
+        srv.stop()
