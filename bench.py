@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this size first, to scale HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
@@ -127,6 +129,16 @@ def main():
     gen_s = time.perf_counter() - t0
     d_in = ctx.alloc(n + 256)
     ctx.htod(d_in, data)
+    if a.calib_copy_mib:
+        # a plain copy of known size (the runtime's copy kernel): FETCH_SIZE / WRITE_SIZE read for it
+        # against calib_copy_mib MiB give the counters' scale
+        cb = a.calib_copy_mib * MiB
+        c_src, c_dst = ctx.alloc(cb), ctx.alloc(cb)
+        hip.L.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+        assert hip.L.hipMemcpyAsync(c_dst, c_src, cb, 3, stream) == 0
+        ctx.sync()
+        ctx.free(c_src)
+        ctx.free(c_dst)
     chunks = [(o, min(chunk, n - o)) for o in range(0, n, chunk)]
     inputs = [(o, s, algo, lv) for lv in levels for (o, s) in chunks]
     batch = gcompress.Batch(ctx, d_in, n + 256, inputs)  # the allocation is readable to its end

@@ -45,14 +45,39 @@
 #define CAND_WAVES (CAND_THREADS / WAVE)
 #define ELEM_THREADS 256
 #define PARSE_WAVES 4
+#ifndef PARSE_UNIFORM_ITEM
+#define PARSE_UNIFORM_ITEM 0
+#endif
 
 // ---------------------------------------------------------------------------
 // Helpers
 // ---------------------------------------------------------------------------
 
-// 4 bytes at base[off], little endian.  Inputs are 4-byte aligned; the value
-// is built from the two aligned dwords around it.  `avail` is the number of
-// readable bytes from `base`.  SAFE callers guarantee avail >= n + 16 for a
+// A sub-block's bytes as seen by one wavefront: a uniform (scalar) base
+// rounded down to 4 bytes and the misalignment, so that every read is a
+// 32-bit offset from a scalar base (no 64-bit address arithmetic per lane).
+// `avail` is the number of readable bytes from the sub-block start.
+struct SrcView {
+    const uint8_t* base;  // sub-block start - mis, 4-byte aligned
+    uint32_t mis;         // sub-block start & 3
+    uint64_t avail;
+};
+
+__device__ __forceinline__ SrcView src_view(const uint8_t* arena, uint64_t src_off, uint64_t avail) {
+    SrcView v;
+    v.mis = (uint32_t)(((uintptr_t)arena + src_off) & 3);
+    v.base = arena + (src_off - v.mis);
+    v.avail = avail;
+    return v;
+}
+
+// The aligned dword at byte offset a (a multiple of 4) of the view's base.
+__device__ __forceinline__ uint32_t dword_at(const SrcView& v, uint32_t a) {
+    return *(const uint32_t*)(v.base + a);
+}
+
+// 4 bytes at sub-block offset `off`, little endian, built from the two
+// aligned dwords around them.  SAFE callers guarantee avail >= n + 16 for a
 // sub-block of n bytes, which covers every read below; otherwise a read that
 // would pass the end is built byte by byte from the bytes that exist.
 __device__ __attribute__((noinline)) uint32_t ld32_tail(const uint8_t* p, uint32_t off, uint64_t avail) {
@@ -63,21 +88,27 @@ __device__ __attribute__((noinline)) uint32_t ld32_tail(const uint8_t* p, uint32
 }
 
 template <bool SAFE>
-__device__ __forceinline__ uint32_t ld32(const uint8_t* base, uint32_t off, uint64_t avail) {
-    const uint8_t* p = base + off;
-    if (SAFE || (uint64_t)off + 8 <= avail) {
-        const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
-        const uint32_t* q = (const uint32_t*)(p - sh);  // derived from base: stays a global access
-        uint32_t lo = q[0];
-        uint32_t hi = q[1];
-        return __builtin_amdgcn_alignbyte(hi, lo, sh);
+__device__ __forceinline__ uint32_t ld32(const SrcView& v, uint32_t off) {
+    if (SAFE || (uint64_t)off + 8 <= v.avail) {
+        const uint32_t t = off + v.mis;
+        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));  // q[0], q[1]: one 8-byte load
+        return __builtin_amdgcn_alignbyte(q[1], q[0], t & 3);
     }
-    return ld32_tail(p, off, avail);  // the last bytes of the arena, byte by byte
+    return ld32_tail(v.base + v.mis + off, off, v.avail);  // the last bytes of the arena, byte by byte
 }
 
 // Per-item reads may skip bounds checks when the arena extends 16 bytes past
 // the sub-block (every sub-block but the last of an arena).
 __device__ __forceinline__ bool item_safe(uint64_t avail, uint32_t n) { return avail >= (uint64_t)n + 16; }
+
+// Item i, every field made wavefront-uniform (held in scalar registers).
+__device__ __forceinline__ gc_item load_item(const gc_item* __restrict__ items, uint32_t i) {
+    gc_item it = items[i];
+    uint32_t* w = (uint32_t*)&it;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(gc_item) / 4); k++) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[k]);
+    return it;
+}
 
 __device__ __forceinline__ uint32_t hash4(uint32_t v) { return (v * 2654435761u) >> (32 - GC_HASH_BITS); }
 
@@ -85,7 +116,7 @@ __device__ __forceinline__ uint32_t lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
-__device__ __forceinline__ uint64_t ballot(bool b) { return __ballot(b); }
+__device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
@@ -102,11 +133,11 @@ __device__ __forceinline__ void lds_barrier() {
 // Common-prefix length of src[a..] and src[b..] from `start` (bytes before
 // it known equal), at most `limit`; one lane.
 template <bool SAFE>
-__device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
-                                             uint32_t start, uint32_t limit) {
+__device__ __forceinline__ uint32_t lcp_lane(const SrcView& src, uint32_t a, uint32_t b, uint32_t start,
+                                             uint32_t limit) {
     uint32_t len = start;
     while (len < limit) {
-        uint32_t x = ld32<SAFE>(src, a + len, avail) ^ ld32<SAFE>(src, b + len, avail);
+        uint32_t x = ld32<SAFE>(src, a + len) ^ ld32<SAFE>(src, b + len);
         if (x) {
             len += (uint32_t)__builtin_ctz(x) >> 3;
             break;
@@ -119,21 +150,23 @@ __device__ __forceinline__ uint32_t lcp_lane(const uint8_t* src, uint64_t avail,
 // Common-prefix length, whole wavefront cooperating (256 bytes per step).
 // a, b, start and limit must be uniform.
 template <bool SAFE>
-__device__ __forceinline__ uint32_t lcp_wave(const uint8_t* src, uint64_t avail, uint32_t a, uint32_t b,
-                                             uint32_t limit, uint32_t start) {
+__device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uint32_t b, uint32_t limit,
+                                             uint32_t start) {
     const uint32_t lane = lane_id();
     uint32_t len = start;
-    while (len < limit) {
-        uint32_t o = len + 4 * lane;
-        uint32_t x = 0;
-        if (o < limit) x = ld32<SAFE>(src, a + o, avail) ^ ld32<SAFE>(src, b + o, avail);
-        uint64_t mm = ballot(x != 0);
-        if (mm) {
-            uint32_t k = (uint32_t)__builtin_ctzll(mm);
-            len += 4 * k + ((uint32_t)__builtin_ctz(bcast(x, k)) >> 3);
-            break;
-        }
-        len += 4 * WAVE;
+    uint64_t mm = 0;
+    uint32_t x = 0;
+    while (len < limit) {  // one exit: the structurizer adds no flags
+        const uint32_t o = len + 4 * lane;
+        x = 0;
+        if (o < limit) x = ld32<SAFE>(src, a + o) ^ ld32<SAFE>(src, b + o);
+        mm = ballot(x != 0);
+        len = mm ? len : len + 4 * WAVE;
+        if (mm) break;
+    }
+    if (mm) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(mm);
+        len += 4 * k + ((uint32_t)__builtin_ctz(bcast(x, k)) >> 3);
     }
     return len < limit ? len : limit;
 }
@@ -210,26 +243,26 @@ __device__ __forceinline__ uint32_t cand_tile(CandLds& s, uint32_t word, uint32_
 template <bool SAFE>
 __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const uint8_t* arena, uint16_t* cand16) {
     const uint32_t tid = threadIdx.x;
-    const uint8_t* src = arena + it.src_off;
+    const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
     uint32_t* t = (uint32_t*)s.table;
     for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
     lds_barrier();
-    uint32_t next = (tid + 3 < n) ? ld32<SAFE>(src, tid, it.avail) : 0;
+    uint32_t next = (tid + 3 < n) ? ld32<SAFE>(src, tid) : 0;
     for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
         const uint32_t word = next;
         const uint32_t q = t0 + CAND_THREADS + tid;  // next tile, loaded while this one runs
-        next = (q + 3 < n) ? ld32<SAFE>(src, q, it.avail) : 0;
+        next = (q + 3 < n) ? ld32<SAFE>(src, q) : 0;
         const uint32_t c = cand_tile(s, word, n, t0);
         if (t0 + tid < n) cand16[it.cpos + t0 + tid] = (uint16_t)c;
     }
 }
 
 extern "C" __global__ void __launch_bounds__(CAND_THREADS)
-gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uint16_t* cand16) {
+gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, uint16_t* cand16) {
     __shared__ CandLds s;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
-        const gc_item it = items[i];
+        const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
             cand_item<true>(s, it, arena, cand16);
         else
@@ -251,16 +284,15 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, uin
 // Sub-blocks of a size that is not a multiple of 4 start unaligned: then
 // each dword is assembled by ld32.
 template <bool SAFE>
-__device__ __forceinline__ void load_strip(const uint8_t* src, uint32_t p0, uint64_t avail, uint32_t d[4]) {
-    if ((((uintptr_t)(src + p0)) & 3) == 0 && (SAFE || (uint64_t)p0 + 16 <= avail)) {
-        const uint32_t* q = (const uint32_t*)(src + p0);
-        d[0] = q[0];
-        d[1] = q[1];
-        d[2] = q[2];
-        d[3] = q[3];
+__device__ __forceinline__ void load_strip(const SrcView& src, uint32_t p0, uint32_t d[4]) {
+    if (src.mis == 0 && (SAFE || (uint64_t)p0 + 16 <= src.avail)) {
+        d[0] = dword_at(src, p0);
+        d[1] = dword_at(src, p0 + 4);
+        d[2] = dword_at(src, p0 + 8);
+        d[3] = dword_at(src, p0 + 12);
         return;
     }
-    for (int k = 0; k < 4; k++) d[k] = ld32<SAFE>(src, p0 + 4 * k, avail);
+    for (int k = 0; k < 4; k++) d[k] = ld32<SAFE>(src, p0 + 4 * k);
 }
 
 // The 4 bytes at p0 + j from the strip's dwords.
@@ -282,14 +314,13 @@ template <bool SAFE>
 __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
                                            uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                                            uint32_t max_matches) {
-    const uint8_t* src = arena + it.src_off;
-    const uint64_t avail = it.avail;
+    const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
     const uint32_t depth = it.depth;
     for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += ELEM_THREADS * STRIP) {
         uint32_t d[4];
-        load_strip<SAFE>(src, p0, avail, d);
-        const uint32_t prev0 = p0 > 0 ? ld32<SAFE>(src, p0 - 1, avail) : 0;
+        load_strip<SAFE>(src, p0, d);
+        const uint32_t prev0 = p0 > 0 ? ld32<SAFE>(src, p0 - 1) : 0;
         uint32_t tgt[STRIP], vc[STRIP];
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) {
@@ -297,7 +328,7 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
             tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32<SAFE>(src, tgt[j], avail) : 0;
+        for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32<SAFE>(src, tgt[j]) : 0;
         uint32_t out[STRIP];
         uint32_t next_t = GC_NONE, next_len = 0;
 #pragma unroll
@@ -318,7 +349,7 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
                     if (next_t != GC_NONE && next_t == t + 1)
                         len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
                     else
-                        len = lcp_lane<SAFE>(src, avail, t, p, 2, maxlen);
+                        len = lcp_lane<SAFE>(src, t, p, 2, maxlen);
                     o = ((p - t) << 16) | len;
                     next_len = len;
                 }
@@ -349,10 +380,11 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
 }
 
 extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
-gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
-                    uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches) {
+gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
+                    const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
+                    uint32_t max_matches) {
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
-        const gc_item it = items[i];
+        const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
             lzma2_item<true>(it, arena, cand16, packed, ref_m, ref_c, max_matches);
         else
@@ -373,12 +405,11 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
     // cpos and opos are multiples of 8 and each item owns its positions
     // rounded up to 8: whole strips of W positions are read and written with
     // vector accesses
-    const uint8_t* src = arena + it.src_off;
-    const uint64_t avail = it.avail;
+    const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
     for (uint32_t p0 = threadIdx.x * W; p0 + 3 < n; p0 += ELEM_THREADS * W) {
         uint32_t d[4];
-        load_strip<SAFE>(src, p0, avail, d);
+        load_strip<SAFE>(src, p0, d);
         uint32_t cv[W / 2];
         if constexpr (W == 8) {
             const uint4 cw = *(const uint4*)(cand16 + it.cpos + p0);
@@ -394,7 +425,7 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
             c[j] = (cj == GC_NONE || p0 + j + 3 >= n) ? 0 : cj;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < W; j++) vc[j] = ld32<SAFE>(src, c[j], avail);
+        for (uint32_t j = 0; j < W; j++) vc[j] = ld32<SAFE>(src, c[j]);
         uint32_t out[W];
         uint32_t next_len = 0;
 #pragma unroll
@@ -406,7 +437,7 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
                 if (j + 1 < (int)W && next_len && c[j + 1] == c[j] + 1)
                     len = next_len + 1 < lim ? next_len + 1 : lim;
                 else
-                    len = lcp_lane<SAFE>(src, avail, c[j], p, 3, lim);
+                    len = lcp_lane<SAFE>(src, c[j], p, 3, lim);
             }
             next_len = len;
             out[j] = (len << 16) | c[j];
@@ -418,10 +449,10 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
 }
 
 template <uint32_t W>
-__device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                                                const uint16_t* cand16, uint32_t* info) {
+__device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* __restrict__ items,
+                                                uint32_t n_items, const uint16_t* cand16, uint32_t* info) {
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
-        const gc_item it = items[i];
+        const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
             zstd_info_item<true, W>(it, arena, cand16, info);
         else
@@ -431,14 +462,14 @@ __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_i
 
 // Strips of 8 or 4 positions: the same words, different register use.
 extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
-gc_zstd_info_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
-                    uint32_t* info) {
+gc_zstd_info_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
+                    const uint16_t* cand16, uint32_t* info) {
     zstd_info_items<8>(arena, items, n_items, cand16, info);
 }
 
 extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
-gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint16_t* cand16,
-                     uint32_t* info) {
+gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
+                     const uint16_t* cand16, uint32_t* info) {
     zstd_info_items<4>(arena, items, n_items, cand16, info);
 }
 
@@ -453,7 +484,13 @@ gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_item
 // verified, or a repeat offset) is the next head: every position before it
 // is a literal.  After a sequence the repeat offsets become (new, old0,
 // old1): each lane renames its two kept flags and gathers the new one, so no
-// data moves between lanes until the walk leaves the window.
+// data moves between lanes until the walk leaves the window.  Sequences are
+// staged one per lane and stored 64 at a time.
+//
+// The walk's decisions are wavefront-uniform and run on the scalar unit,
+// which one CU shares between its four SIMDs: that unit's instruction count
+// per sequence sets this kernel's speed, so per-lane work stays in vector
+// registers and loop-carried flags are plain integers.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t len) {
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
@@ -462,41 +499,53 @@ __device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint
 // 3 bytes at x - r equal w (the 3 bytes at x), for a head candidate x
 // (x + 3 < n); the read is clamped so that it is always in the sub-block.
 template <bool SAFE>
-__device__ __forceinline__ bool rep_eq(const uint8_t* src, uint64_t avail, uint32_t x, uint32_t r, uint32_t w) {
+__device__ __forceinline__ uint32_t rep_eq(const SrcView& src, uint32_t x, uint32_t r, uint32_t w) {
     const bool ok = r <= x;
-    const uint32_t v = ld32<SAFE>(src, ok ? x - r : 0, avail) & 0xFFFFFFu;
-    return ok && v == w;
+    const uint32_t v = ld32<SAFE>(src, ok ? x - r : 0) & 0xFFFFFFu;
+    return (ok && v == w) ? 1u : 0u;
 }
 
-template <bool SAFE>
+template <bool SAFE, uint32_t FMT, bool LITS>
 __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t* arena, const uint32_t* info,
-                                                void* seq_out, uint32_t seq_fmt, uint8_t* literals,
-                                                uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
-    const uint8_t* src = arena + it.src_off;
-    const uint64_t avail = it.avail;
+                                                void* seq_out, uint8_t* literals, uint32_t* seq_counts,
+                                                uint32_t* lit_counts, uint32_t max_seq) {
+    const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
-    const uint32_t depth = it.depth;
+    const uint32_t lazy = it.depth > 1 ? 1u : 0u;
     const uint32_t lane = lane_id();
-    const uint32_t* inf = info + it.opos;
+    const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
     uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0, wbase = 0;
-    uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
+    uint8_t* lit_dst = LITS ? literals + it.lit_pos : nullptr;
+
+    // staged sequences: lane j holds sequence (nseq & ~63) + j
+    uint32_t q_a = 0, q_b = 0, q_c = 0;
+    auto store_staged = [&](uint32_t first) {  // this lane's record: sequence first + lane
+        const uint64_t si = it.seq_pos + first + lane;
+        if constexpr (FMT == GC_SEQ_REF) {
+            gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
+            o->literal_length = q_b;
+            o->match_length_minus3 = q_c - 3;
+            o->offset = q_a;
+        } else {
+            *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(q_a, q_b, q_c, 0);
+        }
+    };
 
     uint32_t e = 0, w = 0, f = 0, ae = 0, aw = 0;
     auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& wv) {
         const bool act = x + 3 < n;
         const uint32_t xc = act ? x : 0;  // reads stay in the sub-block
-        const uint32_t i0 = inf[xc];
-        const uint32_t w0 = ld32<SAFE>(src, xc, avail) & 0xFFFFFFu;
+        const uint32_t i0 = *(const uint32_t*)(inf + (xc << 2));
+        const uint32_t w0 = ld32<SAFE>(src, xc) & 0xFFFFFFu;
         ev = act ? i0 : 0;
         wv = act ? w0 : 0;
     };
     auto flags = [&](uint32_t x, uint32_t wv) -> uint32_t {
         const bool act = x + 3 < n;
         const uint32_t xc = act ? x : 0;
-        const uint32_t fv = (rep_eq<SAFE>(src, avail, xc, rep0, wv) ? 1u : 0u) |
-                            (rep_eq<SAFE>(src, avail, xc, rep1, wv) ? 2u : 0u) |
-                            (rep_eq<SAFE>(src, avail, xc, rep2, wv) ? 4u : 0u);
+        const uint32_t fv = rep_eq<SAFE>(src, xc, rep0, wv) | (rep_eq<SAFE>(src, xc, rep1, wv) << 1) |
+                            (rep_eq<SAFE>(src, xc, rep2, wv) << 2);
         return act ? fv : 0;
     };
     if (n > 3) {
@@ -506,9 +555,12 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         peek(WAVE + lane, ae, aw);
     }
 
-    while (pos + 3 < n && nseq < max_seq) {
+    // One loop exit (the condition): the walk ends by lowering `limit`.
+    uint32_t limit = n;
+    while (pos + 3 < limit) {
         const uint32_t lo = pos - wbase;
-        const uint64_t m = ballot(lane >= lo && ((e >> 16) != 0 || f != 0));
+        const uint32_t cand = lane >= lo ? ((e >> 16) | f) : 0u;
+        const uint64_t m = ballot(cand != 0);
         if (!m) {
             // no head left in the window: every position up to its end is a literal
             wbase += WAVE;
@@ -518,98 +570,83 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             f = flags(wbase + lane, w);
             __asm__ volatile("" ::: "memory");
             peek(wbase + WAVE + lane, ae, aw);
-            continue;
-        }
-        const uint32_t k = (uint32_t)__builtin_ctzll(m);
-        uint32_t head = wbase + k;
-        const uint32_t ek = bcast(e, k);
-        uint32_t len, off;
-        if (ek >> 16) {
-            // hash candidate verified
-            const uint32_t c = ek & 0xFFFFu;
-            off = head - c;
-            len = ek >> 16;
-            if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, avail, c, head, n - head, GC_ZSTD_SHORT);
+        } else if (nseq == max_seq) {
+            limit = 0;
         } else {
-            // the first repeat offset that verifies (rep0, rep1, rep2 in order)
-            const uint32_t fk = bcast(f, k);
-            off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
-            len = lcp_wave<SAFE>(src, avail, head - off, head, n - head, 3);
-        }
-        // one lazy step at head + 1 (zstd_compress.cu:278-307); its info word
-        // is in the next lane, or the first ahead lane
-        if (depth > 1 && head + 4 < n) {
-            const uint32_t e2 = k + 1 < WAVE ? bcast(e, k + 1) : bcast(ae, 0);
-            if (e2 >> 16) {
+            const uint32_t k = (uint32_t)__builtin_ctzll(m);
+            uint32_t head = wbase + k;
+            const uint32_t ek = bcast(e, k);
+            uint32_t len, off;
+            if (ek > 0xFFFFu) {
+                // hash candidate verified
+                const uint32_t c = ek & 0xFFFFu;
+                off = head - c;
+                len = ek >> 16;
+                if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, c, head, n - head, GC_ZSTD_SHORT);
+            } else {
+                // the first repeat offset that verifies (rep0, rep1, rep2 in order)
+                const uint32_t fk = bcast(f, k);
+                off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
+                len = lcp_wave<SAFE>(src, head - off, head, n - head, 3);
+            }
+            // one lazy step at head + 1 (zstd_compress.cu:278-307); its info
+            // word is in the next lane, or the first ahead lane
+            const uint32_t e2 = (lazy != 0 && head + 4 < n) ? (k < WAVE - 1 ? bcast(e, k + 1) : bcast(ae, 0)) : 0u;
+            if (e2 > 0xFFFFu) {
                 const uint32_t c2 = e2 & 0xFFFFu;
                 uint32_t len2 = e2 >> 16;
-                if (len2 >= GC_ZSTD_SHORT)
-                    len2 = lcp_wave<SAFE>(src, avail, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
+                if (len2 >= GC_ZSTD_SHORT) len2 = lcp_wave<SAFE>(src, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
                 if (len2 > len + 1) {
                     head += 1;
                     len = len2;
                     off = head - c2;
                 }
             }
-        }
-        // emit (nseq < max_seq holds here)
-        const uint32_t ll = head - lit_start;
-        if (lit_dst) copy_wave(lit_dst + nlit, src + lit_start, ll);
-        nlit += ll;
-        if (lane == 0) {
-            const uint64_t si = it.seq_pos + nseq;
-            if (seq_fmt == GC_SEQ_REF) {
-                gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
-                o->literal_length = ll;
-                o->match_length_minus3 = len - 3;
-                o->offset = off;
+            // emit (nseq < max_seq holds here)
+            const uint32_t ll = head - lit_start;
+            if constexpr (LITS) copy_wave(lit_dst + nlit, src.base + src.mis + lit_start, ll);
+            nlit += ll;
+            const uint32_t slot = nseq & (WAVE - 1);
+            const bool mine = lane == slot;
+            q_a = mine ? off : q_a;
+            q_b = mine ? ll : q_b;
+            q_c = mine ? len : q_c;
+            nseq++;
+            if (slot == WAVE - 1) store_staged(nseq - WAVE);
+            // repeat offsets (off, o0, o1); which old flag, if any, the new
+            // offset's is: 0..2, or 3 = gather it
+            const uint32_t sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
+            rep2 = rep1;
+            rep1 = rep0;
+            rep0 = off;
+            pos = head + len;
+            lit_start = pos;
+            if (pos >= wbase + WAVE) {
+                // the walk left the window: move to the window holding pos
+                // (reads past the sub-block end are clamped by peek)
+                const uint32_t nb = pos & ~(WAVE - 1);
+                if (nb == wbase + WAVE) {
+                    e = ae;
+                    w = aw;
+                } else {
+                    peek(nb + lane, e, w);
+                }
+                wbase = nb;
+                f = flags(wbase + lane, w);
+                __asm__ volatile("" ::: "memory");
+                peek(wbase + WAVE + lane, ae, aw);
             } else {
-                gc_zstd_sequence* o = (gc_zstd_sequence*)seq_out + si;
-                o->offset = off;
-                o->litLength = ll;
-                o->matchLength = len;
-                o->rep = 0;
+                const uint32_t x = wbase + lane;
+                uint32_t b0 = (f >> sel) & 1u;  // sel 3 reads 0
+                if (sel == 3) b0 = x + 3 < n ? rep_eq<SAFE>(src, x, off, w) : 0u;
+                f = b0 | ((f & 3u) << 1);
             }
-        }
-        nseq++;
-        const uint32_t o0 = rep0, o1 = rep1, o2 = rep2;
-        rep2 = rep1;
-        rep1 = rep0;
-        rep0 = off;
-        pos = head + len;
-        lit_start = pos;
-        if (pos + 3 >= n) break;
-        if (pos >= wbase + WAVE) {
-            // the walk left the window: move to the window holding pos
-            const uint32_t nb = pos & ~(WAVE - 1);
-            if (nb == wbase + WAVE) {
-                e = ae;
-                w = aw;
-            } else {
-                peek(nb + lane, e, w);
-            }
-            wbase = nb;
-            f = flags(wbase + lane, w);
-            __asm__ volatile("" ::: "memory");
-            peek(wbase + WAVE + lane, ae, aw);
-        } else {
-            // flags now for (off, o0, o1)
-            const uint32_t x = wbase + lane;
-            uint32_t b0;
-            if (off == o0)
-                b0 = f & 1;
-            else if (off == o1)
-                b0 = (f >> 1) & 1;
-            else if (off == o2)
-                b0 = (f >> 2) & 1;
-            else
-                b0 = (x + 3 < n && rep_eq<SAFE>(src, avail, x + 3 < n ? x : 0, off, w)) ? 1u : 0u;
-            f = b0 | ((f & 3u) << 1);
         }
     }
+    if (lane < (nseq & (WAVE - 1))) store_staged(nseq & ~(uint32_t)(WAVE - 1));
     // trailing literals (zstd_compress.cu:341-344)
     if (n > lit_start) {
-        if (lit_dst) copy_wave(lit_dst + nlit, src + lit_start, n - lit_start);
+        if constexpr (LITS) copy_wave(lit_dst + nlit, src.base + src.mis + lit_start, n - lit_start);
         nlit += n - lit_start;
     }
     if (lane == 0) {
@@ -618,18 +655,44 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     }
 }
 
-extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
-gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* items, uint32_t n_items, const uint32_t* info,
-                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
-                     uint32_t max_seq) {
-    // the wavefront index is uniform: items are read with scalar loads
+template <uint32_t FMT, bool LITS>
+__device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_item* __restrict__ items,
+                                                 uint32_t n_items, const uint32_t* info, void* seq_out,
+                                                 uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
+                                                 uint32_t max_seq) {
+    // the wavefront index is uniform: items are read into scalar registers
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES) {
-        const gc_item it = items[i];
+#if PARSE_UNIFORM_ITEM
+        const gc_item it = load_item(items, i);
+#else
+        const gc_item it = items[i];  // fields in vector registers: fewer scalar registers in use
+#endif
         if (item_safe(it.avail, it.n))
-            zstd_parse_item<true>(it, arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
+            zstd_parse_item<true, FMT, LITS>(it, arena, info, seq_out, literals, seq_counts, lit_counts, max_seq);
         else
-            zstd_parse_item<false>(it, arena, info, seq_out, seq_fmt, literals, seq_counts, lit_counts, max_seq);
+            zstd_parse_item<false, FMT, LITS>(it, arena, info, seq_out, literals, seq_counts, lit_counts, max_seq);
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
+gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,
+                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
+                     uint32_t max_seq) {
+    if (seq_fmt == GC_SEQ_REF) {
+        if (literals)
+            zstd_parse_items<GC_SEQ_REF, true>(arena, items, n_items, info, seq_out, literals, seq_counts, lit_counts,
+                                               max_seq);
+        else
+            zstd_parse_items<GC_SEQ_REF, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
+                                                lit_counts, max_seq);
+    } else {
+        if (literals)
+            zstd_parse_items<GC_SEQ_ZSTD, true>(arena, items, n_items, info, seq_out, literals, seq_counts,
+                                                lit_counts, max_seq);
+        else
+            zstd_parse_items<GC_SEQ_ZSTD, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
+                                                 lit_counts, max_seq);
     }
 }
 

@@ -19,7 +19,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libgcompress.so")
+# GCOMPRESS_LIB: load another build of the library (kernel variants side by side)
+LIB_PATH = os.environ.get("GCOMPRESS_LIB") or os.path.join(_PKG, "libgcompress.so")
 
 SUB_BLOCK = 65536            # constants.rs:5 / :11
 ZSTD_DEPTH_LAZY = 16         # constants.rs:6

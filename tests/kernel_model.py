@@ -275,3 +275,4 @@ def zstd_parse_fixed(d, info, depth=16, max_seq=None):
                 nf.append(b0 | ((sf & 3) << 1))
             f = nf
     return seqs
+
