@@ -77,9 +77,10 @@ __device__ __forceinline__ uint32_t dword_at(const SrcView& v, uint32_t a) {
 }
 
 // 4 bytes at sub-block offset `off`, little endian, built from the two
-// aligned dwords around them.  SAFE callers guarantee avail >= n + 16 for a
-// sub-block of n bytes, which covers every read below; otherwise a read that
-// would pass the end is built byte by byte from the bytes that exist.
+// aligned dwords around them.  SAFE callers guarantee avail >= n + 32 for a
+// sub-block of n bytes, which covers every read below (at most 24 bytes from
+// an offset below n); otherwise a read that would pass the end is built byte
+// by byte from the bytes that exist.
 __device__ __attribute__((noinline)) uint32_t ld32_tail(const uint8_t* p, uint32_t off, uint64_t avail) {
     uint32_t v = 0;
     for (uint32_t i = 0; i < 4; i++)
@@ -97,9 +98,27 @@ __device__ __forceinline__ uint32_t ld32(const SrcView& v, uint32_t off) {
     return ld32_tail(v.base + v.mis + off, off, v.avail);  // the last bytes of the arena, byte by byte
 }
 
-// Per-item reads may skip bounds checks when the arena extends 16 bytes past
+// 16 bytes at sub-block offset `off` as four little-endian words, from the
+// five aligned dwords around them.
+template <bool SAFE>
+__device__ __forceinline__ void ld128(const SrcView& v, uint32_t off, uint32_t o[4]) {
+    if (SAFE || (uint64_t)off + 24 <= v.avail) {
+        const uint32_t t = off + v.mis;
+        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));
+        const uint32_t sh = t & 3;
+        const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+        o[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+        o[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+        o[2] = __builtin_amdgcn_alignbyte(d3, d2, sh);
+        o[3] = __builtin_amdgcn_alignbyte(d4, d3, sh);
+        return;
+    }
+    for (int k = 0; k < 4; k++) o[k] = ld32<false>(v, off + 4 * k);
+}
+
+// Per-item reads may skip bounds checks when the arena extends 32 bytes past
 // the sub-block (every sub-block but the last of an arena).
-__device__ __forceinline__ bool item_safe(uint64_t avail, uint32_t n) { return avail >= (uint64_t)n + 16; }
+__device__ __forceinline__ bool item_safe(uint64_t avail, uint32_t n) { return avail >= (uint64_t)n + 32; }
 
 // Item i, every field made wavefront-uniform (held in scalar registers).
 __device__ __forceinline__ gc_item load_item(const gc_item* __restrict__ items, uint32_t i) {
@@ -136,13 +155,18 @@ template <bool SAFE>
 __device__ __forceinline__ uint32_t lcp_lane(const SrcView& src, uint32_t a, uint32_t b, uint32_t start,
                                              uint32_t limit) {
     uint32_t len = start;
-    while (len < limit) {
-        uint32_t x = ld32<SAFE>(src, a + len) ^ ld32<SAFE>(src, b + len);
-        if (x) {
-            len += (uint32_t)__builtin_ctz(x) >> 3;
-            break;
-        }
-        len += 4;
+    while (len < limit) {  // 16 bytes per step
+        uint32_t x[4], y[4];
+        ld128<SAFE>(src, a + len, x);
+        ld128<SAFE>(src, b + len, y);
+        const uint32_t d0 = x[0] ^ y[0], d1 = x[1] ^ y[1], d2 = x[2] ^ y[2], d3 = x[3] ^ y[3];
+        const uint32_t k = d0   ? (uint32_t)__builtin_ctz(d0) >> 3
+                           : d1 ? 4 + ((uint32_t)__builtin_ctz(d1) >> 3)
+                           : d2 ? 8 + ((uint32_t)__builtin_ctz(d2) >> 3)
+                           : d3 ? 12 + ((uint32_t)__builtin_ctz(d3) >> 3)
+                                : 16u;
+        len += k;
+        if (k < 16) break;
     }
     return len < limit ? len : limit;
 }
@@ -286,10 +310,11 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
 template <bool SAFE>
 __device__ __forceinline__ void load_strip(const SrcView& src, uint32_t p0, uint32_t d[4]) {
     if (src.mis == 0 && (SAFE || (uint64_t)p0 + 16 <= src.avail)) {
-        d[0] = dword_at(src, p0);
-        d[1] = dword_at(src, p0 + 4);
-        d[2] = dword_at(src, p0 + 8);
-        d[3] = dword_at(src, p0 + 12);
+        const uint32_t* q = (const uint32_t*)(src.base + p0);
+        d[0] = q[0];
+        d[1] = q[1];
+        d[2] = q[2];
+        d[3] = q[3];
         return;
     }
     for (int k = 0; k < 4; k++) d[k] = ld32<SAFE>(src, p0 + 4 * k);
