@@ -86,8 +86,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size-mib", type=int, default=1024)
-    ap.add_argument("--chunk-mib", type=int, default=16)
+    ap.add_argument("--size-mib", type=int, default=None, help="default: 1024 (zstd), 640 (lzma2)")
+    ap.add_argument("--chunk-mib", type=int, default=None, help="default: 16 (zstd), 128 (lzma2)")
     ap.add_argument("--workload", choices=["zstd", "lzma2", "serve"], default="zstd")
     ap.add_argument("--rates", default="500,2000,8000")
     ap.add_argument("--requests", type=int, default=2000, help="requests per serving rate")
@@ -101,6 +101,10 @@ def main():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size first, to scale HBM byte counters")
     a = ap.parse_args()
+    if a.size_mib is None:
+        a.size_mib = 640 if a.workload == "lzma2" else 1024
+    if a.chunk_mib is None:
+        a.chunk_mib = 128 if a.workload == "lzma2" else 16
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
               "(sub-blocks shard with no exchange, see DESIGN.md)", file=sys.stderr)

@@ -84,14 +84,15 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
                      uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                      uint32_t max_matches) {
     void* args[] = {&arena, &items, &n_items, &cand16, &packed, &ref_m, &ref_c, &max_matches};
-    return launch(stream, ctx->k.lzma2_len, grid, 256, args, "launch gc_lzma2_len_kernel");
+    return launch(stream, ctx->k.lzma2_len, grid, 1024, args, "launch gc_lzma2_len_kernel");
 }
 
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                      uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip) {
     void* args[] = {&arena, &items, &n_items, &cand16, &info};
-    if (strip == 4) return launch(stream, ctx->k.zstd_info4, grid, 256, args, "launch gc_zstd_info4_kernel");
-    return launch(stream, ctx->k.zstd_info, grid, 256, args, "launch gc_zstd_info_kernel");
+    // one 1024-thread workgroup per sub-block (the sub-block is staged in LDS)
+    if (strip == 4) return launch(stream, ctx->k.zstd_info4, grid, 1024, args, "launch gc_zstd_info4_kernel");
+    return launch(stream, ctx->k.zstd_info, grid, 1024, args, "launch gc_zstd_info_kernel");
 }
 
 int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,

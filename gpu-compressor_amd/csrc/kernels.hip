@@ -43,7 +43,8 @@
 #define WAVE 64
 #define CAND_THREADS 256
 #define CAND_WAVES (CAND_THREADS / WAVE)
-#define ELEM_THREADS 256
+#define INFO_THREADS 1024                          // per-position kernels: one workgroup per sub-block
+#define STAGE_WORDS ((GC_SUB_BLOCK + 64) / 4)      // a sub-block staged in LDS, with 64 bytes to spare
 #define PARSE_WAVES 4
 #ifndef PARSE_UNIFORM_ITEM
 #define PARSE_UNIFORM_ITEM 0
@@ -149,16 +150,56 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A sub-block staged in LDS (byte 0 = sub-block start, 4-byte aligned, at
+// least 32 readable bytes past the sub-block).
+struct LdsBytes {
+    const uint32_t* w;
+};
+
+__device__ __forceinline__ uint32_t ld32(const LdsBytes& v, uint32_t off) {
+    const uint32_t* q = v.w + (off >> 2);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], off & 3);
+}
+
+__device__ __forceinline__ void ld128(const LdsBytes& v, uint32_t off, uint32_t o[4]) {
+    const uint32_t* q = v.w + (off >> 2);
+    const uint32_t sh = off & 3;
+    const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
+    o[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+    o[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+    o[2] = __builtin_amdgcn_alignbyte(d3, d2, sh);
+    o[3] = __builtin_amdgcn_alignbyte(d4, d3, sh);
+}
+
+// Copies bytes [0, round_up(n + 32, 16)) of a sub-block into LDS (bytes past
+// the arena read as 0); the whole workgroup takes part.
+__device__ __forceinline__ void stage_sub_block(uint32_t* lds, const SrcView& src, uint32_t n) {
+    const uint32_t end = (n + 32 + 15) & ~15u;
+    for (uint32_t o = threadIdx.x * 16; o < end; o += blockDim.x * 16) {
+        uint32_t w[4];
+        ld128<false>(src, o, w);
+        *(uint4*)(lds + o / 4) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
 // Common-prefix length of src[a..] and src[b..] from `start` (bytes before
 // it known equal), at most `limit`; one lane.
 template <bool SAFE>
-__device__ __forceinline__ uint32_t lcp_lane(const SrcView& src, uint32_t a, uint32_t b, uint32_t start,
-                                             uint32_t limit) {
+__device__ __forceinline__ void ld128v(const SrcView& v, uint32_t off, uint32_t o[4]) {
+    ld128<SAFE>(v, off, o);
+}
+template <bool SAFE>
+__device__ __forceinline__ void ld128v(const LdsBytes& v, uint32_t off, uint32_t o[4]) {
+    ld128(v, off, o);
+}
+
+template <bool SAFE, class V>
+__device__ __forceinline__ uint32_t lcp_lane(const V& src, uint32_t a, uint32_t b, uint32_t start, uint32_t limit) {
     uint32_t len = start;
     while (len < limit) {  // 16 bytes per step
         uint32_t x[4], y[4];
-        ld128<SAFE>(src, a + len, x);
-        ld128<SAFE>(src, b + len, y);
+        ld128v<SAFE>(src, a + len, x);
+        ld128v<SAFE>(src, b + len, y);
         const uint32_t d0 = x[0] ^ y[0], d1 = x[1] ^ y[1], d2 = x[2] ^ y[2], d3 = x[3] ^ y[3];
         const uint32_t k = d0   ? (uint32_t)__builtin_ctz(d0) >> 3
                            : d1 ? 4 + ((uint32_t)__builtin_ctz(d1) >> 3)
@@ -304,22 +345,6 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
 // ---------------------------------------------------------------------------
 #define STRIP 8
 
-// The 16 bytes src[p0 .. p0+15] as four dwords; bytes past `avail` read as 0.
-// Sub-blocks of a size that is not a multiple of 4 start unaligned: then
-// each dword is assembled by ld32.
-template <bool SAFE>
-__device__ __forceinline__ void load_strip(const SrcView& src, uint32_t p0, uint32_t d[4]) {
-    if (src.mis == 0 && (SAFE || (uint64_t)p0 + 16 <= src.avail)) {
-        const uint32_t* q = (const uint32_t*)(src.base + p0);
-        d[0] = q[0];
-        d[1] = q[1];
-        d[2] = q[2];
-        d[3] = q[3];
-        return;
-    }
-    for (int k = 0; k < 4; k++) d[k] = ld32<SAFE>(src, p0 + 4 * k);
-}
-
 // The 4 bytes at p0 + j from the strip's dwords.
 __device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) {
     const uint32_t k = j >> 2;
@@ -335,17 +360,20 @@ __device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) 
 // Output: packed[it.opos + p] = distance << 16 | length (0 = none) and/or the
 // reference layout (slot 0 of LzmaMatch[max_matches] and counts).
 // ---------------------------------------------------------------------------
-template <bool SAFE>
-__device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
+// One workgroup of INFO_THREADS per sub-block, staged in LDS first (as for
+// the Zstd info words below).
+__device__ __forceinline__ void lzma2_item(const gc_item& it, const LdsBytes& src, const uint16_t* cand16,
                                            uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                                            uint32_t max_matches) {
-    const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
     const uint32_t depth = it.depth;
-    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += ELEM_THREADS * STRIP) {
+    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
         uint32_t d[4];
-        load_strip<SAFE>(src, p0, d);
-        const uint32_t prev0 = p0 > 0 ? ld32<SAFE>(src, p0 - 1) : 0;
+        d[0] = src.w[p0 / 4];
+        d[1] = src.w[p0 / 4 + 1];
+        d[2] = src.w[p0 / 4 + 2];
+        d[3] = src.w[p0 / 4 + 3];
+        const uint32_t prev0 = p0 > 0 ? ld32(src, p0 - 1) : 0;
         uint32_t tgt[STRIP], vc[STRIP];
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) {
@@ -353,7 +381,7 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
             tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32<SAFE>(src, tgt[j]) : 0;
+        for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32(src, tgt[j]) : 0;
         uint32_t out[STRIP];
         uint32_t next_t = GC_NONE, next_len = 0;
 #pragma unroll
@@ -374,7 +402,7 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
                     if (next_t != GC_NONE && next_t == t + 1)
                         len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
                     else
-                        len = lcp_lane<SAFE>(src, t, p, 2, maxlen);
+                        len = lcp_lane<true>(src, t, p, 2, maxlen);
                     o = ((p - t) << 16) | len;
                     next_len = len;
                 }
@@ -401,19 +429,20 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const uint8_t* are
     }
     // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
     if (ref_c)
-        for (uint32_t p = n + threadIdx.x; p < it.out_span; p += ELEM_THREADS) ref_c[it.opos + p] = 0;
+        for (uint32_t p = n + threadIdx.x; p < it.out_span; p += INFO_THREADS) ref_c[it.opos + p] = 0;
 }
 
-extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                     const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                     uint32_t max_matches) {
+    __shared__ uint32_t sb[STAGE_WORDS];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
-        if (item_safe(it.avail, it.n))
-            lzma2_item<true>(it, arena, cand16, packed, ref_m, ref_c, max_matches);
-        else
-            lzma2_item<false>(it, arena, cand16, packed, ref_m, ref_c, max_matches);
+        __syncthreads();  // the last item's reads of sb are done
+        stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
+        __syncthreads();
+        lzma2_item(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches);
     }
 }
 
@@ -423,24 +452,32 @@ gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uin
 // and whether it verifies 3 bytes (:241-245) with the common length capped
 // at GC_ZSTD_SHORT: info = len << 16 | cand, len 0 = no.  A length equal to
 // the cap is re-measured by the parse.  Written for p + 3 < n.
+//
+// One workgroup of INFO_THREADS per sub-block, the sub-block staged in LDS
+// first: the candidate reads and the length measurements land anywhere in
+// the 64 KiB before p, and from LDS they cost no L2 or HBM traffic (from
+// global memory, the sub-blocks of every workgroup in flight on an XCD do
+// not fit its L2).
 // ---------------------------------------------------------------------------
-template <bool SAFE, uint32_t W>
-__device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t* arena, const uint16_t* cand16,
-                                               uint32_t* info) {
-    // cpos and opos are multiples of 8 and each item owns its positions
-    // rounded up to 8: whole strips of W positions are read and written with
+
+template <uint32_t W>
+__device__ __forceinline__ void zstd_info_lds(const LdsBytes& L, uint32_t n, const uint16_t* cand,
+                                              uint32_t* out) {
+    // cand and out start at multiples of 8 positions and each item owns its
+    // positions rounded up to 8: whole strips are read and written with
     // vector accesses
-    const SrcView src = src_view(arena, it.src_off, it.avail);
-    const uint32_t n = it.n;
-    for (uint32_t p0 = threadIdx.x * W; p0 + 3 < n; p0 += ELEM_THREADS * W) {
+    for (uint32_t p0 = threadIdx.x * W; p0 + 3 < n; p0 += INFO_THREADS * W) {
         uint32_t d[4];
-        load_strip<SAFE>(src, p0, d);
+        d[0] = L.w[p0 / 4];
+        d[1] = L.w[p0 / 4 + 1];
+        d[2] = L.w[p0 / 4 + 2];
+        d[3] = L.w[p0 / 4 + 3];
         uint32_t cv[W / 2];
         if constexpr (W == 8) {
-            const uint4 cw = *(const uint4*)(cand16 + it.cpos + p0);
+            const uint4 cw = *(const uint4*)(cand + p0);
             cv[0] = cw.x, cv[1] = cw.y, cv[2] = cw.z, cv[3] = cw.w;
         } else {
-            const uint2 cw = *(const uint2*)(cand16 + it.cpos + p0);
+            const uint2 cw = *(const uint2*)(cand + p0);
             cv[0] = cw.x, cv[1] = cw.y;
         }
         uint32_t c[W], vc[W];
@@ -450,8 +487,8 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
             c[j] = (cj == GC_NONE || p0 + j + 3 >= n) ? 0 : cj;
         }
 #pragma unroll
-        for (uint32_t j = 0; j < W; j++) vc[j] = ld32<SAFE>(src, c[j]);
-        uint32_t out[W];
+        for (uint32_t j = 0; j < W; j++) vc[j] = ld32(L, c[j]);
+        uint32_t o[W];
         uint32_t next_len = 0;
 #pragma unroll
         for (int j = W - 1; j >= 0; j--) {
@@ -462,37 +499,38 @@ __device__ __forceinline__ void zstd_info_item(const gc_item& it, const uint8_t*
                 if (j + 1 < (int)W && next_len && c[j + 1] == c[j] + 1)
                     len = next_len + 1 < lim ? next_len + 1 : lim;
                 else
-                    len = lcp_lane<SAFE>(src, c[j], p, 3, lim);
+                    len = lcp_lane<true>(L, c[j], p, 3, lim);
             }
             next_len = len;
-            out[j] = (len << 16) | c[j];
+            o[j] = (len << 16) | c[j];
         }
-        uint4* o = (uint4*)(info + it.opos + p0);
-        o[0] = make_uint4(out[0], out[1], out[2], out[3]);
-        if constexpr (W == 8) o[1] = make_uint4(out[4], out[5], out[6], out[7]);
+        uint4* q = (uint4*)(out + p0);
+        q[0] = make_uint4(o[0], o[1], o[2], o[3]);
+        if constexpr (W == 8) q[1] = make_uint4(o[4], o[5], o[6], o[7]);
     }
 }
 
 template <uint32_t W>
 __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                 uint32_t n_items, const uint16_t* cand16, uint32_t* info) {
+    __shared__ uint32_t sb[STAGE_WORDS];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
-        if (item_safe(it.avail, it.n))
-            zstd_info_item<true, W>(it, arena, cand16, info);
-        else
-            zstd_info_item<false, W>(it, arena, cand16, info);
+        __syncthreads();  // the last item's reads of sb are done
+        stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
+        __syncthreads();
+        zstd_info_lds<W>(LdsBytes{sb}, it.n, cand16 + it.cpos, info + it.opos);
     }
 }
 
 // Strips of 8 or 4 positions: the same words, different register use.
-extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_zstd_info_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                     const uint16_t* cand16, uint32_t* info) {
     zstd_info_items<8>(arena, items, n_items, cand16, info);
 }
 
-extern "C" __global__ void __launch_bounds__(ELEM_THREADS)
+extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                      const uint16_t* cand16, uint32_t* info) {
     zstd_info_items<4>(arena, items, n_items, cand16, info);
