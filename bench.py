@@ -51,6 +51,20 @@ class Hip:
         return ms.value
 
 
+def hbm_traffic_per_input_byte(workload: str, kernel: str):
+    """HBM bytes per input byte of one launch of `kernel`, from the committed rocprofv3 summary
+    profiles/<workload>_hbm_traffic.md (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a device copy of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "bytes_per_input_byte:" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline_zstd(data: np.ndarray, levels, sample_bytes: int):
     """The oracle (CPU restatement of the reference kernel) on a bounded sample, all host cores."""
     sys.path.insert(0, ROOT)
@@ -229,6 +243,8 @@ def main():
     dom = max(share, key=lambda i: k_ms[i])
     dom_name, dom_ms, dom_bytes = knames[dom], k_ms[dom], share[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    per_byte = hbm_traffic_per_input_byte(a.workload, dom_name)
+    traffic = int(per_byte * work_bytes) if per_byte is not None else None
 
     cpu = None
     if not a.no_cpu:
@@ -268,7 +284,7 @@ def main():
         "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq,
                     "match_coverage_sampled": round(cover / (work_bytes / 16), 4) if cover else None},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "corpus_gen_s": round(gen_s, 2),

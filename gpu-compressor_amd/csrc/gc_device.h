@@ -4,6 +4,12 @@
 
 #include "../../include/gcompress.h"
 
+// Threads per workgroup of the candidate kernel: one table wavefront and
+// (GC_CAND_THREADS / 64 - 1) analysis wavefronts of 64 positions each.
+#ifndef GC_CAND_THREADS
+#define GC_CAND_THREADS 512
+#endif
+
 // Sequence output formats of the Zstd parse kernel.
 #define GC_SEQ_ZSTD 0u  // ZSTD_Sequence {offset, litLength, matchLength, rep}
 #define GC_SEQ_REF 1u   // reference device record {literal_length, match_length - 3, offset}

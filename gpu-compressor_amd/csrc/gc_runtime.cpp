@@ -77,7 +77,7 @@ static int launch(hipStream_t stream, hipFunction_t f, uint32_t grid, uint32_t b
 int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                 uint32_t grid, uint16_t* cand16) {
     void* args[] = {&arena, &items, &n_items, &cand16};
-    return launch(stream, ctx->k.cand, grid, 256, args, "launch gc_cand_kernel");
+    return launch(stream, ctx->k.cand, grid, GC_CAND_THREADS, args, "launch gc_cand_kernel");
 }
 
 int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,

@@ -41,8 +41,8 @@
 #define GC_ZSTD_SHORT 64u  // Zstd lengths >= this are re-measured by the parse
 
 #define WAVE 64
-#define CAND_THREADS 256
-#define CAND_WAVES (CAND_THREADS / WAVE)
+#define CAND_AWAVES (GC_CAND_THREADS / WAVE - 1)  // candidate kernel: analysis wavefronts (+ 1 table)
+#define CAND_THREADS GC_CAND_THREADS
 #define INFO_THREADS 1024                          // per-position kernels: one workgroup per sub-block
 #define STAGE_WORDS ((GC_SUB_BLOCK + 64) / 4)      // a sub-block staged in LDS, with 64 bytes to spare
 #define PARSE_WAVES 4
@@ -239,21 +239,38 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
 // ---------------------------------------------------------------------------
 // Candidate kernel: cand16[it.cpos + p] = cand(p) (GC_NONE when there is none
 // or p + 3 >= n) for every position of every item.
+//
+// One workgroup per sub-block, tiles of CAND_TILE positions, two roles:
+//  * CAND_AWAVES analysis wavefronts, 64 positions each: hash, and group the
+//    lanes of the wavefront that share a hash, so that each position knows
+//    its predecessor inside its 64 (if any) and whether a later lane of the
+//    64 shares its hash.  Packed words go to LDS.
+//  * One table wavefront applies a tile's CAND_AWAVES steps of 64 to the
+//    reference's table (16384 x u16 in LDS) in order: the first lane of a
+//    group reads the slot, the last one writes it.  The order of table
+//    reads and writes is this wavefront's program order, so the steps need
+//    no barrier between them.
+// The analysis of tile t + 1 runs while the table wavefront works on tile t:
+// one barrier per tile.
 // ---------------------------------------------------------------------------
+#define CAND_TILE (CAND_AWAVES * WAVE)
+
 struct CandLds {
-    uint16_t table[GC_TABLE];              // the reference's 14-bit table
-    uint8_t dup[CAND_WAVES][GC_DUP_SIZE];  // write/read-back duplicate test
-    uint8_t flag[CAND_WAVES][WAVE];        // lanes that won a duplicate test
+    uint16_t table[GC_TABLE];               // the reference's 14-bit table
+    uint32_t ana[2][CAND_TILE];             // analysis words, double buffered
+    uint8_t dup[CAND_AWAVES][GC_DUP_SIZE];  // write/read-back duplicate test
+    uint8_t flag[CAND_AWAVES][WAVE];        // lanes that won a duplicate test
 };
 
-// One tile of 256 positions starting at t0; `word` is this thread's 4 bytes
-// (p = t0 + threadIdx.x), valid when p + 3 < n.
-__device__ __forceinline__ uint32_t cand_tile(CandLds& s, uint32_t word, uint32_t n, uint32_t t0) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t wave = tid / WAVE;
-    const uint32_t lane = tid % WAVE;
-    const uint32_t p = t0 + tid;
-    const bool valid = p + 3 < n;
+// Analysis word of this lane's position (valid when p + 3 < n):
+// hash | has an earlier equal lane << 14 | has a later one << 15 |
+// the earlier lane << 16 | valid << 22.
+#define ANA_BELOW (1u << 14)
+#define ANA_ABOVE (1u << 15)
+#define ANA_VALID (1u << 22)
+
+__device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, bool valid) {
+    const uint32_t lane = lane_id();
     const uint32_t h = hash4(word);
 
     // Duplicate test: every valid lane writes its id under a 10-bit slice of
@@ -269,57 +286,86 @@ __device__ __forceinline__ uint32_t cand_tile(CandLds& s, uint32_t word, uint32_
     s.flag[wave][lane] = 0;
     if (valid) s.dup[wave][slot] = (uint8_t)lane;
     __asm__ volatile("" ::: "memory");
-    uint32_t back = valid ? s.dup[wave][slot] : lane;
-    bool loser = valid && back != lane;
+    const uint32_t back = valid ? s.dup[wave][slot] : lane;
+    const bool loser = valid && back != lane;
     if (loser) s.flag[wave][back & (WAVE - 1)] = 1;
     __asm__ volatile("" ::: "memory");
-    bool dup = loser || (valid && s.flag[wave][lane] != 0);
+    const bool dup = loser || (valid && s.flag[wave][lane] != 0);
 
     // Exact groups among the duplicate lanes (one step per distinct hash).
     const uint64_t lane_bit = 1ull << lane;
     uint64_t group = lane_bit;
     uint64_t rest = ballot(dup);
     while (rest) {
-        uint32_t lead = (uint32_t)__builtin_ctzll(rest);
-        uint32_t hl = bcast(h, lead);
-        uint64_t g = ballot(dup && h == hl);
+        const uint32_t lead = (uint32_t)__builtin_ctzll(rest);
+        const uint32_t hl = bcast(h, lead);
+        const uint64_t g = ballot(dup && h == hl);
         if (dup && h == hl) group = g;
         rest &= ~g;
     }
     const uint64_t below = group & (lane_bit - 1);
     const uint64_t above = group & ~(lane_bit | (lane_bit - 1));
+    const uint32_t pred = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
+    return valid ? (h | (below ? ANA_BELOW : 0u) | (above ? ANA_ABOVE : 0u) | (pred << 16) | ANA_VALID) : 0u;
+}
 
-    uint32_t cand = GC_NONE;
-    if (below) cand = t0 + wave * WAVE + 63u - (uint32_t)__builtin_clzll(below);
-
-    // Table rounds: wavefront r reads and writes the table after every lower
-    // wavefront of the tile has written it.
+// The table wavefront's work on one tile at t0 (analysis words in `ana`).
+__device__ __forceinline__ void cand_table(CandLds& s, const uint32_t* ana, uint32_t t0, uint32_t n, uint16_t* out) {
+    const uint32_t lane = lane_id();
+    uint32_t a[CAND_AWAVES], v[CAND_AWAVES];
 #pragma unroll
-    for (uint32_t r = 0; r < CAND_WAVES; r++) {
-        if (wave == r && valid) {
-            if (!below) cand = s.table[h];
-            if (!above) s.table[h] = (uint16_t)p;
-        }
-        lds_barrier();
+    for (int j = 0; j < CAND_AWAVES; j++) a[j] = ana[j * WAVE + lane];
+#pragma unroll
+    for (int j = 0; j < CAND_AWAVES; j++) {
+        const uint32_t h = a[j] & (GC_TABLE - 1);
+        const bool valid = (a[j] & ANA_VALID) != 0;
+        v[j] = (valid && !(a[j] & ANA_BELOW)) ? s.table[h] : GC_NONE;
+        if (valid && !(a[j] & ANA_ABOVE)) s.table[h] = (uint16_t)(t0 + j * WAVE + lane);
     }
-    return valid ? cand : GC_NONE;
+#pragma unroll
+    for (int j = 0; j < CAND_AWAVES; j++) {
+        const uint32_t p = t0 + j * WAVE + lane;
+        const uint32_t c = (a[j] & ANA_BELOW) ? t0 + j * WAVE + ((a[j] >> 16) & 63u) : v[j];
+        if (p < n) out[p] = (uint16_t)((a[j] & ANA_VALID) ? c : GC_NONE);
+    }
 }
 
 template <bool SAFE>
 __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const uint8_t* arena, uint16_t* cand16) {
     const uint32_t tid = threadIdx.x;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
+    const uint32_t lane = lane_id();
     const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
+    uint16_t* out = cand16 + it.cpos;
     uint32_t* t = (uint32_t*)s.table;
     for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
-    lds_barrier();
-    uint32_t next = (tid + 3 < n) ? ld32<SAFE>(src, tid) : 0;
-    for (uint32_t t0 = 0; t0 < n; t0 += CAND_THREADS) {
-        const uint32_t word = next;
-        const uint32_t q = t0 + CAND_THREADS + tid;  // next tile, loaded while this one runs
+    // analysis of tile 0; words two tiles ahead
+    uint32_t next = 0;
+    if (wave < CAND_AWAVES) {
+        const uint32_t p = wave * WAVE + lane;
+        const uint32_t word = (p + 3 < n) ? ld32<SAFE>(src, p) : 0;
+        const uint32_t q = CAND_TILE + p;
         next = (q + 3 < n) ? ld32<SAFE>(src, q) : 0;
-        const uint32_t c = cand_tile(s, word, n, t0);
-        if (t0 + tid < n) cand16[it.cpos + t0 + tid] = (uint16_t)c;
+        s.ana[0][p] = cand_analyse(s, wave, word, p + 3 < n);
+    }
+    lds_barrier();
+    uint32_t buf = 0;
+    for (uint32_t t0 = 0; t0 < n; t0 += CAND_TILE) {
+        if (wave < CAND_AWAVES) {
+            const uint32_t t1 = t0 + CAND_TILE;
+            if (t1 < n) {
+                const uint32_t p = t1 + wave * WAVE + lane;
+                const uint32_t word = next;
+                const uint32_t q = p + CAND_TILE;
+                next = (q + 3 < n) ? ld32<SAFE>(src, q) : 0;
+                s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p + 3 < n);
+            }
+        } else {
+            cand_table(s, s.ana[buf], t0, n, out);
+        }
+        lds_barrier();
+        buf ^= 1;
     }
 }
 
@@ -332,6 +378,7 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
             cand_item<true>(s, it, arena, cand16);
         else
             cand_item<false>(s, it, arena, cand16);
+        lds_barrier();  // the table is reset for the next item
     }
 }
 
