@@ -22,7 +22,9 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-compressor_amd"))
 
 import gcompress  # noqa: E402
 
+KiB = 1 << 10
 MiB = 1 << 20
+GiB = 1 << 30
 HBM_PEAK_GBS = 8000.0
 
 
@@ -65,34 +67,67 @@ def hbm_traffic_per_input_byte(workload: str, kernel: str):
     return None
 
 
-def cpu_baseline_zstd(data: np.ndarray, levels, sample_bytes: int):
-    """The oracle (CPU restatement of the reference kernel) on a bounded sample, all host cores."""
+def cpu_baseline(data: np.ndarray, inputs, sample_bytes: int):
+    """The oracle (CPU restatement of the reference kernels) on the first inputs up to `sample_bytes`
+    of match-find input, all host cores."""
     sys.path.insert(0, ROOT)
     import oracle
 
     cores = min(os.cpu_count() or 1, 16)
-    sample = data[:sample_bytes]
+    done = 0
+    picked = []
+    for (o, s, algo, lv) in inputs:
+        if done >= sample_bytes:
+            break
+        s = min(s, sample_bytes - done)
+        picked.append((o, s, algo, lv))
+        done += s
     t = time.perf_counter()
-    for lv in levels:
-        oracle.zstd_match_find(sample, depth={1: 16, 2: 64}.get(lv, 1), threads=cores)
+    for (o, s, algo, lv) in picked:
+        if algo == gcompress.ALGO_ZSTD:
+            oracle.zstd_match_find(data[o: o + s], depth={1: 16, 2: 64}.get(lv, 1), threads=cores)
+        else:
+            oracle.lzma2_match_find(data[o: o + s], threads=cores)
     dt = time.perf_counter() - t
-    return {"value": round(sample.size * len(levels) / dt / 1e9, 4), "unit": "GB/s", "cores": cores,
-            "kind": "port",
-            "sample": f"first {sample.size // MiB} MiB of the corpus at levels {list(levels)} "
-                      f"({sample.size * len(levels) // MiB} MiB of match-find input), oracle on {cores} threads"}
+    return {"value": round(done / dt / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"the first {len(picked)} of the inputs, {done // MiB} MiB of match-find input, "
+                      f"oracle on {cores} threads"}
 
 
-def cpu_baseline_lzma2(data: np.ndarray, sample_bytes: int):
-    sys.path.insert(0, ROOT)
-    import oracle
-
-    cores = min(os.cpu_count() or 1, 16)
-    sample = data[:sample_bytes]
-    t = time.perf_counter()
-    oracle.lzma2_match_find(sample, 65536, 1, 32, threads=cores)
-    dt = time.perf_counter() - t
-    return {"value": round(sample.size / dt / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"first {sample.size // MiB} MiB of the corpus, oracle on {cores} threads"}
+def workload_inputs(a):
+    """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
+    if a.workload == "zstd":
+        # 1 GiB four-kind mix in 16 MiB chunks, every chunk at Zstd L1 and L2
+        n, chunk = a.size_mib * MiB, a.chunk_mib * MiB
+        data = gcompress.corpus(n, 2, (1, 1, 1, 1), 65536, True)
+        chunks = [(o, min(chunk, n - o)) for o in range(0, n, chunk)]
+        inputs = [(o, s, gcompress.ALGO_ZSTD, lv) for lv in (1, 2) for (o, s) in chunks]
+        desc = f"drop-in Zstd L1+L2 match-find, {a.size_mib} MiB mix in {a.chunk_mib} MiB chunks"
+    elif a.workload == "lzma2":
+        # disk-image-like mix: 40% structured, 25% text, 20% back-references, 15% random, 1 MiB
+        # segments, 8 MiB zero gaps every 96 MiB; dictionary-size chunks
+        n, chunk = a.size_mib * MiB, a.chunk_mib * MiB
+        data = gcompress.corpus(n, 3, (15, 25, 20, 40), MiB, True, 96 * MiB, 8 * MiB)
+        inputs = [(o, min(chunk, n - o), gcompress.ALGO_LZMA2, 1) for o in range(0, n, chunk)]
+        desc = f"LZMA2 L1 match-find, {a.size_mib} MiB disk-image-like mix in {a.chunk_mib} MiB chunks"
+    else:
+        # many inputs: sizes log-uniform in [4 KiB, 32 MiB], per-input Dirichlet(1,1,1,1) kind weights,
+        # a third each at Zstd L1, Zstd L2 and LZMA2
+        rng = np.random.default_rng(4)
+        sizes = np.exp(rng.uniform(np.log(4 * KiB), np.log(32 * MiB), a.inputs)).astype(np.int64)
+        weights = rng.dirichlet([1, 1, 1, 1], a.inputs)
+        kinds = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)]
+        offs = np.concatenate([[0], np.cumsum((sizes + 255) // 256 * 256)])
+        data = np.empty(int(offs[-1]), np.uint8)
+        inputs = []
+        for i in range(a.inputs):
+            o, s = int(offs[i]), int(sizes[i])
+            data[o: o + s] = gcompress.corpus(s, 1000 + i, weights[i], 65536, False)
+            algo, lv = kinds[i % 3]
+            inputs.append((o, s, algo, lv))
+        desc = (f"{a.inputs} inputs, sizes log-uniform 4 KiB-32 MiB ({int(sizes.sum()) / GiB:.2f} GiB in all), "
+                f"Dirichlet kind mix per input, a third each Zstd L1 / Zstd L2 / LZMA2")
+    return data, inputs, desc
 
 
 def main():
@@ -102,7 +137,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size-mib", type=int, default=None, help="default: 1024 (zstd), 640 (lzma2)")
     ap.add_argument("--chunk-mib", type=int, default=None, help="default: 16 (zstd), 128 (lzma2)")
-    ap.add_argument("--workload", choices=["zstd", "lzma2", "serve"], default="zstd")
+    ap.add_argument("--workload", choices=["zstd", "lzma2", "mixed", "serve"], default="zstd")
+    ap.add_argument("--inputs", type=int, default=4096, help="mixed workload: number of inputs")
     ap.add_argument("--rates", default="500,2000,8000")
     ap.add_argument("--requests", type=int, default=2000, help="requests per serving rate")
     ap.add_argument("--budget-gib", type=int, default=64)
@@ -121,7 +157,7 @@ def main():
         a.chunk_mib = 128 if a.workload == "lzma2" else 16
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
-              "(sub-blocks shard with no exchange, see DESIGN.md)", file=sys.stderr)
+              "(sub-blocks shard by range or by request with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     hip = Hip()
@@ -130,21 +166,10 @@ def main():
         return bench_serve(a, hip, ctx)
     L = gcompress.lib()
     stream = L.gc_stream(ctx.h)
-    n = a.size_mib * MiB
-    chunk = a.chunk_mib * MiB
     t0 = time.perf_counter()
-    seed = 2 if a.workload == "zstd" else 3
-    if a.workload == "zstd":
-        data = gcompress.corpus(n, seed, (1, 1, 1, 1), 65536, True)
-        levels = [1, 2]
-        algo = gcompress.ALGO_ZSTD
-    else:
-        # disk-image-like mix: 40% structured, 25% text, 20% back-references, 15% random, 1 MiB
-        # segments, 8 MiB zero gaps every 96 MiB
-        data = gcompress.corpus(n, seed, (15, 25, 20, 40), MiB, True, 96 * MiB, 8 * MiB)
-        levels = [1]
-        algo = gcompress.ALGO_LZMA2
+    data, inputs, desc = workload_inputs(a)
     gen_s = time.perf_counter() - t0
+    n = data.size
     d_in = ctx.alloc(n + 256)
     ctx.htod(d_in, data)
     if a.calib_copy_mib:
@@ -157,30 +182,28 @@ def main():
         ctx.sync()
         ctx.free(c_src)
         ctx.free(c_dst)
-    chunks = [(o, min(chunk, n - o)) for o in range(0, n, chunk)]
-    inputs = [(o, s, algo, lv) for lv in levels for (o, s) in chunks]
     batch = gcompress.Batch(ctx, d_in, n + 256, inputs)  # the allocation is readable to its end
-    work_bytes = n * len(levels)
+    work_bytes = sum(s for (_, s, _, _) in inputs)
+    zstd_bytes = sum(s for (_, s, al, _) in inputs if al == gcompress.ALGO_ZSTD)
+    lzma_bytes = work_bytes - zstd_bytes
 
-    # ---- plain per-chunk form: one launch sequence per chunk, synchronised per chunk ----
-    nsub_chunk = -(-chunk // 65536)
+    # ---- plain form: one launch sequence per input (chunk), synchronised per input ----
     max_seq = 65536 // 3
-    if algo == gcompress.ALGO_ZSTD:
-        d_seq = ctx.alloc(nsub_chunk * max_seq * 16)
-        d_cnt = ctx.alloc(nsub_chunk * 4)
-    else:
-        d_pk = ctx.alloc(chunk * 4)
+    big = max(s for (_, s, _, _) in inputs)
+    nsub_big = -(-big // 65536)
+    d_seq = ctx.alloc(nsub_big * max_seq * 16) if zstd_bytes else None
+    d_cnt = ctx.alloc(nsub_big * 4) if zstd_bytes else None
+    d_pk = ctx.alloc(big * 4) if lzma_bytes else None
 
     def plain_pass():
-        for lv in levels:
-            for (o, s) in chunks:
-                if algo == gcompress.ALGO_ZSTD:
-                    rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_seq, 0, None, d_cnt, None, 65536,
-                                                  -(-s // 65536), s, {1: 16, 2: 64}.get(lv, 1), max_seq)
-                else:
-                    rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_pk, 65536, s, 32)
-                assert rc == 0, L.gc_last_error()
-                ctx.sync()
+        for (o, s, algo, lv) in inputs:
+            if algo == gcompress.ALGO_ZSTD:
+                rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_seq, 0, None, d_cnt, None, 65536,
+                                              -(-s // 65536), s, {1: 16, 2: 64}.get(lv, 1), max_seq)
+            else:
+                rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_pk, 65536, s, 32)
+            assert rc == 0, L.gc_last_error()
+            ctx.sync()
 
     plain_gbs = None
     if a.plain_steps > 0:
@@ -193,9 +216,8 @@ def main():
 
     # ---- batched form ----
     tune = batch.tune()
-    tuned_ms = None
     if not a.no_autotune:
-        tune, tuned_ms = batch.autotune(reps=1)
+        tune, _ = batch.autotune(reps=1)
     for _ in range(a.warmup):
         batch.run()
     ctx.sync()
@@ -220,39 +242,28 @@ def main():
     # algorithmic bytes and results summary (outside the timed region)
     n_sub = 0
     n_seq = 0
-    cover = 0
-    for i in range(len(inputs)):
+    for i, (_, s, algo, _) in enumerate(inputs):
         if algo == gcompress.ALGO_ZSTD:
-            seqs, counts = batch.zstd(i, with_seqs=(i % 16 == 0))
+            _, counts = batch.zstd(i, with_seqs=False)
             n_sub += counts.size
             n_seq += int(counts.sum())
-            if seqs is not None:
-                for s in range(counts.size):
-                    cover += int(seqs[s, : counts[s], 2].sum())
         else:
             n_sub += batch.n_sub(i)
+    zstd_sub = sum(-(-s // 65536) for (_, s, al, _) in inputs if al == gcompress.ALGO_ZSTD)
     # Algorithmic bytes (what the path must move): Zstd N_in + 16 n_seq + 4 n_sub, LZMA2 5 N_in.
-    # Per kernel, the share it is responsible for: every pass reads the input once; the parse
-    # writes the sequences and counts, the LZMA2 length pass the 4-byte match per position.
-    if algo == gcompress.ALGO_ZSTD:
-        alg_bytes = work_bytes + 16 * n_seq + 4 * n_sub
-        share = {0: work_bytes, 2: work_bytes, 3: alg_bytes}
-    else:
-        alg_bytes = 5 * work_bytes
-        share = {0: work_bytes, 1: alg_bytes}
-    dom = max(share, key=lambda i: k_ms[i])
+    # Per kernel, the share it is responsible for: the candidate pass reads every input once, the
+    # Zstd info pass the Zstd inputs; the parse writes the sequences and counts, the LZMA2 length pass
+    # the 4-byte match per position.
+    zstd_alg = zstd_bytes + 16 * n_seq + 4 * zstd_sub
+    share = {0: work_bytes, 1: 5 * lzma_bytes, 2: zstd_bytes, 3: zstd_alg}
+    dom = max((i for i in share if share[i]), key=lambda i: k_ms[i])
     dom_name, dom_ms, dom_bytes = knames[dom], k_ms[dom], share[dom]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     per_byte = hbm_traffic_per_input_byte(a.workload, dom_name)
     traffic = int(per_byte * work_bytes) if per_byte is not None else None
 
-    cpu = None
-    if not a.no_cpu:
-        if algo == gcompress.ALGO_ZSTD:
-            cpu = cpu_baseline_zstd(data, levels, a.cpu_sample_mib * MiB)
-        else:
-            cpu = cpu_baseline_lzma2(data, min(a.cpu_sample_mib, 256) * MiB)
-
+    cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
+    levels = sorted({lv for (_, _, al, lv) in inputs if al == gcompress.ALGO_ZSTD})
     out = {
         "metric": "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency",
         "value": round(value, 3),
@@ -267,13 +278,13 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded four-kind mix: random, word text, back-references, structured binary)",
         "config": {
-            "workload": (f"drop-in Zstd L1+L2 match-find, {a.size_mib} MiB mix in {a.chunk_mib} MiB chunks"
-                         if algo == gcompress.ALGO_ZSTD else
-                         f"LZMA2 L1 match-find, {a.size_mib} MiB disk-image-like mix"),
-            "form": "batched: every sub-block of every chunk and level in one launch per kernel (grid-stride items)",
+            "workload": desc,
+            "form": "batched: every sub-block of every input in one launch per kernel (grid-stride items)",
+            "inputs": len(inputs),
             "input_bytes_per_step": work_bytes,
             "sub_block": 65536,
-            "levels": levels,
+            "zstd_levels": levels,
+            "lzma2_bytes": lzma_bytes,
             "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "elem_wg_per_cu": tune[2],
                      "info_strip": tune[3]},
         },
@@ -281,8 +292,7 @@ def main():
         "batched_vs_plain": round(value / plain_gbs, 2) if plain_gbs else None,
         "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
         "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)},
-        "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq,
-                    "match_coverage_sampled": round(cover / (work_bytes / 16), 4) if cover else None},
+        "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq},
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},
