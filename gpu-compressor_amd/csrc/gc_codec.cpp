@@ -457,20 +457,21 @@ int gc_nvlz_compress(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, 
     if (!ctx) return fail(GC_EINVAL, "null context");
     if (dict_mb == 0) dict_mb = 128;
     const size_t pchunk = (size_t)dict_mb << 20;  // pipeline_lzma2.rs:40
-    std::vector<uint32_t> packed;
+    std::vector<uint8_t> any_match;
     std::vector<std::vector<uint8_t>> enc;
     for (size_t c0 = 0; c0 < n; c0 += pchunk) {
         size_t len = std::min(pchunk, n - c0);
-        packed.resize(len);
-        int rc = gc_lzma2_find_matches_packed(ctx, src + c0, len, packed.data());
-        if (rc) return rc;
         const size_t ns = (size_t)n_sub_of(len);
+        // the encoder only asks whether a sub-block has any match (compress_lzma2_custom.rs:577): one
+        // byte per sub-block comes back instead of the match lists
+        any_match.resize(ns);
+        int rc = gc_lzma2_triage(ctx, src + c0, len, any_match.data());
+        if (rc) return rc;
         enc.assign(ns, {});
         std::vector<int> errs(ns, 0);
         parallel_for(ns, threads, [&](size_t s) {
             const size_t st = s * GC_SUB_BLOCK, l = std::min<size_t>(GC_SUB_BLOCK, len - st);
-            bool any = false;
-            for (size_t i = 0; i < l && !any; i++) any = packed[st + i] != 0;
+            const bool any = any_match[s] != 0;
             enc[s].resize(lzma2_sub_bound(l));
             size_t w = 0;
             errs[s] = encode_sub_block(src + c0 + st, l, any, enc[s].data(), enc[s].size(), &w);

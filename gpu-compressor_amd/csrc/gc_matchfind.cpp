@@ -51,7 +51,7 @@ int check_sub(uint32_t sub) {
 }
 
 int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t* d_packed,
-              gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth) {
+              gc_lzma_match* d_m, uint32_t* d_c, uint32_t max_matches, uint32_t depth, uint8_t* d_triage = nullptr) {
     std::vector<gc_item> items;
     make_items(items, total, sub, nsub, 0, depth, false);
     int rc = ctx->cand.ensure((size_t)nsub * pitch(sub) * 2 + 256);
@@ -61,7 +61,7 @@ int run_lzma2(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t sub, ui
     rc = launch_cand(ctx, ctx->stream, d_in, d_items, nsub, nsub, (uint16_t*)ctx->cand.p);
     if (rc) return rc;
     return launch_lzma2_len(ctx, ctx->stream, d_in, d_items, nsub, nsub, (const uint16_t*)ctx->cand.p, d_packed, d_m,
-                            d_c, max_matches);
+                            d_c, max_matches, d_triage);
 }
 
 int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, uint32_t nsub, void* d_seq,
@@ -150,6 +150,33 @@ int gc_lzma2_find_matches_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, ui
         hipError_t e = hipMemcpyAsync(packed, ctx->out_a.p, n * 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, "gc_lzma2_find_matches_packed: copy out");
+    }
+    return rc;
+}
+
+int gc_lzma2_triage_dev(gc_ctx* ctx, const uint8_t* d_input, uint64_t n, uint8_t* d_any_match) {
+    if (!ctx || !d_input || !d_any_match) return fail(GC_EINVAL, "null argument");
+    if (!aligned4(d_input)) return fail(GC_EINVAL, "d_input must be 4-byte aligned");
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    if (nsub == 0) return GC_OK;
+    if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
+    return run_lzma2(ctx, d_input, n, GC_SUB_BLOCK, (uint32_t)nsub, nullptr, nullptr, nullptr, 1, GC_LZMA2_DEPTH,
+                     d_any_match);
+}
+
+int gc_lzma2_triage(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* any_match) {
+    if (!ctx || (!chunk && n) || (!any_match && n)) return fail(GC_EINVAL, "null argument");
+    if (n == 0) return GC_OK;
+    const size_t nsub = (size_t)div_up(n, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    if (!rc) rc = ctx->out_a.ensure(nsub);
+    if (!rc) rc = gc_lzma2_triage_dev(ctx, (const uint8_t*)ctx->input.p, n, (uint8_t*)ctx->out_a.p);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(any_match, ctx->out_a.p, nsub, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "gc_lzma2_triage: copy out");
     }
     return rc;
 }

@@ -82,8 +82,8 @@ int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_
 
 int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                      uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
-                     uint32_t max_matches) {
-    void* args[] = {&arena, &items, &n_items, &cand16, &packed, &ref_m, &ref_c, &max_matches};
+                     uint32_t max_matches, uint8_t* triage) {
+    void* args[] = {&arena, &items, &n_items, &cand16, &packed, &ref_m, &ref_c, &max_matches, &triage};
     return launch(stream, ctx->k.lzma2_len, grid, 1024, args, "launch gc_lzma2_len_kernel");
 }
 
@@ -113,6 +113,16 @@ int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint3
                   void* d_out, uint32_t nsub, uint32_t max_seq) {
     void* args[] = {&d_in, &d_counts, &d_offsets, &d_out, &max_seq};
     return launch(stream, ctx->k.gather_seqs, nsub, 256, args, "launch gc_gather_seqs");
+}
+
+int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n) {
+    uint32_t vec4 = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0 ? 1u : 0u;
+    // a few hundred wavefronts keep PCIe busy when dst is host memory, and leave the CUs to the
+    // match-finding kernels running beside them
+    uint64_t blocks = (n / 4 + 255) / 256;
+    uint32_t grid = (uint32_t)(blocks < 256 ? (blocks ? blocks : 1) : 256);
+    void* args[] = {&src, &dst, &n, &vec4};
+    return launch(stream, ctx->k.copy_u32, grid, 256, args, "launch gc_copy_u32");
 }
 
 uint32_t zstd_depth_for_level(uint32_t level) {
@@ -158,6 +168,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.copy_u32, ctx->k.module, "gc_copy_u32");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);

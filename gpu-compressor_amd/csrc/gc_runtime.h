@@ -35,6 +35,7 @@ struct Kernels {
     hipFunction_t zstd_parse = nullptr;
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
+    hipFunction_t copy_u32 = nullptr;
 };
 
 }  // namespace gc
@@ -61,7 +62,8 @@ int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_
                 uint32_t grid, uint16_t* cand16);
 int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                      uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
-                     uint32_t max_matches);
+                     uint32_t max_matches,
+                     uint8_t* triage = nullptr);
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                      uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip = 8);
 int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
@@ -70,6 +72,7 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);
+int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
 
 uint32_t zstd_depth_for_level(uint32_t level);

@@ -12,9 +12,10 @@
 //      packed by gc_gather_seqs first), straight into the caller's buffers
 //      when it gave some (gc_server_submit_into), frees their memory and marks
 //      them done.
-// Per-batch sequence counts come back on the compute stream right after the
-// parse (into pinned memory), so the host never waits on a synchronous copy
-// between batches.  The server thread only polls events: a batch that ends
+// Nothing on the compute stream uses a DMA engine (those carry the inputs in):
+// the batch's work items are read from pinned memory by a copy kernel, and
+// the parse writes its sequence counts straight into pinned memory, so the
+// host reads them when the batch's event completes.  The server thread only polls events: a batch that ends
 // while results of an earlier one are still being copied waits in `finished`
 // and the device goes on with the next batch.
 // So requests that arrive while others run are admitted as memory frees up
@@ -44,6 +45,14 @@ const uint32_t kMaxSeq = GC_SUB_BLOCK / 3;
 const uint64_t kAlign = 256;
 
 uint64_t up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// Device address of pinned host memory from hipHostMalloc.
+template <class T>
+T* dev_addr(T* host) {
+    void* d = nullptr;
+    if (host && hipHostGetDevicePointer(&d, host, 0) == hipSuccess && d) return (T*)d;
+    return host;
+}
 
 double now_s() {
     using namespace std::chrono;
@@ -103,6 +112,7 @@ struct Req {
     uint64_t in_off = 0, cand_off = 0, info_off = 0, out_off = 0;  // byte offsets in the region
     uint32_t item0 = 0;        // first Zstd count index of the request in its batch
     void* dst = nullptr;       // caller's result buffer (gc_server_submit_into), else results kept here
+    void* dst_dev = nullptr;   // the same buffer as the device sees it, when it is pinned host memory
     uint64_t dst_cap = 0;      // its size in bytes
     uint32_t* dst_counts = nullptr;
     hipEvent_t copied = nullptr;
@@ -152,16 +162,19 @@ struct gc_server {
     std::deque<uint64_t> pending;                        // arrival order
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     // batch state (server thread only)
-    std::vector<uint64_t> running, fetching;
-    std::deque<std::vector<uint64_t>> finished;  // batches done on the device, results not yet copied
+    // Requests of the running, finished and fetching batches.  Held by address: `reqs` is
+    // changed by client threads (under `mu`), and a request in a batch cannot be released
+    // until it is done.
+    std::vector<Req*> running, fetching;
+    std::deque<std::vector<Req*>> finished;  // batches done on the device, results not yet copied
     hipStream_t fetch_stream = nullptr;  // results back to the host (the context's stream2 brings inputs in)
     hipEvent_t batch_start = nullptr, batch_done = nullptr, fetch_start = nullptr, fetch_done = nullptr;
-    DevBuf items, counts, meta;   // per-batch sequence counts; counts + offsets for the gathers
-    uint32_t* counts_host = nullptr;  // pinned copy of `counts`
+    DevBuf items;                     // the running batch's work items
+    uint32_t* counts_host = nullptr;  // pinned: per-batch sequence counts, written by the parse
     size_t counts_host_cap = 0;       // in u32
     gc_item* items_host = nullptr;    // pinned staging of `items`
     size_t items_host_cap = 0;        // in items
-    uint8_t* meta_host = nullptr; // pinned staging of `meta`
+    uint8_t* meta_host = nullptr; // pinned: counts and offsets for the gathers
     size_t meta_cap = 0;
     std::vector<gc_item> host_items;
     gc_server_stats stats{};
@@ -171,7 +184,7 @@ struct gc_server {
     void admit_locked();
     bool launch_batch();
     void batch_ended();
-    void finish_batch(std::vector<uint64_t>& done);
+    void finish_batch(std::vector<Req*>& done);
     void complete_fetch();
 };
 
@@ -255,8 +268,7 @@ bool gc_server::launch_batch() {
     const uint32_t n_all = (uint32_t)host_items.size(), n_z = n_all - n_l;
     std::lock_guard<std::mutex> g(ctx->mu);
     int rc = items.ensure(n_all * sizeof(gc_item));
-    if (!rc) rc = counts.ensure((size_t)n_z * 4 + 4);
-    // pinned staging (no batch runs now, so the last batch's copies are done)
+    // pinned staging (no batch runs now, so the last batch's reads and writes are done)
     if (!rc && n_z > counts_host_cap) {
         (void)hipHostFree(counts_host);
         counts_host_cap = 0;
@@ -282,13 +294,17 @@ bool gc_server::launch_batch() {
         memcpy(items_host, host_items.data(), (size_t)n_all * sizeof(gc_item));
         e = hipEventRecord(batch_start, ctx->stream);
         if (e == hipSuccess)
-            e = hipMemcpyAsync(items.p, items_host, n_all * sizeof(gc_item), hipMemcpyHostToDevice, ctx->stream);
+            rc = launch_copy_u32(ctx, ctx->stream, (const uint32_t*)dev_addr(items_host), (uint32_t*)items.p,
+                                 (uint64_t)n_all * sizeof(gc_item) / 4);
     }
     const gc_item* all = (const gc_item*)items.p;
-    const double t = now_s();
-    for (Req* r : batch) {
-        r->state = RUNNING;
-        r->t_start = t;
+    {
+        std::lock_guard<std::mutex> g2(mu);  // clients read the state in gc_server_wait
+        const double t = now_s();
+        for (Req* r : batch) {
+            r->state = RUNNING;
+            r->t_start = t;
+        }
     }
     if (!rc && e == hipSuccess) {
         rc = launch_cand(ctx, ctx->stream, pool, all, n_all, n_all, (uint16_t*)pool);
@@ -300,13 +316,10 @@ bool gc_server::launch_batch() {
                                   4);
         if (!rc && n_z)
             rc = launch_zstd_parse(ctx, ctx->stream, pool, all + n_l, n_z, (n_z + 3) / 4, (const uint32_t*)pool,
-                                   pool, GC_SEQ_FORMAT_ZSTD, nullptr, (uint32_t*)counts.p, nullptr, kMaxSeq);
-        if (!rc && n_z)
-            e = hipMemcpyAsync(counts_host, counts.p, (size_t)n_z * 4, hipMemcpyDeviceToHost, ctx->stream);
+                                   pool, GC_SEQ_FORMAT_ZSTD, nullptr, dev_addr(counts_host), nullptr, kMaxSeq);
         if (!rc && e == hipSuccess) e = hipEventRecord(batch_done, ctx->stream);
     }
-    running.clear();
-    for (Req* r : batch) running.push_back(r->id);
+    running = batch;
     if (rc || e != hipSuccess) {
         std::lock_guard<std::mutex> g2(mu);
         for (Req* r : batch) {
@@ -331,8 +344,8 @@ bool gc_server::launch_batch() {
 void gc_server::batch_ended() {
     float ms = 0;
     if (hipEventElapsedTime(&ms, batch_start, batch_done) == hipSuccess) stats.t_batches += ms * 1e-3;
-    for (uint64_t id : running) {
-        Req& r = *reqs[id];
+    for (Req* rp : running) {
+        Req& r = *rp;
         if (r.algo == GC_ALGO_ZSTD) r.counts.assign(counts_host + r.item0, counts_host + r.item0 + r.nsub);
     }
     finished.emplace_back();
@@ -342,31 +355,34 @@ void gc_server::batch_ended() {
 // Bring a finished batch's results back on the fetch stream (no other fetch
 // is in flight).  Zstd: each request's used sequence slots are packed by
 // gc_gather_seqs (counts and offsets staged in `meta`) and copied.
-void gc_server::finish_batch(std::vector<uint64_t>& done) {
+void gc_server::finish_batch(std::vector<Req*>& done) {
     fetching.swap(done);
     done.clear();
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t cs = fetch_stream;
     (void)hipEventRecord(fetch_start, cs);
     uint32_t n_z = 0;
-    for (uint64_t id : fetching)
-        if (reqs[id]->algo == GC_ALGO_ZSTD) n_z += reqs[id]->nsub;
+    for (Req* r : fetching)
+        if (r->algo == GC_ALGO_ZSTD) n_z += r->nsub;
     // meta: counts (u32) for every Zstd item, then offsets (u64)
     const size_t off_bytes = up((size_t)n_z * 4, 8);
     if (off_bytes + (size_t)n_z * 8 > meta_cap) {  // a batch larger than any before
         (void)hipHostFree(meta_host);
         meta_cap = off_bytes + (size_t)n_z * 8;
-        if (hipHostMalloc((void**)&meta_host, meta_cap, 0) != hipSuccess || meta.ensure(meta_cap)) {
+        if (hipHostMalloc((void**)&meta_host, meta_cap, 0) != hipSuccess) {
             meta_host = nullptr;
             meta_cap = 0;
-            for (uint64_t id : fetching) reqs[id]->error = GC_ENOMEM;
+            for (Req* r : fetching) r->error = GC_ENOMEM;
         }
     }
     uint32_t* mc = (uint32_t*)meta_host;
     uint64_t* mo = (uint64_t*)(meta_host + off_bytes);
-    for (uint64_t id : fetching) {
-        Req& r = *reqs[id];
-        r.state = FETCHING;
+    {
+        std::lock_guard<std::mutex> g2(mu);
+        for (Req* r : fetching) r->state = FETCHING;
+    }
+    for (Req* rp : fetching) {
+        Req& r = *rp;
         if (r.algo != GC_ALGO_ZSTD || !meta_host) continue;
         uint64_t tot = 0;
         for (uint32_t s2 = 0; s2 < r.nsub; s2++) {
@@ -380,21 +396,32 @@ void gc_server::finish_batch(std::vector<uint64_t>& done) {
             if (tot * sizeof(gc_zstd_sequence) > r.dst_cap) r.error = GC_ESIZE;
         }
     }
-    if (n_z && meta_host) (void)hipMemcpyAsync(meta.p, meta_host, off_bytes + (size_t)n_z * 8, hipMemcpyHostToDevice, cs);
-    for (uint64_t id : fetching) {
-        Req& r = *reqs[id];
+    // the gathers read the counts and offsets from pinned memory (no earlier fetch is still reading it)
+    uint8_t* meta_dev = dev_addr(meta_host);
+    for (Req* rp : fetching) {
+        Req& r = *rp;
         if (r.error) continue;
         uint8_t* base = pool + r.dev_off;
+        // pinned caller memory is written by the kernels directly (over PCIe, on the CUs);
+        // anything else is packed on the device and copied by DMA
         if (r.algo == GC_ALGO_ZSTD) {
             if (!r.dst) r.seqs.resize(r.fetch_total);
             if (!r.fetch_total) continue;
+            const uint32_t* mcounts = (const uint32_t*)meta_dev + r.item0;
+            const uint64_t* moffs = (const uint64_t*)(meta_dev + off_bytes) + r.item0;
+            if (r.dst_dev) {
+                (void)launch_gather(ctx, cs, base + r.out_off, mcounts, moffs, r.dst_dev, r.nsub, kMaxSeq);
+                continue;
+            }
             void* to = r.dst ? r.dst : (void*)r.seqs.data();
-            (void)launch_gather(ctx, cs, base + r.out_off, (const uint32_t*)meta.p + r.item0,
-                                (const uint64_t*)((uint8_t*)meta.p + off_bytes) + r.item0, base + r.cand_off, r.nsub,
-                                kMaxSeq);
+            (void)launch_gather(ctx, cs, base + r.out_off, mcounts, moffs, base + r.cand_off, r.nsub, kMaxSeq);
             (void)hipMemcpyAsync(to, base + r.cand_off, r.fetch_total * sizeof(gc_zstd_sequence),
                                  hipMemcpyDeviceToHost, cs);
         } else {
+            if (r.dst_dev) {
+                (void)launch_copy_u32(ctx, cs, (const uint32_t*)(base + r.out_off), (uint32_t*)r.dst_dev, r.n);
+                continue;
+            }
             if (!r.dst) r.packed.resize(r.n);
             void* to = r.dst ? r.dst : (void*)r.packed.data();
             (void)hipMemcpyAsync(to, base + r.out_off, r.n * 4, hipMemcpyDeviceToHost, cs);
@@ -408,8 +435,8 @@ void gc_server::complete_fetch() {
     if (hipEventElapsedTime(&ms, fetch_start, fetch_done) == hipSuccess) stats.t_fetch += ms * 1e-3;
     std::lock_guard<std::mutex> g(mu);
     const double t = now_s();
-    for (uint64_t id : fetching) {
-        Req& r = *reqs[id];
+    for (Req* rp : fetching) {
+        Req& r = *rp;
         r.state = r.error ? FAILED : DONE;
         r.t_done = t;
         alloc.release(r.dev_off, r.dev_len);
@@ -482,7 +509,6 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
     const size_t meta_bytes = (size_t)s->max_batch_items * 12 + 64;
     s->meta_cap = meta_bytes;
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, meta_bytes, 0);
-    if (e == hipSuccess && s->meta.ensure(meta_bytes)) e = hipErrorOutOfMemory;
     if (e != hipSuccess) {
         hip_fail(e, "gc_server_start");
         if (s->pool) (void)hipFree(s->pool);
@@ -509,6 +535,15 @@ int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uin
     auto r = std::make_unique<Req>();
     r->dst = out;
     r->dst_cap = out_bytes;
+    if (out) {
+        // pinned host memory (hipHostMalloc) has a device address: results are written there by
+        // the kernels; other memory gets DMA copies
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, out) == hipSuccess && at.type == hipMemoryTypeHost && at.devicePointer)
+            r->dst_dev = at.hostPointer ? (uint8_t*)at.devicePointer + ((uint8_t*)out - (uint8_t*)at.hostPointer)
+                                        : at.devicePointer;
+        (void)hipGetLastError();  // pageable memory reports an error here: not one of ours
+    }
     r->dst_counts = seq_counts;
     r->data = data;
     r->n = n;
@@ -628,8 +663,6 @@ int gc_server_stop(gc_server* s) {
     if (s->batch_done) (void)hipEventDestroy(s->batch_done);
     if (s->fetch_done) (void)hipEventDestroy(s->fetch_done);
     s->items.release();
-    s->counts.release();
-    s->meta.release();
     if (s->meta_host) (void)hipHostFree(s->meta_host);
     if (s->pool) (void)hipFree(s->pool);
     delete s;

@@ -408,12 +408,17 @@ __device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) 
 // reference layout (slot 0 of LzmaMatch[max_matches] and counts).
 // ---------------------------------------------------------------------------
 // One workgroup of INFO_THREADS per sub-block, staged in LDS first (as for
-// the Zstd info words below).
-__device__ __forceinline__ void lzma2_item(const gc_item& it, const LdsBytes& src, const uint16_t* cand16,
+// the Zstd info words below).  LENGTHS = false: only whether the sub-block
+// has any match (the consumer's triage, compress_lzma2_custom.rs:577), no
+// lengths measured and nothing per position written.  Returns this thread's
+// "some position of mine has a match".
+template <bool LENGTHS>
+__device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& src, const uint16_t* cand16,
                                            uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                                            uint32_t max_matches) {
     const uint32_t n = it.n;
     const uint32_t depth = it.depth;
+    bool any = false;
     for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
         uint32_t d[4];
         d[0] = src.w[p0 / 4];
@@ -445,18 +450,22 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const LdsBytes& sr
                     if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
                 }
                 if (t != GC_NONE) {
-                    uint32_t len;
-                    if (next_t != GC_NONE && next_t == t + 1)
-                        len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
-                    else
-                        len = lcp_lane<true>(src, t, p, 2, maxlen);
-                    o = ((p - t) << 16) | len;
-                    next_len = len;
+                    any = true;
+                    if constexpr (LENGTHS) {
+                        uint32_t len;
+                        if (next_t != GC_NONE && next_t == t + 1)
+                            len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
+                        else
+                            len = lcp_lane<true>(src, t, p, 2, maxlen);
+                        o = ((p - t) << 16) | len;
+                        next_len = len;
+                    }
                 }
             }
             next_t = t;
             out[j] = o;
         }
+        if constexpr (!LENGTHS) continue;
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) {
             const uint32_t p = p0 + j;
@@ -475,21 +484,29 @@ __device__ __forceinline__ void lzma2_item(const gc_item& it, const LdsBytes& sr
         }
     }
     // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
-    if (ref_c)
+    if (LENGTHS && ref_c)
         for (uint32_t p = n + threadIdx.x; p < it.out_span; p += INFO_THREADS) ref_c[it.opos + p] = 0;
+    return any;
 }
 
 extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                     const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
-                    uint32_t max_matches) {
+                    uint32_t max_matches, uint8_t* triage) {
+    // triage (optional): one byte per item at it.count_pos, 1 if any position matches
     __shared__ uint32_t sb[STAGE_WORDS];
+    const bool lengths = packed || ref_c;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
         __syncthreads();  // the last item's reads of sb are done
         stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
         __syncthreads();
-        lzma2_item(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches);
+        const bool any = lengths ? lzma2_item<true>(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches)
+                                 : lzma2_item<false>(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches);
+        if (triage) {
+            const int wg_any = __syncthreads_or(any ? 1 : 0);
+            if (threadIdx.x == 0) triage[it.count_pos] = wg_any ? 1 : 0;
+        }
     }
 }
 
@@ -810,6 +827,22 @@ gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* __restrict__ items, ui
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// n u32 from src to dst (dst may be pinned host memory: the copy then runs
+// on the CUs over PCIe instead of on a DMA engine); vec4: both 16-byte
+// aligned.
+extern "C" __global__ void __launch_bounds__(256) gc_copy_u32(const uint32_t* src, uint32_t* dst, uint64_t n,
+                                                               uint32_t vec4) {
+    const uint64_t tid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t done = 0;
+    if (vec4) {
+        const uint64_t n4 = n / 4;
+        for (uint64_t i = tid; i < n4; i += stride) ((uint4*)dst)[i] = ((const uint4*)src)[i];
+        done = n4 * 4;
+    }
+    for (uint64_t i = done + tid; i < n; i += stride) dst[i] = src[i];
 }
 
 // Packs the used sequence slots of each sub-block back to back:

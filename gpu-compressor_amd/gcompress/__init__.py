@@ -86,6 +86,8 @@ SIGNATURES = [
     ("gc_batch_copy_zstd", C.c_int, [_vp, C.c_uint32, _vp, _vp]),
     ("gc_batch_copy_lzma2", C.c_int, [_vp, C.c_uint32, _vp]),
     ("gc_server_start", _vp, [_vp, C.c_uint64, C.c_uint32]),
+    ("gc_lzma2_triage", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("gc_lzma2_triage_dev", C.c_int, [_vp, _vp, C.c_uint64, _vp]),
     ("gc_server_submit", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_server_submit_into", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp, C.c_uint64, _vp]),
     ("gc_server_wait", C.c_int, [_vp, C.c_uint64, C.c_int, _vp]),
@@ -182,6 +184,13 @@ class Context:
 
     def __exit__(self, *exc):
         self.close()
+
+    def lzma2_triage(self, data) -> np.ndarray:
+        """One byte per 64 KiB sub-block: 1 if any position has an LZMA2 match (gc_lzma2_triage)."""
+        d = _as_u8(data)
+        out = np.zeros(max(-(-d.size // 65536), 1), np.uint8)
+        _check(lib().gc_lzma2_triage(self.h, _ptr(d), d.size, _ptr(out)), "gc_lzma2_triage")
+        return out[: -(-d.size // 65536)]
 
     # -- device memory ---------------------------------------------------
     def alloc(self, nbytes: int) -> int:

@@ -132,6 +132,12 @@ int gc_gpu_find_matches(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t* m
 
 /* Same matches, compact: packed[n] as in gc_lzma2_match_find_packed_dev. */
 int gc_lzma2_find_matches_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t* packed);
+/* Device-side triage: any_match[s] = 1 if some position of 64 KiB sub-block s
+ * has an LZMA2 match, else 0 (what encode_single_sub_block asks of the match
+ * lists, compress_lzma2_custom.rs:577).  No lengths are measured and one byte
+ * per sub-block comes back instead of the lists. */
+int gc_lzma2_triage(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* any_match);
+int gc_lzma2_triage_dev(gc_ctx* ctx, const uint8_t* d_input, uint64_t n, uint8_t* d_any_match);
 
 /* The sequences of compress_chunk_zstd_custom (compress_zstd_custom.rs:
  * 79-153) before entropy coding: seqs holds n_sub * 21845 records

@@ -47,6 +47,34 @@ def test_lzma2_packed_matches_oracle(gctx, name, data):
     assert bad.size == 0, f"{bad.size} positions differ, first {bad[:5]}: got {got[bad[:5]]} want {want[bad[:5]]}"
 
 
+def _triage_cases():
+    rng = np.random.default_rng(21)
+    out = [(name, data) for name, data in CASES]
+    # sub-blocks with and without matches side by side: random bytes have none, a zero run has
+    out.append(("rand+zero+rand", np.concatenate([rng.integers(0, 256, 65536, dtype=np.uint8),
+                                                  np.zeros(65536, np.uint8),
+                                                  rng.integers(0, 256, 100000, dtype=np.uint8)])))
+    # a single repeat of 5 bytes at the end of an otherwise match-free sub-block
+    lone = rng.permutation(256).astype(np.uint8)
+    lone = np.concatenate([lone, lone[:5]])
+    out.append(("lone-pair", lone))
+    return out
+
+
+TRIAGE = _triage_cases()
+
+
+@pytest.mark.parametrize("name,data", TRIAGE, ids=[c[0] for c in TRIAGE])
+def test_lzma2_triage_matches_oracle(gctx, name, data):
+    """Device-side triage: one byte per sub-block, 1 iff some position has a match (the oracle's lists)."""
+    print(f"[lzma2 triage] {name} n={data.size}", flush=True)
+    got = gctx.lzma2_triage(data)
+    want = oracle.lzma2_packed(data)
+    nsub = -(-data.size // 65536)
+    want_any = np.array([int(np.any(want[s * 65536:(s + 1) * 65536] != 0)) for s in range(nsub)], np.uint8)
+    np.testing.assert_array_equal(got, want_any)
+
+
 @pytest.mark.parametrize("level", [1, 2, 3])
 @pytest.mark.parametrize("name,data", CASES, ids=[c[0] for c in CASES])
 def test_zstd_sequences_match_oracle(gctx, name, data, level):

@@ -1,5 +1,7 @@
 """Served form: each request's result is the same alone and in a concurrent mix, equal to the
 oracle; admission under a tight HBM budget; oversized requests are rejected."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -120,5 +122,96 @@ def test_submit_into_caller_buffers(gctx):
         srv.release(ok)
         with pytest.raises(gcompress.GcError, match="4 bytes per input byte"):
             srv.submit_into(text, gcompress.ALGO_LZMA2, 1, np.zeros(10, np.uint32))
+    finally:
+        srv.stop()
+
+
+def _pinned(nbytes):
+    """Pinned host memory (hipHostMalloc) as a uint8 array, and its pointer for hipHostFree."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipHostMalloc.argtypes = [C.c_void_p, C.c_size_t, C.c_uint]
+    p = C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(p), nbytes, 0) == 0
+    return np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(p.value)), hip, p
+
+
+def test_submit_into_pinned_buffers(gctx):
+    """Pinned result buffers are written by the kernels (no DMA copy); same results as the oracle,
+    for buffers at offsets inside one pinned allocation."""
+    reqs = _requests(4, 12, 300000)
+    need = sum(max(d.size * 4, (d.size // 3 + 64) * 16) + 256 for d, _, _ in reqs)
+    ring, hip, ptr = _pinned(need)
+    srv = gcompress.Server(gctx, 1 << 30)
+    try:
+        print("[serve] submit_into pinned", flush=True)
+        outs, o = [], 16  # not 256-aligned: the copy kernel's unaligned path too
+        for data, algo, level in reqs:
+            nsub = -(-data.size // SUB)
+            if algo == gcompress.ALGO_LZMA2:
+                nb, counts = data.size * 4, None
+            else:
+                nb, counts = (data.size // 3 + 64) * 16, np.zeros(nsub, np.uint32)
+            buf = ring[o: o + nb].view(np.uint32)
+            buf[:] = 0xDEADBEEF
+            outs.append((srv.submit_into(data, algo, level, buf, counts), buf, counts))
+            o += nb + 256
+        for (rid, buf, counts), (data, algo, level) in zip(outs, reqs):
+            info = srv.wait(rid)
+            if algo == gcompress.ALGO_LZMA2:
+                np.testing.assert_array_equal(buf, oracle.lzma2_packed(data))
+            else:
+                oseq, osc, _, _ = oracle.zstd_match_find(data, depth={1: 16, 2: 64}[level])
+                np.testing.assert_array_equal(counts, osc)
+                want = oracle.zstd_as_zstd_sequences(oseq, osc)
+                want = np.concatenate(want) if want else np.zeros((0, 4), np.uint32)
+                assert info.n_sequences == want.shape[0]
+                np.testing.assert_array_equal(buf.reshape(-1, 4)[:want.shape[0]], want)
+            srv.release(rid)
+    finally:
+        srv.stop()
+        hip.hipHostFree.argtypes = [C.c_void_p]
+        hip.hipHostFree(ptr)
+
+
+def test_concurrent_clients(gctx):
+    """Clients on several threads submit, wait and release while the server thread batches (the server
+    must not look requests up in the shared table while clients change it)."""
+    import threading
+
+    srv = gcompress.Server(gctx, 256 << 20)
+    errors = []
+    rng = np.random.default_rng(8)
+    pool = gcompress.corpus(1 << 20, 88, [1, 1, 1, 1])
+    plans = []
+    for t in range(6):
+        plan = []
+        for i in range(150):
+            n = int(rng.integers(1, 40000))
+            o = int(rng.integers(0, pool.size - n)) & ~3
+            algo, level = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)][(t + i) % 3]
+            plan.append((pool[o:o + n], algo, level))
+        plans.append(plan)
+
+    def client(plan, check_every):
+        try:
+            for i, (data, algo, level) in enumerate(plan):
+                rid = srv.submit(data, algo, level)
+                if i % check_every == 0:
+                    _check(srv, rid, data, algo, level)  # waits, compares with the oracle, releases
+                else:
+                    srv.wait(rid)
+                    srv.release(rid)
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    try:
+        print("[serve] concurrent clients", flush=True)
+        ths = [threading.Thread(target=client, args=(p, 10)) for p in plans]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        assert not errors, errors[0]
+        assert srv.stats()["completed"] == 6 * 150
     finally:
         srv.stop()
