@@ -24,6 +24,7 @@ struct gc_batch {
     uint32_t n_z = 0, n_l = 0;       // items: LZMA2 at [0, n_l), Zstd at [n_l, n_l + n_z)
     uint64_t z_bytes = 0, l_bytes = 0;
     DevBuf items, cand, info, seq, counts, packed;
+    DevBuf cost, order;  // Zstd: per-item parse cost (verified positions) and the parse's item order
     gc_tune tune{};
     hipEvent_t ev[6] = {};
 };
@@ -66,14 +67,15 @@ int enqueue(gc_batch* b, bool timed) {
     if (timed) (void)hipEventRecord(b->ev[2], st);
     if (b->n_z) {
         rc = launch_zstd_info(ctx, st, b->arena, zit, b->n_z, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
-                              (uint32_t*)b->info.p, b->tune.info_strip);
+                              (uint32_t*)b->info.p, b->tune.info_strip, (uint32_t*)b->cost.p);
+        if (!rc) rc = launch_order(ctx, st, (const uint32_t*)b->cost.p, b->n_z, (uint32_t*)b->order.p);
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[3], st);
     if (b->n_z) {
         rc = launch_zstd_parse(ctx, st, b->arena, zit, b->n_z, grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
                                (const uint32_t*)b->info.p, b->seq.p, GC_SEQ_FORMAT_ZSTD, nullptr,
-                               (uint32_t*)b->counts.p, nullptr, kMaxSeq);
+                               (uint32_t*)b->counts.p, nullptr, kMaxSeq, (const uint32_t*)b->order.p);
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[4], st);
@@ -160,6 +162,8 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     if (!rc && b->n_z) rc = b->info.ensure(zpos * 4);
     if (!rc && b->n_z) rc = b->seq.ensure(zseq * sizeof(gc_zstd_sequence));
     if (!rc && b->n_z) rc = b->counts.ensure(zcnt * 4);
+    if (!rc && b->n_z) rc = b->cost.ensure((size_t)b->n_z * 4);
+    if (!rc && b->n_z) rc = b->order.ensure((size_t)b->n_z * 4);
     if (!rc && b->n_l) rc = b->packed.ensure(lpos * 4);
     for (int k = 0; !rc && k < 6; k++) {
         hipError_t e = hipEventCreate(&b->ev[k]);
@@ -186,6 +190,8 @@ int gc_batch_destroy(gc_batch* b) {
     b->seq.release();
     b->counts.release();
     b->packed.release();
+    b->cost.release();
+    b->order.release();
     delete b;
     return GC_OK;
 }

@@ -88,8 +88,8 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
 }
 
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip) {
-    void* args[] = {&arena, &items, &n_items, &cand16, &info};
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost) {
+    void* args[] = {&arena, &items, &n_items, &cand16, &info, &cost};
     // one 1024-thread workgroup per sub-block (the sub-block is staged in LDS)
     if (strip == 4) return launch(stream, ctx->k.zstd_info4, grid, 1024, args, "launch gc_zstd_info4_kernel");
     return launch(stream, ctx->k.zstd_info, grid, 1024, args, "launch gc_zstd_info_kernel");
@@ -97,8 +97,8 @@ int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
 
 int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                       uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
-                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq) {
-    void* args[] = {&arena, &items, &n_items, &info, &seq, &seq_fmt, &lit, &seq_counts, &lit_counts, &max_seq};
+                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq, const uint32_t* order) {
+    void* args[] = {&arena, &items, &n_items, &info, &seq, &seq_fmt, &lit, &seq_counts, &lit_counts, &max_seq, &order};
     return launch(stream, ctx->k.zstd_parse, grid, 256, args, "launch gc_zstd_parse_kernel");
 }
 
@@ -113,6 +113,11 @@ int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint3
                   void* d_out, uint32_t nsub, uint32_t max_seq) {
     void* args[] = {&d_in, &d_counts, &d_offsets, &d_out, &max_seq};
     return launch(stream, ctx->k.gather_seqs, nsub, 256, args, "launch gc_gather_seqs");
+}
+
+int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order) {
+    void* args[] = {&cost, &n, &order};
+    return launch(stream, ctx->k.order_items, 1, 1024, args, "launch gc_order_items");
 }
 
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -169,6 +174,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.copy_u32, ctx->k.module, "gc_copy_u32");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.order_items, ctx->k.module, "gc_order_items");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);

@@ -36,6 +36,7 @@ struct Kernels {
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
     hipFunction_t copy_u32 = nullptr;
+    hipFunction_t order_items = nullptr;
 };
 
 }  // namespace gc
@@ -65,13 +66,16 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
                      uint32_t max_matches,
                      uint8_t* triage = nullptr);
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip = 8);
+                     uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip = 8,
+                     uint32_t* cost = nullptr);
 int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                       uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
-                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq);
+                      uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq,
+                      const uint32_t* order = nullptr);
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);
+int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order);
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
 

@@ -170,6 +170,7 @@ struct gc_server {
     hipStream_t fetch_stream = nullptr;  // results back to the host (the context's stream2 brings inputs in)
     hipEvent_t batch_start = nullptr, batch_done = nullptr, fetch_start = nullptr, fetch_done = nullptr;
     DevBuf items;                     // the running batch's work items
+    DevBuf cost, order;               // Zstd items: parse cost and the parse's order (costliest first)
     uint32_t* counts_host = nullptr;  // pinned: per-batch sequence counts, written by the parse
     size_t counts_host_cap = 0;       // in u32
     gc_item* items_host = nullptr;    // pinned staging of `items`
@@ -268,6 +269,8 @@ bool gc_server::launch_batch() {
     const uint32_t n_all = (uint32_t)host_items.size(), n_z = n_all - n_l;
     std::lock_guard<std::mutex> g(ctx->mu);
     int rc = items.ensure(n_all * sizeof(gc_item));
+    if (!rc) rc = cost.ensure((size_t)n_z * 4 + 4);
+    if (!rc) rc = order.ensure((size_t)n_z * 4 + 4);
     // pinned staging (no batch runs now, so the last batch's reads and writes are done)
     if (!rc && n_z > counts_host_cap) {
         (void)hipHostFree(counts_host);
@@ -313,10 +316,12 @@ bool gc_server::launch_batch() {
                                   nullptr, nullptr, 1);
         if (!rc && n_z)
             rc = launch_zstd_info(ctx, ctx->stream, pool, all + n_l, n_z, n_z, (const uint16_t*)pool, (uint32_t*)pool,
-                                  4);
+                                  4, (uint32_t*)cost.p);
+        if (!rc && n_z) rc = launch_order(ctx, ctx->stream, (const uint32_t*)cost.p, n_z, (uint32_t*)order.p);
         if (!rc && n_z)
             rc = launch_zstd_parse(ctx, ctx->stream, pool, all + n_l, n_z, (n_z + 3) / 4, (const uint32_t*)pool,
-                                   pool, GC_SEQ_FORMAT_ZSTD, nullptr, dev_addr(counts_host), nullptr, kMaxSeq);
+                                   pool, GC_SEQ_FORMAT_ZSTD, nullptr, dev_addr(counts_host), nullptr, kMaxSeq,
+                                   (const uint32_t*)order.p);
         if (!rc && e == hipSuccess) e = hipEventRecord(batch_done, ctx->stream);
     }
     running = batch;
@@ -663,6 +668,8 @@ int gc_server_stop(gc_server* s) {
     if (s->batch_done) (void)hipEventDestroy(s->batch_done);
     if (s->fetch_done) (void)hipEventDestroy(s->fetch_done);
     s->items.release();
+    s->cost.release();
+    s->order.release();
     if (s->meta_host) (void)hipHostFree(s->meta_host);
     if (s->pool) (void)hipFree(s->pool);
     delete s;

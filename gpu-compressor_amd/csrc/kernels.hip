@@ -524,9 +524,11 @@ gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uin
 // not fit its L2).
 // ---------------------------------------------------------------------------
 
+// Returns how many of this thread's positions have a verified candidate.
 template <uint32_t W>
-__device__ __forceinline__ void zstd_info_lds(const LdsBytes& L, uint32_t n, const uint16_t* cand,
-                                              uint32_t* out) {
+__device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n, const uint16_t* cand,
+                                                  uint32_t* out) {
+    uint32_t verified = 0;
     // cand and out start at multiples of 8 positions and each item owns its
     // positions rounded up to 8: whole strips are read and written with
     // vector accesses
@@ -566,38 +568,73 @@ __device__ __forceinline__ void zstd_info_lds(const LdsBytes& L, uint32_t n, con
                     len = lcp_lane<true>(L, c[j], p, 3, lim);
             }
             next_len = len;
+            verified += len ? 1u : 0u;
             o[j] = (len << 16) | c[j];
         }
         uint4* q = (uint4*)(out + p0);
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
         if constexpr (W == 8) q[1] = make_uint4(o[4], o[5], o[6], o[7]);
     }
+    return verified;
 }
 
+// cost (optional): per item (at it.count_pos) the number of positions whose
+// candidate verified, which is what the parse's work grows with.
 template <uint32_t W>
 __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* __restrict__ items,
-                                                uint32_t n_items, const uint16_t* cand16, uint32_t* info) {
+                                                uint32_t n_items, const uint16_t* cand16, uint32_t* info,
+                                                uint32_t* cost) {
     __shared__ uint32_t sb[STAGE_WORDS];
+    __shared__ uint32_t verified;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
-        __syncthreads();  // the last item's reads of sb are done
+        __syncthreads();  // the last item's reads of sb and `verified` are done
+        if (threadIdx.x == 0) verified = 0;
         stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
         __syncthreads();
-        zstd_info_lds<W>(LdsBytes{sb}, it.n, cand16 + it.cpos, info + it.opos);
+        const uint32_t v = zstd_info_lds<W>(LdsBytes{sb}, it.n, cand16 + it.cpos, info + it.opos);
+        if (cost) {
+            if (v) atomicAdd(&verified, v);
+            __syncthreads();
+            if (threadIdx.x == 0) cost[it.count_pos] = verified;
+        }
     }
 }
 
 // Strips of 8 or 4 positions: the same words, different register use.
 extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_zstd_info_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
-                    const uint16_t* cand16, uint32_t* info) {
-    zstd_info_items<8>(arena, items, n_items, cand16, info);
+                    const uint16_t* cand16, uint32_t* info, uint32_t* cost) {
+    zstd_info_items<8>(arena, items, n_items, cand16, info, cost);
 }
 
 extern "C" __global__ void __launch_bounds__(INFO_THREADS)
 gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
-                     const uint16_t* cand16, uint32_t* info) {
-    zstd_info_items<4>(arena, items, n_items, cand16, info);
+                     const uint16_t* cand16, uint32_t* info, uint32_t* cost) {
+    zstd_info_items<4>(arena, items, n_items, cand16, info, cost);
+}
+
+// The parse's item order, costliest first (counting sort of cost >> 9 into
+// 129 buckets, descending): the sub-blocks that take longest start first and
+// the launch does not end on a few long ones.  One workgroup.
+#define ORDER_BUCKETS 129
+extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t* cost, uint32_t n, uint32_t* order) {
+    __shared__ uint32_t start[ORDER_BUCKETS];
+    for (uint32_t b = threadIdx.x; b < ORDER_BUCKETS; b += blockDim.x) start[b] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&start[min(cost[i] >> 9, ORDER_BUCKETS - 1u)], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int b = ORDER_BUCKETS - 1; b >= 0; b--) {
+            const uint32_t c = start[b];
+            start[b] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        order[atomicAdd(&start[min(cost[i] >> 9, ORDER_BUCKETS - 1u)], 1u)] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -786,14 +823,15 @@ template <uint32_t FMT, bool LITS>
 __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                  uint32_t n_items, const uint32_t* info, void* seq_out,
                                                  uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
-                                                 uint32_t max_seq) {
+                                                 uint32_t max_seq, const uint32_t* __restrict__ order) {
     // the wavefront index is uniform: items are read into scalar registers
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES) {
+        const uint32_t k = order ? order[i] : i;
 #if PARSE_UNIFORM_ITEM
-        const gc_item it = load_item(items, i);
+        const gc_item it = load_item(items, k);
 #else
-        const gc_item it = items[i];  // fields in vector registers: fewer scalar registers in use
+        const gc_item it = items[k];  // fields in vector registers: fewer scalar registers in use
 #endif
         if (item_safe(it.avail, it.n))
             zstd_parse_item<true, FMT, LITS>(it, arena, info, seq_out, literals, seq_counts, lit_counts, max_seq);
@@ -805,21 +843,22 @@ __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_
 extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
 gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,
                      void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
-                     uint32_t max_seq) {
+                     uint32_t max_seq, const uint32_t* __restrict__ order) {
+    // order (optional): the items in the order to take them
     if (seq_fmt == GC_SEQ_REF) {
         if (literals)
             zstd_parse_items<GC_SEQ_REF, true>(arena, items, n_items, info, seq_out, literals, seq_counts, lit_counts,
-                                               max_seq);
+                                               max_seq, order);
         else
             zstd_parse_items<GC_SEQ_REF, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                lit_counts, max_seq);
+                                                lit_counts, max_seq, order);
     } else {
         if (literals)
             zstd_parse_items<GC_SEQ_ZSTD, true>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                lit_counts, max_seq);
+                                                lit_counts, max_seq, order);
         else
             zstd_parse_items<GC_SEQ_ZSTD, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                 lit_counts, max_seq);
+                                                 lit_counts, max_seq, order);
     }
 }
 
