@@ -94,6 +94,81 @@ def cpu_baseline(data: np.ndarray, inputs, sample_bytes: int):
                       f"oracle on {cores} threads"}
 
 
+def cpu_baselines_more(data: np.ndarray, workload: str, sample_bytes: int):
+    """The other CPU baselines, on a bounded sample of the workload's data (first bytes):
+    Zstd: the oracle on 1 core, and oracle match finding (all cores) + ZSTD_compressSequences with 1 encode
+    thread (the reference's shape) and with all cores;  LZMA2: the oracle on 1 core, oracle + triage +
+    liblzma preset 9 on a pool of min(cores, 8) (the reference's encoder), and level 0 (liblzma raw encoder,
+    min(cores, 8) workers on dictionary chunks).  GB/s of input."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    L = gcompress.lib()
+    cores = min(os.cpu_count() or 1, 16)
+    pool_n = min(cores, 8)
+    x = np.ascontiguousarray(data[:sample_bytes])
+    nsub = -(-x.size // 65536)
+    out = {"sample_mib": x.size // MiB, "cores": cores}
+
+    def rate(t):
+        return round(x.size / t / 1e9, 4)
+
+    if workload != "lzma2":
+        t = time.perf_counter()
+        oracle.zstd_match_find(x, depth=16, threads=1)
+        out["zstd_matchfind_1core_gbs"] = rate(time.perf_counter() - t)
+        t = time.perf_counter()
+        oseq, osc, _, _ = oracle.zstd_match_find(x, depth=16, threads=cores)
+        find_s = time.perf_counter() - t
+        z = oracle.zstd_as_zstd_sequences(oseq, osc)
+        seqs = np.ascontiguousarray(np.concatenate(z) if z else np.zeros((1, 4), np.uint32))
+        counts = np.ascontiguousarray(osc.astype(np.uint32))
+        cap = L.gc_zstd_custom_bound(x.size)
+        buf = np.empty(cap, np.uint8)
+        sizes = np.empty(nsub, np.uint64)
+        olen = C.c_size_t()
+        for threads in (1, cores):
+            t = time.perf_counter()
+            rc = L.gc_zstd_encode_sequences(x.ctypes.data, x.size, seqs.ctypes.data, counts.ctypes.data,
+                                            buf.ctypes.data, cap, sizes.ctypes.data, C.byref(olen), threads)
+            assert rc == 0, L.gc_last_error()
+            enc_s = time.perf_counter() - t
+            out[f"zstd_find_and_encode_{threads}_encode_threads_gbs"] = rate(find_s + enc_s)
+        out["zstd_ratio"] = round(x.size / olen.value, 4)
+    if workload != "zstd":
+        t = time.perf_counter()
+        oracle.lzma2_match_find(x, max_matches=1, threads=1)
+        out["lzma2_matchfind_1core_gbs"] = rate(time.perf_counter() - t)
+        t = time.perf_counter()
+        _, c = oracle.lzma2_match_find(x, max_matches=1, threads=cores)
+        counts = np.ascontiguousarray(c)
+
+        def enc(s):
+            st = s * 65536
+            ln = min(65536, x.size - st)
+            b = np.empty(2 * ln + 1024, np.uint8)
+            n_out = C.c_size_t()
+            rc = L.gc_encode_single_sub_block(x[st:].ctypes.data, ln, counts[st:].ctypes.data, b.ctypes.data,
+                                              b.size, C.byref(n_out))
+            assert rc == 0, L.gc_last_error()
+            return n_out.value
+
+        with ThreadPoolExecutor(pool_n) as ex:
+            total = sum(ex.map(enc, range(nsub)))
+        out["lzma2_find_triage_preset9_gbs"] = rate(time.perf_counter() - t)
+        out["lzma2_preset9_ratio"] = round(x.size / total, 4)
+        cap = L.gc_nvlz_l0_bound(x.size, 128)
+        buf = np.empty(cap, np.uint8)
+        olen = C.c_size_t()
+        t = time.perf_counter()
+        rc = L.gc_nvlz_compress_l0(x.ctypes.data, x.size, 128, 0, pool_n, buf.ctypes.data, cap, C.byref(olen))
+        assert rc == 0, L.gc_last_error()
+        out["lzma2_level0_gbs"] = rate(time.perf_counter() - t)
+    return out
+
+
 def workload_inputs(a):
     """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
     if a.workload == "zstd":
@@ -263,6 +338,9 @@ def main():
     traffic = int(per_byte * work_bytes) if per_byte is not None else None
 
     cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
+    cpu_more = None
+    if not a.no_cpu and a.workload in ("zstd", "lzma2"):
+        cpu_more = cpu_baselines_more(data, a.workload, (64 if a.workload == "zstd" else 16) * MiB)
     levels = sorted({lv for (_, _, al, lv) in inputs if al == gcompress.ALGO_ZSTD})
     out = {
         "metric": "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency",
@@ -297,6 +375,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,
+        "cpu_baselines_more": cpu_more,
         "corpus_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)
