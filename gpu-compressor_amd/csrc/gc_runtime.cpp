@@ -111,8 +111,10 @@ int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64
 
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq) {
-    void* args[] = {&d_in, &d_counts, &d_offsets, &d_out, &max_seq};
-    return launch(stream, ctx->k.gather_seqs, nsub, 256, args, "launch gc_gather_seqs");
+    // at most 64 workgroups: when d_out is host memory the copy is PCIe-bound, and the CUs stay
+    // free for the match-finding kernels running beside it
+    void* args[] = {&d_in, &d_counts, &d_offsets, &d_out, &max_seq, &nsub};
+    return launch(stream, ctx->k.gather_seqs, nsub < 64 ? nsub : 64, 256, args, "launch gc_gather_seqs");
 }
 
 int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order) {
@@ -122,10 +124,10 @@ int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t
 
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n) {
     uint32_t vec4 = (((uintptr_t)src | (uintptr_t)dst) & 15) == 0 ? 1u : 0u;
-    // a few hundred wavefronts keep PCIe busy when dst is host memory, and leave the CUs to the
-    // match-finding kernels running beside them
+    // 32 workgroups keep PCIe busy when dst is host memory (the writes are posted), and leave the
+    // CUs to the match-finding kernels running beside them
     uint64_t blocks = (n / 4 + 255) / 256;
-    uint32_t grid = (uint32_t)(blocks < 256 ? (blocks ? blocks : 1) : 256);
+    uint32_t grid = (uint32_t)(blocks < 32 ? (blocks ? blocks : 1) : 32);
     void* args[] = {&src, &dst, &n, &vec4};
     return launch(stream, ctx->k.copy_u32, grid, 256, args, "launch gc_copy_u32");
 }

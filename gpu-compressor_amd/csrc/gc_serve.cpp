@@ -407,8 +407,8 @@ void gc_server::finish_batch(std::vector<Req*>& done) {
         Req& r = *rp;
         if (r.error) continue;
         uint8_t* base = pool + r.dev_off;
-        // pinned caller memory is written by the kernels directly (over PCIe, on the CUs);
-        // anything else is packed on the device and copied by DMA
+        // pinned caller memory is written by the gather / copy kernels directly (over PCIe, on a
+        // few hundred wavefronts); anything else is packed on the device and copied by DMA
         if (r.algo == GC_ALGO_ZSTD) {
             if (!r.dst) r.seqs.resize(r.fetch_total);
             if (!r.fetch_total) continue;

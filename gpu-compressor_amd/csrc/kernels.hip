@@ -887,10 +887,12 @@ extern "C" __global__ void __launch_bounds__(256) gc_copy_u32(const uint32_t* sr
 // Packs the used sequence slots of each sub-block back to back:
 // out[offsets[s] + i] = in[s * max_seq + i] for i < counts[s].
 extern "C" __global__ void __launch_bounds__(256)
-gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets, uint4* out, uint32_t max_seq) {
-    const uint32_t s = blockIdx.x;
-    const uint32_t c = counts[s];
-    const uint4* src = in + (uint64_t)s * max_seq;
-    uint4* dst = out + offsets[s];
-    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
+gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets, uint4* out, uint32_t max_seq,
+               uint32_t nsub) {
+    for (uint32_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+        const uint32_t c = counts[s];
+        const uint4* src = in + (uint64_t)s * max_seq;
+        uint4* dst = out + offsets[s];
+        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
+    }
 }
