@@ -169,6 +169,58 @@ def cpu_baselines_more(data: np.ndarray, workload: str, sample_bytes: int):
     return out
 
 
+def equal_ratio_check(ctx, batch, data, inputs, d_in, sample_bytes):
+    """Batched results against the plain per-chunk form on a sample of inputs (first inputs of each
+    algorithm/level, up to sample_bytes each); the Zstd ratio from the GPU's own sequences through
+    ZSTD_compressSequences. Identical sequences and matches give identical compressed output."""
+    L = gcompress.lib()
+    max_seq = 65536 // 3
+    picked, seen = [], {}
+    for i, (o, s, algo, lv) in enumerate(inputs):
+        key = (algo, lv)
+        if seen.get(key, 0) < sample_bytes:
+            seen[key] = seen.get(key, 0) + s
+            picked.append(i)
+    cmp_bytes = 0
+    zin = zout = 0
+    identical = True
+    for i in picked:
+        o, s, algo, lv = inputs[i]
+        ns = -(-s // 65536)
+        if algo == gcompress.ALGO_ZSTD:
+            d_s, d_c = ctx.alloc(ns * max_seq * 16), ctx.alloc(ns * 4)
+            rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_s, 0, None, d_c, None, 65536, ns, s,
+                                          {1: 16, 2: 64}.get(lv, 1), max_seq)
+            assert rc == 0, L.gc_last_error()
+            ps = np.empty((ns, max_seq, 4), np.uint32)
+            pc = np.empty(ns, np.uint32)
+            ctx.dtoh(ps, d_s)
+            ctx.dtoh(pc, d_c)
+            ctx.free(d_s)
+            ctx.free(d_c)
+            bs, bc = batch.zstd(i)
+            same = np.array_equal(pc, bc) and all(np.array_equal(ps[b, :pc[b]], bs[b, :bc[b]]) for b in range(ns))
+            seqs = np.concatenate([bs[b, :bc[b]] for b in range(ns)] + [np.zeros((1, 4), np.uint32)])
+            _, sizes = gcompress.zstd_encode_sequences(data[o:o + s], seqs, bc, threads=min(os.cpu_count() or 1, 16))
+            zin += s
+            zout += sum(sizes)
+        else:
+            d_p = ctx.alloc(4 * s + 64)
+            rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_p, 65536, s, 32)
+            assert rc == 0, L.gc_last_error()
+            pp = np.empty(s, np.uint32)
+            ctx.dtoh(pp, d_p)
+            ctx.free(d_p)
+            same = np.array_equal(pp, batch.lzma2(i))
+        identical &= bool(same)
+        cmp_bytes += s
+    assert identical, "batched results differ from the plain per-chunk form"
+    out = {"inputs_compared": len(picked), "bytes_compared": cmp_bytes, "identical_to_plain": identical}
+    if zin:
+        out["zstd_ratio_from_gpu_sequences"] = round(zin / zout, 4)
+    return out
+
+
 def workload_inputs(a):
     """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
     if a.workload == "zstd":
@@ -223,6 +275,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check-sample-mib", type=int, default=32,
+                    help="bytes per algorithm/level compared against the plain form and encoded for the ratio")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="one device-to-device copy of this size first, to scale HBM byte counters")
     a = ap.parse_args()
@@ -337,6 +391,7 @@ def main():
     per_byte = hbm_traffic_per_input_byte(a.workload, dom_name)
     traffic = int(per_byte * work_bytes) if per_byte is not None else None
 
+    equal = equal_ratio_check(ctx, batch, data, inputs, d_in, a.check_sample_mib * MiB)
     cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
     cpu_more = None
     if not a.no_cpu and a.workload in ("zstd", "lzma2"):
@@ -371,6 +426,7 @@ def main():
         "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
         "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)},
         "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq},
+        "equal_ratio": equal,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "algorithmic_bytes": dom_bytes},

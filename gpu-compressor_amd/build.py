@@ -4,7 +4,8 @@
     python gpu-compressor_amd/build.py --lib-only
 
 Steps:
-  1. hipcc --genco --offload-arch=gfx950 csrc/kernels.hip -> _build/kernels.hsaco
+  1. hipcc --genco --offload-arch=gfx950 csrc/kernels.hip -> _build/kernels.hsaco and
+     _build/kernels_parse.hsaco (the parse kernels, built with PARSE_FLAGS)
   2. g++ the host sources (the code object is embedded with .incbin and
      loaded at run time with hipModuleLoadData) -> libgcompress.so
   3. gcc oracle/gc_oracle.c -> oracle/liboracle.so  (test infrastructure)
@@ -50,22 +51,33 @@ def run(cmd):
     subprocess.check_call(cmd)
 
 
-def build_code_object():
-    os.makedirs(BUILD, exist_ok=True)
-    out = os.path.join(BUILD, "kernels.hsaco")
+# The parse kernels' code object: uniform branches are left unstructurized
+# (LLVM's StructurizeCFG otherwise rebuilds the walk's uniform control flow
+# with extra flow blocks, whose phis become register copies on every path).
+PARSE_FLAGS = ["-mllvm", "-structurizecfg-skip-uniform-regions"]
+
+
+def build_code_objects(flags=(), defines=(), out_dir=BUILD):
+    """(main, parse) code objects from kernels.hip."""
+    os.makedirs(out_dir, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in KERNEL_SOURCES]
-    run([HIPCC, "--genco", *HIP_FLAGS, "-I", CSRC, *srcs, "-o", out])
-    return out
+    defs = [f"-D{d}" for d in defines]
+    main_co = os.path.join(out_dir, "kernels.hsaco")
+    parse_co = os.path.join(out_dir, "kernels_parse.hsaco")
+    run([HIPCC, "--genco", *HIP_FLAGS, *flags, *defs, "-DGC_BUILD_PARSE=0", "-I", CSRC, *srcs, "-o", main_co])
+    run([HIPCC, "--genco", *HIP_FLAGS, *PARSE_FLAGS, *flags, *defs, "-DGC_BUILD_MAIN=0", "-I", CSRC, *srcs,
+         "-o", parse_co])
+    return main_co, parse_co
 
 
-def build_library(co_path):
+def build_library(cos):
     objs = []
     for s in HOST_SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
         run([
             "g++", "-c", "-O2", "-g", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
             "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-I", CSRC,
-            f'-DGC_CO_PATH="{co_path}"',
+            f'-DGC_CO_PATH="{cos[0]}"', f'-DGC_PARSE_CO_PATH="{cos[1]}"',
             os.path.join(CSRC, s), "-o", obj,
         ])
         objs.append(obj)
@@ -88,8 +100,7 @@ def main(argv=None):
     ap.add_argument("--oracle-only", action="store_true")
     a = ap.parse_args(argv)
     if not a.oracle_only:
-        co = build_code_object()
-        build_library(co)
+        build_library(build_code_objects())
     if not a.lib_only:
         build_oracle()
 

@@ -6,10 +6,11 @@
 
 #include "../../include/gcompress.h"
 
-// The gfx950 code object built from kernels.hip is linked into this library
-// and loaded with hipModuleLoadData (no files, no JIT at run time).
-#ifndef GC_CO_PATH
-#error "GC_CO_PATH must name the code object to embed"
+// The gfx950 code objects built from kernels.hip (the parse kernels, and
+// the rest) are linked into this library and loaded with hipModuleLoadData
+// (no files, no JIT at run time).
+#if !defined(GC_CO_PATH) || !defined(GC_PARSE_CO_PATH)
+#error "GC_CO_PATH and GC_PARSE_CO_PATH must name the code objects to embed"
 #endif
 __asm__(".section .rodata\n"
         ".balign 4096\n"
@@ -17,8 +18,14 @@ __asm__(".section .rodata\n"
         "gc_code_object:\n"
         ".incbin \"" GC_CO_PATH "\"\n"
         ".byte 0\n"
+        ".balign 4096\n"
+        ".globl gc_parse_code_object\n"
+        "gc_parse_code_object:\n"
+        ".incbin \"" GC_PARSE_CO_PATH "\"\n"
+        ".byte 0\n"
         ".previous\n");
 extern "C" const unsigned char gc_code_object[];
+extern "C" const unsigned char gc_parse_code_object[];
 
 namespace gc {
 
@@ -99,7 +106,8 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
                       uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
                       uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq, const uint32_t* order) {
     void* args[] = {&arena, &items, &n_items, &info, &seq, &seq_fmt, &lit, &seq_counts, &lit_counts, &max_seq, &order};
-    return launch(stream, ctx->k.zstd_parse, grid, 256, args, "launch gc_zstd_parse_kernel");
+    const int ref = seq_fmt == GC_SEQ_REF ? 1 : 0, lits = lit ? 1 : 0;
+    return launch(stream, ctx->k.zstd_parse[ref][lits], grid, 256, args, "launch gc_zstd_parse_kernel");
 }
 
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n) {
@@ -172,7 +180,12 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len, ctx->k.module, "gc_lzma2_len_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info, ctx->k.module, "gc_zstd_info_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info4, ctx->k.module, "gc_zstd_info4_kernel");
-    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse, ctx->k.module, "gc_zstd_parse_kernel");
+    if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module_parse, gc_parse_code_object);
+    const char* parse_names[2][2] = {{"gc_zstd_parse_kernel", "gc_zstd_parse_lits_kernel"},
+                                     {"gc_zstd_parse_ref_kernel", "gc_zstd_parse_ref_lits_kernel"}};
+    for (int r = 0; r < 2; r++)
+        for (int l = 0; l < 2; l++)
+            if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse[r][l], ctx->k.module_parse, parse_names[r][l]);
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.copy_u32, ctx->k.module, "gc_copy_u32");
@@ -198,6 +211,7 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_c.release();
     ctx->out_d.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
+    if (ctx->k.module_parse) (void)hipModuleUnload(ctx->k.module_parse);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);

@@ -27,12 +27,13 @@ struct DevBuf {
 
 // Loaded kernels.
 struct Kernels {
-    hipModule_t module = nullptr;
+    hipModule_t module = nullptr;        // every kernel but the parse
+    hipModule_t module_parse = nullptr;  // the Zstd parse kernels
     hipFunction_t cand = nullptr;
     hipFunction_t lzma2_len = nullptr;
     hipFunction_t zstd_info = nullptr;
     hipFunction_t zstd_info4 = nullptr;
-    hipFunction_t zstd_parse = nullptr;
+    hipFunction_t zstd_parse[2][2] = {};  // [reference layout][literals]
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
     hipFunction_t copy_u32 = nullptr;

@@ -32,6 +32,17 @@
 
 #include "gc_device.h"
 
+// Two code objects are built from this file: the parse kernels
+// (GC_BUILD_MAIN=0), whose walk is compiled without structurizing uniform
+// branches (fewer register copies on the loop's paths), and the others
+// (GC_BUILD_PARSE=0).  A plain compile has both.
+#ifndef GC_BUILD_MAIN
+#define GC_BUILD_MAIN 1
+#endif
+#ifndef GC_BUILD_PARSE
+#define GC_BUILD_PARSE 1
+#endif
+
 #define GC_HASH_BITS 14
 #define GC_TABLE (1u << GC_HASH_BITS)
 #define GC_NONE 0xFFFFu
@@ -138,6 +149,14 @@ __device__ __forceinline__ uint32_t lane_id() {
 
 __device__ __forceinline__ uint64_t ballot(bool b) { return __builtin_amdgcn_ballot_w64(b); }
 
+// A wavefront-uniform value kept in a vector register, and back.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
+    uint32_t v;
+    __asm__("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
@@ -236,6 +255,7 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
     return len < limit ? len : limit;
 }
 
+#if GC_BUILD_MAIN
 // ---------------------------------------------------------------------------
 // Candidate kernel: cand16[it.cpos + p] = cand(p) (GC_NONE when there is none
 // or p + 3 >= n) for every position of every item.
@@ -637,6 +657,9 @@ extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t
         order[atomicAdd(&start[min(cost[i] >> 9, ORDER_BUCKETS - 1u)], 1u)] = i;
 }
 
+#endif  // GC_BUILD_MAIN
+
+#if GC_BUILD_PARSE
 // ---------------------------------------------------------------------------
 // Zstd parse (zstd_compress.cu:230-338), one wavefront per sub-block.
 //
@@ -669,17 +692,55 @@ __device__ __forceinline__ uint32_t rep_eq(const SrcView& src, uint32_t x, uint3
     return (ok && v == w) ? 1u : 0u;
 }
 
-template <bool SAFE, uint32_t FMT, bool LITS>
+// The parse's recent history in LDS: a ring of PARSE_RING bytes per
+// wavefront, byte p of the sub-block at index p % PARSE_RING, plus a copy of
+// its first 8 bytes past the end so that a dword pair read at any index stays
+// inside.  While the walk is in the window [wbase, wbase + 64) the ring holds
+// every byte of [wbase + 72 - PARSE_RING, wbase + 66) (the last lane's
+// 3-byte word gives the 2 bytes past the window): a repeat offset of at most
+// RING_REACH is compared for every lane of the window without a global load,
+// which on this path is a dependent round trip to HBM.
+#define PARSE_RING 4096u
+#define RING_REACH (PARSE_RING - WAVE - 8)
+
+#define RING_BYTES (PARSE_RING + 16)  // + the copy of bytes 0..7 + a junk slot
+#define RING_JUNK (PARSE_RING + 8)
+
+// Writes of lanes that have nothing to write go to the junk slot: no
+// branches around LDS accesses, whose waits would then not overlap.
+__device__ __forceinline__ void ring_put8(uint8_t* ring, uint32_t p, uint32_t byte) {
+    const uint32_t j = p & (PARSE_RING - 1);
+    ring[j] = (uint8_t)byte;
+    ring[j < 8 ? PARSE_RING + j : RING_JUNK] = (uint8_t)byte;
+}
+
+// 2 bytes at p, a multiple of 64, from the lanes with `on` set.
+__device__ __forceinline__ void ring_put16(uint8_t* ring, uint32_t p, uint32_t v, bool on) {
+    const uint32_t j = p & (PARSE_RING - 1);
+    *(uint16_t*)(ring + (on ? j : RING_JUNK)) = (uint16_t)v;
+    *(uint16_t*)(ring + (on && j == 0 ? PARSE_RING : RING_JUNK)) = (uint16_t)v;
+}
+
+// The 3 bytes at p (little endian) from the ring.
+__device__ __forceinline__ uint32_t ring_w24(const uint8_t* ring, uint32_t p) {
+    const uint32_t i = p & (PARSE_RING - 1);
+    const uint32_t* r = (const uint32_t*)(ring + (i & ~3u));
+    return __builtin_amdgcn_alignbyte(r[1], r[0], i & 3) & 0xFFFFFFu;
+}
+
+template <bool SAFE, uint32_t FMT, bool LITS, bool LAZY>
 __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t* arena, const uint32_t* info,
                                                 void* seq_out, uint8_t* literals, uint32_t* seq_counts,
-                                                uint32_t* lit_counts, uint32_t max_seq) {
+                                                uint32_t* lit_counts, uint32_t max_seq, uint8_t* ring) {
     const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
-    const uint32_t lazy = it.depth > 1 ? 1u : 0u;
     const uint32_t lane = lane_id();
     const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
-    uint32_t pos = 0, lit_start = 0, nseq = 0, nlit = 0, wbase = 0;
+    uint32_t pos = 0, nseq = 0, wbase = 0;
+    // literal bookkeeping in vector registers (the same in every lane): the
+    // scalar register count limits this kernel's occupancy
+    uint32_t lit_start = in_vgpr(0u), nlit = in_vgpr(0u);
     uint8_t* lit_dst = LITS ? literals + it.lit_pos : nullptr;
 
     // staged sequences: lane j holds sequence (nseq & ~63) + j
@@ -696,126 +757,208 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
     };
 
-    uint32_t e = 0, w = 0, f = 0, ae = 0, aw = 0;
-    auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& wv) {
+    // per lane, for x = wbase + lane: e = info(x), e1 = info(x + 1) (the
+    // lazy step's word), w = the 3 bytes at x (0 past the end; the low byte
+    // goes to the ring), f = repeat flags; ae, ae1, aw the same for the next
+    // window.  Info words of positions with x + 3 >= n read as 0.
+    uint32_t e = 0, e1 = 0, w = 0, f = 0, ae = 0, ae1 = 0, aw = 0;
+    uint32_t xv = lane;  // wbase + lane
+    auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& ev1, uint32_t& wv) {
         const bool act = x + 3 < n;
-        const uint32_t xc = act ? x : 0;  // reads stay in the sub-block
-        const uint32_t i0 = *(const uint32_t*)(inf + (xc << 2));
-        const uint32_t w0 = ld32<SAFE>(src, xc) & 0xFFFFFFu;
+        const bool in = x < n;
+        const uint32_t xc = act ? x : 0;  // reads stay in the sub-block (x + 1 < n here)
+        const uint32_t* ip = (const uint32_t*)(inf + (xc << 2));
+        const uint32_t i0 = ip[0];
+        const uint32_t w0 = ld32<SAFE>(src, in ? x : 0u) & 0xFFFFFFu;
         ev = act ? i0 : 0;
-        wv = act ? w0 : 0;
+        wv = in ? w0 : 0;
+        if constexpr (LAZY) {
+            const uint32_t i1 = ip[1];
+            ev1 = x + 4 < n ? i1 : 0;
+        }
     };
+    // the repeat flags of the window at xv - lane, whose bytes are in the ring
     auto flags = [&](uint32_t x, uint32_t wv) -> uint32_t {
         const bool act = x + 3 < n;
-        const uint32_t xc = act ? x : 0;
-        const uint32_t fv = rep_eq<SAFE>(src, xc, rep0, wv) | (rep_eq<SAFE>(src, xc, rep1, wv) << 1) |
-                            (rep_eq<SAFE>(src, xc, rep2, wv) << 2);
+        uint32_t fv;
+        if (max(rep0, max(rep1, rep2)) <= RING_REACH) {
+            // every lane reads (indexes wrap inside the ring), then masks
+            const uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1),
+                           v2 = ring_w24(ring, x - rep2);
+            fv = ((x >= rep0) & (v0 == wv) ? 1u : 0u) | ((x >= rep1) & (v1 == wv) ? 2u : 0u) |
+                 ((x >= rep2) & (v2 == wv) ? 4u : 0u);
+        } else {
+            const uint32_t xc = act ? x : 0;
+            const uint32_t wc = wv & 0xFFFFFFu;
+            fv = rep_eq<SAFE>(src, xc, rep0, wc) | (rep_eq<SAFE>(src, xc, rep1, wc) << 1) |
+                 (rep_eq<SAFE>(src, xc, rep2, wc) << 2);
+        }
         return act ? fv : 0;
     };
-    if (n > 3) {
-        peek(lane, e, w);
+    // entering the window at nb (w already holds its bytes): the bytes from
+    // `from` (a multiple of 64) up to nb are fetched into the ring first
+    auto enter = [&](uint32_t nb, uint32_t from) {
+        for (uint32_t p0 = from; p0 < nb; p0 += 4 * WAVE) {
+            const uint32_t p = p0 + 4 * lane;
+            if (p < nb) {  // p + 3 < nb <= n
+                const uint32_t v = ld32<SAFE>(src, p);
+                const uint32_t j = p & (PARSE_RING - 1);
+                *(uint32_t*)(ring + j) = v;
+                if (j < 8) *(uint32_t*)(ring + PARSE_RING + j) = v;
+            }
+        }
+        ring_put8(ring, nb + lane, w);
+        ring_put16(ring, nb + WAVE, w >> 8, lane == WAVE - 1);
+        __asm__ volatile("" ::: "memory");  // ring writes before the reads
+    };
+    // heads satisfy pos + 3 < n; the walk ends by lowering lim
+    uint32_t lim = (n > 3 && max_seq > 0) ? n - 3 : 0;
+    if (lim) {
+        peek(lane, e, e1, w);
+        enter(0, 0);
         f = flags(lane, w);
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
-        peek(WAVE + lane, ae, aw);
+        peek(WAVE + lane, ae, ae1, aw);
     }
-
-    // One loop exit (the condition): the walk ends by lowering `limit`.
-    uint32_t limit = n;
-    while (pos + 3 < limit) {
-        const uint32_t lo = pos - wbase;
-        const uint32_t cand = lane >= lo ? ((e >> 16) | f) : 0u;
+    // next sequence count at which the staged records are stored or the
+    // walk stops (max_seq)
+    uint32_t ev_at = max_seq < WAVE ? max_seq : WAVE;
+    uint32_t wend = WAVE;
+    while (pos < lim) {
+        const uint32_t cand = xv >= pos ? ((e >> 16) | f) : 0u;
         const uint64_t m = ballot(cand != 0);
         if (!m) {
             // no head left in the window: every position up to its end is a literal
-            wbase += WAVE;
+            wbase = wend;
+            wend += WAVE;
             pos = wbase;
+            xv += WAVE;
             e = ae;
+            e1 = ae1;
             w = aw;
-            f = flags(wbase + lane, w);
+            enter(wbase, wbase);
+            f = flags(xv, w);
             __asm__ volatile("" ::: "memory");
-            peek(wbase + WAVE + lane, ae, aw);
-        } else if (nseq == max_seq) {
-            limit = 0;
+            peek(wend + lane, ae, ae1, aw);
         } else {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
             const uint32_t ek = bcast(e, k);
-            uint32_t len, off;
+            const uint32_t fk = bcast(f, k);
+            uint32_t len, off, sel;
             if (ek > 0xFFFFu) {
                 // hash candidate verified
                 const uint32_t c = ek & 0xFFFFu;
                 off = head - c;
                 len = ek >> 16;
                 if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, c, head, n - head, GC_ZSTD_SHORT);
+                // an offset equal to a repeat offset r has r's flag set at head
+                // (the 3 bytes verified): with no flag set it is a new one
+                sel = 3;
+                if (fk) sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
             } else {
                 // the first repeat offset that verifies (rep0, rep1, rep2 in order)
-                const uint32_t fk = bcast(f, k);
+                sel = (fk & 1) ? 0u : ((fk & 2) ? 1u : 2u);
                 off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
-                len = lcp_wave<SAFE>(src, head - off, head, n - head, 3);
+                if (off <= RING_REACH) {
+                    // the bytes up to the window end are in the ring
+                    const uint32_t span = (wend < n ? wend : n) - head;
+                    const uint32_t ba = ring[(head - off + lane) & (PARSE_RING - 1)];
+                    const uint32_t bb = ring[(head + lane) & (PARSE_RING - 1)];
+                    const uint64_t mm = ballot((lane >= 3) & (lane < span) & (ba != bb));
+                    if (mm)
+                        len = (uint32_t)__builtin_ctzll(mm);
+                    else
+                        len = lcp_wave<SAFE>(src, head - off, head, n - head, span > 3 ? span : 3u);
+                } else {
+                    len = lcp_wave<SAFE>(src, head - off, head, n - head, 3);
+                }
             }
-            // one lazy step at head + 1 (zstd_compress.cu:278-307); its info
-            // word is in the next lane, or the first ahead lane
-            const uint32_t e2 = (lazy != 0 && head + 4 < n) ? (k < WAVE - 1 ? bcast(e, k + 1) : bcast(ae, 0)) : 0u;
-            if (e2 > 0xFFFFu) {
-                const uint32_t c2 = e2 & 0xFFFFu;
-                uint32_t len2 = e2 >> 16;
-                if (len2 >= GC_ZSTD_SHORT) len2 = lcp_wave<SAFE>(src, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
-                if (len2 > len + 1) {
-                    head += 1;
-                    len = len2;
-                    off = head - c2;
+            if constexpr (LAZY) {
+                // one lazy step at head + 1 (zstd_compress.cu:278-307)
+                const uint32_t e2 = bcast(e1, k);
+                if (e2 > 0xFFFFu) {
+                    const uint32_t c2 = e2 & 0xFFFFu;
+                    uint32_t len2 = e2 >> 16;
+                    if (len2 >= GC_ZSTD_SHORT)
+                        len2 = lcp_wave<SAFE>(src, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
+                    if (len2 > len + 1) {
+                        head += 1;
+                        len = len2;
+                        off = head - c2;
+                        sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
+                    }
                 }
             }
             // emit (nseq < max_seq holds here)
             const uint32_t ll = head - lit_start;
-            if constexpr (LITS) copy_wave(lit_dst + nlit, src.base + src.mis + lit_start, ll);
+            if constexpr (LITS)
+                copy_wave(lit_dst + uniform(nlit), src.base + src.mis + uniform(lit_start), uniform(ll));
             nlit += ll;
-            const uint32_t slot = nseq & (WAVE - 1);
-            const bool mine = lane == slot;
+            const bool mine = lane == (nseq & (WAVE - 1));
             q_a = mine ? off : q_a;
             q_b = mine ? ll : q_b;
             q_c = mine ? len : q_c;
             nseq++;
-            if (slot == WAVE - 1) store_staged(nseq - WAVE);
-            // repeat offsets (off, o0, o1); which old flag, if any, the new
-            // offset's is: 0..2, or 3 = gather it
-            const uint32_t sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
+            if (nseq == ev_at) {
+                if ((nseq & (WAVE - 1)) == 0) store_staged(nseq - WAVE);
+                if (nseq == max_seq) lim = 0;  // the reference stops at the next head
+                ev_at = nseq + WAVE < max_seq ? nseq + WAVE : max_seq;
+            }
+            // repeat offsets (off, o0, o1); sel: which old flag, if any, the
+            // new offset's is (0..2), or 3 = gather it
             rep2 = rep1;
             rep1 = rep0;
             rep0 = off;
             pos = head + len;
-            lit_start = pos;
-            if (pos >= wbase + WAVE) {
+            lit_start = in_vgpr(pos);
+            if (pos >= wend) {
                 // the walk left the window: move to the window holding pos
                 // (reads past the sub-block end are clamped by peek)
                 const uint32_t nb = pos & ~(WAVE - 1);
-                if (nb == wbase + WAVE) {
+                uint32_t from = nb;
+                if (nb == wend) {
                     e = ae;
+                    e1 = ae1;
                     w = aw;
                 } else {
-                    peek(nb + lane, e, w);
+                    peek(nb + lane, e, e1, w);
+                    // the windows jumped over, as far as the ring reaches
+                    from = nb + WAVE - PARSE_RING < wend || nb + WAVE < PARSE_RING ? wend : nb + WAVE - PARSE_RING;
                 }
                 wbase = nb;
-                f = flags(wbase + lane, w);
+                wend = nb + WAVE;
+                xv = nb + lane;
+                enter(nb, from);
+                f = flags(xv, w);
                 __asm__ volatile("" ::: "memory");
-                peek(wbase + WAVE + lane, ae, aw);
+                peek(wend + lane, ae, ae1, aw);
             } else {
-                const uint32_t x = wbase + lane;
                 uint32_t b0 = (f >> sel) & 1u;  // sel 3 reads 0
-                if (sel == 3) b0 = x + 3 < n ? rep_eq<SAFE>(src, x, off, w) : 0u;
+                if (sel == 3) {
+                    const bool act = xv + 3 < n;
+                    if (off <= RING_REACH) {
+                        const uint32_t v = ring_w24(ring, xv - off);
+                        b0 = act & (xv >= off) & (v == w) ? 1u : 0u;
+                    }
+                    else
+                        b0 = rep_eq<SAFE>(src, act ? xv : 0u, off, w & 0xFFFFFFu) & (act ? 1u : 0u);
+                }
                 f = b0 | ((f & 3u) << 1);
             }
         }
     }
     if (lane < (nseq & (WAVE - 1))) store_staged(nseq & ~(uint32_t)(WAVE - 1));
     // trailing literals (zstd_compress.cu:341-344)
-    if (n > lit_start) {
-        if constexpr (LITS) copy_wave(lit_dst + nlit, src.base + src.mis + lit_start, n - lit_start);
-        nlit += n - lit_start;
+    const uint32_t lit_end = uniform(lit_start);
+    uint32_t nlit_total = uniform(nlit);
+    if (n > lit_end) {
+        if constexpr (LITS) copy_wave(lit_dst + nlit_total, src.base + src.mis + lit_end, n - lit_end);
+        nlit_total += n - lit_end;
     }
     if (lane == 0) {
         seq_counts[it.count_pos] = nseq;
-        if (lit_counts) lit_counts[it.count_pos] = nlit;
+        if (lit_counts) lit_counts[it.count_pos] = nlit_total;
     }
 }
 
@@ -823,9 +966,11 @@ template <uint32_t FMT, bool LITS>
 __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                  uint32_t n_items, const uint32_t* info, void* seq_out,
                                                  uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
-                                                 uint32_t max_seq, const uint32_t* __restrict__ order) {
+                                                 uint32_t max_seq, const uint32_t* __restrict__ order,
+                                                 uint32_t (*ring_mem)[RING_BYTES / 4]) {
     // the wavefront index is uniform: items are read into scalar registers
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    uint8_t* ring = (uint8_t*)ring_mem[wave];
     for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES) {
         const uint32_t k = order ? order[i] : i;
 #if PARSE_UNIFORM_ITEM
@@ -833,35 +978,45 @@ __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_
 #else
         const gc_item it = items[k];  // fields in vector registers: fewer scalar registers in use
 #endif
-        if (item_safe(it.avail, it.n))
-            zstd_parse_item<true, FMT, LITS>(it, arena, info, seq_out, literals, seq_counts, lit_counts, max_seq);
-        else
-            zstd_parse_item<false, FMT, LITS>(it, arena, info, seq_out, literals, seq_counts, lit_counts, max_seq);
+        const bool safe = item_safe(it.avail, it.n);
+        if (it.depth > 1) {
+            if (safe)
+                zstd_parse_item<true, FMT, LITS, true>(it, arena, info, seq_out, literals, seq_counts, lit_counts,
+                                                       max_seq, ring);
+            else
+                zstd_parse_item<false, FMT, LITS, true>(it, arena, info, seq_out, literals, seq_counts, lit_counts,
+                                                        max_seq, ring);
+        } else {
+            if (safe)
+                zstd_parse_item<true, FMT, LITS, false>(it, arena, info, seq_out, literals, seq_counts, lit_counts,
+                                                        max_seq, ring);
+            else
+                zstd_parse_item<false, FMT, LITS, false>(it, arena, info, seq_out, literals, seq_counts, lit_counts,
+                                                         max_seq, ring);
+        }
     }
 }
 
-extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)
-gc_zstd_parse_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,
-                     void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
-                     uint32_t max_seq, const uint32_t* __restrict__ order) {
-    // order (optional): the items in the order to take them
-    if (seq_fmt == GC_SEQ_REF) {
-        if (literals)
-            zstd_parse_items<GC_SEQ_REF, true>(arena, items, n_items, info, seq_out, literals, seq_counts, lit_counts,
-                                               max_seq, order);
-        else
-            zstd_parse_items<GC_SEQ_REF, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                lit_counts, max_seq, order);
-    } else {
-        if (literals)
-            zstd_parse_items<GC_SEQ_ZSTD, true>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                lit_counts, max_seq, order);
-        else
-            zstd_parse_items<GC_SEQ_ZSTD, false>(arena, items, n_items, info, seq_out, literals, seq_counts,
-                                                 lit_counts, max_seq, order);
+// One kernel per record layout and literal output, each with its own
+// register allocation (scalar registers bound this kernel's occupancy).
+// order (optional): the items in the order to take them.
+#define GC_PARSE_KERNEL(NAME, FMT, LITS)                                                                          \
+    extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)                                              \
+        NAME(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,    \
+             void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,     \
+             uint32_t max_seq, const uint32_t* __restrict__ order) {                                             \
+        __shared__ uint32_t ring_mem[PARSE_WAVES][RING_BYTES / 4];                                          \
+        zstd_parse_items<FMT, LITS>(arena, items, n_items, info, seq_out, literals, seq_counts, lit_counts,      \
+                                    max_seq, order, ring_mem);                                                   \
     }
-}
+GC_PARSE_KERNEL(gc_zstd_parse_kernel, GC_SEQ_ZSTD, false)
+GC_PARSE_KERNEL(gc_zstd_parse_lits_kernel, GC_SEQ_ZSTD, true)
+GC_PARSE_KERNEL(gc_zstd_parse_ref_kernel, GC_SEQ_REF, false)
+GC_PARSE_KERNEL(gc_zstd_parse_ref_lits_kernel, GC_SEQ_REF, true)
 
+#endif  // GC_BUILD_PARSE
+
+#if GC_BUILD_MAIN
 // Small fill kernel.
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -896,3 +1051,4 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
         for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
     }
 }
+#endif  // GC_BUILD_MAIN
