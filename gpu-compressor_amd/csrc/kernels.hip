@@ -737,23 +737,25 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     const uint32_t lane = lane_id();
     const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
-    uint32_t pos = 0, nseq = 0, wbase = 0;
+    uint32_t pos = 0, nseq = 0, wbase = 0, wend = WAVE;
     // literal bookkeeping in vector registers (the same in every lane): the
     // scalar register count limits this kernel's occupancy
     uint32_t lit_start = in_vgpr(0u), nlit = in_vgpr(0u);
     uint8_t* lit_dst = LITS ? literals + it.lit_pos : nullptr;
 
     // staged sequences: lane j holds sequence (nseq & ~63) + j
-    uint32_t q_a = 0, q_b = 0, q_c = 0;
+    // (offset | length << 16, literal length); both fit 16 bits: a head is
+    // at position >= 1, so offset and length are below 65536
+    uint32_t q_ol = 0, q_b = 0;
     auto store_staged = [&](uint32_t first) {  // this lane's record: sequence first + lane
         const uint64_t si = it.seq_pos + first + lane;
         if constexpr (FMT == GC_SEQ_REF) {
             gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
             o->literal_length = q_b;
-            o->match_length_minus3 = q_c - 3;
-            o->offset = q_a;
+            o->match_length_minus3 = (q_ol >> 16) - 3;
+            o->offset = q_ol & 0xFFFFu;
         } else {
-            *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(q_a, q_b, q_c, 0);
+            *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(q_ol & 0xFFFFu, q_b, q_ol >> 16, 0);
         }
     };
 
@@ -777,22 +779,43 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             ev1 = x + 4 < n ? i1 : 0;
         }
     };
-    // the repeat flags of the window at xv - lane, whose bytes are in the ring
-    auto flags = [&](uint32_t x, uint32_t wv) -> uint32_t {
+    // Repeat offsets beyond the ring's reach: at each window entry, for the
+    // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
+    // for the next window (y = wend + lane), like the info words.  They stay
+    // in place (a register whose load is in flight is not copied) and serve
+    // the next window's flags for whichever offsets are still in use.
+    // (The tags sr0..sr2 are uniform but live in vector registers: scalar
+    // registers bound the occupancy.)
+    uint32_t sh0 = 0, sh1 = 0, sh2 = 0, sr0 = in_vgpr(0u), sr1 = in_vgpr(0u), sr2 = in_vgpr(0u);
+    auto far_w24 = [&](uint32_t y, uint32_t r) -> uint32_t {  // the 3 bytes at y - r (clamped read)
+        return ld32<SAFE>(src, (y >= r && y < n) ? y - r : 0u) & 0xFFFFFFu;
+    };
+    auto load_shadows = [&]() {  // for the window after the current one
+        sr0 = in_vgpr(rep0 > RING_REACH ? rep0 : 0u);
+        sr1 = in_vgpr(rep1 > RING_REACH ? rep1 : 0u);
+        sr2 = in_vgpr(rep2 > RING_REACH ? rep2 : 0u);
+        if (rep0 > RING_REACH) sh0 = far_w24(wend + lane, rep0);
+        if (rep1 > RING_REACH) sh1 = far_w24(wend + lane, rep1);
+        if (rep2 > RING_REACH) sh2 = far_w24(wend + lane, rep2);
+    };
+    auto far_flag_word = [&](uint32_t x, uint32_t r, bool shadows) -> uint32_t {
+        if (shadows && r == uniform(sr0)) return sh0;
+        if (shadows && r == uniform(sr1)) return sh1;
+        if (shadows && r == uniform(sr2)) return sh2;
+        return far_w24(x, r);
+    };
+    // the repeat flags of the window at x - lane: from the ring, or for a
+    // far offset from a shadow (`shadows`: the window is the one they were
+    // loaded for) or a global read
+    auto flags = [&](uint32_t x, uint32_t wv, bool shadows) -> uint32_t {
         const bool act = x + 3 < n;
-        uint32_t fv;
-        if (max(rep0, max(rep1, rep2)) <= RING_REACH) {
-            // every lane reads (indexes wrap inside the ring), then masks
-            const uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1),
-                           v2 = ring_w24(ring, x - rep2);
-            fv = ((x >= rep0) & (v0 == wv) ? 1u : 0u) | ((x >= rep1) & (v1 == wv) ? 2u : 0u) |
-                 ((x >= rep2) & (v2 == wv) ? 4u : 0u);
-        } else {
-            const uint32_t xc = act ? x : 0;
-            const uint32_t wc = wv & 0xFFFFFFu;
-            fv = rep_eq<SAFE>(src, xc, rep0, wc) | (rep_eq<SAFE>(src, xc, rep1, wc) << 1) |
-                 (rep_eq<SAFE>(src, xc, rep2, wc) << 2);
-        }
+        // every lane reads (indexes wrap inside the ring), then masks
+        uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1), v2 = ring_w24(ring, x - rep2);
+        if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
+        if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
+        if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
+        const uint32_t fv = ((x >= rep0) & (v0 == wv) ? 1u : 0u) | ((x >= rep1) & (v1 == wv) ? 2u : 0u) |
+                            ((x >= rep2) & (v2 == wv) ? 4u : 0u);
         return act ? fv : 0;
     };
     // entering the window at nb (w already holds its bytes): the bytes from
@@ -816,14 +839,13 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     if (lim) {
         peek(lane, e, e1, w);
         enter(0, 0);
-        f = flags(lane, w);
+        f = flags(lane, w, false);
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
         peek(WAVE + lane, ae, ae1, aw);
     }
     // next sequence count at which the staged records are stored or the
     // walk stops (max_seq)
     uint32_t ev_at = max_seq < WAVE ? max_seq : WAVE;
-    uint32_t wend = WAVE;
     while (pos < lim) {
         const uint32_t cand = xv >= pos ? ((e >> 16) | f) : 0u;
         const uint64_t m = ballot(cand != 0);
@@ -837,8 +859,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             e1 = ae1;
             w = aw;
             enter(wbase, wbase);
-            f = flags(xv, w);
+            f = flags(xv, w, true);
             __asm__ volatile("" ::: "memory");
+            load_shadows();
             peek(wend + lane, ae, ae1, aw);
         } else {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
@@ -896,9 +919,8 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 copy_wave(lit_dst + uniform(nlit), src.base + src.mis + uniform(lit_start), uniform(ll));
             nlit += ll;
             const bool mine = lane == (nseq & (WAVE - 1));
-            q_a = mine ? off : q_a;
+            q_ol = mine ? (off | (len << 16)) : q_ol;
             q_b = mine ? ll : q_b;
-            q_c = mine ? len : q_c;
             nseq++;
             if (nseq == ev_at) {
                 if ((nseq & (WAVE - 1)) == 0) store_staged(nseq - WAVE);
@@ -916,8 +938,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 // the walk left the window: move to the window holding pos
                 // (reads past the sub-block end are clamped by peek)
                 const uint32_t nb = pos & ~(WAVE - 1);
+                const bool next = nb == wend;  // the window the shadows are for
                 uint32_t from = nb;
-                if (nb == wend) {
+                if (next) {
                     e = ae;
                     e1 = ae1;
                     w = aw;
@@ -930,8 +953,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 wend = nb + WAVE;
                 xv = nb + lane;
                 enter(nb, from);
-                f = flags(xv, w);
+                f = flags(xv, w, next);
                 __asm__ volatile("" ::: "memory");
+                load_shadows();
                 peek(wend + lane, ae, ae1, aw);
             } else {
                 uint32_t b0 = (f >> sel) & 1u;  // sel 3 reads 0
