@@ -360,13 +360,15 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
     uint16_t* out = cand16 + it.cpos;
     uint32_t* t = (uint32_t*)s.table;
     for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
-    // analysis of tile 0; words two tiles ahead
+    // analysis of tile 0; words two tiles ahead.  The loads are unconditional
+    // (positions past the end read position 0, masked when used): a load
+    // under a per-lane condition is waited for where the branches join.
     uint32_t next = 0;
     if (wave < CAND_AWAVES) {
         const uint32_t p = wave * WAVE + lane;
-        const uint32_t word = (p + 3 < n) ? ld32<SAFE>(src, p) : 0;
+        const uint32_t word = ld32<SAFE>(src, p + 3 < n ? p : 0u);
         const uint32_t q = CAND_TILE + p;
-        next = (q + 3 < n) ? ld32<SAFE>(src, q) : 0;
+        next = ld32<SAFE>(src, q + 3 < n ? q : 0u);
         s.ana[0][p] = cand_analyse(s, wave, word, p + 3 < n);
     }
     lds_barrier();
@@ -378,7 +380,7 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
                 const uint32_t p = t1 + wave * WAVE + lane;
                 const uint32_t word = next;
                 const uint32_t q = p + CAND_TILE;
-                next = (q + 3 < n) ? ld32<SAFE>(src, q) : 0;
+                next = ld32<SAFE>(src, q + 3 < n ? q : 0u);
                 s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p + 3 < n);
             }
         } else {
