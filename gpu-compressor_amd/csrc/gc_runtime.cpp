@@ -91,7 +91,8 @@ int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, cons
                      uint32_t grid, const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                      uint32_t max_matches, uint8_t* triage) {
     void* args[] = {&arena, &items, &n_items, &cand16, &packed, &ref_m, &ref_c, &max_matches, &triage};
-    return launch(stream, ctx->k.lzma2_len, grid, 1024, args, "launch gc_lzma2_len_kernel");
+    hipFunction_t f = ref_c ? ctx->k.lzma2_len_ref : (packed ? ctx->k.lzma2_len : ctx->k.lzma2_triage);
+    return launch(stream, f, grid, 1024, args, "launch gc_lzma2_len_kernel");
 }
 
 int launch_zstd_info(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
@@ -178,6 +179,8 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module, gc_code_object);
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand, ctx->k.module, "gc_cand_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len, ctx->k.module, "gc_lzma2_len_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_ref, ctx->k.module, "gc_lzma2_len_ref_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_triage, ctx->k.module, "gc_lzma2_triage_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info, ctx->k.module, "gc_zstd_info_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info4, ctx->k.module, "gc_zstd_info4_kernel");
     if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module_parse, gc_parse_code_object);

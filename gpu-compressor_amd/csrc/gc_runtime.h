@@ -30,7 +30,9 @@ struct Kernels {
     hipModule_t module = nullptr;        // every kernel but the parse
     hipModule_t module_parse = nullptr;  // the Zstd parse kernels
     hipFunction_t cand = nullptr;
-    hipFunction_t lzma2_len = nullptr;
+    hipFunction_t lzma2_len = nullptr;      // packed words
+    hipFunction_t lzma2_len_ref = nullptr;  // reference layout
+    hipFunction_t lzma2_triage = nullptr;   // triage byte only
     hipFunction_t zstd_info = nullptr;
     hipFunction_t zstd_info4 = nullptr;
     hipFunction_t zstd_parse[2][2] = {};  // [reference layout][literals]

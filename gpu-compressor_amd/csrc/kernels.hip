@@ -110,6 +110,24 @@ __device__ __forceinline__ uint32_t ld32(const SrcView& v, uint32_t off) {
     return ld32_tail(v.base + v.mis + off, off, v.avail);  // the last bytes of the arena, byte by byte
 }
 
+// ld32 in two halves, for a prefetch: the load now (the two aligned dwords
+// and the shift), the byte alignment when the word is used.  Computing it
+// right away would wait for the load there.
+struct Raw32 {
+    uint32_t lo, hi, sh;
+    __device__ __forceinline__ uint32_t word() const { return __builtin_amdgcn_alignbyte(hi, lo, sh); }
+};
+template <bool SAFE>
+__device__ __forceinline__ Raw32 ld32_raw(const SrcView& v, uint32_t off) {
+    if constexpr (SAFE) {
+        const uint32_t t = off + v.mis;
+        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));
+        return Raw32{q[0], q[1], t & 3};
+    } else {
+        return Raw32{ld32<false>(v, off), 0u, 0u};
+    }
+}
+
 // 16 bytes at sub-block offset `off` as four little-endian words, from the
 // five aligned dwords around them.
 template <bool SAFE>
@@ -192,9 +210,28 @@ __device__ __forceinline__ void ld128(const LdsBytes& v, uint32_t off, uint32_t 
 
 // Copies bytes [0, round_up(n + 32, 16)) of a sub-block into LDS (bytes past
 // the arena read as 0); the whole workgroup takes part.
+// With room past the end (the usual case) every thread issues all its loads
+// (clamped, unconditional) before the first LDS write, so their latencies
+// overlap; otherwise the reads near the arena's end go byte by byte.
+template <uint32_t THREADS>
 __device__ __forceinline__ void stage_sub_block(uint32_t* lds, const SrcView& src, uint32_t n) {
     const uint32_t end = (n + 32 + 15) & ~15u;
-    for (uint32_t o = threadIdx.x * 16; o < end; o += blockDim.x * 16) {
+    constexpr uint32_t PASSES = (GC_SUB_BLOCK + 32 + 15) / (16 * THREADS) + 1;
+    if (src.avail >= (uint64_t)end + 8) {
+        uint32_t w[PASSES][4];
+#pragma unroll
+        for (uint32_t k = 0; k < PASSES; k++) {
+            const uint32_t o = (threadIdx.x + k * THREADS) * 16;
+            ld128<true>(src, o < end ? o : 0u, w[k]);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < PASSES; k++) {
+            const uint32_t o = (threadIdx.x + k * THREADS) * 16;
+            if (o < end) *(uint4*)(lds + o / 4) = make_uint4(w[k][0], w[k][1], w[k][2], w[k][3]);
+        }
+        return;
+    }
+    for (uint32_t o = threadIdx.x * 16; o < end; o += THREADS * 16) {
         uint32_t w[4];
         ld128<false>(src, o, w);
         *(uint4*)(lds + o / 4) = make_uint4(w[0], w[1], w[2], w[3]);
@@ -363,12 +400,12 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
     // analysis of tile 0; words two tiles ahead.  The loads are unconditional
     // (positions past the end read position 0, masked when used): a load
     // under a per-lane condition is waited for where the branches join.
-    uint32_t next = 0;
+    Raw32 next{0u, 0u, 0u};
     if (wave < CAND_AWAVES) {
         const uint32_t p = wave * WAVE + lane;
         const uint32_t word = ld32<SAFE>(src, p + 3 < n ? p : 0u);
         const uint32_t q = CAND_TILE + p;
-        next = ld32<SAFE>(src, q + 3 < n ? q : 0u);
+        next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
         s.ana[0][p] = cand_analyse(s, wave, word, p + 3 < n);
     }
     lds_barrier();
@@ -378,9 +415,9 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
             const uint32_t t1 = t0 + CAND_TILE;
             if (t1 < n) {
                 const uint32_t p = t1 + wave * WAVE + lane;
-                const uint32_t word = next;
+                const uint32_t word = next.word();
                 const uint32_t q = p + CAND_TILE;
-                next = ld32<SAFE>(src, q + 3 < n ? q : 0u);
+                next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
                 s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p + 3 < n);
             }
         } else {
@@ -434,7 +471,7 @@ __device__ __forceinline__ uint32_t strip_word(const uint32_t d[4], uint32_t j) 
 // has any match (the consumer's triage, compress_lzma2_custom.rs:577), no
 // lengths measured and nothing per position written.  Returns this thread's
 // "some position of mine has a match".
-template <bool LENGTHS>
+template <bool LENGTHS, bool REF>
 __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& src, const uint16_t* cand16,
                                            uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
                                            uint32_t max_matches) {
@@ -493,8 +530,8 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
             const uint32_t p = p0 + j;
             if (p >= n) break;
             const uint64_t q = it.opos + p;
-            if (packed) packed[q] = out[j];
-            if (ref_c) {
+            if (!REF || packed) packed[q] = out[j];
+            if (REF) {
                 ref_c[q] = out[j] ? 1u : 0u;
                 if (out[j]) {
                     gc_lzma_match m;
@@ -506,31 +543,44 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
         }
     }
     // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
-    if (LENGTHS && ref_c)
+    if (LENGTHS && REF)
         for (uint32_t p = n + threadIdx.x; p < it.out_span; p += INFO_THREADS) ref_c[it.opos + p] = 0;
     return any;
 }
 
-extern "C" __global__ void __launch_bounds__(INFO_THREADS)
-gc_lzma2_len_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
-                    const uint16_t* cand16, uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c,
-                    uint32_t max_matches, uint8_t* triage) {
-    // triage (optional): one byte per item at it.count_pos, 1 if any position matches
+// One kernel per output (scalar registers bound the occupancy): packed
+// words (gc_lzma2_len_kernel), the reference layout with optional packed
+// words (gc_lzma2_len_ref_kernel), or the triage byte alone
+// (gc_lzma2_triage_kernel).  triage (optional): one byte per item at
+// it.count_pos, 1 if any position matches.
+template <bool LENGTHS, bool REF>
+__device__ __forceinline__ void lzma2_len_items(const uint8_t* arena, const gc_item* __restrict__ items,
+                                                uint32_t n_items, const uint16_t* cand16, uint32_t* packed,
+                                                gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches,
+                                                uint8_t* triage) {
     __shared__ uint32_t sb[STAGE_WORDS];
-    const bool lengths = packed || ref_c;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
         __syncthreads();  // the last item's reads of sb are done
-        stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
+        stage_sub_block<INFO_THREADS>(sb, src_view(arena, it.src_off, it.avail), it.n);
         __syncthreads();
-        const bool any = lengths ? lzma2_item<true>(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches)
-                                 : lzma2_item<false>(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches);
+        const bool any = lzma2_item<LENGTHS, REF>(it, LdsBytes{sb}, cand16, packed, ref_m, ref_c, max_matches);
         if (triage) {
             const int wg_any = __syncthreads_or(any ? 1 : 0);
             if (threadIdx.x == 0) triage[it.count_pos] = wg_any ? 1 : 0;
         }
     }
 }
+
+#define GC_LZMA2_KERNEL(NAME, LENGTHS, REF)                                                                        \
+    extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)                                                \
+        NAME(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint16_t* cand16,  \
+             uint32_t* packed, gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches, uint8_t* triage) {   \
+        lzma2_len_items<LENGTHS, REF>(arena, items, n_items, cand16, packed, ref_m, ref_c, max_matches, triage); \
+    }
+GC_LZMA2_KERNEL(gc_lzma2_len_kernel, true, false)
+GC_LZMA2_KERNEL(gc_lzma2_len_ref_kernel, true, true)
+GC_LZMA2_KERNEL(gc_lzma2_triage_kernel, false, false)
 
 // ---------------------------------------------------------------------------
 // Zstd per position, given cand(p): the candidate as the reference's table
@@ -612,7 +662,7 @@ __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_i
         const gc_item it = load_item(items, i);
         __syncthreads();  // the last item's reads of sb and `verified` are done
         if (threadIdx.x == 0) verified = 0;
-        stage_sub_block(sb, src_view(arena, it.src_off, it.avail), it.n);
+        stage_sub_block<INFO_THREADS>(sb, src_view(arena, it.src_off, it.avail), it.n);
         __syncthreads();
         const uint32_t v = zstd_info_lds<W>(LdsBytes{sb}, it.n, cand16 + it.cpos, info + it.opos);
         if (cost) {
