@@ -46,7 +46,9 @@
 #define GC_HASH_BITS 14
 #define GC_TABLE (1u << GC_HASH_BITS)
 #define GC_NONE 0xFFFFu
-#define GC_DUP_BITS 10
+#ifndef GC_DUP_BITS
+#define GC_DUP_BITS 9
+#endif
 #define GC_DUP_SIZE (1u << GC_DUP_BITS)
 #define GC_LZMA_MAX 273u
 #define GC_ZSTD_SHORT 64u  // Zstd lengths >= this are re-measured by the parse
@@ -316,7 +318,6 @@ struct CandLds {
     uint16_t table[GC_TABLE];               // the reference's 14-bit table
     uint32_t ana[2][CAND_TILE];             // analysis words, double buffered
     uint8_t dup[CAND_AWAVES][GC_DUP_SIZE];  // write/read-back duplicate test
-    uint8_t flag[CAND_AWAVES][WAVE];        // lanes that won a duplicate test
 };
 
 // Analysis word of this lane's position (valid when p + 3 < n):
@@ -330,34 +331,32 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     const uint32_t lane = lane_id();
     const uint32_t h = hash4(word);
 
-    // Duplicate test: every valid lane writes its id under a 10-bit slice of
-    // its hash (one slot array per wavefront) and reads it back.  A lane that
-    // reads another lane's id shares the slice with it and marks it; both are
-    // then resolved exactly below.  Lanes that share a whole hash always
-    // share a slice, so none of them escapes the exact path.
+    // Duplicate test: every valid lane writes its id under a slice of its
+    // hash (one slot array per wavefront) and reads it back; one lane of each
+    // slice wins.  Lanes that share a whole hash share a slice, so every
+    // group of two or more equal hashes has a lane that lost: the losers'
+    // hashes seed the exact groups below, which take in every lane with the
+    // same hash (winners too).
     // Other lanes write the same bytes between this lane's store and load,
     // which the compiler cannot see: the empty asm with a memory clobber
     // keeps it from forwarding the store.  LDS operations of one wavefront
     // execute in program order, so the load sees every lane's store.
     const uint32_t slot = h & (GC_DUP_SIZE - 1);
-    s.flag[wave][lane] = 0;
     if (valid) s.dup[wave][slot] = (uint8_t)lane;
     __asm__ volatile("" ::: "memory");
     const uint32_t back = valid ? s.dup[wave][slot] : lane;
     const bool loser = valid && back != lane;
-    if (loser) s.flag[wave][back & (WAVE - 1)] = 1;
-    __asm__ volatile("" ::: "memory");
-    const bool dup = loser || (valid && s.flag[wave][lane] != 0);
 
-    // Exact groups among the duplicate lanes (one step per distinct hash).
+    // Exact groups (one step per distinct hash among the losers).
     const uint64_t lane_bit = 1ull << lane;
     uint64_t group = lane_bit;
-    uint64_t rest = ballot(dup);
+    uint64_t rest = ballot(loser);
     while (rest) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(rest);
         const uint32_t hl = bcast(h, lead);
-        const uint64_t g = ballot(dup && h == hl);
-        if (dup && h == hl) group = g;
+        const bool in = valid && h == hl;
+        const uint64_t g = ballot(in);
+        if (in) group = g;
         rest &= ~g;
     }
     const uint64_t below = group & (lane_bit - 1);
