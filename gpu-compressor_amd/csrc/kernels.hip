@@ -602,14 +602,11 @@ __device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n,
     uint32_t verified = 0;
     // cand and out start at multiples of 8 positions and each item owns its
     // positions rounded up to 8: whole strips are read and written with
-    // vector accesses
-    for (uint32_t p0 = threadIdx.x * W; p0 + 3 < n; p0 += INFO_THREADS * W) {
-        uint32_t d[4];
-        d[0] = L.w[p0 / 4];
-        d[1] = L.w[p0 / 4 + 1];
-        d[2] = L.w[p0 / 4 + 2];
-        d[3] = L.w[p0 / 4 + 3];
-        uint32_t cv[W / 2];
+    // vector accesses.  The next strip's candidates are loaded one strip
+    // early into the other of two register sets (a strip past the end
+    // reloads the current one): a loop-carried copy of registers whose load
+    // is in flight would wait for it.
+    auto load_cand = [&](uint32_t p0, uint32_t cv[W / 2]) {
         if constexpr (W == 8) {
             const uint4 cw = *(const uint4*)(cand + p0);
             cv[0] = cw.x, cv[1] = cw.y, cv[2] = cw.z, cv[3] = cw.w;
@@ -617,6 +614,13 @@ __device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n,
             const uint2 cw = *(const uint2*)(cand + p0);
             cv[0] = cw.x, cv[1] = cw.y;
         }
+    };
+    auto strip = [&](uint32_t p0, const uint32_t cv[W / 2]) {
+        uint32_t d[4];
+        d[0] = L.w[p0 / 4];
+        d[1] = L.w[p0 / 4 + 1];
+        d[2] = L.w[p0 / 4 + 2];
+        d[3] = L.w[p0 / 4 + 3];
         uint32_t c[W], vc[W];
 #pragma unroll
         for (uint32_t j = 0; j < W; j++) {
@@ -645,6 +649,23 @@ __device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n,
         uint4* q = (uint4*)(out + p0);
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
         if constexpr (W == 8) q[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    };
+    constexpr uint32_t STEP = INFO_THREADS * W;
+    uint32_t ca[W / 2], cb[W / 2];
+    uint32_t p0 = threadIdx.x * W;
+    load_cand(p0 + 3 < n ? p0 : 0u, ca);
+    // (the empty asm keeps each prefetch where it is: scheduled later, the
+    // next use would wait for it)
+    for (; p0 + 3 < n; p0 += 2 * STEP) {
+        const uint32_t p1 = p0 + STEP;
+        load_cand(p1 + 3 < n ? p1 : p0, cb);
+        __asm__ volatile("" ::: "memory");
+        strip(p0, ca);
+        if (p1 + 3 >= n) break;
+        const uint32_t p2 = p1 + STEP;
+        load_cand(p2 + 3 < n ? p2 : p1, ca);
+        __asm__ volatile("" ::: "memory");
+        strip(p1, cb);
     }
     return verified;
 }
@@ -673,13 +694,13 @@ __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_i
 }
 
 // Strips of 8 or 4 positions: the same words, different register use.
-extern "C" __global__ void __launch_bounds__(INFO_THREADS)
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
 gc_zstd_info_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                     const uint16_t* cand16, uint32_t* info, uint32_t* cost) {
     zstd_info_items<8>(arena, items, n_items, cand16, info, cost);
 }
 
-extern "C" __global__ void __launch_bounds__(INFO_THREADS)
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
 gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                      const uint16_t* cand16, uint32_t* info, uint32_t* cost) {
     zstd_info_items<4>(arena, items, n_items, cand16, info, cost);
