@@ -221,6 +221,38 @@ def equal_ratio_check(ctx, batch, data, inputs, d_in, sample_bytes):
     return out
 
 
+def encode_from_gpu_matches(batch, data, inputs, sample_bytes):
+    """The LZMA2 consumer of the GPU's match lists (gc_cpu_encode_from_matches: one verified stream per
+    sub-block, min(cores, 8) threads) on the first LZMA2 input's first sample_bytes, with the batched
+    run's own packed matches; its ratio beside liblzma preset 9's on the first 4 MiB of the same bytes."""
+    L = gcompress.lib()
+    i = next(k for k, (_, _, al, _) in enumerate(inputs) if al == gcompress.ALGO_LZMA2)
+    o, s, _, _ = inputs[i]
+    m = min(s, sample_bytes)
+    x = np.ascontiguousarray(data[o:o + m])
+    pk = np.ascontiguousarray(batch.lzma2(i)[:m])
+    threads = min(os.cpu_count() or 1, 8)
+    nsub = -(-m // 65536)
+    cap = nsub * L.gc_lzma2_encode_bound(65536)
+    out = np.empty(cap, np.uint8)
+    sizes = np.zeros(nsub, np.uint64)
+    olen = C.c_size_t()
+    t = time.perf_counter()
+    rc = L.gc_cpu_encode_from_matches(x.ctypes.data, m, pk.ctypes.data, threads, out.ctypes.data, cap,
+                                      sizes.ctypes.data, C.byref(olen))
+    dt = time.perf_counter() - t
+    assert rc == 0, L.gc_last_error()
+    k = min(m, 4 * MiB)
+    p9 = 0
+    for st in range(0, k, 65536):
+        sub = x[st:st + 65536]
+        p9 += len(gcompress.encode_single_sub_block(sub, (pk[st:st + sub.size] != 0).astype(np.uint32)))
+    ours_k = int(sizes[: -(-k // 65536)].sum())
+    return {"sample_mib": m // MiB, "threads": threads, "gbs": round(m / dt / 1e9, 4),
+            "ratio": round(m / olen.value, 4), "ratio_first_4mib": round(k / ours_k, 4),
+            "preset9_ratio_first_4mib": round(k / p9, 4)}
+
+
 def workload_inputs(a):
     """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
     if a.workload == "zstd":
@@ -392,6 +424,7 @@ def main():
     traffic = int(per_byte * work_bytes) if per_byte is not None else None
 
     equal = equal_ratio_check(ctx, batch, data, inputs, d_in, a.check_sample_mib * MiB)
+    enc_m = None if a.no_cpu or not lzma_bytes else encode_from_gpu_matches(batch, data, inputs, 16 * MiB)
     cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
     cpu_more = None
     if not a.no_cpu and a.workload in ("zstd", "lzma2"):
@@ -432,6 +465,7 @@ def main():
                      "algorithmic_bytes": dom_bytes},
         "cpu_baseline": cpu,
         "cpu_baselines_more": cpu_more,
+        "lzma2_encode_from_gpu_matches": enc_m,
         "corpus_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)

@@ -34,6 +34,7 @@ HOST_SOURCES = [
     "gc_corpus.cpp",
     "gc_codec.cpp",
     "gc_serve.cpp",
+    "gc_lzma_enc.cpp",
 ]
 KERNEL_SOURCES = ["kernels.hip"]
 

@@ -886,8 +886,8 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
         if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
         if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
-        const uint32_t fv = ((x >= rep0) & (v0 == wv) ? 1u : 0u) | ((x >= rep1) & (v1 == wv) ? 2u : 0u) |
-                            ((x >= rep2) & (v2 == wv) ? 4u : 0u);
+        const uint32_t fv = (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
+                            (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
         return act ? fv : 0;
     };
     // entering the window at nb (w already holds its bytes): the bytes from
@@ -1035,7 +1035,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                     const bool act = xv + 3 < n;
                     if (off <= RING_REACH) {
                         const uint32_t v = ring_w24(ring, xv - off);
-                        b0 = act & (xv >= off) & (v == w) ? 1u : 0u;
+                        b0 = (act & (xv >= off) & (v == w)) ? 1u : 0u;
                     }
                     else
                         b0 = rep_eq<SAFE>(src, act ? xv : 0u, off, w & 0xFFFFFFu) & (act ? 1u : 0u);

@@ -66,6 +66,10 @@ SIGNATURES = [
     ("gc_lzma2_raw_encode", C.c_int, [_vp, _sz, C.c_uint32, _vp, _sz, _vp]),
     ("gc_lzma2_raw_decode", C.c_int, [_vp, _sz, _vp, _sz, _vp]),
     ("gc_encode_single_sub_block", C.c_int, [_vp, _sz, _vp, _vp, _sz, _vp]),
+    ("gc_lzma2_encode_bound", _sz, [_sz]),
+    ("gc_lzma2_encode_matches", C.c_int, [_vp, _sz, _vp, C.c_int, _vp, _sz, _vp]),
+    ("gc_cpu_encode_from_matches", C.c_int, [_vp, _sz, _vp, C.c_int, _vp, _sz, _vp, _vp]),
+    ("gc_nvlz_compress_matches", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_container_bound", _sz, [_sz, C.c_uint32]),
     ("gc_nvzs_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_nvlz_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
@@ -373,6 +377,52 @@ def encode_single_sub_block(sub_data, match_counts) -> bytes:
     n = C.c_size_t()
     _check(lib().gc_encode_single_sub_block(_ptr(src), src.size, _ptr(mc), _ptr(out), cap, C.byref(n)),
            "gc_encode_single_sub_block")
+    return out[: n.value].tobytes()
+
+
+def lzma2_encode_matches(sub_data, packed, verify: bool = True) -> bytes:
+    """One sub-block (<= 65536 bytes) as an LZMA2 stream coded from its packed matches
+    (compress_lzma2_custom.rs:344-465, with the LZMA format's full coding)."""
+    src = _as_u8(sub_data)
+    pk = np.ascontiguousarray(packed, dtype=np.uint32)
+    if pk.size < src.size:
+        raise ValueError("one packed match word per byte")
+    cap = lib().gc_lzma2_encode_bound(src.size)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_lzma2_encode_matches(_ptr(src), src.size, _ptr(pk), int(verify), _ptr(out), cap, C.byref(n)),
+           "gc_lzma2_encode_matches")
+    return out[: n.value].tobytes()
+
+
+def cpu_encode_from_matches(chunk_data, packed, threads: int = 0):
+    """compress_lzma2_custom.rs:601-664: one verified LZMA2 stream per 64 KiB sub-block -> (streams, sizes)."""
+    src = _as_u8(chunk_data)
+    pk = np.ascontiguousarray(packed, dtype=np.uint32)
+    if pk.size < src.size:
+        raise ValueError("one packed match word per byte")
+    nsub = -(-src.size // SUB_BLOCK)
+    cap = max(nsub, 1) * lib().gc_lzma2_encode_bound(SUB_BLOCK)
+    out = np.empty(cap, np.uint8)
+    sizes = np.zeros(max(nsub, 1), np.uint64)
+    n = C.c_size_t()
+    _check(lib().gc_cpu_encode_from_matches(_ptr(src), src.size, _ptr(pk), threads, _ptr(out), cap, _ptr(sizes),
+                                            C.byref(n)), "gc_cpu_encode_from_matches")
+    streams, o = [], 0
+    for z in sizes[:nsub]:
+        streams.append(out[o:o + int(z)].tobytes())
+        o += int(z)
+    return streams, [int(z) for z in sizes[:nsub]]
+
+
+def nvlz_compress_matches(data, dict_mb: int = 128, device_id: int = 0, threads: int = 0) -> bytes:
+    """NVLZ container with sub-blocks coded from the GPU's matches (no second search)."""
+    src = _as_u8(data)
+    cap = lib().gc_container_bound(src.size, ALGO_LZMA2)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvlz_compress_matches(context(device_id).h, _ptr(src), src.size, dict_mb, threads, _ptr(out),
+                                          cap, C.byref(n)), "gc_nvlz_compress_matches")
     return out[: n.value].tobytes()
 
 

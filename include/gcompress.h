@@ -189,6 +189,32 @@ int gc_lzma2_raw_decode(const uint8_t* src, size_t n, uint8_t* out, size_t expec
 int gc_encode_single_sub_block(const uint8_t* sub, size_t n, const uint32_t* match_counts, uint8_t* out, size_t cap,
                                size_t* out_len);
 
+/* LZMA2 coded from the match lists (compress_lzma2_custom.rs:344-465
+ * encode_lzma2_block, :601-664 cpu_encode_from_matches), with the
+ * matched-literal, special-distance and repeat-match coding of the LZMA
+ * format: one stream (one chunk + the 0x00 end marker) per sub-block of at
+ * most 65536 bytes, from its packed matches (gc_lzma2_find_matches_packed
+ * layout: distance << 16 | length per position, 0 = none).  verify != 0
+ * decodes the stream with liblzma and falls back to raw chunks if it does
+ * not round-trip (:640-647); a stream that does not fit one chunk or is not
+ * smaller than the raw chunks is replaced by them as well.
+ * cap >= gc_lzma2_encode_bound(n) always suffices. */
+size_t gc_lzma2_encode_bound(size_t n);
+int gc_lzma2_encode_matches(const uint8_t* sub, size_t n, const uint32_t* packed, int verify, uint8_t* out,
+                            size_t cap, size_t* out_len);
+/* A whole chunk: one verified stream per 64 KiB sub-block, concatenated,
+ * sizes[s] = bytes of stream s; threads <= 0: min(cores, 8).  cap >= the sum
+ * of gc_lzma2_encode_bound over the sub-blocks suffices. */
+int gc_cpu_encode_from_matches(const uint8_t* src, size_t n, const uint32_t* packed, int threads, uint8_t* out,
+                               size_t cap, uint64_t* sizes, size_t* out_len);
+/* NVLZ container (chunk size 65536) whose sub-blocks are coded from the GPU's
+ * matches by gc_cpu_encode_from_matches instead of liblzma preset 9:
+ * dictionary-size chunks (dict_mb, 0 = 128) match-found on the GPU.  No
+ * counterpart in the reference's pipeline (its combined API,
+ * compress_lzma2_custom.rs:667-673, is not called).  Decoded by gc_decompress. */
+int gc_nvlz_compress_matches(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, int threads, uint8_t* out,
+                             size_t cap, size_t* out_len);
+
 /* Largest NVZS (algo GC_ALGO_ZSTD) / NVLZ level >= 1 container for n bytes. */
 size_t gc_container_bound(size_t n, uint32_t algo);
 

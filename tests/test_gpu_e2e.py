@@ -69,3 +69,18 @@ def test_nvlz_round_trip_and_triage(n):
             sub = data[s * SUB:(s + 1) * SUB]
             assert entry == gcompress.encode_single_sub_block(sub, c[s * SUB:s * SUB + sub.size])
             pos += sizes[s]
+
+
+@pytest.mark.parametrize("n", [0, 5, SUB, 3 * SUB + 99])
+def test_nvlz_from_gpu_matches_round_trip(n):
+    """NVLZ whose sub-blocks are coded from the GPU's matches: the container reads back, and each entry
+    is exactly the stream the CPU encoder makes from the oracle's matches (the GPU's are identical)."""
+    data = gcompress.corpus(n, 32, (15, 25, 20, 40), 4096) if n else np.zeros(0, np.uint8)
+    print(f"[e2e nvlz matches] n={n}", flush=True)
+    blob = gcompress.nvlz_compress_matches(data, dict_mb=1, threads=3)
+    algo, orig, ch, nch = gcompress.container_info(blob)
+    assert (algo, orig, ch, nch) == (gcompress.ALGO_LZMA2, n, SUB, -(-n // SUB))
+    assert gcompress.decompress(blob) == data.tobytes()
+    if n:
+        streams, _ = gcompress.cpu_encode_from_matches(data, oracle.lzma2_packed(data), threads=1)
+        assert blob[28 + 8 * nch:] == b"".join(streams)
