@@ -35,6 +35,7 @@ HOST_SOURCES = [
     "gc_codec.cpp",
     "gc_serve.cpp",
     "gc_lzma_enc.cpp",
+    "gc_stream.cpp",
 ]
 KERNEL_SOURCES = ["kernels.hip"]
 

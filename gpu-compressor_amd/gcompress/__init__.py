@@ -70,6 +70,7 @@ SIGNATURES = [
     ("gc_lzma2_encode_matches", C.c_int, [_vp, _sz, _vp, C.c_int, _vp, _sz, _vp]),
     ("gc_cpu_encode_from_matches", C.c_int, [_vp, _sz, _vp, C.c_int, _vp, _sz, _vp, _vp]),
     ("gc_nvlz_compress_matches", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
+    ("gc_decompress_file", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, _vp]),
     ("gc_container_bound", _sz, [_sz, C.c_uint32]),
     ("gc_nvzs_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_nvlz_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
@@ -454,6 +455,14 @@ def nvlz_compress_l0(data, dict_mb: int = 128, preset: int = 0, threads: int = 0
     _check(lib().gc_nvlz_compress_l0(_ptr(src), src.size, dict_mb, preset, threads, _ptr(out), cap, C.byref(n)),
            "gc_nvlz_compress_l0")
     return out[: n.value].tobytes()
+
+
+def decompress_file(in_path, out_path, threads: int = 0) -> int:
+    """decompress.rs:9-143, pipelined: container file -> original bytes in out_path; returns bytes written."""
+    n = C.c_uint64()
+    _check(lib().gc_decompress_file(os.fsencode(in_path), os.fsencode(out_path), threads, C.byref(n)),
+           "gc_decompress_file")
+    return n.value
 
 
 def container_info(blob: bytes):

@@ -59,6 +59,7 @@ typedef struct {
 #define GC_EFORMAT (-5) /* malformed NVZS / NVLZ container */
 #define GC_ESIZE (-6)   /* output buffer too small */
 #define GC_EBUDGET (-7) /* request larger than the admission budget */
+#define GC_EIO (-8)     /* file could not be opened, read or written */
 
 /* constants.rs:5-6,10-13 */
 #define GC_SUB_BLOCK 65536u
@@ -234,6 +235,13 @@ int gc_nvlz_compress(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, 
 size_t gc_nvlz_l0_bound(size_t n, uint32_t dict_mb);
 int gc_nvlz_compress_l0(const uint8_t* src, size_t n, uint32_t dict_mb, uint32_t preset, int threads, uint8_t* out,
                         size_t cap, size_t* out_len);
+
+/* decompress.rs:9-143 as a pipeline: the container file at in_path decoded
+ * into out_path, frames read by one thread, decoded by `threads` (<= 0:
+ * min(cores, 8)) and written in order, at most 2 x threads frames in memory.
+ * *out_bytes (optional) = bytes written.  GC_EIO for files that cannot be
+ * opened or written, GC_EFORMAT for a malformed or truncated container. */
+int gc_decompress_file(const char* in_path, const char* out_path, int threads, uint64_t* out_bytes);
 
 /* decompress.rs:35-143: header of an NVZS / NVLZ container. */
 int gc_container_info(const uint8_t* in, size_t n, uint32_t* algo, uint64_t* original_size, uint64_t* chunk_size,

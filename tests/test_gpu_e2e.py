@@ -84,3 +84,17 @@ def test_nvlz_from_gpu_matches_round_trip(n):
     if n:
         streams, _ = gcompress.cpu_encode_from_matches(data, oracle.lzma2_packed(data), threads=1)
         assert blob[28 + 8 * nch:] == b"".join(streams)
+
+
+@pytest.mark.parametrize("algo", ["nvzs", "nvlz", "nvlz_matches"])
+def test_containers_through_the_streaming_reader(tmp_path, algo):
+    """GPU-made containers written to a file and read back by the pipelined reader (decompress.rs:9-143)."""
+    data = gcompress.corpus(6 * SUB + 1000, 33, (1, 1, 1, 1), 4096)
+    print(f"[e2e stream] {algo}", flush=True)
+    blob = {"nvzs": lambda: gcompress.nvzs_compress(data, 1, threads=2),
+            "nvlz": lambda: gcompress.nvlz_compress(data, 1, dict_mb=1, threads=2),
+            "nvlz_matches": lambda: gcompress.nvlz_compress_matches(data, dict_mb=1, threads=2)}[algo]()
+    src, dst = tmp_path / "c.nv", tmp_path / "d.bin"
+    src.write_bytes(blob)
+    assert gcompress.decompress_file(src, dst, 4) == data.size
+    assert dst.read_bytes() == data.tobytes()
