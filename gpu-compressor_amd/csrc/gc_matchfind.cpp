@@ -270,4 +270,111 @@ int gc_zstd_find_sequences_packed(gc_ctx* ctx, const uint8_t* chunk, size_t n, u
     return e == hipSuccess ? GC_OK : hip_fail(e, "gc_zstd_find_sequences_packed: sequences");
 }
 
+int gc_zstd_encode_block_dev(gc_ctx* ctx, const uint8_t* d_input, const void* d_sequences, uint32_t seq_format,
+                             const uint8_t* d_literals, const uint32_t* d_seq_counts, const uint32_t* d_lit_counts,
+                             uint8_t* d_output, uint32_t* d_output_sizes, uint32_t chunk_size, uint32_t num_chunks,
+                             uint64_t total_input_size, uint32_t max_sequences_per_chunk, uint32_t max_frame_size) {
+    if (!ctx || !d_input || !d_sequences || !d_literals || !d_seq_counts || !d_lit_counts || !d_output ||
+        !d_output_sizes)
+        return fail(GC_EINVAL, "null argument");
+    if (int rc = check_sub(chunk_size)) return rc;
+    if (seq_format > GC_SEQ_FORMAT_REF) return fail(GC_EINVAL, "unknown seq_format");
+    if ((uint64_t)max_frame_size < (uint64_t)chunk_size + 16)
+        return fail(GC_EINVAL, "max_frame_size must be at least chunk_size + 16 (the raw frame)");
+    if ((uint64_t)num_chunks > div_up(total_input_size, chunk_size))
+        return fail(GC_EINVAL, "num_chunks exceeds the sub-chunks of total_input_size");
+    if (num_chunks == 0) return GC_OK;
+    std::vector<gc_item> items;
+    make_items(items, total_input_size, chunk_size, num_chunks, max_sequences_per_chunk, 0, true);
+    int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
+    if (rc) return rc;
+    return launch_zstd_encode(ctx, ctx->stream, d_input, (const gc_item*)ctx->items.p, num_chunks, d_sequences,
+                              seq_format, d_literals, d_seq_counts, d_lit_counts, d_output, d_output_sizes,
+                              max_frame_size, max_sequences_per_chunk);
+}
+
+int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out, size_t cap,
+                       uint64_t* sizes, size_t* out_len) {
+    if (!ctx || (!chunk && n) || !out || !sizes || !out_len) return fail(GC_EINVAL, "null argument");
+    *out_len = 0;
+    if (n == 0) return GC_OK;
+    const uint32_t max_seq = GC_SUB_BLOCK / 3;  // compress_zstd_custom.rs:55
+    const uint32_t slot = GC_SUB_BLOCK + 16;
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    if (!rc) rc = ctx->out_a.ensure((size_t)nsub * max_seq * sizeof(gc_zstd_sequence));
+    if (!rc) rc = ctx->out_b.ensure(nsub * 4);
+    if (!rc) rc = ctx->out_c.ensure((size_t)nsub * slot);  // literals, then the packed frames
+    if (!rc) rc = ctx->out_d.ensure(nsub * 4);
+    if (!rc) rc = ctx->out_e.ensure((size_t)nsub * slot);
+    if (!rc) rc = ctx->out_f.ensure(nsub * 12 + 16);
+    if (rc) return rc;
+    const uint8_t* d_in = (const uint8_t*)ctx->input.p;
+    uint32_t* d_sizes = (uint32_t*)ctx->out_f.p;
+    uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
+    rc = run_zstd(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p, GC_SEQ_FORMAT_ZSTD, (uint8_t*)ctx->out_c.p,
+                  (uint32_t*)ctx->out_b.p, (uint32_t*)ctx->out_d.p, zstd_depth_for_level(level), max_seq);
+    if (!rc)  // the items run_zstd uploaded describe the same sub-blocks
+        rc = launch_zstd_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, ctx->out_a.p,
+                                GC_SEQ_FORMAT_ZSTD, (const uint8_t*)ctx->out_c.p, (const uint32_t*)ctx->out_b.p,
+                                (const uint32_t*)ctx->out_d.p, (uint8_t*)ctx->out_e.p, d_sizes, slot, max_seq);
+    if (rc) return rc;
+    std::vector<uint32_t> fs(nsub);
+    hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: frame sizes");
+    std::vector<uint64_t> offs(nsub);
+    uint64_t t = 0;
+    for (uint64_t s = 0; s < nsub; s++) {
+        offs[s] = t;
+        sizes[s] = fs[s];
+        t += fs[s];
+    }
+    if (t > cap) return fail(GC_ESIZE, "output capacity below the frames' total size");
+    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: offsets");
+    // the packed frames go where the literals were (the input is no larger than the literal slots)
+    uint8_t* d_packed = (uint8_t*)ctx->out_c.p;
+    rc = launch_gather_bytes(ctx, ctx->stream, (const uint8_t*)ctx->out_e.p, d_sizes, d_offs, d_packed, slot,
+                             (uint32_t)nsub);
+    if (rc) return rc;
+    e = hipMemcpyAsync(out, d_packed, t, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: frames");
+    *out_len = (size_t)t;
+    return GC_OK;
+}
+
+int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
+                         size_t* out_len) {
+    if (!ctx || (!src && n) || !out || !out_len) return fail(GC_EINVAL, "null argument");
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    const size_t head = 28 + 8 * nsub;
+    if (cap < head + n + 16 * nsub) return fail(GC_ESIZE, "output capacity below gc_nvzs_gpu_bound");
+    memcpy(out, "NVZS", 4);  // pipeline.rs:187-260
+    auto put64 = [](uint8_t* p, uint64_t v) {
+        for (int i = 0; i < 8; i++) p[i] = (uint8_t)(v >> (8 * i));
+    };
+    put64(out + 4, n);
+    put64(out + 12, GC_SUB_BLOCK);
+    put64(out + 20, nsub);
+    std::vector<uint64_t> sizes(nsub, 0);
+    size_t o = head;
+    const size_t piece = (size_t)16 << 20;  // device scratch per call: sequence slots of 256 sub-blocks
+    for (size_t c0 = 0; c0 < n; c0 += piece) {
+        const size_t len = n - c0 < piece ? n - c0 : piece;
+        size_t w = 0;
+        int rc = gc_zstd_frames_gpu(ctx, src + c0, len, level, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
+        if (rc) return rc;
+        o += w;
+    }
+    for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
+    *out_len = o;
+    return GC_OK;
+}
+
+size_t gc_nvzs_gpu_bound(size_t n) { return 28 + 8 * (size_t)div_up(n, GC_SUB_BLOCK) + n + 16 * (size_t)div_up(n, GC_SUB_BLOCK); }
+
 }  // extern "C"

@@ -111,6 +111,27 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
     return launch(stream, ctx->k.zstd_parse[ref][lits], grid, 256, args, "launch gc_zstd_parse_kernel");
 }
 
+int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                       const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
+                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
+                       uint32_t max_seq) {
+    void* args[] = {&arena, &items,      &n_items, &seqs,  &seq_fmt,   &lits,
+                    &seq_counts, &lit_counts, &out, &sizes, &max_frame, &max_seq};
+    // four wavefronts (items) per workgroup; each workgroup first tabulates the FSE tables in LDS
+    uint32_t grid = (n_items + 3) / 4;
+    const uint32_t cap = (uint32_t)ctx->cu_count * 8;
+    if (grid > cap) grid = cap;
+    if (grid == 0) grid = 1;
+    return launch(stream, ctx->k.zstd_encode, grid, 256, args, "launch gc_zstd_encode_kernel");
+}
+
+int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
+                        const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub) {
+    void* args[] = {&in, &sizes, &offsets, &out, &slot, &nsub};
+    uint32_t grid = nsub < 4096 ? (nsub ? nsub : 1) : 4096;
+    return launch(stream, ctx->k.gather_bytes, grid, 256, args, "launch gc_gather_bytes");
+}
+
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n) {
     uint64_t blocks = (n + 255) / 256;
     uint32_t grid = (uint32_t)(blocks < 4096 ? (blocks ? blocks : 1) : 4096);
@@ -193,6 +214,8 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.copy_u32, ctx->k.module, "gc_copy_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.order_items, ctx->k.module, "gc_order_items");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_encode, ctx->k.module, "gc_zstd_encode_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_bytes, ctx->k.module, "gc_gather_bytes");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);
@@ -213,6 +236,8 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_b.release();
     ctx->out_c.release();
     ctx->out_d.release();
+    ctx->out_e.release();
+    ctx->out_f.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
     if (ctx->k.module_parse) (void)hipModuleUnload(ctx->k.module_parse);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);

@@ -40,6 +40,8 @@ struct Kernels {
     hipFunction_t gather_seqs = nullptr;
     hipFunction_t copy_u32 = nullptr;
     hipFunction_t order_items = nullptr;
+    hipFunction_t zstd_encode = nullptr;
+    hipFunction_t gather_bytes = nullptr;
 };
 
 }  // namespace gc
@@ -55,7 +57,7 @@ struct gc_ctx {
     gc::DevBuf cand;       // per-position candidates (u16)
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
-    gc::DevBuf out_a, out_b, out_c, out_d;
+    gc::DevBuf out_a, out_b, out_c, out_d, out_e, out_f;
     std::mutex mu;
 };
 
@@ -79,6 +81,14 @@ int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);
 int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order);
+// Zstd frames from sequences and literals (one wavefront per item), and the
+// packing of frames written in fixed slots.
+int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                       const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
+                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
+                       uint32_t max_seq);
+int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
+                        const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub);
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
 

@@ -1113,6 +1113,296 @@ GC_PARSE_KERNEL(gc_zstd_parse_ref_lits_kernel, GC_SEQ_REF, true)
 #endif  // GC_BUILD_PARSE
 
 #if GC_BUILD_MAIN
+// ---------------------------------------------------------------------------
+// Zstd frames on the GPU (the capability of zstd_compress.cu:756-1038): one
+// frame per sub-block from its sequences and literals, the sequences coded
+// with the format's predefined FSE distributions (RFC 8878 3.1.1.3.2.2) and
+// the literals raw, or a raw block when that is not smaller.
+//
+// One wavefront per sub-block.  The literal section is copied by the whole
+// wavefront.  Sequences are taken 64 at a time from the last: each lane
+// computes its sequence's three codes and extra-bit fields, then the FSE
+// state chain (serial by nature: each state follows from the next) runs on
+// the scalar unit, reading the lanes' fields with readlane and one LDS entry
+// per symbol: for (symbol, next state) the cell to move to and the bits to
+// write, tabulated per workgroup from the decoder's tables.  The bits go into
+// a 64-bit scalar accumulator; whole dwords are staged one per lane and
+// stored 256 bytes at a time.  Offsets are never coded as repeats
+// (Offset_Value = offset + 3).
+// ---------------------------------------------------------------------------
+#define ZE_THREADS 256
+#define ZE_WAVES (ZE_THREADS / WAVE)
+#define ZE_ENC_ML (36 * 64)
+#define ZE_ENC_OF (36 * 64 + 53 * 64)
+#define ZE_ENC_SIZE (36 * 64 + 53 * 64 + 29 * 32)
+
+__constant__ uint32_t ze_ll_base[36] = {0,  1,  2,   3,   4,   5,   6,    7,    8,    9,     10,    11,
+                                        12, 13, 14,  15,  16,  18,  20,   22,   24,   28,    32,    40,
+                                        48, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536};
+__constant__ uint8_t ze_ll_bits[36] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  1,  1,
+                                       1, 1, 2, 2, 3, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+__constant__ uint32_t ze_ml_base[53] = {3,  4,  5,  6,  7,  8,   9,   10,  11,  12,   13,   14,   15,   16,
+                                       17, 18, 19, 20, 21, 22,  23,  24,  25,  26,   27,   28,   29,   30,
+                                       31, 32, 33, 34, 35, 37,  39,  41,  43,  47,   51,   59,   67,   83,
+                                       99, 131, 259, 515, 1027, 2051, 4099, 8195, 16387, 32771, 65539};
+__constant__ uint8_t ze_ml_bits[53] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                       0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1,
+                                       2, 2, 3, 3, 4, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+// predefined distributions, LL then ML then OF (-1: one cell, "less than 1")
+__constant__ int8_t ze_norm[36 + 53 + 29] = {
+    4, 3, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 1, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 2, 3, 2, 1, 1, 1, 1, 1, -1, -1, -1, -1,
+    1, 4, 3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+    1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1,
+    1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1};
+
+struct ZeLds {
+    uint16_t enc[ZE_ENC_SIZE];  // per (symbol, next state): cell | bits << 6 | value << 9
+    uint8_t sym[160];           // decoder cells: LL 0..63, ML 64..127, OF 128..159
+    uint8_t nb[160];
+    uint16_t base[160];
+};
+
+// table t (0 LL, 1 ML, 2 OF): first cell, accuracy log, symbols, first norm
+__device__ __forceinline__ void ze_table(uint32_t t, uint32_t& cell0, uint32_t& log, uint32_t& nsym, uint32_t& norm0) {
+    cell0 = t == 0 ? 0u : (t == 1 ? 64u : 128u);
+    log = t == 2 ? 5u : 6u;
+    nsym = t == 0 ? 36u : (t == 1 ? 53u : 29u);
+    norm0 = t == 0 ? 0u : (t == 1 ? 36u : 89u);
+}
+
+__device__ void ze_build(ZeLds& s) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < 3) {  // the cells' symbols: "less than 1" symbols at the top, the rest spread
+        uint32_t c0, log, nsym, n0;
+        ze_table(tid, c0, log, nsym, n0);
+        const uint32_t size = 1u << log, mask = size - 1;
+        uint32_t high = size - 1;
+        for (uint32_t k = 0; k < nsym; k++)
+            if (ze_norm[n0 + k] == -1) s.sym[c0 + high--] = (uint8_t)k;
+        const uint32_t step = (size >> 1) + (size >> 3) + 3;
+        uint32_t pos = 0;
+        for (uint32_t k = 0; k < nsym; k++)
+            for (int r = 0; r < ze_norm[n0 + k]; r++) {
+                s.sym[c0 + pos] = (uint8_t)k;
+                do pos = (pos + step) & mask;
+                while (pos > high);
+            }
+    }
+    __syncthreads();
+    for (uint32_t c = tid; c < 160; c += blockDim.x) {  // bits and baseline of each cell
+        const uint32_t t = c < 64 ? 0u : (c < 128 ? 1u : 2u);
+        uint32_t c0, log, nsym, n0;
+        ze_table(t, c0, log, nsym, n0);
+        const uint32_t k = s.sym[c];
+        uint32_t x = ze_norm[n0 + k] == -1 ? 1u : (uint32_t)ze_norm[n0 + k];
+        for (uint32_t c2 = c0; c2 < c; c2++) x += s.sym[c2] == k ? 1u : 0u;
+        const uint32_t bits = log - (31u - (uint32_t)__builtin_clz(x));
+        s.nb[c] = (uint8_t)bits;
+        s.base[c] = (uint16_t)((x << bits) - (1u << log));
+    }
+    __syncthreads();
+    for (uint32_t e = tid; e < ZE_ENC_SIZE; e += blockDim.x) {  // the encoder's view
+        const uint32_t t = e < ZE_ENC_ML ? 0u : (e < ZE_ENC_OF ? 1u : 2u);
+        uint32_t c0, log, nsym, n0;
+        ze_table(t, c0, log, nsym, n0);
+        const uint32_t r = e - (t == 0 ? 0u : (t == 1 ? ZE_ENC_ML : ZE_ENC_OF));
+        const uint32_t k = r >> log, target = r & ((1u << log) - 1);
+        uint32_t v = 0;
+        for (uint32_t c = c0; c < c0 + (1u << log); c++)
+            if (s.sym[c] == k && s.base[c] <= target && target < (uint32_t)s.base[c] + (1u << s.nb[c]))
+                v = (c - c0) | ((uint32_t)s.nb[c] << 6) | ((target - s.base[c]) << 9);
+        s.enc[e] = (uint16_t)v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t ze_code(const uint32_t* base, uint32_t ncode, uint32_t v) {
+    uint32_t c = 0;
+    for (uint32_t k = 1; k < ncode; k++) c = base[k] <= v ? k : c;
+    return c;
+}
+
+__device__ __forceinline__ void ze_copy_bytes(uint8_t* dst, const uint8_t* src, uint32_t len) {
+    const uint32_t lane = lane_id();
+    uint32_t k = lane;
+    for (; k + 3 * WAVE < len; k += 4 * WAVE) {  // four loads in flight per lane
+        const uint8_t a = src[k], b = src[k + WAVE], c = src[k + 2 * WAVE], d = src[k + 3 * WAVE];
+        dst[k] = a, dst[k + WAVE] = b, dst[k + 2 * WAVE] = c, dst[k + 3 * WAVE] = d;
+    }
+    for (; k < len; k += WAVE) dst[k] = src[k];
+}
+
+template <uint32_t FMT>
+__device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, const void* seqs, const uint8_t* literals,
+                         const uint32_t* seq_counts, const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes,
+                         uint32_t max_frame, uint32_t max_seq) {
+    const uint32_t lane = lane_id();
+    const uint32_t n = it.n;
+    const uint32_t nseq = min(uniform(seq_counts[it.count_pos]), max_seq);  // reads stay in the item's slots
+    const uint32_t nlit = uniform(lit_counts[it.count_pos]);
+    uint8_t* dst = out + (uint64_t)it.count_pos * max_frame;
+    // frame header: magic, descriptor (single segment), content size
+    const uint32_t hdr = n >= 256 ? 7u : 6u;
+    if (lane < hdr) {
+        const uint32_t fcs = n >= 256 ? n - 256 : n;
+        const uint32_t b = lane < 4 ? ((0xFD2FB528u >> (8 * lane)) & 0xFF)
+                                    : (lane == 4 ? (n >= 256 ? 0x60u : 0x20u) : ((fcs >> (8 * (lane - 5))) & 0xFF));
+        dst[lane] = (uint8_t)b;
+    }
+    const uint32_t start = hdr + 3;  // after the block header
+    const uint32_t limit = start + n;  // a compressed block must end before this
+    bool raw = nlit + 2 >= n;         // literals alone would not be smaller
+    uint32_t end = 0;
+    if (!raw) {
+        const uint32_t lh = nlit < 32 ? 1u : (nlit < 4096 ? 2u : 3u);
+        const uint32_t sh = nseq < 128 ? 1u : 2u;
+        // literal section header (raw literals, 5-, 12- or 20-bit size)
+        if (lane < lh) {
+            const uint32_t v = lh == 1 ? (nlit << 3) : (lh == 2 ? ((nlit << 4) | 4u) : ((nlit << 4) | 12u));
+            dst[start + lane] = (uint8_t)(v >> (8 * lane));
+        }
+        ze_copy_bytes(dst + start + lh, literals + it.lit_pos, nlit);
+        uint32_t o = start + lh + nlit;
+        if (lane < sh) {
+            const uint32_t v = sh == 1 ? nseq : (lane == 0 ? (nseq >> 8) + 0x80 : nseq & 0xFF);
+            dst[o + lane] = (uint8_t)v;
+        }
+        o += sh;
+        if (nseq) {
+            if (lane == 0) dst[o] = 0;  // predefined modes for LL, OF and ML
+            o += 1;
+        }
+        // the bitstream: scalar accumulator, dwords staged one per lane
+        uint64_t acc = 0;
+        uint32_t nbits = 0, nd = 0, st = 0, wpos = o;
+        bool over = false;
+        auto put = [&](uint32_t v, uint32_t k) {
+            acc |= (uint64_t)v << nbits;
+            nbits += k;
+            if (nbits >= 32) {
+                st = lane == (nd & (WAVE - 1)) ? (uint32_t)acc : st;
+                nd++;
+                acc >>= 32;
+                nbits -= 32;
+                if ((nd & (WAVE - 1)) == 0) {
+                    if (wpos + 4 * WAVE < limit) {
+                        uint8_t* q = dst + wpos + 4 * lane;
+                        q[0] = (uint8_t)st, q[1] = (uint8_t)(st >> 8), q[2] = (uint8_t)(st >> 16), q[3] = (uint8_t)(st >> 24);
+                    } else {
+                        over = true;
+                    }
+                    wpos += 4 * WAVE;
+                }
+            }
+        };
+        uint32_t sl = 0, sm = 0, so = 0;
+        for (int32_t top = (int32_t)nseq; top > 0 && !over; top -= WAVE) {
+            const int32_t i0 = top > WAVE ? top - WAVE : 0;
+            const uint32_t cnt = (uint32_t)(top - i0);
+            uint32_t codes = 0, xl = 0, xm = 0, xo = 0;
+            if (lane < cnt) {
+                uint32_t off, ll, ml;
+                const uint64_t r = it.seq_pos + (uint64_t)i0 + lane;
+                if constexpr (FMT == GC_SEQ_REF) {
+                    const gc_ref_sequence q = ((const gc_ref_sequence*)seqs)[r];
+                    ll = q.literal_length, ml = q.match_length_minus3 + 3, off = q.offset;
+                } else {
+                    const uint4 q = ((const uint4*)seqs)[r];
+                    off = q.x, ll = q.y, ml = q.z;
+                }
+                const uint32_t lc = ze_code(ze_ll_base, 36, ll), mc = ze_code(ze_ml_base, 53, ml);
+                const uint32_t ofv = off + 3, oc = min(31u - (uint32_t)__builtin_clz(ofv), 28u);  // 16 at most here
+                codes = lc | (mc << 8) | (oc << 16);
+                xl = (ll - ze_ll_base[lc]) | ((uint32_t)ze_ll_bits[lc] << 16);
+                xm = (ml - ze_ml_base[mc]) | ((uint32_t)ze_ml_bits[mc] << 16);
+                xo = (ofv - (1u << oc)) | (oc << 16);
+            }
+            for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
+                const uint32_t cd = bcast(codes, (uint32_t)j);
+                const uint32_t lc = cd & 0xFF, mc = (cd >> 8) & 0xFF, oc = cd >> 16;
+                if (i0 + j == (int32_t)nseq - 1) {
+                    // the last sequence's states: the cells reached from state 0
+                    sl = uniform(s.enc[lc * 64]) & 63;
+                    sm = uniform(s.enc[ZE_ENC_ML + mc * 64]) & 63;
+                    so = uniform(s.enc[ZE_ENC_OF + oc * 32]) & 63;
+                } else {
+                    const uint32_t el = uniform(s.enc[lc * 64 + sl]);
+                    const uint32_t em = uniform(s.enc[ZE_ENC_ML + mc * 64 + sm]);
+                    const uint32_t eo = uniform(s.enc[ZE_ENC_OF + oc * 32 + so]);
+                    // state updates, read as LL, ML, OF: written in reverse
+                    put(eo >> 9, (eo >> 6) & 7);
+                    put(em >> 9, (em >> 6) & 7);
+                    put(el >> 9, (el >> 6) & 7);
+                    sl = el & 63, sm = em & 63, so = eo & 63;
+                }
+                // extra bits, read as OF, ML, LL: written in reverse
+                const uint32_t vl = bcast(xl, (uint32_t)j), vm = bcast(xm, (uint32_t)j), vo = bcast(xo, (uint32_t)j);
+                put(vl & 0xFFFF, vl >> 16);
+                put(vm & 0xFFFF, vm >> 16);
+                put(vo & 0xFFFF, vo >> 16);
+            }
+        }
+        if (nseq) {
+            // initial states, read as LL, OF, ML; then the end mark
+            put(sm, 6);
+            put(so, 5);
+            put(sl, 6);
+            put(1, 1);
+        }
+        const uint32_t pend = nd & (WAVE - 1), tail = (nbits + 7) / 8;
+        end = wpos + 4 * pend + tail;
+        if (over || end >= limit) {
+            raw = true;
+        } else {
+            if (lane < pend) {
+                uint8_t* q = dst + wpos + 4 * lane;
+                q[0] = (uint8_t)st, q[1] = (uint8_t)(st >> 8), q[2] = (uint8_t)(st >> 16), q[3] = (uint8_t)(st >> 24);
+            }
+            if (lane < tail) dst[wpos + 4 * pend + lane] = (uint8_t)(acc >> (8 * lane));
+        }
+        if (raw) __threadfence();  // the raw block below overwrites the bytes written so far
+    }
+    if (raw) {
+        ze_copy_bytes(dst + start, arena + it.src_off, n);
+        end = start + n;
+    }
+    if (lane < 3) {
+        const uint32_t bh = raw ? (1u | (n << 3)) : (1u | (2u << 1) | ((end - start) << 3));
+        dst[hdr + lane] = (uint8_t)(bh >> (8 * lane));
+    }
+    if (lane == 0) sizes[it.count_pos] = end;
+}
+
+extern "C" __global__ void __launch_bounds__(ZE_THREADS)
+gc_zstd_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const void* seqs,
+                      uint32_t seq_fmt, const uint8_t* literals, const uint32_t* seq_counts,
+                      const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
+                      uint32_t max_seq) {
+    __shared__ ZeLds s;
+    ze_build(s);
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    for (uint32_t i = blockIdx.x * ZE_WAVES + wave; i < n_items; i += gridDim.x * ZE_WAVES) {
+        const gc_item it = load_item(items, i);
+        if (seq_fmt == GC_SEQ_REF)
+            ze_frame<GC_SEQ_REF>(s, it, arena, seqs, literals, seq_counts, lit_counts, out, sizes, max_frame, max_seq);
+        else
+            ze_frame<GC_SEQ_ZSTD>(s, it, arena, seqs, literals, seq_counts, lit_counts, out, sizes, max_frame,
+                                  max_seq);
+    }
+}
+
+// Packs frames: out[offsets[s] + i] = in[s * slot + i] for i < sizes[s].
+extern "C" __global__ void __launch_bounds__(256)
+gc_gather_bytes(const uint8_t* in, const uint32_t* sizes, const uint64_t* offsets, uint8_t* out, uint32_t slot,
+                uint32_t nsub) {
+    for (uint32_t s = blockIdx.x; s < nsub; s += gridDim.x) {
+        const uint32_t c = sizes[s];
+        const uint8_t* src = in + (uint64_t)s * slot;
+        uint8_t* dst = out + offsets[s];
+        for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) dst[i] = src[i];
+    }
+}
+
 // Small fill kernel.
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)

@@ -71,6 +71,11 @@ SIGNATURES = [
     ("gc_cpu_encode_from_matches", C.c_int, [_vp, _sz, _vp, C.c_int, _vp, _sz, _vp, _vp]),
     ("gc_nvlz_compress_matches", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_decompress_file", C.c_int, [C.c_char_p, C.c_char_p, C.c_int, _vp]),
+    ("gc_zstd_encode_block_dev", C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp, _vp, _vp, _vp, _vp, C.c_uint32,
+                                           C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("gc_zstd_frames_gpu", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp, _vp]),
+    ("gc_nvzs_gpu_bound", _sz, [_sz]),
+    ("gc_nvzs_compress_gpu", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp]),
     ("gc_container_bound", _sz, [_sz, C.c_uint32]),
     ("gc_nvzs_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_nvlz_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
@@ -424,6 +429,34 @@ def nvlz_compress_matches(data, dict_mb: int = 128, device_id: int = 0, threads:
     n = C.c_size_t()
     _check(lib().gc_nvlz_compress_matches(context(device_id).h, _ptr(src), src.size, dict_mb, threads, _ptr(out),
                                           cap, C.byref(n)), "gc_nvlz_compress_matches")
+    return out[: n.value].tobytes()
+
+
+def zstd_frames_gpu(data, level: int = 1, device_id: int = 0):
+    """Match finding and Zstd frame coding on the GPU -> (frames, sizes), one frame per 64 KiB."""
+    src = _as_u8(data)
+    nsub = -(-src.size // SUB_BLOCK)
+    cap = src.size + 16 * max(nsub, 1)
+    out = np.empty(max(cap, 1), np.uint8)
+    sizes = np.zeros(max(nsub, 1), np.uint64)
+    n = C.c_size_t()
+    _check(lib().gc_zstd_frames_gpu(context(device_id).h, _ptr(src), src.size, level, _ptr(out), cap, _ptr(sizes),
+                                    C.byref(n)), "gc_zstd_frames_gpu")
+    frames, o = [], 0
+    for z in sizes[:nsub]:
+        frames.append(out[o:o + int(z)].tobytes())
+        o += int(z)
+    return frames, [int(z) for z in sizes[:nsub]]
+
+
+def nvzs_compress_gpu(data, level: int = 1, device_id: int = 0) -> bytes:
+    """NVZS container whose frames are coded on the GPU."""
+    src = _as_u8(data)
+    cap = lib().gc_nvzs_gpu_bound(src.size)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvzs_compress_gpu(context(device_id).h, _ptr(src), src.size, level, _ptr(out), cap, C.byref(n)),
+           "gc_nvzs_compress_gpu")
     return out[: n.value].tobytes()
 
 

@@ -216,6 +216,30 @@ int gc_cpu_encode_from_matches(const uint8_t* src, size_t n, const uint32_t* pac
 int gc_nvlz_compress_matches(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, int threads, uint8_t* out,
                              size_t cap, size_t* out_len);
 
+/* zstd_encode_block (zstd_compress.cu:756-1038, not called by the reference's
+ * pipeline): one Zstd frame per sub-chunk from the match finder's sequences
+ * (seq_format as gc_zstd_match_find_dev) and literals (d_literals at
+ * s * chunk_size, d_lit_counts), written at d_output + s * max_frame_size,
+ * its size in d_output_sizes[s].  The sequences are coded with the format's
+ * predefined FSE distributions and the literals raw; a sub-chunk whose frame
+ * would not be smaller than its bytes gets a raw block (d_input).  Device
+ * pointers, asynchronous on the context's stream; max_frame_size >=
+ * chunk_size + 16.  Frames are decodable by any Zstd decoder. */
+int gc_zstd_encode_block_dev(gc_ctx* ctx, const uint8_t* d_input, const void* d_sequences, uint32_t seq_format,
+                             const uint8_t* d_literals, const uint32_t* d_seq_counts, const uint32_t* d_lit_counts,
+                             uint8_t* d_output, uint32_t* d_output_sizes, uint32_t chunk_size, uint32_t num_chunks,
+                             uint64_t total_input_size, uint32_t max_sequences_per_chunk, uint32_t max_frame_size);
+/* Host chunk -> match finding and frame coding on the GPU: one frame per
+ * 64 KiB sub-block, packed in order into out, sizes[s]; cap >= n + 16 per
+ * sub-block suffices.  The GPU counterpart of gc_compress_chunk_zstd_custom
+ * (libzstd's entropy coding on the CPU): faster, larger. */
+int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out, size_t cap,
+                       uint64_t* sizes, size_t* out_len);
+/* NVZS container with GPU-coded frames (decoded by gc_decompress). */
+size_t gc_nvzs_gpu_bound(size_t n);
+int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
+                         size_t* out_len);
+
 /* Largest NVZS (algo GC_ALGO_ZSTD) / NVLZ level >= 1 container for n bytes. */
 size_t gc_container_bound(size_t n, uint32_t algo);
 

@@ -30,6 +30,8 @@ def lib():
         L.orc_hash4.argtypes = [vp]
         L.orc_zstd_match_find.restype = C.c_int
         L.orc_zstd_match_find.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_int]
+        L.orc_zstd_frame_predef.restype = C.c_size_t
+        L.orc_zstd_frame_predef.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, C.c_size_t]
         L.orc_lzma2_match_find.restype = C.c_int
         L.orc_lzma2_match_find.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_int]
         _lib = L
@@ -101,3 +103,18 @@ def zstd_as_zstd_sequences(seq: np.ndarray, counts: np.ndarray):
         z[:, 2] = r[:, 1] + 3
         out.append(z)
     return out
+
+
+def zstd_frame_predef(sub, rows) -> bytes:
+    """One Zstd frame for a sub-block (<= 65536 bytes) from its ZSTD_Sequence rows {offset, litLength,
+    matchLength, rep}, with the format's predefined FSE tables and raw literals: the checker of the GPU
+    sequence encoder."""
+    src = _u8(sub)
+    r = np.ascontiguousarray(rows, dtype=np.uint32).reshape(-1, 4)
+    cap = src.size + 64
+    out = np.empty(cap, np.uint8)
+    n = lib().orc_zstd_frame_predef(src.ctypes.data, src.size, r.ctypes.data if r.size else None, r.shape[0],
+                                    out.ctypes.data, cap)
+    if n == 0:
+        raise RuntimeError("frame did not fit")
+    return out[:n].tobytes()

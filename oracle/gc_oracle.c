@@ -289,3 +289,216 @@ int orc_lzma2_match_find(const uint8_t* in, uint64_t total, uint32_t sub, uint32
     j.cnt_a = counts;
     return orc_run(&j, threads);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Zstd frame of one sub-block from its sequences, with the format's         */
+/* predefined FSE distributions and raw literals (RFC 8878 3.1.1.3.2.2,      */
+/* 4.1.1).  Not a restatement of the reference (its GPU encoder,             */
+/* zstd_compress.cu:756-1038, is dead code): the checker of the GPU encoder, */
+/* itself checked by decoding its frames with libzstd.                       */
+/* ------------------------------------------------------------------------ */
+
+/* Literals_Length and Match_Length codes: baseline and extra bits. */
+static const uint32_t orc_ll_base[36] = {0,  1,  2,   3,   4,   5,   6,    7,    8,    9,     10,    11,
+                                         12, 13, 14,  15,  16,  18,  20,   22,   24,   28,    32,    40,
+                                         48, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536};
+static const uint8_t orc_ll_bits[36] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  1,  1,
+                                        1, 1, 2, 2, 3, 3, 4, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+static const uint32_t orc_ml_base[53] = {3,  4,  5,  6,  7,  8,   9,   10,  11,  12,   13,   14,   15,   16,
+                                         17, 18, 19, 20, 21, 22,  23,  24,  25,  26,   27,   28,   29,   30,
+                                         31, 32, 33, 34, 35, 37,  39,  41,  43,  47,   51,   59,   67,   83,
+                                         99, 131, 259, 515, 1027, 2051, 4099, 8195, 16387, 32771, 65539};
+static const uint8_t orc_ml_bits[53] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,
+                                        0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1,
+                                        2, 2, 3, 3, 4, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+/* Predefined distributions (-1: a "less than 1" probability, one cell). */
+static const int16_t orc_ll_norm[36] = {4, 3, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 1, 1, 1, 2, 2,
+                                        2, 2, 2, 2, 2, 2, 2, 3, 2, 1, 1, 1, 1, 1, -1, -1, -1, -1};
+static const int16_t orc_ml_norm[53] = {1, 4, 3, 2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                        1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1,
+                                        1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1};
+static const int16_t orc_of_norm[29] = {1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1,
+                                        1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1};
+
+/* An FSE table as the decoder builds it: per cell its symbol, bits to read
+ * and baseline of the next state. */
+typedef struct {
+    uint32_t log, size;
+    uint8_t sym[64], nb[64];
+    uint16_t base[64];
+} orc_fse;
+
+static void orc_fse_build(orc_fse* t, const int16_t* norm, uint32_t nsym, uint32_t log) {
+    const uint32_t size = 1u << log, mask = size - 1;
+    uint32_t high = size - 1, next[64];
+    t->log = log;
+    t->size = size;
+    for (uint32_t s = 0; s < nsym; s++) {
+        if (norm[s] == -1) {
+            t->sym[high--] = (uint8_t)s;
+            next[s] = 1;
+        } else {
+            next[s] = (uint32_t)norm[s];
+        }
+    }
+    const uint32_t step = (size >> 1) + (size >> 3) + 3;
+    uint32_t pos = 0;
+    for (uint32_t s = 0; s < nsym; s++)
+        for (int k = 0; k < norm[s]; k++) {
+            t->sym[pos] = (uint8_t)s;
+            do pos = (pos + step) & mask;
+            while (pos > high);
+        }
+    for (uint32_t c = 0; c < size; c++) {
+        const uint32_t x = next[t->sym[c]]++;
+        uint32_t hb = 31;
+        while (!(x >> hb)) hb--;
+        t->nb[c] = (uint8_t)(log - hb);
+        t->base[c] = (uint16_t)((x << t->nb[c]) - size);
+    }
+}
+
+/* The cell of symbol s whose next-state range holds `target`. */
+static uint32_t orc_fse_cell(const orc_fse* t, uint32_t s, uint32_t target) {
+    for (uint32_t c = 0; c < t->size; c++)
+        if (t->sym[c] == s && t->base[c] <= target && target < (uint32_t)t->base[c] + (1u << t->nb[c])) return c;
+    return 0xFFFFFFFFu;
+}
+
+typedef struct {
+    uint8_t* p;
+    size_t n, cap;
+    uint64_t acc;
+    uint32_t bits;
+    int full;
+} orc_bits;
+
+static void orc_put_bits(orc_bits* b, uint32_t v, uint32_t nb) {
+    b->acc |= (uint64_t)v << b->bits;
+    b->bits += nb;
+    while (b->bits >= 8) {
+        if (b->n < b->cap)
+            b->p[b->n++] = (uint8_t)b->acc;
+        else
+            b->full = 1;
+        b->acc >>= 8;
+        b->bits -= 8;
+    }
+}
+
+static uint32_t orc_code(const uint32_t* base, uint32_t ncode, uint32_t v) {
+    uint32_t c = 0;
+    while (c + 1 < ncode && base[c + 1] <= v) c++;
+    return c;
+}
+
+static uint32_t orc_highbit(uint32_t v) {
+    uint32_t h = 0;
+    while (v >> (h + 1)) h++;
+    return h;
+}
+
+/*
+ * One frame for src[0..n) (n <= 65536) from nseq ZSTD_Sequence rows
+ * {offset, litLength, matchLength, rep} (offsets never coded as repeats:
+ * Offset_Value = offset + 3).  Returns the frame size, or 0 if it does not
+ * fit `cap`.
+ */
+size_t orc_zstd_frame_predef(const uint8_t* src, uint32_t n, const uint32_t* seqs, uint32_t nseq, uint8_t* out,
+                             size_t cap) {
+    orc_fse ll, ml, of;
+    orc_fse_build(&ll, orc_ll_norm, 36, 6);
+    orc_fse_build(&ml, orc_ml_norm, 53, 6);
+    orc_fse_build(&of, orc_of_norm, 29, 5);
+    if (cap < 16) return 0;
+    size_t o = 0;
+    out[o++] = 0x28, out[o++] = 0xB5, out[o++] = 0x2F, out[o++] = 0xFD;
+    if (n >= 256) {
+        out[o++] = 0x60; /* 2-byte content size, single segment */
+        out[o++] = (uint8_t)(n - 256), out[o++] = (uint8_t)((n - 256) >> 8);
+    } else {
+        out[o++] = 0x20;
+        out[o++] = (uint8_t)n;
+    }
+    const size_t bh = o; /* block header */
+    o += 3;
+    const size_t start = o;
+    /* literals: every byte outside the matches, in order */
+    uint32_t nlit = 0, p = 0;
+    for (uint32_t i = 0; i < nseq; i++) {
+        nlit += seqs[4 * i + 1];
+        p += seqs[4 * i + 1] + seqs[4 * i + 2];
+    }
+    nlit += n - p;
+    int compressed = 1;
+    if (o + 3 + nlit + 3 > cap) compressed = 0;
+    if (compressed) {
+        if (nlit < 32) {
+            out[o++] = (uint8_t)(nlit << 3);
+        } else if (nlit < 4096) {
+            out[o++] = (uint8_t)((nlit << 4) | 4), out[o++] = (uint8_t)(nlit >> 4);
+        } else {
+            out[o++] = (uint8_t)((nlit << 4) | 12), out[o++] = (uint8_t)(nlit >> 4), out[o++] = (uint8_t)(nlit >> 12);
+        }
+        p = 0;
+        for (uint32_t i = 0; i < nseq; i++) {
+            memcpy(out + o, src + p, seqs[4 * i + 1]);
+            o += seqs[4 * i + 1];
+            p += seqs[4 * i + 1] + seqs[4 * i + 2];
+        }
+        memcpy(out + o, src + p, n - p);
+        o += n - p;
+        /* sequences section */
+        if (nseq < 128) {
+            out[o++] = (uint8_t)nseq;
+        } else {
+            out[o++] = (uint8_t)((nseq >> 8) + 0x80), out[o++] = (uint8_t)nseq;
+        }
+        if (nseq) {
+            out[o++] = 0x00; /* predefined modes */
+            orc_bits b = {out + o, 0, cap - o, 0, 0, 0};
+            uint32_t sl = 0, sm = 0, so = 0;
+            for (int32_t i = (int32_t)nseq - 1; i >= 0; i--) {
+                const uint32_t lit = seqs[4 * i + 1], mlen = seqs[4 * i + 2], ofv = seqs[4 * i] + 3;
+                const uint32_t lc = orc_code(orc_ll_base, 36, lit), mc = orc_code(orc_ml_base, 53, mlen);
+                const uint32_t oc = orc_highbit(ofv);
+                if (i == (int32_t)nseq - 1) {
+                    /* the last sequence's states: any cell of its symbols (the one reached from 0) */
+                    sl = orc_fse_cell(&ll, lc, 0), sm = orc_fse_cell(&ml, mc, 0), so = orc_fse_cell(&of, oc, 0);
+                } else {
+                    const uint32_t cl = orc_fse_cell(&ll, lc, sl), cm = orc_fse_cell(&ml, mc, sm),
+                                   co = orc_fse_cell(&of, oc, so);
+                    /* state updates, read as LL, ML, OF: written in reverse */
+                    orc_put_bits(&b, so - of.base[co], of.nb[co]);
+                    orc_put_bits(&b, sm - ml.base[cm], ml.nb[cm]);
+                    orc_put_bits(&b, sl - ll.base[cl], ll.nb[cl]);
+                    sl = cl, sm = cm, so = co;
+                }
+                /* extra bits, read as OF, ML, LL: written in reverse */
+                orc_put_bits(&b, lit - orc_ll_base[lc], orc_ll_bits[lc]);
+                orc_put_bits(&b, mlen - orc_ml_base[mc], orc_ml_bits[mc]);
+                orc_put_bits(&b, ofv - (1u << oc), oc);
+            }
+            /* initial states, read as LL, OF, ML */
+            orc_put_bits(&b, sm, ml.log);
+            orc_put_bits(&b, so, of.log);
+            orc_put_bits(&b, sl, ll.log);
+            orc_put_bits(&b, 1, 1); /* end mark */
+            if (b.bits) orc_put_bits(&b, 0, 8 - b.bits);
+            if (b.full) compressed = 0;
+            o += b.n;
+        }
+        if (o - start >= n) compressed = 0;
+    }
+    if (!compressed) { /* raw block */
+        if (start + n > cap) return 0;
+        memcpy(out + start, src, n);
+        o = start + n;
+        const uint32_t h = 1u | (0u << 1) | (n << 3);
+        out[bh] = (uint8_t)h, out[bh + 1] = (uint8_t)(h >> 8), out[bh + 2] = (uint8_t)(h >> 16);
+        return o;
+    }
+    const uint32_t h = 1u | (2u << 1) | ((uint32_t)(o - start) << 3);
+    out[bh] = (uint8_t)h, out[bh + 1] = (uint8_t)(h >> 8), out[bh + 2] = (uint8_t)(h >> 16);
+    return o;
+}

@@ -1,0 +1,105 @@
+"""GPU Zstd frame coding (gc_zstd_encode_block_dev, gc_zstd_frames_gpu; the capability of
+zstd_compress.cu:756-1038): every frame equals the oracle's frame for the same sequences byte for byte
+(oracle/gc_oracle.c orc_zstd_frame_predef, itself checked by libzstd) and decodes with libzstd."""
+import numpy as np
+import pytest
+
+import datagen
+import gcompress
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+SUB = 65536
+
+
+def _cases():
+    rng = np.random.default_rng(77)
+    out = [(f"mix{n}", datagen.mix(n, n % 97)) for n in (1, 3, 100, 255, 256, 4095, 4096, 65535, 65536,
+                                                          3 * 65536 + 777)]
+    out += [("text", datagen.text(2 * SUB + 5, 1)), ("backrefs", datagen.backrefs(2 * SUB, 2)),
+            ("structured", datagen.structured(2 * SUB, 3)), ("random", datagen.random_bytes(SUB, 4)),
+            ("zero", np.zeros(SUB + 3, np.uint8)), ("period5", datagen.periodic(SUB, 5, 5))]
+    # sub-blocks of every kind side by side
+    out.append(("kinds", np.concatenate([datagen.random_bytes(SUB, 8), np.zeros(SUB, np.uint8),
+                                         datagen.text(SUB, 9), rng.integers(0, 4, SUB, dtype=np.uint8)])))
+    return out
+
+
+CASES = _cases()
+
+
+def oracle_frames(data, level):
+    depth = {1: 16, 2: 64}.get(level, 1)
+    seq, sc, _, _ = oracle.zstd_match_find(data, depth=depth)
+    rows = oracle.zstd_as_zstd_sequences(seq, sc)
+    return [oracle.zstd_frame_predef(data[s * SUB:(s + 1) * SUB], rows[s]) for s in range(len(rows))]
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+@pytest.mark.parametrize("name,data", CASES, ids=[c[0] for c in CASES])
+def test_frames_equal_oracle_and_decode(gctx, name, data, level):
+    print(f"[zenc] {name} n={data.size} level={level}", flush=True)
+    frames, sizes = gcompress.zstd_frames_gpu(data, level)
+    want = oracle_frames(data, level)
+    assert len(frames) == len(want)
+    for s, (f, w) in enumerate(zip(frames, want)):
+        sub = data[s * SUB:(s + 1) * SUB].tobytes()
+        assert len(f) == sizes[s]
+        assert f == w, f"sub-block {s}: frame differs from the oracle's ({len(f)} vs {len(w)} bytes)"
+        assert gcompress.zstd_frame_decode(f, len(sub)) == sub
+
+
+def _dev(ctx, a):
+    p = ctx.alloc(max(a.nbytes, 1) + 64)
+    if a.nbytes:
+        ctx.htod(p, a)
+    return p
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+@pytest.mark.parametrize("chunk", [65536, 4096, 999])
+def test_encode_block_dev_reference_abi(gctx, fmt, chunk):
+    """gc_zstd_match_find_dev (with literals) then gc_zstd_encode_block_dev: the reference's pair of kernels,
+    both record layouts, sub-chunk sizes other than 64 KiB."""
+    print(f"[zenc abi] fmt={fmt} chunk={chunk}", flush=True)
+    data = datagen.mix(2 * 65536 + 4321, 12)
+    n = data.size
+    nsub = -(-n // chunk)
+    max_seq = chunk // 3
+    rec = 16 if fmt == 0 else 12
+    slot = chunk + 16
+    L = gcompress.lib()
+    d_in = _dev(gctx, data)
+    d_s, d_l = gctx.alloc(nsub * max_seq * rec), gctx.alloc(nsub * chunk)
+    d_sc, d_lc = gctx.alloc(nsub * 4), gctx.alloc(nsub * 4)
+    d_o, d_os = gctx.alloc(nsub * slot), gctx.alloc(nsub * 4)
+    rc = L.gc_zstd_match_find_dev(gctx.h, d_in, d_s, fmt, d_l, d_sc, d_lc, chunk, nsub, n, 16, max_seq)
+    assert rc == 0, L.gc_last_error()
+    rc = L.gc_zstd_encode_block_dev(gctx.h, d_in, d_s, fmt, d_l, d_sc, d_lc, d_o, d_os, chunk, nsub, n, max_seq, slot)
+    assert rc == 0, L.gc_last_error()
+    frames = np.zeros(nsub * slot, np.uint8)
+    sizes = np.zeros(nsub, np.uint32)
+    gctx.dtoh(frames, d_o)
+    gctx.dtoh(sizes, d_os)
+    for p in (d_in, d_s, d_l, d_sc, d_lc, d_o, d_os):
+        gctx.free(p)
+    seq, sc, _, _ = oracle.zstd_match_find(data, chunk, 16, max_seq)
+    rows = oracle.zstd_as_zstd_sequences(seq, sc)
+    for s in range(nsub):
+        sub = data[s * chunk:(s + 1) * chunk]
+        f = frames[s * slot: s * slot + sizes[s]].tobytes()
+        assert f == oracle.zstd_frame_predef(sub, rows[s])
+        assert gcompress.zstd_frame_decode(f, sub.size) == sub.tobytes()
+    # a frame slot smaller than the raw frame is refused
+    assert L.gc_zstd_encode_block_dev(gctx.h, 8, 8, fmt, 8, 8, 8, 8, 8, chunk, nsub, n, max_seq, chunk) != 0
+
+
+@pytest.mark.parametrize("n", [0, 7, SUB, 5 * SUB + 9])
+def test_nvzs_gpu_container(gctx, n):
+    data = gcompress.corpus(n, 34, (1, 1, 1, 1), 4096) if n else np.zeros(0, np.uint8)
+    print(f"[zenc nvzs] n={n}", flush=True)
+    blob = gcompress.nvzs_compress_gpu(data, 1)
+    algo, orig, ch, nch = gcompress.container_info(blob)
+    assert (algo, orig, ch, nch) == (gcompress.ALGO_ZSTD, n, SUB, -(-n // SUB))
+    assert gcompress.decompress(blob) == data.tobytes()
