@@ -253,6 +253,50 @@ def encode_from_gpu_matches(batch, data, inputs, sample_bytes):
             "preset9_ratio_first_4mib": round(k / p9, 4)}
 
 
+def gpu_frames(ctx, hip, stream, d_in, data, inputs, sample_bytes):
+    """Zstd frames coded on the GPU (gc_zstd_encode_block_dev) for the first Zstd L1 input's bytes, up to
+    sample_bytes from its start, as one buffer of 64 KiB sub-chunks: the match finder with literals first
+    (untimed), then the coding kernel timed with HIP events (3 runs).  Ratio of the frames beside
+    libzstd's (ZSTD_compressSequences on the same sequences, first 16 MiB)."""
+    L = gcompress.lib()
+    o, s, _, _ = next(x for x in inputs if x[2] == gcompress.ALGO_ZSTD)
+    n = min(sample_bytes, data.size - o)
+    nsub = -(-n // 65536)
+    max_seq = 65536 // 3
+    slot = 65536 + 16
+    d_s, d_l = ctx.alloc(nsub * max_seq * 16), ctx.alloc(nsub * 65536)
+    d_sc, d_lc = ctx.alloc(nsub * 4), ctx.alloc(nsub * 4)
+    d_o, d_os = ctx.alloc(nsub * slot), ctx.alloc(nsub * 4)
+    rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_s, 0, d_l, d_sc, d_lc, 65536, nsub, n, 16, max_seq)
+    assert rc == 0, L.gc_last_error()
+    ctx.sync()
+    ms = []
+    for _ in range(3):
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        rc = L.gc_zstd_encode_block_dev(ctx.h, d_in + o, d_s, 0, d_l, d_sc, d_lc, d_o, d_os, 65536, nsub, n, max_seq,
+                                        slot)
+        assert rc == 0, L.gc_last_error()
+        hip.record(e1, stream)
+        ms.append(hip.elapsed_ms(e0, e1))
+    sizes = np.zeros(nsub, np.uint32)
+    ctx.dtoh(sizes, d_os)
+    k = min(nsub, 256)
+    seqs = np.zeros((k, max_seq, 4), np.uint32)
+    counts = np.zeros(k, np.uint32)
+    ctx.dtoh(seqs, d_s)
+    ctx.dtoh(counts, d_sc)
+    for p_ in (d_s, d_l, d_sc, d_lc, d_o, d_os):
+        ctx.free(p_)
+    x = np.ascontiguousarray(data[o:o + min(n, k * 65536)])
+    rows = np.concatenate([seqs[b, :counts[b]] for b in range(k)] + [np.zeros((1, 4), np.uint32)])
+    _, zs = gcompress.zstd_encode_sequences(x, rows, counts, threads=min(os.cpu_count() or 1, 16))
+    kms = float(np.median(ms))
+    return {"sample_mib": n // MiB, "kernel_ms": round(kms, 3), "encode_gbs": round(n / (kms / 1e3) / 1e9, 2),
+            "ratio": round(n / int(sizes.sum()), 4), "ratio_first_16mib": round(x.size / int(sizes[:k].sum()), 4),
+            "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4)}
+
+
 def workload_inputs(a):
     """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
     if a.workload == "zstd":
@@ -425,6 +469,7 @@ def main():
 
     equal = equal_ratio_check(ctx, batch, data, inputs, d_in, a.check_sample_mib * MiB)
     enc_m = None if a.no_cpu or not lzma_bytes else encode_from_gpu_matches(batch, data, inputs, 16 * MiB)
+    zframes = None if a.no_cpu or not zstd_bytes else gpu_frames(ctx, hip, stream, d_in, data, inputs, 256 * MiB)
     cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
     cpu_more = None
     if not a.no_cpu and a.workload in ("zstd", "lzma2"):
@@ -466,6 +511,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baselines_more": cpu_more,
         "lzma2_encode_from_gpu_matches": enc_m,
+        "zstd_gpu_frames": zframes,
         "corpus_gen_s": round(gen_s, 2),
     }
     print(json.dumps(out), flush=True)
