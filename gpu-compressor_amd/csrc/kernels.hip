@@ -181,6 +181,34 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
 
+// Lane i gets lane i - 1's v, lane 0 gets first (DPP wave_shr:1).
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v, uint32_t first) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)first, (int)v, 0x138, 0xF, 0xF, false);
+}
+
+// Sum over the wavefront, uniform result: pairs, quads (quad_perm), 8 and
+// 16 lanes (row half mirror, row mirror), then rows 1 and 3 take rows 0
+// and 2 (row_bcast:15) and row 3 the first half (row_bcast:31); lane 63 holds
+// the total.  DPP needs no lane-address registers.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return bcast(v, WAVE - 1);
+}
+
+// lane `lane` of a set to sa and of b to sb (all uniform): one instruction
+// per register, where a compare and a select would be two or three.  (Two
+// scalar operands exceed the constant bus: the lane index is passed in M0.)
+__device__ __forceinline__ void write_lane2(uint32_t& a, uint32_t sa, uint32_t& b, uint32_t sb, uint32_t lane) {
+    __asm__("v_writelane_b32 %0, %2, %4\n\tv_writelane_b32 %1, %3, %4"
+            : "+v"(a), "+v"(b)
+            : "s"(sa), "s"(sb), "{m0}"(lane));
+}
+
 // Barrier for LDS traffic only: outstanding global loads (prefetches) stay
 // in flight across it.
 __device__ __forceinline__ void lds_barrier() {
@@ -810,31 +838,45 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
     uint32_t pos = 0, nseq = 0, wbase = 0, wend = WAVE;
-    // literal bookkeeping in vector registers (the same in every lane): the
-    // scalar register count limits this kernel's occupancy
+    // literal bookkeeping for the literal copy, in vector registers (the same
+    // in every lane): the scalar register count limits this kernel's occupancy
     uint32_t lit_start = in_vgpr(0u), nlit = in_vgpr(0u);
     uint8_t* lit_dst = LITS ? literals + it.lit_pos : nullptr;
 
-    // staged sequences: lane j holds sequence (nseq & ~63) + j
-    // (offset | length << 16, literal length); both fit 16 bits: a head is
-    // at position >= 1, so offset and length are below 65536
-    uint32_t q_ol = 0, q_b = 0;
-    auto store_staged = [&](uint32_t first) {  // this lane's record: sequence first + lane
-        const uint64_t si = it.seq_pos + first + lane;
-        if constexpr (FMT == GC_SEQ_REF) {
-            gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
-            o->literal_length = q_b;
-            o->match_length_minus3 = (q_ol >> 16) - 3;
-            o->offset = q_ol & 0xFFFFu;
-        } else {
-            *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(q_ol & 0xFFFFu, q_b, q_ol >> 16, 0);
+    // staged sequences: lane j holds sequence (nseq & ~63) + j as
+    // offset | length << 16 (both fit 16 bits: a head is at position >= 1, so
+    // offset and length are below 65536) and its head position, each written
+    // into its lane by one instruction.  Literal lengths follow when the
+    // records are stored: head minus the end of the sequence before.
+    uint32_t q_ol = 0, q_h = 0;
+    // (uniform, in vector registers like the literal bookkeeping)
+    uint32_t batch_end = in_vgpr(0u);  // end of the last sequence before the staged ones
+    uint32_t matched = in_vgpr(0u);    // bytes covered by the stored sequences
+    auto store_staged = [&](uint32_t first, uint32_t cnt) {  // sequences first .. first + cnt - 1
+        const uint32_t len = q_ol >> 16;
+        const uint32_t end = q_h + len;
+        const uint32_t ll = q_h - wave_shr1(end, batch_end);
+        if (lane < cnt) {
+            const uint64_t si = it.seq_pos + first + lane;
+            if constexpr (FMT == GC_SEQ_REF) {
+                gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
+                o->literal_length = ll;
+                o->match_length_minus3 = len - 3;
+                o->offset = q_ol & 0xFFFFu;
+            } else {
+                *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(q_ol & 0xFFFFu, ll, len, 0);
+            }
         }
+        matched = in_vgpr(uniform(matched) + wave_sum(lane < cnt ? len : 0u));
+        batch_end = in_vgpr(bcast(end, cnt - 1));
     };
 
     // per lane, for x = wbase + lane: e = info(x), e1 = info(x + 1) (the
     // lazy step's word), w = the 3 bytes at x (0 past the end; the low byte
-    // goes to the ring), f = repeat flags; ae, ae1, aw the same for the next
-    // window.  Info words of positions with x + 3 >= n read as 0.
+    // goes to the ring) with bit 24 set where x + 3 >= n, so that no 3-byte
+    // comparison with it succeeds there; f = repeat flags; ae, ae1, aw the
+    // same for the next window.  Info words of positions with x + 3 >= n read
+    // as 0.
     uint32_t e = 0, e1 = 0, w = 0, f = 0, ae = 0, ae1 = 0, aw = 0;
     uint32_t xv = lane;  // wbase + lane
     auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& ev1, uint32_t& wv) {
@@ -845,7 +887,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         const uint32_t i0 = ip[0];
         const uint32_t w0 = ld32<SAFE>(src, in ? x : 0u) & 0xFFFFFFu;
         ev = act ? i0 : 0;
-        wv = in ? w0 : 0;
+        wv = (in ? w0 : 0) | (act ? 0 : 0x1000000u);
         if constexpr (LAZY) {
             const uint32_t i1 = ip[1];
             ev1 = x + 4 < n ? i1 : 0;
@@ -919,8 +961,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // walk stops (max_seq)
     uint32_t ev_at = max_seq < WAVE ? max_seq : WAVE;
     while (pos < lim) {
-        const uint32_t cand = xv >= pos ? ((e >> 16) | f) : 0u;
-        const uint64_t m = ballot(cand != 0);
+        // two ballots of plain compares: a ballot of their conjunction is
+        // built as a select and a third compare
+        const uint64_t m = ballot(xv >= pos) & ballot(((e >> 16) | f) != 0);
         if (!m) {
             // no head left in the window: every position up to its end is a literal
             wbase = wend;
@@ -947,10 +990,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 off = head - c;
                 len = ek >> 16;
                 if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, c, head, n - head, GC_ZSTD_SHORT);
-                // an offset equal to a repeat offset r has r's flag set at head
-                // (the 3 bytes verified): with no flag set it is a new one
-                sel = 3;
-                if (fk) sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
+                sel = 3;  // the new offset's flags are computed below
             } else {
                 // the first repeat offset that verifies (rep0, rep1, rep2 in order)
                 sel = (fk & 1) ? 0u : ((fk & 2) ? 1u : 2u);
@@ -981,31 +1021,30 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                         head += 1;
                         len = len2;
                         off = head - c2;
-                        sel = off == rep0 ? 0u : (off == rep1 ? 1u : (off == rep2 ? 2u : 3u));
+                        sel = 3;
                     }
                 }
             }
             // emit (nseq < max_seq holds here)
-            const uint32_t ll = head - lit_start;
-            if constexpr (LITS)
+            if constexpr (LITS) {
+                const uint32_t ll = head - lit_start;
                 copy_wave(lit_dst + uniform(nlit), src.base + src.mis + uniform(lit_start), uniform(ll));
-            nlit += ll;
-            const bool mine = lane == (nseq & (WAVE - 1));
-            q_ol = mine ? (off | (len << 16)) : q_ol;
-            q_b = mine ? ll : q_b;
+                nlit += ll;
+            }
+            write_lane2(q_ol, off | (len << 16), q_h, head, nseq & (WAVE - 1));
             nseq++;
             if (nseq == ev_at) {
-                if ((nseq & (WAVE - 1)) == 0) store_staged(nseq - WAVE);
+                if ((nseq & (WAVE - 1)) == 0) store_staged(nseq - WAVE, WAVE);
                 if (nseq == max_seq) lim = 0;  // the reference stops at the next head
                 ev_at = nseq + WAVE < max_seq ? nseq + WAVE : max_seq;
             }
-            // repeat offsets (off, o0, o1); sel: which old flag, if any, the
-            // new offset's is (0..2), or 3 = gather it
+            // repeat offsets (off, o0, o1); sel: which old flag the new
+            // offset's is (0..2: a repeat match), or 3 = compute it
             rep2 = rep1;
             rep1 = rep0;
             rep0 = off;
             pos = head + len;
-            lit_start = in_vgpr(pos);
+            if constexpr (LITS) lit_start = in_vgpr(pos);
             if (pos >= wend) {
                 // the walk left the window: move to the window holding pos
                 // (reads past the sub-block end are clamped by peek)
@@ -1035,23 +1074,26 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                     const bool act = xv + 3 < n;
                     if (off <= RING_REACH) {
                         const uint32_t v = ring_w24(ring, xv - off);
-                        b0 = (act & (xv >= off) & (v == w)) ? 1u : 0u;
-                    }
-                    else
+                        b0 = ((xv >= off) & (v == w)) ? 1u : 0u;  // (w has bit 24 set where !act)
+                    } else if (off == rep1) {  // a far offset equal to the old rep0 or rep1: its flags
+                        b0 = f & 1u;
+                    } else if (off == rep2) {
+                        b0 = (f >> 1) & 1u;
+                    } else {
                         b0 = rep_eq<SAFE>(src, act ? xv : 0u, off, w & 0xFFFFFFu) & (act ? 1u : 0u);
+                    }
                 }
                 f = b0 | ((f & 3u) << 1);
             }
         }
     }
-    if (lane < (nseq & (WAVE - 1))) store_staged(nseq & ~(uint32_t)(WAVE - 1));
-    // trailing literals (zstd_compress.cu:341-344)
-    const uint32_t lit_end = uniform(lit_start);
-    uint32_t nlit_total = uniform(nlit);
-    if (n > lit_end) {
-        if constexpr (LITS) copy_wave(lit_dst + nlit_total, src.base + src.mis + lit_end, n - lit_end);
-        nlit_total += n - lit_end;
-    }
+    if (nseq & (WAVE - 1)) store_staged(nseq & ~(uint32_t)(WAVE - 1), nseq & (WAVE - 1));
+    // trailing literals (zstd_compress.cu:341-344): every byte not covered
+    // by a sequence is a literal
+    const uint32_t lit_end = nseq ? uniform(batch_end) : 0u;
+    const uint32_t nlit_total = n - uniform(matched);
+    if constexpr (LITS)
+        if (n > lit_end) copy_wave(lit_dst + uniform(nlit), src.base + src.mis + lit_end, n - lit_end);
     if (lane == 0) {
         seq_counts[it.count_pos] = nseq;
         if (lit_counts) lit_counts[it.count_pos] = nlit_total;
@@ -1096,8 +1138,14 @@ __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_
 // One kernel per record layout and literal output, each with its own
 // register allocation (scalar registers bound this kernel's occupancy).
 // order (optional): the items in the order to take them.
+//
+// Scalar registers are capped at 96: on MI355X a wavefront that uses 97-100
+// runs at 7 waves per SIMD although the compiler counts it as 8 (measured:
+// the same parse at 98 registers took 12.4 ms per 2 GiB launch, capped at 96
+// with 4 registers spilled to vector lanes 11.1 ms).
+#define PARSE_SGPRS 96
 #define GC_PARSE_KERNEL(NAME, FMT, LITS)                                                                          \
-    extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE)                                              \
+    extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE) __attribute__((amdgpu_num_sgpr(PARSE_SGPRS))) \
         NAME(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,    \
              void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,     \
              uint32_t max_seq, const uint32_t* __restrict__ order) {                                             \
