@@ -1421,7 +1421,8 @@ __device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, cons
     if (lane == 0) sizes[it.count_pos] = end;
 }
 
-extern "C" __global__ void __launch_bounds__(ZE_THREADS)
+// (scalar registers capped at 96, as the parse's: 8 waves per SIMD)
+extern "C" __global__ void __launch_bounds__(ZE_THREADS) __attribute__((amdgpu_num_sgpr(96)))
 gc_zstd_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const void* seqs,
                       uint32_t seq_fmt, const uint8_t* literals, const uint32_t* seq_counts,
                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
