@@ -504,6 +504,15 @@ def main():
         "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
         "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)},
         "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq},
+        # results moved per input byte, this layout against the reference's: Zstd, 16 B records +
+        # 4 B count per sub-block (the reference copies back all 21845 12-byte slots of every 64 KiB
+        # sub-chunk and its count, compress_zstd_custom.rs:82-84,115-116); LZMA2, one 4 B match per
+        # position (the reference: 8 slots of 8 B plus a 4 B count, compress_lzma2_custom.rs:511-512,540)
+        "output_bytes_per_input_byte": {
+            **({"zstd": {"ours": round((16 * n_seq + 4 * zstd_sub) / zstd_bytes, 4),
+                         "reference_layout": round((12 * 21845 + 4) / 65536, 4)}} if zstd_bytes else {}),
+            **({"lzma2": {"ours": 4.0, "reference_layout": 68.0}} if lzma_bytes else {}),
+        },
         "equal_ratio": equal,
         "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
