@@ -837,7 +837,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     const uint32_t lane = lane_id();
     const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
-    uint32_t pos = 0, nseq = 0, wbase = 0, wend = WAVE;
+    uint32_t pos = 0, nseq = 0, wbase = 0;  // the window is [wbase, wbase + WAVE)
     // literal bookkeeping for the literal copy, in vector registers (the same
     // in every lane): the scalar register count limits this kernel's occupancy
     uint32_t lit_start = in_vgpr(0u), nlit = in_vgpr(0u);
@@ -895,7 +895,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     };
     // Repeat offsets beyond the ring's reach: at each window entry, for the
     // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
-    // for the next window (y = wend + lane), like the info words.  They stay
+    // for the next window (y = wbase + WAVE + lane), like the info words.  They stay
     // in place (a register whose load is in flight is not copied) and serve
     // the next window's flags for whichever offsets are still in use.
     // (The tags sr0..sr2 are uniform but live in vector registers: scalar
@@ -908,9 +908,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         sr0 = in_vgpr(rep0 > RING_REACH ? rep0 : 0u);
         sr1 = in_vgpr(rep1 > RING_REACH ? rep1 : 0u);
         sr2 = in_vgpr(rep2 > RING_REACH ? rep2 : 0u);
-        if (rep0 > RING_REACH) sh0 = far_w24(wend + lane, rep0);
-        if (rep1 > RING_REACH) sh1 = far_w24(wend + lane, rep1);
-        if (rep2 > RING_REACH) sh2 = far_w24(wend + lane, rep2);
+        if (rep0 > RING_REACH) sh0 = far_w24(wbase + WAVE + lane, rep0);
+        if (rep1 > RING_REACH) sh1 = far_w24(wbase + WAVE + lane, rep1);
+        if (rep2 > RING_REACH) sh2 = far_w24(wbase + WAVE + lane, rep2);
     };
     auto far_flag_word = [&](uint32_t x, uint32_t r, bool shadows) -> uint32_t {
         if (shadows && r == uniform(sr0)) return sh0;
@@ -957,17 +957,90 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
         peek(WAVE + lane, ae, ae1, aw);
     }
+    // move to the window at nb (a multiple of 64 past the current one); with
+    // `prefetched`, the next window's prefetched words serve it when nb is
+    // that window
+    auto move_to = [&](uint32_t nb, bool prefetched) {
+        const uint32_t wend = wbase + WAVE;
+        const bool next = prefetched && nb == wend;  // the window the shadows are for
+        uint32_t from = nb;
+        if (next) {
+            e = ae;
+            e1 = ae1;
+            w = aw;
+        } else {
+            peek(nb + lane, e, e1, w);
+            // the windows jumped over, as far as the ring reaches
+            from = nb + WAVE - PARSE_RING < wend || nb + WAVE < PARSE_RING ? wend : nb + WAVE - PARSE_RING;
+        }
+        wbase = nb;
+        xv = nb + lane;
+        enter(nb, from);
+        f = flags(xv, w, next);
+        __asm__ volatile("" ::: "memory");
+        load_shadows();
+        peek(nb + WAVE + lane, ae, ae1, aw);
+    };
+    // Look-ahead over a stretch of literals: from sb (a window start), 256
+    // positions per step, 4 per lane, with every load of a step issued
+    // together (a window-by-window slide waits for one load per window).
+    // Returns the first window that may hold a head at or after sb (a
+    // verified hash candidate, or a position whose 3 bytes equal those at
+    // one of the current repeat offsets), or lim if there is none before
+    // lim.  A lane whose positions straddle a repeat offset counts as a hit:
+    // the window's own flags then decide.
+    auto scan = [&](uint32_t sb) -> uint32_t {
+        for (; sb < lim; sb += 4 * WAVE) {
+            const uint32_t x0 = sb + 4 * lane;
+            const uint32_t xc = x0 < lim ? x0 : 0u;  // x0 + 3 < n: reads stay in the sub-block
+            const uint4 iw = *(const uint4*)(inf + (xc << 2));
+            const uint32_t o0 = ld32<SAFE>(src, xc), o1 = ld32<SAFE>(src, xc + 4);
+            const uint32_t ya = xc >= rep0 ? xc - rep0 : 0u, yb = xc >= rep1 ? xc - rep1 : 0u,
+                           yc = xc >= rep2 ? xc - rep2 : 0u;
+            const uint32_t a0 = ld32<SAFE>(src, ya), a1 = ld32<SAFE>(src, ya + 4);
+            const uint32_t b0 = ld32<SAFE>(src, yb), b1 = ld32<SAFE>(src, yb + 4);
+            const uint32_t c0 = ld32<SAFE>(src, yc), c1 = ld32<SAFE>(src, yc + 4);
+            bool hit = false;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                const uint32_t wj = __builtin_amdgcn_alignbyte(o1, o0, j) & 0xFFFFFFu;
+                const uint32_t ij = j == 0 ? iw.x : j == 1 ? iw.y : j == 2 ? iw.z : iw.w;
+                const bool m0 = ((__builtin_amdgcn_alignbyte(a1, a0, j) & 0xFFFFFFu) == wj) & (xc >= rep0);
+                const bool m1 = ((__builtin_amdgcn_alignbyte(b1, b0, j) & 0xFFFFFFu) == wj) & (xc >= rep1);
+                const bool m2 = ((__builtin_amdgcn_alignbyte(c1, c0, j) & 0xFFFFFFu) == wj) & (xc >= rep2);
+                hit |= (x0 + j < lim) & ((ij > 0xFFFFu) | m0 | m1 | m2);
+            }
+            hit |= (x0 < lim) & (((x0 < rep0) & (x0 + 3 >= rep0)) | ((x0 < rep1) & (x0 + 3 >= rep1)) |
+                                 ((x0 < rep2) & (x0 + 3 >= rep2)));
+            const uint64_t hm = ballot(hit);
+            if (hm) return (sb + 4 * (uint32_t)__builtin_ctzll(hm)) & ~(WAVE - 1);
+        }
+        return lim;
+    };
     // next sequence count at which the staged records are stored or the
     // walk stops (max_seq)
     uint32_t ev_at = max_seq < WAVE ? max_seq : WAVE;
+    bool slid = false;  // the last window change was a slide past a window with no head
     while (pos < lim) {
         // two ballots of plain compares: a ballot of their conjunction is
         // built as a select and a third compare
         const uint64_t m = ballot(xv >= pos) & ballot(((e >> 16) | f) != 0);
         if (!m) {
-            // no head left in the window: every position up to its end is a literal
-            wbase = wend;
-            wend += WAVE;
+            // no head left in the window: every position up to its end is a
+            // literal.  After a slide, look further ahead at once.
+            if (slid) {
+                const uint32_t nb = scan(wbase + WAVE);
+                slid = false;
+                if (nb >= lim) {
+                    pos = lim;  // literals to the end
+                } else {
+                    move_to(nb, false);  // (the prefetched words are not kept across the scan)
+                    pos = nb;
+                }
+                continue;
+            }
+            slid = true;
+            wbase += WAVE;
             pos = wbase;
             xv += WAVE;
             e = ae;
@@ -977,8 +1050,9 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             f = flags(xv, w, true);
             __asm__ volatile("" ::: "memory");
             load_shadows();
-            peek(wend + lane, ae, ae1, aw);
+            peek(wbase + WAVE + lane, ae, ae1, aw);
         } else {
+            slid = false;
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
             const uint32_t ek = bcast(e, k);
@@ -997,7 +1071,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
                 if (off <= RING_REACH) {
                     // the bytes up to the window end are in the ring
-                    const uint32_t span = (wend < n ? wend : n) - head;
+                    const uint32_t span = (wbase + WAVE < n ? wbase + WAVE : n) - head;
                     const uint32_t ba = ring[(head - off + lane) & (PARSE_RING - 1)];
                     const uint32_t bb = ring[(head + lane) & (PARSE_RING - 1)];
                     const uint64_t mm = ballot((lane >= 3) & (lane < span) & (ba != bb));
@@ -1045,29 +1119,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             rep0 = off;
             pos = head + len;
             if constexpr (LITS) lit_start = in_vgpr(pos);
-            if (pos >= wend) {
+            if (pos >= wbase + WAVE) {
                 // the walk left the window: move to the window holding pos
                 // (reads past the sub-block end are clamped by peek)
-                const uint32_t nb = pos & ~(WAVE - 1);
-                const bool next = nb == wend;  // the window the shadows are for
-                uint32_t from = nb;
-                if (next) {
-                    e = ae;
-                    e1 = ae1;
-                    w = aw;
-                } else {
-                    peek(nb + lane, e, e1, w);
-                    // the windows jumped over, as far as the ring reaches
-                    from = nb + WAVE - PARSE_RING < wend || nb + WAVE < PARSE_RING ? wend : nb + WAVE - PARSE_RING;
-                }
-                wbase = nb;
-                wend = nb + WAVE;
-                xv = nb + lane;
-                enter(nb, from);
-                f = flags(xv, w, next);
-                __asm__ volatile("" ::: "memory");
-                load_shadows();
-                peek(wend + lane, ae, ae1, aw);
+                move_to(pos & ~(WAVE - 1), true);
             } else {
                 uint32_t b0 = (f >> sel) & 1u;  // sel 3 reads 0
                 if (sel == 3) {
@@ -1139,13 +1194,14 @@ __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_
 // register allocation (scalar registers bound this kernel's occupancy).
 // order (optional): the items in the order to take them.
 //
-// Scalar registers are capped at 96: on MI355X a wavefront that uses 97-100
-// runs at 7 waves per SIMD although the compiler counts it as 8 (measured:
-// the same parse at 98 registers took 12.4 ms per 2 GiB launch, capped at 96
-// with 4 registers spilled to vector lanes 11.1 ms).
-#define PARSE_SGPRS 96
+// 8 waves per SIMD are asked for (the second launch bound): the compiler then
+// keeps to 64 vector registers (the look-ahead scan would take 75) and to 78
+// scalar ones, spilling about 28 scalar values to vector lanes.  Scalar
+// registers are what this kernel's occupancy has turned on: the same parse
+// at 98 took 12.4 ms per 2 GiB launch, at 94 11.0 ms and at 78 10.8 ms
+// (the compiler counts each of them as 8 waves per SIMD).
 #define GC_PARSE_KERNEL(NAME, FMT, LITS)                                                                          \
-    extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE) __attribute__((amdgpu_num_sgpr(PARSE_SGPRS))) \
+    extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE, 8)                                          \
         NAME(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,    \
              void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,     \
              uint32_t max_seq, const uint32_t* __restrict__ order) {                                             \

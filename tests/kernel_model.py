@@ -184,10 +184,14 @@ def zstd_parse(d, info, depth=16, max_seq=None):
     return seqs
 
 
-def zstd_parse_fixed(d, info, depth=16, max_seq=None):
+def zstd_parse_fixed(d, info, depth=16, max_seq=None, scan=False):
     """gc_zstd_parse_kernel, aligned-window form: lanes hold positions wbase + i (wbase a multiple
-    of 64) and the next window ahead; lanes before pos are masked out of the search."""
+    of 64) and the next window ahead; lanes before pos are masked out of the search.  With `scan`,
+    a second window in a row with no head starts the look-ahead: 256 positions per step, 4 per lane,
+    to the first window that may hold a head (a lane whose 4 positions straddle a repeat offset
+    counts as a hit)."""
     n = len(d)
+    lim = n - 3
     if max_seq is None:
         max_seq = 65536 // 3
     rep = [1, 4, 8]
@@ -205,6 +209,25 @@ def zstd_parse_fixed(d, info, depth=16, max_seq=None):
     def flags(x, w):
         return (1 if eq(x, rep[0], w) else 0) | (2 if eq(x, rep[1], w) else 0) | (4 if eq(x, rep[2], w) else 0)
 
+    def look_ahead(sb):
+        while sb < lim:
+            hits = []
+            for lane in range(WAVE):
+                x0 = sb + 4 * lane
+                hit = False
+                if x0 < lim:
+                    for x in range(x0, min(x0 + 4, lim)):
+                        if info[x] >> 16 or any(x0 >= r and v3(d, x - r) == v3(d, x) for r in rep):
+                            hit = True
+                    if any(x0 < r <= x0 + 3 for r in rep):
+                        hit = True
+                hits.append(hit)
+            if any(hits):
+                return (sb + 4 * hits.index(True)) & ~(WAVE - 1)
+            sb += 4 * WAVE
+        return lim
+
+    slid = False
     wbase = 0
     e, w = zip(*[peek(i) for i in range(WAVE)]) if n > 3 else ([0] * WAVE, [(0, 0, 0)] * WAVE)
     e, w = list(e), list(w)
@@ -214,12 +237,22 @@ def zstd_parse_fixed(d, info, depth=16, max_seq=None):
         lo = pos - wbase
         m = [i for i in range(lo, WAVE) if (e[i] >> 16) or f[i]]
         if not m:
-            wbase += WAVE
-            pos = wbase
-            e, w = ae, aw
+            if scan and slid:
+                slid = False
+                nb = look_ahead(wbase + WAVE)
+                if nb >= lim:
+                    break
+                wbase = pos = nb
+                e, w = map(list, zip(*[peek(nb + i) for i in range(WAVE)]))
+            else:
+                slid = True
+                wbase += WAVE
+                pos = wbase
+                e, w = ae, aw
             f = [flags(wbase + i, w[i]) for i in range(WAVE)]
             ae, aw = map(list, zip(*[peek(wbase + WAVE + i) for i in range(WAVE)]))
             continue
+        slid = False
         k = m[0]
         head = wbase + k
         ek = e[k]

@@ -23,6 +23,16 @@ def inputs():
     out.append(("period200", long_rep))
     out.append(("period70", np.resize(datagen.random_bytes(70, 9), 9000)))
     out.append(("short", datagen.mix(100, 10)))
+    # long literal stretches (the look-ahead): random bytes with a few planted repeats, at repeat
+    # offsets 1/4/8 and at new offsets, some straddling a 4-position lane group or a 256-position step
+    r = datagen.random_bytes(20000, 12).copy()
+    for at, off, ln in [(700, 4, 5), (3001, 8, 4), (5000, 1, 6), (7777, 300, 9), (9300, 9300 - 7777, 7),
+                        (12543, 1, 3), (15000, 4000, 12), (19990, 5, 6)]:
+        r[at:at + ln] = r[at - off:at - off + ln] if off > ln else np.resize(r[at - off:at], ln)
+    out.append(("random+planted", r))
+    out.append(("random", datagen.random_bytes(9000, 13)))
+    out.append(("text+random", np.concatenate([datagen.text(3000, 14), datagen.random_bytes(6000, 15),
+                                               datagen.text(2000, 16)])))
     return out
 
 
@@ -44,10 +54,12 @@ def test_info_strips_equal_direct(name, data, strip):
 
 @pytest.mark.parametrize("name,data", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("depth", [1, 16])
-@pytest.mark.parametrize("form", ["sliding", "aligned"])
+@pytest.mark.parametrize("form", ["sliding", "aligned", "aligned+scan"])
 def test_parse_window_equals_oracle(name, data, depth, form):
     d = data.tolist()
-    parse = km.zstd_parse if form == "sliding" else km.zstd_parse_fixed
-    got = parse(d, km.zstd_info(d), depth)
+    if form == "sliding":
+        got = km.zstd_parse(d, km.zstd_info(d), depth)
+    else:
+        got = km.zstd_parse_fixed(d, km.zstd_info(d), depth, scan=form == "aligned+scan")
     seq, sc, _, _ = oracle.zstd_match_find(data, depth=depth)
     assert got == [tuple(x) for x in seq[0, : sc[0]].tolist()]
