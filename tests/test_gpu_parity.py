@@ -32,6 +32,23 @@ def cases():
     out.append(("longcopies", np.concatenate(parts)))
     for k in (1, 2, 3, 4, 7, 8, 9, 16, 63, 64, 65, 200):
         out.append((f"period{k}", datagen.periodic(65536 + 17, k, k)))
+    # long literal stretches (the parse's look-ahead): random bytes with short repeats planted at
+    # new offsets and then at the offsets those make current, some across a 4- or 256-position step
+    # edge and in the last bytes of the input; and random bytes with a short text snippet every few KiB
+    r = datagen.random_bytes(3 * 65536 + 999, 31).copy()
+    at = 300
+    for off, ln in [(77, 6), (77, 4), (1000, 5), (77, 3), (5000, 8), (1000, 4), (3, 9), (40000, 6), (5000, 3)] * 9:
+        at += int(rng.integers(700, 9000))
+        if at + ln >= r.size:
+            break
+        r[at:at + ln] = np.resize(r[at - off:at], ln) if off < ln else r[at - off:at - off + ln]
+    r[-7:] = r[-7 - 64:-64]
+    out.append(("planted", r))
+    s = datagen.random_bytes(3 * 65536 + 31, 32).copy()
+    snippet = datagen.text(40, 33)
+    for at in range(1000, s.size - 40, 3001):
+        s[at:at + 40] = snippet
+    out.append(("sparsetext", s))
     return out
 
 
