@@ -311,7 +311,7 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
         const uint32_t o = len + 4 * lane;
         x = 0;
         if (o < limit) x = ld32<SAFE>(src, a + o) ^ ld32<SAFE>(src, b + o);
-        mm = ballot(x != 0);
+        mm = __builtin_amdgcn_uicmp(x, 0u, 33);  // the lanes whose words differ
         len = mm ? len : len + 4 * WAVE;
         if (mm) break;
     }
@@ -355,9 +355,10 @@ struct CandLds {
 #define ANA_ABOVE (1u << 15)
 #define ANA_VALID (1u << 22)
 
-__device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, bool valid) {
+__device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, uint32_t p, uint32_t n) {
     const uint32_t lane = lane_id();
     const uint32_t h = hash4(word);
+    const bool valid = p + 3 < n;
 
     // Duplicate test: every valid lane writes its id under a slice of its
     // hash (one slot array per wavefront) and reads it back; one lane of each
@@ -372,19 +373,22 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     const uint32_t slot = h & (GC_DUP_SIZE - 1);
     if (valid) s.dup[wave][slot] = (uint8_t)lane;
     __asm__ volatile("" ::: "memory");
-    const uint32_t back = valid ? s.dup[wave][slot] : lane;
-    const bool loser = valid && back != lane;
+    const uint32_t back = valid ? s.dup[wave][slot] : lane;  // (an invalid lane never loses)
 
-    // Exact groups (one step per distinct hash among the losers).
+    // Exact groups (one step per distinct hash among the losers).  The lane
+    // masks come straight from compares (uicmp) and are combined on the scalar
+    // unit; a group's mask selects its lanes as it is (inverse ballot): a
+    // ballot of a conjunction, or a per-lane test of a mask bit, would cost
+    // further vector instructions in every step.
+    const uint64_t vmask = __builtin_amdgcn_uicmp(p + 3, n, 36);  // unsigned <
     const uint64_t lane_bit = 1ull << lane;
     uint64_t group = lane_bit;
-    uint64_t rest = ballot(loser);
+    uint64_t rest = __builtin_amdgcn_uicmp(back, lane, 33) & vmask;  // !=
     while (rest) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(rest);
         const uint32_t hl = bcast(h, lead);
-        const bool in = valid && h == hl;
-        const uint64_t g = ballot(in);
-        if (in) group = g;
+        const uint64_t g = __builtin_amdgcn_uicmp(h, hl, 32) & vmask;  // ==
+        if (__builtin_amdgcn_inverse_ballot_w64(g)) group = g;
         rest &= ~g;
     }
     const uint64_t below = group & (lane_bit - 1);
@@ -433,7 +437,7 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
         const uint32_t word = ld32<SAFE>(src, p + 3 < n ? p : 0u);
         const uint32_t q = CAND_TILE + p;
         next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
-        s.ana[0][p] = cand_analyse(s, wave, word, p + 3 < n);
+        s.ana[0][p] = cand_analyse(s, wave, word, p, n);
     }
     lds_barrier();
     uint32_t buf = 0;
@@ -445,7 +449,7 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
                 const uint32_t word = next.word();
                 const uint32_t q = p + CAND_TILE;
                 next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
-                s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p + 3 < n);
+                s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p, n);
             }
         } else {
             cand_table(s, s.ana[buf], t0, n, out);
@@ -1074,7 +1078,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                     const uint32_t span = (wbase + WAVE < n ? wbase + WAVE : n) - head;
                     const uint32_t ba = ring[(head - off + lane) & (PARSE_RING - 1)];
                     const uint32_t bb = ring[(head + lane) & (PARSE_RING - 1)];
-                    const uint64_t mm = ballot((lane >= 3) & (lane < span) & (ba != bb));
+                    // lanes 3 .. span - 1 whose bytes differ (masks from compares, combined
+                    // on the scalar unit)
+                    const uint64_t mm = __builtin_amdgcn_uicmp(ba, bb, 33) & __builtin_amdgcn_uicmp(lane, span, 36) &
+                                        ~7ull;
                     if (mm)
                         len = (uint32_t)__builtin_ctzll(mm);
                     else
