@@ -362,10 +362,16 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
 
     // Duplicate test: every valid lane writes its id under a slice of its
     // hash (one slot array per wavefront) and reads it back; one lane of each
-    // slice wins.  Lanes that share a whole hash share a slice, so every
-    // group of two or more equal hashes has a lane that lost: the losers'
-    // hashes seed the exact groups below, which take in every lane with the
-    // same hash (winners too).
+    // slot wins.  Lanes that share a whole hash share the slot, so every
+    // group of two or more equal hashes has a lane that lost.  A loser reads
+    // the winner's hash across the wavefront: if it is its own, the loser
+    // seeds its group.  Lanes that lost to another hash repeat the test among
+    // themselves under another slice of their hash: a group none of whose
+    // lanes won the first slot lies wholly among them and shares the second
+    // slot, so one of its lanes loses there and seeds it.  Every group of two
+    // or more gets a seed whichever lane the hardware lets win a slot, and a
+    // lane alone with its hash seeds nothing unless it loses both tests (the
+    // seeds are what the exact groups below iterate over).
     // Other lanes write the same bytes between this lane's store and load,
     // which the compiler cannot see: the empty asm with a memory clobber
     // keeps it from forwarding the store.  LDS operations of one wavefront
@@ -374,8 +380,15 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     if (valid) s.dup[wave][slot] = (uint8_t)lane;
     __asm__ volatile("" ::: "memory");
     const uint32_t back = valid ? s.dup[wave][slot] : lane;  // (an invalid lane never loses)
+    const uint32_t hw = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(back << 2), (int)h);
+    const bool other = back != lane && hw != h;
+    const uint32_t slot2 = (h >> (GC_HASH_BITS - GC_DUP_BITS)) & (GC_DUP_SIZE - 1);
+    __asm__ volatile("" ::: "memory");
+    if (other) s.dup[wave][slot2] = (uint8_t)lane;
+    __asm__ volatile("" ::: "memory");
+    const uint32_t back2 = other ? s.dup[wave][slot2] : lane;
 
-    // Exact groups (one step per distinct hash among the losers).  The lane
+    // Exact groups (one step per distinct hash among the seeds).  The lane
     // masks come straight from compares (uicmp) and are combined on the scalar
     // unit; a group's mask selects its lanes as it is (inverse ballot): a
     // ballot of a conjunction, or a per-lane test of a mask bit, would cost
@@ -383,7 +396,9 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     const uint64_t vmask = __builtin_amdgcn_uicmp(p + 3, n, 36);  // unsigned <
     const uint64_t lane_bit = 1ull << lane;
     uint64_t group = lane_bit;
-    uint64_t rest = __builtin_amdgcn_uicmp(back, lane, 33) & vmask;  // !=
+    uint64_t rest = ((__builtin_amdgcn_uicmp(back, lane, 33) & __builtin_amdgcn_uicmp(hw, h, 32)) |  // !=, ==
+                     __builtin_amdgcn_uicmp(back2, lane, 33)) &
+                    vmask;
     while (rest) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(rest);
         const uint32_t hl = bcast(h, lead);
