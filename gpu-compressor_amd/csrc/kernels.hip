@@ -348,12 +348,16 @@ struct CandLds {
     uint8_t dup[CAND_AWAVES][GC_DUP_SIZE];  // write/read-back duplicate test
 };
 
-// Analysis word of this lane's position (valid when p + 3 < n):
-// hash | has an earlier equal lane << 14 | has a later one << 15 |
-// the earlier lane << 16 | valid << 22.
-#define ANA_BELOW (1u << 14)
-#define ANA_ABOVE (1u << 15)
-#define ANA_VALID (1u << 22)
+// Analysis word of a position, decoded as far as the table wavefront's
+// steps need: the byte offset of its hash's table entry (2 x hash, bits
+// 0-14), the candidate unless it comes from the table (bits 15-30: the
+// earlier lane's position when one shares the hash, GC_NONE when the
+// position is not valid, else ANA_FROM_TABLE, which no position reaches: a
+// candidate is at most n - 5), and whether the table entry is written (the
+// sign bit, one compare: the position is valid and no later lane of its tile
+// step shares its hash).
+#define ANA_WRITE (1u << 31)
+#define ANA_FROM_TABLE 0xFFFEu
 
 __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, uint32_t p, uint32_t n) {
     const uint32_t lane = lane_id();
@@ -408,8 +412,10 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     }
     const uint64_t below = group & (lane_bit - 1);
     const uint64_t above = group & ~(lane_bit | (lane_bit - 1));
-    const uint32_t pred = below ? 63u - (uint32_t)__builtin_clzll(below) : 0u;
-    return valid ? (h | (below ? ANA_BELOW : 0u) | (above ? ANA_ABOVE : 0u) | (pred << 16) | ANA_VALID) : 0u;
+    // (the earlier lane's position, used when there is one; | 1 keeps clz defined)
+    const uint32_t pred = p - lane + 63u - (uint32_t)__builtin_clzll(below | 1u);
+    const uint32_t c = !valid ? GC_NONE : below ? pred : ANA_FROM_TABLE;
+    return (valid && !above ? ANA_WRITE : 0u) | (c << 15) | (h << 1);
 }
 
 // The table wavefront's work on one tile at t0 (analysis words in `ana`).
@@ -420,16 +426,19 @@ __device__ __forceinline__ void cand_table(CandLds& s, const uint32_t* ana, uint
     for (int j = 0; j < CAND_AWAVES; j++) a[j] = ana[j * WAVE + lane];
 #pragma unroll
     for (int j = 0; j < CAND_AWAVES; j++) {
-        const uint32_t h = a[j] & (GC_TABLE - 1);
-        const bool valid = (a[j] & ANA_VALID) != 0;
-        v[j] = (valid && !(a[j] & ANA_BELOW)) ? s.table[h] : GC_NONE;
-        if (valid && !(a[j] & ANA_ABOVE)) s.table[h] = (uint16_t)(t0 + j * WAVE + lane);
+        // every lane reads its entry (lanes whose candidate is decided already
+        // ignore it); the last lane of each hash writes its position
+        const uint16_t* e = (const uint16_t*)((const uint8_t*)s.table + (a[j] & (2 * GC_TABLE - 2)));
+        v[j] = *e;
+        if ((int32_t)a[j] < 0) *(uint16_t*)e = (uint16_t)(t0 + j * WAVE + lane);  // ANA_WRITE
     }
+    // (one base address and immediate offsets; the bound on the scalar unit)
+    uint16_t* o = out + t0 + lane;
+    const int32_t left = (int32_t)(n - t0);
 #pragma unroll
     for (int j = 0; j < CAND_AWAVES; j++) {
-        const uint32_t p = t0 + j * WAVE + lane;
-        const uint32_t c = (a[j] & ANA_BELOW) ? t0 + j * WAVE + ((a[j] >> 16) & 63u) : v[j];
-        if (p < n) out[p] = (uint16_t)((a[j] & ANA_VALID) ? c : GC_NONE);
+        const uint32_t c = (a[j] >> 15) & 0xFFFFu;
+        if ((int32_t)lane < left - j * (int32_t)WAVE) o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
     }
 }
 
