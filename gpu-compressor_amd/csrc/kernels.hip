@@ -345,7 +345,7 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
 struct CandLds {
     uint16_t table[GC_TABLE];               // the reference's 14-bit table
     uint32_t ana[2][CAND_TILE];             // analysis words, double buffered
-    uint8_t dup[CAND_AWAVES][GC_DUP_SIZE];  // write/read-back duplicate test
+    uint8_t dup[CAND_AWAVES][GC_DUP_SIZE + 4];  // write/read-back duplicate test (+ a junk slot)
 };
 
 // Analysis word of a position, decoded as far as the table wavefront's
@@ -380,29 +380,32 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     // which the compiler cannot see: the empty asm with a memory clobber
     // keeps it from forwarding the store.  LDS operations of one wavefront
     // execute in program order, so the load sees every lane's store.
-    const uint32_t slot = h & (GC_DUP_SIZE - 1);
-    if (valid) s.dup[wave][slot] = (uint8_t)lane;
-    __asm__ volatile("" ::: "memory");
-    const uint32_t back = valid ? s.dup[wave][slot] : lane;  // (an invalid lane never loses)
-    const uint32_t hw = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(back << 2), (int)h);
-    const bool other = back != lane && hw != h;
-    const uint32_t slot2 = (h >> (GC_HASH_BITS - GC_DUP_BITS)) & (GC_DUP_SIZE - 1);
-    __asm__ volatile("" ::: "memory");
-    if (other) s.dup[wave][slot2] = (uint8_t)lane;
-    __asm__ volatile("" ::: "memory");
-    const uint32_t back2 = other ? s.dup[wave][slot2] : lane;
-
-    // Exact groups (one step per distinct hash among the seeds).  The lane
-    // masks come straight from compares (uicmp) and are combined on the scalar
-    // unit; a group's mask selects its lanes as it is (inverse ballot): a
-    // ballot of a conjunction, or a per-lane test of a mask bit, would cost
-    // further vector instructions in every step.
+    // Lanes that take no part in a test write and read the junk slot instead
+    // of branching around the LDS access; the lane masks below leave them out.
+    // The lane masks come straight from compares (uicmp) and are combined on
+    // the scalar unit; a group's mask selects its lanes as it is (inverse
+    // ballot): a ballot of a conjunction, or a per-lane test of a mask bit,
+    // would cost further vector instructions.
+    uint8_t* dup = s.dup[wave];
     const uint64_t vmask = __builtin_amdgcn_uicmp(p + 3, n, 36);  // unsigned <
+    const uint32_t slot = valid ? (h & (GC_DUP_SIZE - 1)) : GC_DUP_SIZE;
+    dup[slot] = (uint8_t)lane;
+    __asm__ volatile("" ::: "memory");
+    const uint32_t back = dup[slot];
+    const uint32_t hw = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(back << 2), (int)h);
+    const uint64_t lost = __builtin_amdgcn_uicmp(back, lane, 33) & vmask;  // !=
+    const uint64_t other = lost & __builtin_amdgcn_uicmp(hw, h, 33);       // lost to another hash
+    const uint32_t slot2 =
+        __builtin_amdgcn_inverse_ballot_w64(other) ? (h >> (GC_HASH_BITS - GC_DUP_BITS)) & (GC_DUP_SIZE - 1) : GC_DUP_SIZE;
+    __asm__ volatile("" ::: "memory");
+    dup[slot2] = (uint8_t)lane;
+    __asm__ volatile("" ::: "memory");
+    const uint32_t back2 = dup[slot2];
+
+    // Exact groups (one step per distinct hash among the seeds).
     const uint64_t lane_bit = 1ull << lane;
     uint64_t group = lane_bit;
-    uint64_t rest = ((__builtin_amdgcn_uicmp(back, lane, 33) & __builtin_amdgcn_uicmp(hw, h, 32)) |  // !=, ==
-                     __builtin_amdgcn_uicmp(back2, lane, 33)) &
-                    vmask;
+    uint64_t rest = (lost & ~other) | (__builtin_amdgcn_uicmp(back2, lane, 33) & other);
     while (rest) {
         const uint32_t lead = (uint32_t)__builtin_ctzll(rest);
         const uint32_t hl = bcast(h, lead);
