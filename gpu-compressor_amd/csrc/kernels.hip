@@ -536,6 +536,11 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
     const uint32_t n = it.n;
     const uint32_t depth = it.depth;
     bool any = false;
+    // a whole strip's candidates in one 16-byte load and its packed words in
+    // two 16-byte stores where the item's slots are 16-byte aligned (the
+    // candidate slots are; an output at a caller's offset may not be)
+    const bool wide = ((uintptr_t)(cand16 + it.cpos) & 15) == 0 &&
+                      (!LENGTHS || REF || ((uintptr_t)(packed + it.opos) & 15) == 0);
     for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
         uint32_t d[4];
         d[0] = src.w[p0 / 4];
@@ -543,11 +548,20 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
         d[2] = src.w[p0 / 4 + 2];
         d[3] = src.w[p0 / 4 + 3];
         const uint32_t prev0 = p0 > 0 ? ld32(src, p0 - 1) : 0;
+        const bool full = wide && p0 + STRIP <= n;
         uint32_t tgt[STRIP], vc[STRIP];
+        if (full) {
+            const uint4 cw = *(const uint4*)(cand16 + it.cpos + p0);
+            const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
 #pragma unroll
-        for (uint32_t j = 0; j < STRIP; j++) {
-            const uint32_t p = p0 + j;
-            tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
+            for (uint32_t j = 0; j < STRIP; j++)
+                tgt[j] = (p0 + j + 3 < n && depth > 0) ? (cv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu : GC_NONE;
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < STRIP; j++) {
+                const uint32_t p = p0 + j;
+                tgt[j] = (p + 3 < n && depth > 0) ? cand16[it.cpos + p] : GC_NONE;
+            }
         }
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32(src, tgt[j]) : 0;
@@ -583,6 +597,12 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
             out[j] = o;
         }
         if constexpr (!LENGTHS) continue;
+        if (!REF && full) {
+            uint4* q = (uint4*)(packed + it.opos + p0);
+            q[0] = make_uint4(out[0], out[1], out[2], out[3]);
+            q[1] = make_uint4(out[4], out[5], out[6], out[7]);
+            continue;
+        }
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) {
             const uint32_t p = p0 + j;
@@ -952,16 +972,16 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // the repeat flags of the window at x - lane: from the ring, or for a
     // far offset from a shadow (`shadows`: the window is the one they were
     // loaded for) or a global read
+    // (wv comes from peek: bit 24 is set where x + 3 >= n, so no flag is set
+    // there)
     auto flags = [&](uint32_t x, uint32_t wv, bool shadows) -> uint32_t {
-        const bool act = x + 3 < n;
         // every lane reads (indexes wrap inside the ring), then masks
         uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1), v2 = ring_w24(ring, x - rep2);
         if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
         if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
         if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
-        const uint32_t fv = (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
-                            (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
-        return act ? fv : 0;
+        return (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
+               (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
     };
     // entering the window at nb (w already holds its bytes): the bytes from
     // `from` (a multiple of 64) up to nb are fetched into the ring first
