@@ -435,13 +435,22 @@ __device__ __forceinline__ void cand_table(CandLds& s, const uint32_t* ana, uint
         v[j] = *e;
         if ((int32_t)a[j] < 0) *(uint16_t*)e = (uint16_t)(t0 + j * WAVE + lane);  // ANA_WRITE
     }
-    // (one base address and immediate offsets; the bound on the scalar unit)
+    // (one base address and immediate offsets; the bound on the scalar unit,
+    // tested per lane only in a sub-block's last tile)
     uint16_t* o = out + t0 + lane;
     const int32_t left = (int32_t)(n - t0);
+    if (left >= (int32_t)CAND_TILE) {
 #pragma unroll
-    for (int j = 0; j < CAND_AWAVES; j++) {
-        const uint32_t c = (a[j] >> 15) & 0xFFFFu;
-        if ((int32_t)lane < left - j * (int32_t)WAVE) o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
+        for (int j = 0; j < CAND_AWAVES; j++) {
+            const uint32_t c = (a[j] >> 15) & 0xFFFFu;
+            o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < CAND_AWAVES; j++) {
+            const uint32_t c = (a[j] >> 15) & 0xFFFFu;
+            if ((int32_t)lane < left - j * (int32_t)WAVE) o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
+        }
     }
 }
 
@@ -857,21 +866,6 @@ __device__ __forceinline__ uint32_t rep_eq(const SrcView& src, uint32_t x, uint3
 #define RING_BYTES (PARSE_RING + 16)  // + the copy of bytes 0..7 + a junk slot
 #define RING_JUNK (PARSE_RING + 8)
 
-// Writes of lanes that have nothing to write go to the junk slot: no
-// branches around LDS accesses, whose waits would then not overlap.
-__device__ __forceinline__ void ring_put8(uint8_t* ring, uint32_t p, uint32_t byte) {
-    const uint32_t j = p & (PARSE_RING - 1);
-    ring[j] = (uint8_t)byte;
-    ring[j < 8 ? PARSE_RING + j : RING_JUNK] = (uint8_t)byte;
-}
-
-// 2 bytes at p, a multiple of 64, from the lanes with `on` set.
-__device__ __forceinline__ void ring_put16(uint8_t* ring, uint32_t p, uint32_t v, bool on) {
-    const uint32_t j = p & (PARSE_RING - 1);
-    *(uint16_t*)(ring + (on ? j : RING_JUNK)) = (uint16_t)v;
-    *(uint16_t*)(ring + (on && j == 0 ? PARSE_RING : RING_JUNK)) = (uint16_t)v;
-}
-
 // The 3 bytes at p (little endian) from the ring.
 __device__ __forceinline__ uint32_t ring_w24(const uint8_t* ring, uint32_t p) {
     const uint32_t i = p & (PARSE_RING - 1);
@@ -995,8 +989,19 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 if (j < 8) *(uint32_t*)(ring + PARSE_RING + j) = v;
             }
         }
-        ring_put8(ring, nb + lane, w);
-        ring_put16(ring, nb + WAVE, w >> 8, lane == WAVE - 1);
+        // the window's bytes, and from the last lane the 2 bytes past it (the
+        // other lanes write the junk slot rather than branch around the
+        // write); the copy of bytes 0..7 past the ring's end only in the
+        // windows that write them (one in 64, a uniform branch)
+        const uint32_t jw = nb & (PARSE_RING - 1), je = (nb + WAVE) & (PARSE_RING - 1);
+        ring[jw + lane] = (uint8_t)w;
+        *(uint16_t*)(ring + (lane == WAVE - 1 ? je : RING_JUNK)) = (uint16_t)(w >> 8);
+        if (jw == 0) {
+            if (lane < 8) ring[PARSE_RING + lane] = (uint8_t)w;
+        }
+        if (je == 0) {
+            if (lane == WAVE - 1) *(uint16_t*)(ring + PARSE_RING) = (uint16_t)(w >> 8);
+        }
         __asm__ volatile("" ::: "memory");  // ring writes before the reads
     };
     // heads satisfy pos + 3 < n; the walk ends by lowering lim
