@@ -1112,7 +1112,6 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
             const uint32_t ek = bcast(e, k);
-            const uint32_t fk = bcast(f, k);
             uint32_t len, off, sel;
             if (ek > 0xFFFFu) {
                 // hash candidate verified
@@ -1123,6 +1122,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 sel = 3;  // the new offset's flags are computed below
             } else {
                 // the first repeat offset that verifies (rep0, rep1, rep2 in order)
+                const uint32_t fk = bcast(f, k);  // (read here: only this path needs the head's flags)
                 sel = (fk & 1) ? 0u : ((fk & 2) ? 1u : 2u);
                 off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
                 if (off <= RING_REACH) {
