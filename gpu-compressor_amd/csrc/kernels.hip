@@ -550,21 +550,25 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
     // candidate slots are; an output at a caller's offset may not be)
     const bool wide = ((uintptr_t)(cand16 + it.cpos) & 15) == 0 &&
                       (!LENGTHS || REF || ((uintptr_t)(packed + it.opos) & 15) == 0);
-    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
+    // INNER: every position of the wavefront's strips lies in a full, wide
+    // strip with p > 0 and p + GC_LZMA_MAX <= n (the end tests and the length
+    // cap drop out); chosen per step for the wavefront (base: lane 0's strip)
+    auto strip = [&](uint32_t p0, auto inner) {
+        constexpr bool INNER = decltype(inner)::value;
         uint32_t d[4];
         d[0] = src.w[p0 / 4];
         d[1] = src.w[p0 / 4 + 1];
         d[2] = src.w[p0 / 4 + 2];
         d[3] = src.w[p0 / 4 + 3];
-        const uint32_t prev0 = p0 > 0 ? ld32(src, p0 - 1) : 0;
-        const bool full = wide && p0 + STRIP <= n;
+        const uint32_t prev0 = (INNER || p0 > 0) ? ld32(src, p0 - 1) : 0;
+        const bool full = INNER || (wide && p0 + STRIP <= n);
         uint32_t tgt[STRIP], vc[STRIP];
         if (full) {
             const uint4 cw = *(const uint4*)(cand16 + it.cpos + p0);
             const uint32_t cv[4] = {cw.x, cw.y, cw.z, cw.w};
 #pragma unroll
             for (uint32_t j = 0; j < STRIP; j++)
-                tgt[j] = (p0 + j + 3 < n && depth > 0) ? (cv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu : GC_NONE;
+                tgt[j] = ((INNER || p0 + j + 3 < n) && depth > 0) ? (cv[j >> 1] >> (16 * (j & 1))) & 0xFFFFu : GC_NONE;
         } else {
 #pragma unroll
             for (uint32_t j = 0; j < STRIP; j++) {
@@ -580,12 +584,12 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
         for (int j = STRIP - 1; j >= 0; j--) {
             const uint32_t p = p0 + j;
             uint32_t o = 0, t = GC_NONE;
-            if (p + 1 < n) {
+            if (INNER || p + 1 < n) {
                 const uint32_t cur = strip_word(d, j);
-                const uint32_t maxlen = n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX;
+                const uint32_t maxlen = INNER ? GC_LZMA_MAX : (n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX);
                 if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
                     t = tgt[j];  // the probe reports a match only from 2 common bytes on
-                } else if (p > 0) {
+                } else if (INNER || p > 0) {
                     const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
                     if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
                 }
@@ -605,12 +609,12 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
             next_t = t;
             out[j] = o;
         }
-        if constexpr (!LENGTHS) continue;
+        if constexpr (!LENGTHS) return;
         if (!REF && full) {
             uint4* q = (uint4*)(packed + it.opos + p0);
             q[0] = make_uint4(out[0], out[1], out[2], out[3]);
             q[1] = make_uint4(out[4], out[5], out[6], out[7]);
-            continue;
+            return;
         }
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) {
@@ -628,6 +632,13 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
                 }
             }
         }
+    };
+    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
+        const uint32_t base = uniform(p0);
+        if (wide && base > 0 && base + WAVE * STRIP + GC_LZMA_MAX <= n)
+            strip(p0, std::true_type{});
+        else
+            strip(p0, std::false_type{});
     }
     // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
     if (LENGTHS && REF)
