@@ -359,10 +359,13 @@ struct CandLds {
 #define ANA_WRITE (1u << 31)
 #define ANA_FROM_TABLE 0xFFFEu
 
+// ALL: every lane's position is valid (p + 3 < n), as in all but a
+// sub-block's last tiles: the validity masks and selects drop out.
+template <bool ALL>
 __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, uint32_t p, uint32_t n) {
     const uint32_t lane = lane_id();
     const uint32_t h = hash4(word);
-    const bool valid = p + 3 < n;
+    const bool valid = ALL || p + 3 < n;
 
     // Duplicate test: every valid lane writes its id under a slice of its
     // hash (one slot array per wavefront) and reads it back; one lane of each
@@ -387,7 +390,7 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
     // ballot): a ballot of a conjunction, or a per-lane test of a mask bit,
     // would cost further vector instructions.
     uint8_t* dup = s.dup[wave];
-    const uint64_t vmask = __builtin_amdgcn_uicmp(p + 3, n, 36);  // unsigned <
+    const uint64_t vmask = ALL ? ~0ull : __builtin_amdgcn_uicmp(p + 3, n, 36);  // unsigned <
     const uint32_t slot = valid ? (h & (GC_DUP_SIZE - 1)) : GC_DUP_SIZE;
     dup[slot] = (uint8_t)lane;
     __asm__ volatile("" ::: "memory");
@@ -464,6 +467,11 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
     uint16_t* out = cand16 + it.cpos;
     uint32_t* t = (uint32_t*)s.table;
     for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
+    // (lane 0's position decides for the wavefront whether all 64 are valid)
+    auto analyse = [&](uint32_t word, uint32_t p) {
+        return uniform(p) + (WAVE - 1) + 3 < n ? cand_analyse<true>(s, wave, word, p, n)
+                                               : cand_analyse<false>(s, wave, word, p, n);
+    };
     // analysis of tile 0; words two tiles ahead.  The loads are unconditional
     // (positions past the end read position 0, masked when used): a load
     // under a per-lane condition is waited for where the branches join.
@@ -473,7 +481,7 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
         const uint32_t word = ld32<SAFE>(src, p + 3 < n ? p : 0u);
         const uint32_t q = CAND_TILE + p;
         next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
-        s.ana[0][p] = cand_analyse(s, wave, word, p, n);
+        s.ana[0][p] = analyse(word, p);
     }
     lds_barrier();
     uint32_t buf = 0;
@@ -485,7 +493,7 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
                 const uint32_t word = next.word();
                 const uint32_t q = p + CAND_TILE;
                 next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
-                s.ana[buf ^ 1][wave * WAVE + lane] = cand_analyse(s, wave, word, p, n);
+                s.ana[buf ^ 1][wave * WAVE + lane] = analyse(word, p);
             }
         } else {
             cand_table(s, s.ana[buf], t0, n, out);
