@@ -200,6 +200,19 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return bcast(v, WAVE - 1);
 }
 
+// Inclusive prefix sum over the wavefront's lanes (lane i: v of lanes 0..i):
+// shifts within rows of 16 (row_shr 1, 2, 4, 8), then rows 1 and 3 add the
+// row before (row_bcast:15) and rows 2 and 3 the first half (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
 // lane `lane` of a set to sa and of b to sb (all uniform): one instruction
 // per register, where a compare and a select would be two or three.  (Two
 // scalar operands exceed the constant bus: the lane index is passed in M0.)
@@ -1317,6 +1330,9 @@ GC_PARSE_KERNEL(gc_zstd_parse_ref_lits_kernel, GC_SEQ_REF, true)
 #define ZE_ENC_ML (36 * 64)
 #define ZE_ENC_OF (36 * 64 + 53 * 64)
 #define ZE_ENC_SIZE (36 * 64 + 53 * 64 + 29 * 32)
+// per-wavefront bitstream buffer: carried bits (< 32) + 64 sequences of at
+// most 65 bits, + the 2 dwords past the last that a lane's 3-dword OR reaches
+#define ZE_BITBUF 136
 
 __constant__ uint32_t ze_ll_base[36] = {0,  1,  2,   3,   4,   5,   6,    7,    8,    9,     10,    11,
                                         12, 13, 14,  15,  16,  18,  20,   22,   24,   28,    32,    40,
@@ -1415,9 +1431,9 @@ __device__ __forceinline__ void ze_copy_bytes(uint8_t* dst, const uint8_t* src, 
 }
 
 template <uint32_t FMT>
-__device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, const void* seqs, const uint8_t* literals,
-                         const uint32_t* seq_counts, const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes,
-                         uint32_t max_frame, uint32_t max_seq) {
+__device__ void ze_frame(ZeLds& s, uint32_t* bits, const gc_item& it, const uint8_t* arena, const void* seqs,
+                         const uint8_t* literals, const uint32_t* seq_counts, const uint32_t* lit_counts, uint8_t* out,
+                         uint32_t* sizes, uint32_t max_frame, uint32_t max_seq) {
     const uint32_t lane = lane_id();
     const uint32_t n = it.n;
     const uint32_t nseq = min(uniform(seq_counts[it.count_pos]), max_seq);  // reads stay in the item's slots
@@ -1454,29 +1470,19 @@ __device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, cons
             if (lane == 0) dst[o] = 0;  // predefined modes for LL, OF and ML
             o += 1;
         }
-        // the bitstream: scalar accumulator, dwords staged one per lane
-        uint64_t acc = 0;
-        uint32_t nbits = 0, nd = 0, st = 0, wpos = o;
+        // The bitstream, 64 sequences at a time from the last: the FSE state
+        // chain (each state follows from the next sequence's) runs serially
+        // and records each sequence's three table entries in the sequence's
+        // lane; then every lane assembles its sequence's fields (state
+        // updates OF, ML, LL, extra bits LL, ML, OF: at most 65 bits), a
+        // prefix sum over the lanes gives each its bit offset (the last
+        // sequence first), and the lanes OR their bits into the wavefront's
+        // LDS buffer, whose whole dwords then go to the frame.  bits[0] holds
+        // the `carry` bits not yet written, which start at frame byte wpos.
+        uint32_t carry = 0, wpos = o;
         bool over = false;
-        auto put = [&](uint32_t v, uint32_t k) {
-            acc |= (uint64_t)v << nbits;
-            nbits += k;
-            if (nbits >= 32) {
-                st = lane == (nd & (WAVE - 1)) ? (uint32_t)acc : st;
-                nd++;
-                acc >>= 32;
-                nbits -= 32;
-                if ((nd & (WAVE - 1)) == 0) {
-                    if (wpos + 4 * WAVE < limit) {
-                        uint8_t* q = dst + wpos + 4 * lane;
-                        q[0] = (uint8_t)st, q[1] = (uint8_t)(st >> 8), q[2] = (uint8_t)(st >> 16), q[3] = (uint8_t)(st >> 24);
-                    } else {
-                        over = true;
-                    }
-                    wpos += 4 * WAVE;
-                }
-            }
-        };
+        for (uint32_t k = lane; k < ZE_BITBUF; k += WAVE) bits[k] = 0;
+        __asm__ volatile("" ::: "memory");
         uint32_t sl = 0, sm = 0, so = 0;
         for (int32_t top = (int32_t)nseq; top > 0 && !over; top -= WAVE) {
             const int32_t i0 = top > WAVE ? top - WAVE : 0;
@@ -1499,11 +1505,14 @@ __device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, cons
                 xm = (ml - ze_ml_base[mc]) | ((uint32_t)ze_ml_bits[mc] << 16);
                 xo = (ofv - (1u << oc)) | (oc << 16);
             }
+            // the state chain; lane j gets LL | ML << 16 and OF table entries
+            // (none, so no bits, for the frame's last sequence, whose states
+            // are the cells reached from state 0)
+            uint32_t rlm = 0, ro = 0;
             for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
                 const uint32_t cd = bcast(codes, (uint32_t)j);
                 const uint32_t lc = cd & 0xFF, mc = (cd >> 8) & 0xFF, oc = cd >> 16;
                 if (i0 + j == (int32_t)nseq - 1) {
-                    // the last sequence's states: the cells reached from state 0
                     sl = uniform(s.enc[lc * 64]) & 63;
                     sm = uniform(s.enc[ZE_ENC_ML + mc * 64]) & 63;
                     so = uniform(s.enc[ZE_ENC_OF + oc * 32]) & 63;
@@ -1511,36 +1520,73 @@ __device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, cons
                     const uint32_t el = uniform(s.enc[lc * 64 + sl]);
                     const uint32_t em = uniform(s.enc[ZE_ENC_ML + mc * 64 + sm]);
                     const uint32_t eo = uniform(s.enc[ZE_ENC_OF + oc * 32 + so]);
-                    // state updates, read as LL, ML, OF: written in reverse
-                    put(eo >> 9, (eo >> 6) & 7);
-                    put(em >> 9, (em >> 6) & 7);
-                    put(el >> 9, (el >> 6) & 7);
+                    write_lane2(rlm, el | (em << 16), ro, eo, (uint32_t)j);
                     sl = el & 63, sm = em & 63, so = eo & 63;
                 }
-                // extra bits, read as OF, ML, LL: written in reverse
-                const uint32_t vl = bcast(xl, (uint32_t)j), vm = bcast(xm, (uint32_t)j), vo = bcast(xo, (uint32_t)j);
-                put(vl & 0xFFFF, vl >> 16);
-                put(vm & 0xFFFF, vm >> 16);
-                put(vo & 0xFFFF, vo >> 16);
             }
+            // this lane's fields, lowest bits first (fields 1-5 fill at most 49
+            // bits, so only the last can cross into the high word)
+            const uint32_t el = rlm & 0xFFFFu, em = rlm >> 16;
+            uint64_t lo = 0, hi = 0;
+            uint32_t nbit = 0;
+            auto field = [&](uint32_t v, uint32_t k) {
+                lo |= (uint64_t)v << nbit;
+                if (nbit + k > 64) hi = (uint64_t)(v >> (64 - nbit));
+                nbit += k;
+            };
+            field(ro >> 9, (ro >> 6) & 7);
+            field(em >> 9, (em >> 6) & 7);
+            field(el >> 9, (el >> 6) & 7);
+            field(xl & 0xFFFFu, xl >> 16);
+            field(xm & 0xFFFFu, xm >> 16);
+            field(xo & 0xFFFFu, xo >> 16);
+            const uint32_t incl = wave_scan(nbit);
+            const uint32_t batch = bcast(incl, WAVE - 1);
+            const uint32_t at = carry + (batch - incl);  // the lanes above come first
+            const uint32_t d = at >> 5, sh = at & 31;
+            const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32), l2 = (uint32_t)hi;
+            const uint32_t w0 = l0 << sh;
+            const uint32_t w1 = sh ? __builtin_amdgcn_alignbit(l1, l0, 32 - sh) : l1;
+            const uint32_t w2 = sh ? __builtin_amdgcn_alignbit(l2, l1, 32 - sh) : l2;
+            atomicOr(&bits[d], w0);
+            atomicOr(&bits[d + 1], w1);
+            atomicOr(&bits[d + 2], w2);
+            __asm__ volatile("" ::: "memory");
+            // the whole dwords to the frame (bytes: the frame is not aligned)
+            const uint32_t total = carry + batch, full = total >> 5;
+            if (wpos + 4 * full >= limit) {
+                over = true;
+            } else {
+                for (uint32_t k = lane; k < full; k += WAVE) {
+                    const uint32_t v = bits[k];
+                    uint8_t* q = dst + wpos + 4 * k;
+                    q[0] = (uint8_t)v, q[1] = (uint8_t)(v >> 8), q[2] = (uint8_t)(v >> 16), q[3] = (uint8_t)(v >> 24);
+                }
+            }
+            const uint32_t part = bits[full];
+            __asm__ volatile("" ::: "memory");
+            for (uint32_t k = lane; k <= full + 2; k += WAVE) bits[k] = k == 0 ? part : 0u;
+            __asm__ volatile("" ::: "memory");
+            carry = total & 31;
+            wpos += 4 * full;
         }
-        if (nseq) {
+        uint32_t total = carry;
+        if (nseq && !over) {
             // initial states, read as LL, OF, ML; then the end mark
-            put(sm, 6);
-            put(so, 5);
-            put(sl, 6);
-            put(1, 1);
+            const uint64_t v = (uint64_t)(sm | (so << 6) | (sl << 11) | (1u << 17)) << carry;
+            if (lane == 0) {
+                bits[0] |= (uint32_t)v;
+                bits[1] |= (uint32_t)(v >> 32);
+            }
+            __asm__ volatile("" ::: "memory");
+            total = carry + 18;
         }
-        const uint32_t pend = nd & (WAVE - 1), tail = (nbits + 7) / 8;
-        end = wpos + 4 * pend + tail;
+        const uint32_t tail = (total + 7) / 8;
+        end = wpos + tail;
         if (over || end >= limit) {
             raw = true;
         } else {
-            if (lane < pend) {
-                uint8_t* q = dst + wpos + 4 * lane;
-                q[0] = (uint8_t)st, q[1] = (uint8_t)(st >> 8), q[2] = (uint8_t)(st >> 16), q[3] = (uint8_t)(st >> 24);
-            }
-            if (lane < tail) dst[wpos + 4 * pend + lane] = (uint8_t)(acc >> (8 * lane));
+            if (lane < tail) dst[wpos + lane] = (uint8_t)(bits[lane >> 2] >> (8 * (lane & 3)));
         }
         if (raw) __threadfence();  // the raw block below overwrites the bytes written so far
     }
@@ -1555,22 +1601,25 @@ __device__ void ze_frame(ZeLds& s, const gc_item& it, const uint8_t* arena, cons
     if (lane == 0) sizes[it.count_pos] = end;
 }
 
-// (scalar registers capped at 96, as the parse's: 8 waves per SIMD)
+// (scalar registers capped at 96: above that the hardware runs fewer waves per
+// SIMD than the compiler counts)
 extern "C" __global__ void __launch_bounds__(ZE_THREADS) __attribute__((amdgpu_num_sgpr(96)))
 gc_zstd_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const void* seqs,
                       uint32_t seq_fmt, const uint8_t* literals, const uint32_t* seq_counts,
                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
                       uint32_t max_seq) {
     __shared__ ZeLds s;
+    __shared__ uint32_t bitbuf[ZE_WAVES][ZE_BITBUF];
     ze_build(s);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     for (uint32_t i = blockIdx.x * ZE_WAVES + wave; i < n_items; i += gridDim.x * ZE_WAVES) {
         const gc_item it = load_item(items, i);
         if (seq_fmt == GC_SEQ_REF)
-            ze_frame<GC_SEQ_REF>(s, it, arena, seqs, literals, seq_counts, lit_counts, out, sizes, max_frame, max_seq);
+            ze_frame<GC_SEQ_REF>(s, bitbuf[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
+                                 max_frame, max_seq);
         else
-            ze_frame<GC_SEQ_ZSTD>(s, it, arena, seqs, literals, seq_counts, lit_counts, out, sizes, max_frame,
-                                  max_seq);
+            ze_frame<GC_SEQ_ZSTD>(s, bitbuf[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
+                                  max_frame, max_seq);
     }
 }
 
