@@ -112,6 +112,21 @@ __device__ __forceinline__ uint32_t ld32(const SrcView& v, uint32_t off) {
     return ld32_tail(v.base + v.mis + off, off, v.avail);  // the last bytes of the arena, byte by byte
 }
 
+// ld32 as one unaligned dword load (gfx950 under ROCm runs with unaligned
+// memory access enabled: a dword load at any byte address returns the 4
+// bytes there), for a prefetch whose word is used as loaded.  The bytes read
+// lie inside those of ld32's aligned pair, so the same SAFE bound covers
+// them.
+template <bool SAFE>
+__device__ __forceinline__ uint32_t ld32u(const SrcView& v, uint32_t off) {
+    if (SAFE || (uint64_t)off + 8 <= v.avail) {
+        uint32_t w;
+        __builtin_memcpy(&w, v.base + (off + v.mis), 4);  // scalar base + 32-bit lane offset
+        return w;
+    }
+    return ld32_tail(v.base + v.mis + off, off, v.avail);
+}
+
 // ld32 in two halves, for a prefetch: the load now (the two aligned dwords
 // and the shift), the byte alignment when the word is used.  Computing it
 // right away would wait for the load there.
@@ -956,19 +971,26 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // as 0.
     uint32_t e = 0, e1 = 0, w = 0, f = 0, ae = 0, ae1 = 0, aw = 0;
     uint32_t xv = lane;  // wbase + lane
+    // peek issues the loads only: the words go straight into the registers
+    // that carry them (for the next window, ae, ae1, aw) and are masked by
+    // finish where they are used.  Masking them at the load would wait for
+    // the loads there, a full memory round trip on every window change.
+    // The 4 bytes at x come from one unaligned dword load (gfx950 under ROCm
+    // runs with unaligned memory access enabled), so no byte shift is left
+    // to do either.
     auto peek = [&](uint32_t x, uint32_t& ev, uint32_t& ev1, uint32_t& wv) {
         const bool act = x + 3 < n;
-        const bool in = x < n;
         const uint32_t xc = act ? x : 0;  // reads stay in the sub-block (x + 1 < n here)
         const uint32_t* ip = (const uint32_t*)(inf + (xc << 2));
-        const uint32_t i0 = ip[0];
-        const uint32_t w0 = ld32<SAFE>(src, in ? x : 0u) & 0xFFFFFFu;
-        ev = act ? i0 : 0;
-        wv = (in ? w0 : 0) | (act ? 0 : 0x1000000u);
-        if constexpr (LAZY) {
-            const uint32_t i1 = ip[1];
-            ev1 = x + 4 < n ? i1 : 0;
-        }
+        ev = ip[0];
+        wv = ld32u<SAFE>(src, x < n ? x : 0u);
+        if constexpr (LAZY) ev1 = ip[1];
+    };
+    auto finish = [&](uint32_t x) {  // e, e1, w of the window at x - lane, as peek left them
+        const bool act = x + 3 < n;
+        e = act ? e : 0;
+        w = (x < n ? (w & 0xFFFFFFu) : 0) | (act ? 0 : 0x1000000u);
+        if constexpr (LAZY) e1 = x + 4 < n ? e1 : 0;
     };
     // Repeat offsets beyond the ring's reach: at each window entry, for the
     // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
@@ -1040,6 +1062,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     uint32_t lim = (n > 3 && max_seq > 0) ? n - 3 : 0;
     if (lim) {
         peek(lane, e, e1, w);
+        finish(lane);
         enter(0, 0);
         f = flags(lane, w, false);
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
@@ -1063,6 +1086,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
         wbase = nb;
         xv = nb + lane;
+        finish(xv);
         enter(nb, from);
         f = flags(xv, w, next);
         __asm__ volatile("" ::: "memory");
@@ -1134,6 +1158,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             e = ae;
             e1 = ae1;
             w = aw;
+            finish(xv);
             enter(wbase, wbase);
             f = flags(xv, w, true);
             __asm__ volatile("" ::: "memory");
