@@ -1132,26 +1132,16 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // next sequence count at which the staged records are stored or the
     // walk stops (max_seq)
     uint32_t ev_at = max_seq < WAVE ? max_seq : WAVE;
-    bool slid = false;  // the last window change was a slide past a window with no head
     while (pos < lim) {
         // two ballots of plain compares: a ballot of their conjunction is
         // built as a select and a third compare
         const uint64_t m = ballot(xv >= pos) & ballot(((e >> 16) | f) != 0);
         if (!m) {
             // no head left in the window: every position up to its end is a
-            // literal.  After a slide, look further ahead at once.
-            if (slid) {
-                const uint32_t nb = scan(wbase + WAVE);
-                slid = false;
-                if (nb >= lim) {
-                    pos = lim;  // literals to the end
-                } else {
-                    move_to(nb, false);  // (the prefetched words are not kept across the scan)
-                    pos = nb;
-                }
-                continue;
-            }
-            slid = true;
+            // literal.  Slide to the next window (its words are prefetched);
+            // if that one has no head either, look further ahead at once.
+            // (Testing the new window here keeps a "just slid" flag, and its
+            // reset on every head, out of the walk.)
             wbase += WAVE;
             pos = wbase;
             xv += WAVE;
@@ -1164,23 +1154,29 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             __asm__ volatile("" ::: "memory");
             load_shadows();
             peek(wbase + WAVE + lane, ae, ae1, aw);
+            if (pos < lim && ballot(((e >> 16) | f) != 0) == 0) {
+                const uint32_t nb = scan(wbase + WAVE);
+                if (nb >= lim) {
+                    pos = lim;  // literals to the end
+                } else {
+                    move_to(nb, false);  // (the prefetched words are not kept across the scan)
+                    pos = nb;
+                }
+            }
         } else {
-            slid = false;
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
             const uint32_t ek = bcast(e, k);
-            uint32_t len, off, sel;
+            uint32_t len, off;
             if (ek > 0xFFFFu) {
                 // hash candidate verified
                 const uint32_t c = ek & 0xFFFFu;
                 off = head - c;
                 len = ek >> 16;
                 if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, c, head, n - head, GC_ZSTD_SHORT);
-                sel = 3;  // the new offset's flags are computed below
             } else {
                 // the first repeat offset that verifies (rep0, rep1, rep2 in order)
                 const uint32_t fk = bcast(f, k);  // (read here: only this path needs the head's flags)
-                sel = (fk & 1) ? 0u : ((fk & 2) ? 1u : 2u);
                 off = (fk & 1) ? rep0 : ((fk & 2) ? rep1 : rep2);
                 if (off <= RING_REACH) {
                     // the bytes up to the window end are in the ring
@@ -1211,7 +1207,6 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                         head += 1;
                         len = len2;
                         off = head - c2;
-                        sel = 3;
                     }
                 }
             }
@@ -1228,8 +1223,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 if (nseq == max_seq) lim = 0;  // the reference stops at the next head
                 ev_at = nseq + WAVE < max_seq ? nseq + WAVE : max_seq;
             }
-            // repeat offsets (off, o0, o1); sel: which old flag the new
-            // offset's is (0..2: a repeat match), or 3 = compute it
+            // repeat offsets (off, o0, o1)
             rep2 = rep1;
             rep1 = rep0;
             rep0 = off;
@@ -1240,19 +1234,21 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 // (reads past the sub-block end are clamped by peek)
                 move_to(pos & ~(WAVE - 1), true);
             } else {
-                uint32_t b0 = (f >> sel) & 1u;  // sel 3 reads 0
-                if (sel == 3) {
+                // the new offset's flags (a flag depends only on the offset
+                // and the position, so a repeat match's offset gets the same
+                // bits it had: recomputed, not renamed, which keeps the rare
+                // repeat path's choice out of the common one)
+                uint32_t b0;
+                if (off <= RING_REACH) {
+                    const uint32_t v = ring_w24(ring, xv - off);
+                    b0 = ((xv >= off) & (v == w)) ? 1u : 0u;  // (w has bit 24 set where xv + 3 >= n)
+                } else if (off == rep1) {  // a far offset equal to the old rep0 or rep1: its flags
+                    b0 = f & 1u;
+                } else if (off == rep2) {
+                    b0 = (f >> 1) & 1u;
+                } else {
                     const bool act = xv + 3 < n;
-                    if (off <= RING_REACH) {
-                        const uint32_t v = ring_w24(ring, xv - off);
-                        b0 = ((xv >= off) & (v == w)) ? 1u : 0u;  // (w has bit 24 set where !act)
-                    } else if (off == rep1) {  // a far offset equal to the old rep0 or rep1: its flags
-                        b0 = f & 1u;
-                    } else if (off == rep2) {
-                        b0 = (f >> 1) & 1u;
-                    } else {
-                        b0 = rep_eq<SAFE>(src, act ? xv : 0u, off, w & 0xFFFFFFu) & (act ? 1u : 0u);
-                    }
+                    b0 = rep_eq<SAFE>(src, act ? xv : 0u, off, w & 0xFFFFFFu) & (act ? 1u : 0u);
                 }
                 f = b0 | ((f & 3u) << 1);
             }
