@@ -986,11 +986,19 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         wv = ld32u<SAFE>(src, x < n ? x : 0u);
         if constexpr (LAZY) ev1 = ip[1];
     };
+    // finish also turns each word's candidate into its offset: the low half
+    // of e becomes x - cand (1..65535 where a candidate verified, the high
+    // half, the length, untouched), so that a head's offset is one scalar
+    // AND; e1's becomes x + 1 - cand.  The info array itself is unchanged.
     auto finish = [&](uint32_t x) {  // e, e1, w of the window at x - lane, as peek left them
         const bool act = x + 3 < n;
         e = act ? e : 0;
+        e = (e & 0xFFFF0000u) | ((x - e) & 0xFFFFu);
         w = (x < n ? (w & 0xFFFFFFu) : 0) | (act ? 0 : 0x1000000u);
-        if constexpr (LAZY) e1 = x + 4 < n ? e1 : 0;
+        if constexpr (LAZY) {
+            e1 = x + 4 < n ? e1 : 0;
+            e1 = (e1 & 0xFFFF0000u) | ((x + 1 - e1) & 0xFFFFu);
+        }
     };
     // Repeat offsets beyond the ring's reach: at each window entry, for the
     // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
@@ -1167,13 +1175,17 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
             const uint32_t ek = bcast(e, k);
-            uint32_t len, off;
+            uint32_t len, off, ol;  // ol: the staged record word, off | len << 16
             if (ek > 0xFFFFu) {
-                // hash candidate verified
-                const uint32_t c = ek & 0xFFFFu;
-                off = head - c;
+                // hash candidate verified (ek's low half is its offset, so ek
+                // is the record word unless the length is measured further)
+                off = ek & 0xFFFFu;
                 len = ek >> 16;
-                if (len >= GC_ZSTD_SHORT) len = lcp_wave<SAFE>(src, c, head, n - head, GC_ZSTD_SHORT);
+                ol = ek;
+                if (len >= GC_ZSTD_SHORT) {
+                    len = lcp_wave<SAFE>(src, head - off, head, n - head, GC_ZSTD_SHORT);
+                    ol = off | (len << 16);
+                }
             } else {
                 // the first repeat offset that verifies (rep0, rep1, rep2 in order)
                 const uint32_t fk = bcast(f, k);  // (read here: only this path needs the head's flags)
@@ -1194,19 +1206,21 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 } else {
                     len = lcp_wave<SAFE>(src, head - off, head, n - head, 3);
                 }
+                ol = off | (len << 16);
             }
             if constexpr (LAZY) {
                 // one lazy step at head + 1 (zstd_compress.cu:278-307)
                 const uint32_t e2 = bcast(e1, k);
                 if (e2 > 0xFFFFu) {
-                    const uint32_t c2 = e2 & 0xFFFFu;
+                    const uint32_t off2 = e2 & 0xFFFFu;  // (the offset from head + 1)
                     uint32_t len2 = e2 >> 16;
                     if (len2 >= GC_ZSTD_SHORT)
-                        len2 = lcp_wave<SAFE>(src, c2, head + 1, n - head - 1, GC_ZSTD_SHORT);
+                        len2 = lcp_wave<SAFE>(src, head + 1 - off2, head + 1, n - head - 1, GC_ZSTD_SHORT);
                     if (len2 > len + 1) {
                         head += 1;
                         len = len2;
-                        off = head - c2;
+                        off = off2;
+                        ol = off | (len << 16);
                     }
                 }
             }
@@ -1216,7 +1230,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 copy_wave(lit_dst + uniform(nlit), src.base + src.mis + uniform(lit_start), uniform(ll));
                 nlit += ll;
             }
-            write_lane2(q_ol, off | (len << 16), q_h, head, nseq & (WAVE - 1));
+            write_lane2(q_ol, ol, q_h, head, nseq & (WAVE - 1));
             nseq++;
             if (nseq == ev_at) {
                 if ((nseq & (WAVE - 1)) == 0) store_staged(nseq - WAVE, WAVE);
