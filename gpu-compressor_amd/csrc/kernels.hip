@@ -1011,7 +1011,14 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     auto far_w24 = [&](uint32_t y, uint32_t r) -> uint32_t {  // the 3 bytes at y - r (clamped read)
         return ld32<SAFE>(src, (y >= r && y < n) ? y - r : 0u) & 0xFFFFFFu;
     };
-    auto load_shadows = [&]() {  // for the window after the current one
+    // (far: some current offset is beyond the ring's reach; one test per
+    // window change instead of one per offset in flags and load_shadows)
+    auto any_far = [&]() { return max(rep0, max(rep1, rep2)) > RING_REACH; };
+    auto load_shadows = [&](bool far) {  // for the window after the current one
+        if (!far) {  // (no tag may match a later far offset: the shadows are stale)
+            sr0 = sr1 = sr2 = in_vgpr(0u);
+            return;
+        }
         sr0 = in_vgpr(rep0 > RING_REACH ? rep0 : 0u);
         sr1 = in_vgpr(rep1 > RING_REACH ? rep1 : 0u);
         sr2 = in_vgpr(rep2 > RING_REACH ? rep2 : 0u);
@@ -1030,12 +1037,14 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // loaded for) or a global read
     // (wv comes from peek: bit 24 is set where x + 3 >= n, so no flag is set
     // there)
-    auto flags = [&](uint32_t x, uint32_t wv, bool shadows) -> uint32_t {
+    auto flags = [&](uint32_t x, uint32_t wv, bool shadows, bool far) -> uint32_t {
         // every lane reads (indexes wrap inside the ring), then masks
         uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1), v2 = ring_w24(ring, x - rep2);
-        if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
-        if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
-        if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
+        if (far) {
+            if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
+            if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
+            if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
+        }
         return (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
                (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
     };
@@ -1072,7 +1081,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         peek(lane, e, e1, w);
         finish(lane);
         enter(0, 0);
-        f = flags(lane, w, false);
+        f = flags(lane, w, false, false);  // (offsets 1, 4, 8)
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
         peek(WAVE + lane, ae, ae1, aw);
     }
@@ -1096,9 +1105,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         xv = nb + lane;
         finish(xv);
         enter(nb, from);
-        f = flags(xv, w, next);
+        const bool far = any_far();
+        f = flags(xv, w, next, far);
         __asm__ volatile("" ::: "memory");
-        load_shadows();
+        load_shadows(far);
         peek(nb + WAVE + lane, ae, ae1, aw);
     };
     // Look-ahead over a stretch of literals: from sb (a window start), 256
@@ -1158,9 +1168,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             w = aw;
             finish(xv);
             enter(wbase, wbase);
-            f = flags(xv, w, true);
+            const bool far = any_far();
+            f = flags(xv, w, true, far);
             __asm__ volatile("" ::: "memory");
-            load_shadows();
+            load_shadows(far);
             peek(wbase + WAVE + lane, ae, ae1, aw);
             if (pos < lim && ballot(((e >> 16) | f) != 0) == 0) {
                 const uint32_t nb = scan(wbase + WAVE);
