@@ -1222,7 +1222,13 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             if constexpr (LAZY) {
                 // one lazy step at head + 1 (zstd_compress.cu:278-307)
                 const uint32_t e2 = bcast(e1, k);
-                if (e2 > 0xFFFFu) {
+                // it can win only if the word at head + 1 is at least
+                // min(len + 2, GC_ZSTD_SHORT) long (a length of GC_ZSTD_SHORT
+                // or more reads as GC_ZSTD_SHORT): one compare of packed words
+                // rules out most heads (ol >> 16 is len; an overflow only lets
+                // a head through to the exact test below)
+                const uint32_t need = min((ol & 0xFFFF0000u) + 0x20000u, GC_ZSTD_SHORT << 16);
+                if (e2 >= need) {
                     const uint32_t off2 = e2 & 0xFFFFu;  // (the offset from head + 1)
                     uint32_t len2 = e2 >> 16;
                     if (len2 >= GC_ZSTD_SHORT)
