@@ -378,8 +378,13 @@ int gc_lzma2_raw_decode(const uint8_t* src, size_t n, uint8_t* out, size_t expec
     s.avail_out = expected;
     int ret = L.code(&s, kLzmaFinish);
     size_t got = expected - s.avail_out;
+    const size_t left = s.avail_in;
     L.end(&s);
-    if (ret != kLzmaStreamEnd && ret != kLzmaOk) return fail(GC_EFORMAT, "LZMA2 decode failed");
+    // The entry must be one whole stream (its end marker read) that decodes
+    // into `expected` bytes: a stream still going when the output is full, or
+    // bytes after the end marker, is a damaged entry, not a short read.
+    if (ret != kLzmaStreamEnd) return fail(GC_EFORMAT, "LZMA2 decode failed or the entry decodes past its size");
+    if (left) return fail(GC_EFORMAT, "LZMA2 entry has bytes after its end marker");
     *out_len = got;
     return GC_OK;
 }

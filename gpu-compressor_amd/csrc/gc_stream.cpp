@@ -12,8 +12,10 @@
 // caller's thread, with at most 2 x threads frames in memory.
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
+#include <new>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
@@ -65,7 +67,14 @@ extern "C" int gc_decompress_file(const char* in_path, const char* out_path, int
     const uint64_t orig = le64(head + 4), ch = le64(head + 12), nch = le64(head + 20);
     if (ch == 0 ? orig != 0 || nch != 0 : nch != (orig + ch - 1) / ch)
         return fail(GC_EFORMAT, "entry count does not match original and chunk sizes");
-    if (nch > (1ull << 40)) return fail(GC_EFORMAT, "entry count out of range");
+    // Sizes come from the file and are bounded before anything is allocated:
+    // a chunk larger than 4 GiB is no container this library or the
+    // reference writes, and every entry needs 8 bytes of the size table.
+    struct stat st;
+    if (fstat(fileno(in.f), &st) != 0) return fail(GC_EIO, std::string("cannot stat ") + in_path);
+    const uint64_t file_size = (uint64_t)st.st_size;
+    if (ch > (4ull << 30)) return fail(GC_EFORMAT, "chunk size out of range (above 4 GiB)");
+    if (nch > (file_size - 28) / 8) return fail(GC_EFORMAT, "truncated size table");
     std::vector<uint64_t> csize(nch);
     {
         std::vector<uint8_t> table(8 * nch);
@@ -106,7 +115,12 @@ extern "C" int gc_decompress_file(const char* in_path, const char* out_path, int
                 std::lock_guard<std::mutex> lk(mu);
                 return set_err(GC_EFORMAT, "a frame size is out of range");
             }
-            s.in.resize(csize[c]);
+            try {
+                s.in.resize(csize[c]);
+            } catch (const std::bad_alloc&) {
+                std::lock_guard<std::mutex> lk(mu);
+                return set_err(GC_ENOMEM, "no memory for a frame");
+            }
             const bool ok = fread(s.in.data(), 1, csize[c], in.f) == csize[c];
             std::lock_guard<std::mutex> lk(mu);
             if (!ok) return set_err(GC_EFORMAT, "truncated payload");  // read_exact, decompress.rs:81
@@ -133,7 +147,12 @@ extern "C" int gc_decompress_file(const char* in_path, const char* out_path, int
                 }
                 Slot& s = slots[c % window];
                 const uint64_t d = dsize_of(c);
-                s.out.resize(d);
+                try {
+                    s.out.resize(d);
+                } catch (const std::bad_alloc&) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    return set_err(GC_ENOMEM, "no memory for a decoded frame");
+                }
                 size_t got = 0;
                 const int rc = algo == GC_ALGO_LZMA2
                                    ? gc_lzma2_raw_decode(s.in.data(), s.in.size(), s.out.data(), d, &got)

@@ -71,6 +71,19 @@ def test_lzma2_triage_raw_and_compressed():
     assert gcompress.lzma2_raw_decode(blob, SUB) == txt.tobytes()
 
 
+def test_lzma2_decode_rejects_long_or_trailing_entries():
+    # an entry that decodes to more than the expected size, or has bytes after its end marker, is
+    # damaged: it is not truncated to fit
+    txt = datagen.text(SUB, 6)
+    counts = np.ones(SUB, np.uint32)
+    blob = gcompress.encode_single_sub_block(txt, counts)
+    assert gcompress.lzma2_raw_decode(blob, SUB) == txt.tobytes()
+    with pytest.raises(gcompress.GcError, match="past its size"):
+        gcompress.lzma2_raw_decode(blob, SUB - 1)
+    with pytest.raises(gcompress.GcError, match="after its end marker"):
+        gcompress.lzma2_raw_decode(blob + b"\x00\x00", SUB)
+
+
 def test_lzma2_triage_matches_oracle_counts():
     for data in (datagen.mix(SUB, 3), datagen.random_bytes(SUB, 4), datagen.structured(SUB, 5)):
         _, c = oracle.lzma2_match_find(data, SUB, 8, 32)

@@ -78,12 +78,19 @@ def test_damaged_files(tmp_path):
         "payload": good[:-10],
         "frame": good[:-40] + bytes(40),
         "size": good[:28] + struct.pack("<Q", 1 << 40) + good[36:],
+        # a huge chunk of one entry: refused before any allocation of its size
+        "huge_chunk": b"NVZS" + struct.pack("<QQQ", 1 << 40, 1 << 40, 1) + struct.pack("<Q", 100) + bytes(100),
+        # an entry count the file cannot hold a size table for
+        "huge_count": b"NVZS" + struct.pack("<QQQ", (1 << 40) * SUB, SUB, 1 << 40) + bytes(64),
     }
     for name, blob in cases.items():
         p = tmp_path / f"{name}.nv"
         p.write_bytes(blob)
         with pytest.raises(gcompress.GcError):
             gcompress.decompress_file(p, out, 3)
+        if name in ("huge_chunk", "huge_count"):
+            with pytest.raises(gcompress.GcError, match="out of range|truncated"):
+                gcompress.decompress_file(p, out, 3)
     with pytest.raises(gcompress.GcError, match="cannot open"):
         gcompress.decompress_file(tmp_path / "missing.nv", out, 1)
     p = tmp_path / "good.nv"
