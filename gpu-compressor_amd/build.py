@@ -38,6 +38,9 @@ HOST_SOURCES = [
     "gc_stream.cpp",
 ]
 KERNEL_SOURCES = ["kernels.hip"]
+# HIP sources compiled for host and device into the library itself (device
+# variables registered with the runtime, looked up with hipGetSymbolAddress)
+HOST_HIP_SOURCES = ["gc_noalloc.hip"]
 
 # No packed-FP32 instructions anywhere (none are needed: the work is integer).
 HIP_FLAGS = [
@@ -82,6 +85,10 @@ def build_library(cos):
             f'-DGC_CO_PATH="{cos[0]}"', f'-DGC_PARSE_CO_PATH="{cos[1]}"',
             os.path.join(CSRC, s), "-o", obj,
         ])
+        objs.append(obj)
+    for s in HOST_HIP_SOURCES:
+        obj = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
+        run([HIPCC, "-c", "-fPIC", *HIP_FLAGS, "-I", CSRC, os.path.join(CSRC, s), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     run(["g++", "-shared", "-o", tmp, *objs, f"-L{ROCM}/lib", f"-Wl,-rpath,{ROCM}/lib",

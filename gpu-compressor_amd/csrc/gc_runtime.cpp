@@ -29,19 +29,29 @@ extern "C" const unsigned char gc_parse_code_object[];
 
 namespace gc {
 
-static thread_local std::string t_error;
+// A fixed buffer: reporting an error allocates nothing (the allocation-free
+// entry points stay so on their error paths too).
+static thread_local char t_error[512];
 
-void set_error(const std::string& msg) { t_error = msg; }
+void set_error(const char* msg) {
+    size_t k = 0;
+    for (; msg && msg[k] && k + 1 < sizeof(t_error); k++) t_error[k] = msg[k];
+    t_error[k] = 0;
+}
 
-int fail(int code, const std::string& msg) {
-    t_error = msg;
+void set_error(const std::string& msg) { set_error(msg.c_str()); }
+
+int fail(int code, const char* msg) {
+    set_error(msg);
     return code;
 }
+
+int fail(int code, const std::string& msg) { return fail(code, msg.c_str()); }
 
 int hip_fail(hipError_t e, const char* what) {
     char buf[256];
     snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
-    t_error = buf;
+    set_error(buf);
     return GC_EHIP;
 }
 
@@ -162,6 +172,12 @@ int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32
     return launch(stream, ctx->k.copy_u32, grid, 256, args, "launch gc_copy_u32");
 }
 
+int launch_make_items(gc_ctx* ctx, hipStream_t stream, gc_item* items, uint64_t total, uint32_t sub, uint32_t nsub,
+                      uint32_t max_seq, uint32_t depth) {
+    void* args[] = {&items, &total, &sub, &nsub, &max_seq, &depth};
+    return launch(stream, ctx->k.make_items, (nsub + 255) / 256, 256, args, "launch gc_make_items");
+}
+
 uint32_t zstd_depth_for_level(uint32_t level) {
     // compress_zstd_custom.rs:57-61
     if (level == 1) return GC_ZSTD_DEPTH_LAZY;
@@ -175,7 +191,7 @@ using namespace gc;
 
 extern "C" {
 
-const char* gc_last_error(void) { return t_error.c_str(); }
+const char* gc_last_error(void) { return t_error; }
 
 int gc_device_count(void) {
     int n = 0;
@@ -216,11 +232,15 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.order_items, ctx->k.module, "gc_order_items");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_encode, ctx->k.module, "gc_zstd_encode_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_bytes, ctx->k.module, "gc_gather_bytes");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.make_items, ctx->k.module, "gc_make_items");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);
         return nullptr;
     }
+    // the allocation-free entry point's scratch on this device (a failure is
+    // reported by that entry point; every other entry point works without it)
+    (void)noalloc_attach(device);
     return ctx;
 }
 

@@ -12,8 +12,11 @@
 
 namespace gc {
 
-// Thread-local error message behind gc_last_error().
+// Thread-local error message behind gc_last_error() (a fixed buffer; the
+// const char* forms allocate nothing).
+void set_error(const char* msg);
 void set_error(const std::string& msg);
+int fail(int code, const char* msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
 
@@ -42,6 +45,7 @@ struct Kernels {
     hipFunction_t order_items = nullptr;
     hipFunction_t zstd_encode = nullptr;
     hipFunction_t gather_bytes = nullptr;
+    hipFunction_t make_items = nullptr;
 };
 
 }  // namespace gc
@@ -90,6 +94,12 @@ int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
 int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
                         const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub);
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);
+// Items of a buffer tiled in sub-blocks from its start, written on the device.
+int launch_make_items(gc_ctx* ctx, hipStream_t stream, gc_item* items, uint64_t total, uint32_t sub, uint32_t nsub,
+                      uint32_t max_seq, uint32_t depth);
+// Per-device scratch of the allocation-free entry point (gc_noalloc.hip):
+// looked up once per device when a context opens.
+int noalloc_attach(int device);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
 
 uint32_t zstd_depth_for_level(uint32_t level);

@@ -1687,6 +1687,30 @@ gc_gather_bytes(const uint8_t* in, const uint32_t* sizes, const uint64_t* offset
     }
 }
 
+// Work items of a buffer tiled in sub-blocks of `sub` bytes from its start,
+// scratch positions at a pitch of sub rounded up to 8 (the host's make_items
+// for the kernel-level Zstd entry point, built on the device so that a call
+// needs no host-to-device copy).
+extern "C" __global__ void __launch_bounds__(256)
+gc_make_items(gc_item* items, uint64_t total, uint32_t sub, uint32_t nsub, uint32_t max_seq, uint32_t depth) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    const uint64_t start = (uint64_t)s * sub;
+    const uint64_t pitch = ((uint64_t)sub + 7) & ~(uint64_t)7;
+    gc_item it;
+    it.src_off = start;
+    it.avail = total - start;
+    it.cpos = (uint64_t)s * pitch;
+    it.opos = (uint64_t)s * pitch;
+    it.seq_pos = (uint64_t)s * max_seq;
+    it.lit_pos = start;
+    it.count_pos = s;
+    it.n = (uint32_t)(total - start < sub ? total - start : sub);
+    it.depth = depth;
+    it.out_span = sub;
+    items[s] = it;
+}
+
 // Small fill kernel.
 extern "C" __global__ void gc_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)

@@ -52,6 +52,8 @@ SIGNATURES = [
     ("gc_stream", _vp, [_vp]),
     ("gc_zstd_match_find_dev", C.c_int,
      [_vp, _vp, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("gc_zstd_match_find_dev_noalloc", C.c_int,
+     [_vp, _vp, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_lzma2_match_find_dev", C.c_int,
      [_vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_lzma2_match_find_packed_dev", C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint64, C.c_uint32]),

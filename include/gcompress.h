@@ -108,6 +108,22 @@ int gc_zstd_match_find_dev(gc_ctx* ctx, const uint8_t* d_input, void* d_sequence
                            uint32_t num_chunks, uint64_t total_input_size, uint32_t search_depth,
                            uint32_t max_sequences_per_chunk);
 
+/* No counterpart in the reference: the same as gc_zstd_match_find_dev, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing, copies
+ * nothing and does not synchronise: its scratch is one fixed device array per
+ * GPU, held until the process ends, so num_chunks may be at most
+ * GC_NOALLOC_MAX_SUB_BLOCKS and num_chunks * round_up(chunk_size, 8) at most
+ * GC_NOALLOC_MAX_POSITIONS (8 MiB of input in 64 KiB sub-chunks); larger
+ * calls fail with GC_EINVAL.  Calls that overlap on one GPU (other streams or
+ * threads) share that array and race; calls ordered on one stream do not. */
+#define GC_NOALLOC_MAX_SUB_BLOCKS 4096u
+#define GC_NOALLOC_MAX_POSITIONS (8u << 20)
+int gc_zstd_match_find_dev_noalloc(gc_ctx* ctx, const uint8_t* d_input, void* d_sequences, uint32_t seq_format,
+                                   uint8_t* d_literals, uint32_t* d_seq_counts, uint32_t* d_lit_counts,
+                                   uint32_t chunk_size, uint32_t num_chunks, uint64_t total_input_size,
+                                   uint32_t search_depth, uint32_t max_sequences_per_chunk);
+
 /* lzma2_match_find (lzma2_match_find.cu:54-63): writes slot 0 of
  * d_matches[(s*sub + p)*max_matches] where a match exists and every
  * d_match_counts[s*sub + p]; other slots are left untouched (the reference
