@@ -1,8 +1,21 @@
 // gc_batch.cpp -- batched form: many inputs of uneven size, every 64 KiB
 // sub-block of every input in one launch per kernel.
 //
+// Two ways to run a pass:
+//  * static (GC_BATCH_STATIC): grid = items (or the tuned workgroups per
+//    CU), workgroup b takes items b, b + G, ...;
+//  * persistent (GC_BATCH_PERSISTENT): a resident grid of each kernel drains
+//    a device work queue (gc_queue: one counter per kernel, the item counts
+//    in device memory), and the pass -- candidates, LZMA2 lengths, Zstd info,
+//    the parse order and the parse, in that order on one stream -- is
+//    captured once in a hipGraph and replayed.  The queue's counters are set
+//    back to zero by the last workgroup of each kernel, so the graph holds
+//    kernel nodes only (no memset or reset node), and every queue loop is
+//    bounded: an error bit, read by gc_batch_check, replaces any spin.
+//
 // Per input the results are exactly those of the plain per-chunk form:
 // sub-blocks are tiled from each input's own start and are independent.
+#include <stddef.h>
 #include <string.h>
 
 #include <algorithm>
@@ -25,7 +38,11 @@ struct gc_batch {
     uint64_t z_bytes = 0, l_bytes = 0;
     DevBuf items, cand, info, seq, counts, packed;
     DevBuf cost, order;  // Zstd: per-item parse cost (verified positions) and the parse's item order
+    DevBuf queue;        // persistent form: the device work queue (gc_queue)
     gc_tune tune{};
+    uint32_t mode = GC_BATCH_STATIC;
+    hipGraph_t graph = nullptr;  // persistent form: one pass, captured
+    hipGraphExec_t exec = nullptr;
     hipEvent_t ev[6] = {};
 };
 
@@ -37,6 +54,92 @@ uint32_t grid_for(gc_batch* b, uint32_t per_cu, uint32_t items) {
     uint64_t g = (uint64_t)per_cu * (uint64_t)b->ctx->cu_count;
     if (g > items) g = items;
     return (uint32_t)g;
+}
+
+void drop_graph(gc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Persistent form: the same kernels fed from the device work queue, on the
+// main stream.  Grids are resident workgroups (the tuned count per CU); a
+// kernel with no items of its kind is not launched.
+int enqueue_persistent(gc_batch* b, bool timed) {
+    gc_ctx* ctx = b->ctx;
+    hipStream_t st = ctx->stream;
+    gc_queue* q = (gc_queue*)b->queue.p;
+    const gc_item* all = (const gc_item*)b->items.p;
+    const gc_item* zit = all + b->n_l;
+    const uint32_t n_all = b->n_l + b->n_z;
+    const uint16_t* cand = (const uint16_t*)b->cand.p;
+    if (timed) (void)hipEventRecord(b->ev[0], st);
+    int rc = GC_OK;
+    if (n_all) rc = launch_cand_q(ctx, st, b->arena, all, q, grid_for(b, b->tune.cand_wg_per_cu, n_all), (uint16_t*)b->cand.p);
+    if (rc) return rc;
+    if (timed) (void)hipEventRecord(b->ev[1], st);
+    if (b->n_l) {
+        rc = launch_lzma2_len_q(ctx, st, b->arena, all, q, grid_for(b, b->tune.elem_wg_per_cu, b->n_l), cand,
+                                (uint32_t*)b->packed.p);
+        if (rc) return rc;
+    }
+    if (timed) (void)hipEventRecord(b->ev[2], st);
+    if (b->n_z) {
+        rc = launch_zstd_info_q(ctx, st, b->arena, zit, q, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
+                                (uint32_t*)b->info.p, b->tune.info_strip, (uint32_t*)b->cost.p);
+        if (!rc) rc = launch_order(ctx, st, (const uint32_t*)b->cost.p, b->n_z, (uint32_t*)b->order.p, q);
+        if (rc) return rc;
+    }
+    if (timed) (void)hipEventRecord(b->ev[3], st);
+    if (b->n_z) {
+        rc = launch_zstd_parse_q(ctx, st, b->arena, zit, q, grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
+                                 (const uint32_t*)b->info.p, b->seq.p, (uint32_t*)b->counts.p, kMaxSeq,
+                                 (const uint32_t*)b->order.p);
+        if (rc) return rc;
+    }
+    if (timed) (void)hipEventRecord(b->ev[4], st);
+    return GC_OK;
+}
+
+// The persistent pass captured on the main stream (one stream, kernel nodes
+// only), instantiated once and replayed by gc_batch_run.
+int build_graph(gc_batch* b) {
+    hipStream_t st = b->ctx->stream;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    int rc = enqueue_persistent(b, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(b);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return GC_OK;
+}
+
+// The queue's error word after the passes queued so far (synchronises the
+// stream); cleared once read.
+int check_queue(gc_batch* b) {
+    if (!b->queue.p) return GC_OK;
+    hipStream_t st = b->ctx->stream;
+    uint32_t err = 0;
+    hipError_t e = hipMemcpyAsync(&err, (uint8_t*)b->queue.p + offsetof(gc_queue, err), 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "gc_batch_check");
+    if (!err) return GC_OK;
+    const uint32_t zero = 0;
+    e = hipMemcpyAsync((uint8_t*)b->queue.p + offsetof(gc_queue, err), &zero, 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return fail(GC_EHIP, (err & 1u) ? "device work queue: a persistent loop passed its trip bound"
+                                    : "device work queue: a kernel did not take every item");
 }
 
 // Enqueue one full pass on the main stream.  Items of all inputs go to
@@ -165,6 +268,16 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
     if (!rc && b->n_z) rc = b->cost.ensure((size_t)b->n_z * 4);
     if (!rc && b->n_z) rc = b->order.ensure((size_t)b->n_z * 4);
     if (!rc && b->n_l) rc = b->packed.ensure(lpos * 4);
+    if (!rc) rc = b->queue.ensure(sizeof(gc_queue));
+    if (!rc) {
+        // counters at zero; the item count of each kernel
+        gc_queue q0{};
+        q0.n[GC_Q_CAND] = (uint32_t)all.size();
+        q0.n[GC_Q_LZMA2] = b->n_l;
+        q0.n[GC_Q_INFO] = b->n_z;
+        q0.n[GC_Q_PARSE] = b->n_z;
+        rc = upload(ctx, b->queue, &q0, sizeof(q0));
+    }
     for (int k = 0; !rc && k < 6; k++) {
         hipError_t e = hipEventCreate(&b->ev[k]);
         if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate");
@@ -182,6 +295,7 @@ int gc_batch_destroy(gc_batch* b) {
     if (!b) return GC_OK;
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     b->items.release();
@@ -192,6 +306,7 @@ int gc_batch_destroy(gc_batch* b) {
     b->packed.release();
     b->cost.release();
     b->order.release();
+    b->queue.release();
     delete b;
     return GC_OK;
 }
@@ -203,6 +318,7 @@ int gc_batch_set_tune(gc_batch* b, const gc_tune* t) {
         return fail(GC_EINVAL, "tune out of range (1..64 workgroups per CU)");
     if (t->info_strip != 4 && t->info_strip != 8) return fail(GC_EINVAL, "info_strip must be 4 or 8");
     b->tune = *t;
+    drop_graph(b);  // grid sizes are part of the captured pass
     return GC_OK;
 }
 
@@ -216,24 +332,47 @@ int gc_batch_run(gc_batch* b) {
     if (!b) return fail(GC_EINVAL, "null batch");
     std::lock_guard<std::mutex> g(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
-    return enqueue(b, false);
+    if (b->mode != GC_BATCH_PERSISTENT) return enqueue(b, false);
+    if (!b->exec) {
+        int rc = build_graph(b);
+        if (rc) return rc;
+    }
+    hipError_t e = hipGraphLaunch(b->exec, b->ctx->stream);
+    return e == hipSuccess ? GC_OK : hip_fail(e, "hipGraphLaunch");
 }
 
 int gc_batch_set_mode(gc_batch* b, uint32_t mode) {
     if (!b) return fail(GC_EINVAL, "null batch");
-    if (mode == GC_BATCH_STATIC) return GC_OK;
     if (mode == GC_BATCH_QUEUE_GRAPH)
         return fail(GC_EINVAL,
-                    "the device-queue + hipGraph form is switched off: it hung on its first replay on MI355X "
-                    "and the cause has not been found");
-    return fail(GC_EINVAL, "unknown batch mode");
+                    "the first device-queue + hipGraph form (LZMA2 on a forked stream inside the capture, "
+                    "queues reset by a captured kernel) hung on its first replay on MI355X with no cause "
+                    "found and stays switched off; GC_BATCH_PERSISTENT is the persistent form");
+    if (mode != GC_BATCH_STATIC && mode != GC_BATCH_PERSISTENT) return fail(GC_EINVAL, "unknown batch mode");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    if (mode != b->mode) {
+        drop_graph(b);
+        // resident grids for the persistent kernels: 4 candidate workgroups per CU (LDS), 2 of the
+        // 1024-thread per-position kernels, 8 four-wavefront parse workgroups (8 waves per SIMD);
+        // the static form's grid is one workgroup per item
+        b->tune = mode == GC_BATCH_PERSISTENT ? gc_tune{4, 8, 2, 8} : gc_tune{64, 64, 64, 8};
+        b->mode = mode;
+    }
+    return GC_OK;
+}
+
+int gc_batch_check(gc_batch* b) {
+    if (!b) return fail(GC_EINVAL, "null batch");
+    std::lock_guard<std::mutex> g(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    return check_queue(b);
 }
 
 int gc_batch_run_timed(gc_batch* b, float* ms) {
     if (!b || !ms) return fail(GC_EINVAL, "null argument");
     std::lock_guard<std::mutex> g(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
-    int rc = enqueue(b, true);
+    int rc = b->mode == GC_BATCH_PERSISTENT ? enqueue_persistent(b, true) : enqueue(b, true);
     if (rc) return rc;
     hipError_t e = hipEventSynchronize(b->ev[4]);
     if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
@@ -262,6 +401,7 @@ int gc_batch_copy_zstd(gc_batch* b, uint32_t i, gc_zstd_sequence* seqs, uint32_t
     if (!b || i >= b->inputs.size() || !counts) return fail(GC_EINVAL, "bad argument");
     if (b->inputs[i].algo != GC_ALGO_ZSTD) return fail(GC_EINVAL, "input is not a Zstd input");
     std::lock_guard<std::mutex> g(b->ctx->mu);
+    if (int rc = check_queue(b)) return rc;
     hipStream_t s = b->ctx->stream;
     const uint32_t ns = b->nsub[i];
     hipError_t e = hipMemcpyAsync(counts, (uint32_t*)b->counts.p + b->cnt_base[i], ns * 4, hipMemcpyDeviceToHost, s);
@@ -279,6 +419,7 @@ int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed) {
     if (!b || i >= b->inputs.size() || !packed) return fail(GC_EINVAL, "bad argument");
     if (b->inputs[i].algo != GC_ALGO_LZMA2) return fail(GC_EINVAL, "input is not an LZMA2 input");
     std::lock_guard<std::mutex> g(b->ctx->mu);
+    if (int rc = check_queue(b)) return rc;
     hipStream_t s = b->ctx->stream;
     hipError_t e = hipMemcpyAsync(packed, (uint32_t*)b->packed.p + b->out_base[i], b->inputs[i].size * 4,
                                   hipMemcpyDeviceToHost, s);

@@ -30,3 +30,23 @@ typedef struct {
     uint32_t depth;     // Zstd: > 1 enables the lazy step (zstd_compress.cu:278); LZMA2: 0 = no probe
     uint32_t out_span;  // LZMA2 reference layout: positions whose counts are written (the full sub-block)
 } gc_item;
+
+// Device work queue of the persistent kernels: one counter set per kernel
+// (phase).  A workgroup (the parse: a wavefront) takes item next[ph]++ until
+// the value reaches n[ph]; the last one to leave checks that n[ph] items were
+// taken and sets next, taken and done back to 0, so the queue is ready for
+// the next launch without a reset kernel or memset.  err (never cleared by
+// the device): bit 0 a loop passed its trip bound, bit 1 a phase did not take
+// exactly n[ph] items.
+#define GC_Q_CAND 0u
+#define GC_Q_LZMA2 1u
+#define GC_Q_INFO 2u
+#define GC_Q_PARSE 3u
+typedef struct {
+    uint32_t next[4];
+    uint32_t done[4];
+    uint32_t taken[4];
+    uint32_t n[4];
+    uint32_t err;
+    uint32_t pad[15];
+} gc_queue;

@@ -157,9 +157,36 @@ int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint3
     return launch(stream, ctx->k.gather_seqs, nsub < 64 ? nsub : 64, 256, args, "launch gc_gather_seqs");
 }
 
-int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order) {
-    void* args[] = {&cost, &n, &order};
+int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order,
+                 const gc_queue* q) {
+    void* args[] = {&cost, &n, &order, &q};
     return launch(stream, ctx->k.order_items, 1, 1024, args, "launch gc_order_items");
+}
+
+int launch_cand_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                  uint32_t grid, uint16_t* cand16) {
+    void* args[] = {&arena, &items, &q, &cand16};
+    return launch(stream, ctx->k.cand_q, grid, GC_CAND_THREADS, args, "launch gc_cand_q_kernel");
+}
+
+int launch_lzma2_len_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                       uint32_t grid, const uint16_t* cand16, uint32_t* packed) {
+    void* args[] = {&arena, &items, &q, &cand16, &packed};
+    return launch(stream, ctx->k.lzma2_len_q, grid, 1024, args, "launch gc_lzma2_len_q_kernel");
+}
+
+int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost) {
+    void* args[] = {&arena, &items, &q, &cand16, &info, &cost};
+    return launch(stream, strip == 4 ? ctx->k.zstd_info4_q : ctx->k.zstd_info_q, grid, 1024, args,
+                  "launch gc_zstd_info_q_kernel");
+}
+
+int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                        uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
+                        const uint32_t* order) {
+    void* args[] = {&arena, &items, &q, &info, &seq, &seq_counts, &max_seq, &order};
+    return launch(stream, ctx->k.zstd_parse_q, grid, 256, args, "launch gc_zstd_parse_q_kernel");
 }
 
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -233,6 +260,12 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_encode, ctx->k.module, "gc_zstd_encode_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_bytes, ctx->k.module, "gc_gather_bytes");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.make_items, ctx->k.module, "gc_make_items");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand_q, ctx->k.module, "gc_cand_q_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_q, ctx->k.module, "gc_lzma2_len_q_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info_q, ctx->k.module, "gc_zstd_info_q_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info4_q, ctx->k.module, "gc_zstd_info4_q_kernel");
+    if (e == hipSuccess)
+        e = hipModuleGetFunction(&ctx->k.zstd_parse_q, ctx->k.module_parse, "gc_zstd_parse_q_kernel");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);

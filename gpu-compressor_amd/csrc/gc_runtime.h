@@ -46,6 +46,12 @@ struct Kernels {
     hipFunction_t zstd_encode = nullptr;
     hipFunction_t gather_bytes = nullptr;
     hipFunction_t make_items = nullptr;
+    // persistent forms: items from a device work queue (gc_queue)
+    hipFunction_t cand_q = nullptr;
+    hipFunction_t lzma2_len_q = nullptr;
+    hipFunction_t zstd_info_q = nullptr;
+    hipFunction_t zstd_info4_q = nullptr;
+    hipFunction_t zstd_parse_q = nullptr;
 };
 
 }  // namespace gc
@@ -84,7 +90,19 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);
-int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order);
+int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order,
+                 const gc_queue* q = nullptr);
+// Persistent kernels: grid workgroups take items from the queue q (item
+// counts in q->n); nothing else differs from the static launches.
+int launch_cand_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                  uint32_t grid, uint16_t* cand16);
+int launch_lzma2_len_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                       uint32_t grid, const uint16_t* cand16, uint32_t* packed);
+int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost);
+int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
+                        uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
+                        const uint32_t* order);
 // Zstd frames from sequences and literals (one wavefront per item), and the
 // packing of frames written in fixed slots.
 int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,

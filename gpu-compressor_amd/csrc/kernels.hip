@@ -245,6 +245,54 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// ---------------------------------------------------------------------------
+// Item sources.  Static: workgroup b takes items b, b + G, b + 2G, ...
+// Queue (persistent kernels): each workgroup (or, in the parse, each
+// wavefront) takes the next item from the device work queue until it runs
+// dry.  A taker can take at most n items, so its loop is bounded by n + 1
+// fetches; passing that bound sets error bit 0 and ends the loop.  No taker
+// ever waits for another: every loop ends whatever the other workgroups do.
+// ---------------------------------------------------------------------------
+// The taker that leaves last (of `takers`) finds every other one gone: it
+// checks the count and sets the counters back to 0 for the next launch.
+__device__ __forceinline__ void queue_leave(gc_queue* q, uint32_t ph, uint32_t n, uint32_t mine, uint32_t takers) {
+    atomicAdd(&q->taken[ph], mine);
+    __threadfence();
+    if (atomicAdd(&q->done[ph], 1u) == takers - 1) {
+        __threadfence();
+        if (atomicAdd(&q->taken[ph], 0u) != n) atomicOr(&q->err, 2u);
+        atomicExch(&q->next[ph], 0u);
+        atomicExch(&q->taken[ph], 0u);
+        atomicExch(&q->done[ph], 0u);
+    }
+}
+
+// Workgroup-level: body(i) for every item i the workgroup takes; all threads
+// call this, body may use barriers.
+template <class F>
+__device__ __forceinline__ void wg_queue_items(gc_queue* q, uint32_t ph, F&& body) {
+    __shared__ uint32_t s_take;
+    const uint32_t n = uniform(__atomic_load_n(&q->n[ph], __ATOMIC_RELAXED));
+    uint32_t mine = 0;
+    for (uint32_t iter = 0;; iter++) {
+        if (threadIdx.x == 0) {
+            uint32_t i = atomicAdd(&q->next[ph], 1u);
+            if (iter > n) {
+                atomicOr(&q->err, 1u);
+                i = n;
+            }
+            s_take = i;
+        }
+        __syncthreads();
+        const uint32_t i = uniform(s_take);
+        __syncthreads();
+        if (i >= n) break;
+        body(i);
+        mine++;
+    }
+    if (threadIdx.x == 0) queue_leave(q, ph, n, mine, gridDim.x);
+}
+
 // A sub-block staged in LDS (byte 0 = sub-block start, 4-byte aligned, at
 // least 32 readable bytes past the sub-block).
 struct LdsBytes {
@@ -544,6 +592,20 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
     }
 }
 
+// Persistent form: items from the device work queue (phase GC_Q_CAND).
+extern "C" __global__ void __launch_bounds__(CAND_THREADS)
+gc_cand_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, uint16_t* cand16) {
+    __shared__ CandLds s;
+    wg_queue_items(q, GC_Q_CAND, [&](uint32_t i) {
+        const gc_item it = load_item(items, i);
+        if (item_safe(it.avail, it.n))
+            cand_item<true>(s, it, arena, cand16);
+        else
+            cand_item<false>(s, it, arena, cand16);
+        lds_barrier();
+    });
+}
+
 // ---------------------------------------------------------------------------
 // Per-position passes.  Each thread takes a strip of STRIP consecutive
 // positions and walks it backwards: where position p+1 matches along the same
@@ -691,9 +753,9 @@ template <bool LENGTHS, bool REF>
 __device__ __forceinline__ void lzma2_len_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                 uint32_t n_items, const uint16_t* cand16, uint32_t* packed,
                                                 gc_lzma_match* ref_m, uint32_t* ref_c, uint32_t max_matches,
-                                                uint8_t* triage) {
+                                                uint8_t* triage, gc_queue* q = nullptr) {
     __shared__ uint32_t sb[STAGE_WORDS];
-    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+    auto body = [&](uint32_t i) {
         const gc_item it = load_item(items, i);
         __syncthreads();  // the last item's reads of sb are done
         stage_sub_block<INFO_THREADS>(sb, src_view(arena, it.src_off, it.avail), it.n);
@@ -703,7 +765,11 @@ __device__ __forceinline__ void lzma2_len_items(const uint8_t* arena, const gc_i
             const int wg_any = __syncthreads_or(any ? 1 : 0);
             if (threadIdx.x == 0) triage[it.count_pos] = wg_any ? 1 : 0;
         }
-    }
+    };
+    if (q)
+        wg_queue_items(q, GC_Q_LZMA2, body);
+    else
+        for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) body(i);
 }
 
 #define GC_LZMA2_KERNEL(NAME, LENGTHS, REF)                                                                        \
@@ -715,6 +781,14 @@ __device__ __forceinline__ void lzma2_len_items(const uint8_t* arena, const gc_i
 GC_LZMA2_KERNEL(gc_lzma2_len_kernel, true, false)
 GC_LZMA2_KERNEL(gc_lzma2_len_ref_kernel, true, true)
 GC_LZMA2_KERNEL(gc_lzma2_triage_kernel, false, false)
+
+// Persistent form of the packed-word kernel: items from the device work queue
+// (phase GC_Q_LZMA2).
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
+gc_lzma2_len_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
+                      uint32_t* packed) {
+    lzma2_len_items<true, false>(arena, items, 0, cand16, packed, nullptr, nullptr, 1, nullptr, q);
+}
 
 // ---------------------------------------------------------------------------
 // Zstd per position, given cand(p): the candidate as the reference's table
@@ -807,13 +881,15 @@ __device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n,
 
 // cost (optional): per item (at it.count_pos) the number of positions whose
 // candidate verified, which is what the parse's work grows with.
+// (cost is indexed by the item's index in `items`, the index the parse's
+// order refers to)
 template <uint32_t W>
 __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                 uint32_t n_items, const uint16_t* cand16, uint32_t* info,
-                                                uint32_t* cost) {
+                                                uint32_t* cost, gc_queue* q = nullptr) {
     __shared__ uint32_t sb[STAGE_WORDS];
     __shared__ uint32_t verified;
-    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+    auto body = [&](uint32_t i) {
         const gc_item it = load_item(items, i);
         __syncthreads();  // the last item's reads of sb and `verified` are done
         if (threadIdx.x == 0) verified = 0;
@@ -823,9 +899,13 @@ __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_i
         if (cost) {
             if (v) atomicAdd(&verified, v);
             __syncthreads();
-            if (threadIdx.x == 0) cost[it.count_pos] = verified;
+            if (threadIdx.x == 0) cost[i] = verified;
         }
-    }
+    };
+    if (q)
+        wg_queue_items(q, GC_Q_INFO, body);
+    else
+        for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) body(i);
 }
 
 // Strips of 8 or 4 positions: the same words, different register use.
@@ -841,12 +921,28 @@ gc_zstd_info4_kernel(const uint8_t* arena, const gc_item* __restrict__ items, ui
     zstd_info_items<4>(arena, items, n_items, cand16, info, cost);
 }
 
+// Persistent forms: items from the device work queue (phase GC_Q_INFO).
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
+gc_zstd_info_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
+                      uint32_t* info, uint32_t* cost) {
+    zstd_info_items<8>(arena, items, 0, cand16, info, cost, q);
+}
+
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
+gc_zstd_info4_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
+                       uint32_t* info, uint32_t* cost) {
+    zstd_info_items<4>(arena, items, 0, cand16, info, cost, q);
+}
+
 // The parse's item order, costliest first (counting sort of cost >> 9 into
 // 129 buckets, descending): the sub-blocks that take longest start first and
 // the launch does not end on a few long ones.  One workgroup.
 #define ORDER_BUCKETS 129
-extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t* cost, uint32_t n, uint32_t* order) {
+// (q: the item count is the queue's parse count)
+extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t* cost, uint32_t n, uint32_t* order,
+                                                                  const gc_queue* q) {
     __shared__ uint32_t start[ORDER_BUCKETS];
+    if (q) n = uniform(__atomic_load_n(&q->n[GC_Q_PARSE], __ATOMIC_RELAXED));
     for (uint32_t b = threadIdx.x; b < ORDER_BUCKETS; b += blockDim.x) start[b] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&start[min(cost[i] >> 9, ORDER_BUCKETS - 1u)], 1u);
@@ -1299,6 +1395,12 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
 }
 
 template <uint32_t FMT, bool LITS>
+__device__ __forceinline__ void zstd_parse_one(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t k,
+                                               const uint32_t* info, void* seq_out, uint8_t* literals,
+                                               uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq,
+                                               uint8_t* ring);
+
+template <uint32_t FMT, bool LITS>
 __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_item* __restrict__ items,
                                                  uint32_t n_items, const uint32_t* info, void* seq_out,
                                                  uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
@@ -1307,8 +1409,44 @@ __device__ __forceinline__ void zstd_parse_items(const uint8_t* arena, const gc_
     // the wavefront index is uniform: items are read into scalar registers
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     uint8_t* ring = (uint8_t*)ring_mem[wave];
-    for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES) {
-        const uint32_t k = order ? order[i] : i;
+    for (uint32_t i = blockIdx.x * PARSE_WAVES + wave; i < n_items; i += gridDim.x * PARSE_WAVES)
+        zstd_parse_one<FMT, LITS>(arena, items, order ? order[i] : i, info, seq_out, literals, seq_counts, lit_counts,
+                                  max_seq, ring);
+}
+
+// Persistent form: each wavefront takes items from the device work queue
+// (phase GC_Q_PARSE) in `order`, at most n + 1 fetches.
+template <uint32_t FMT, bool LITS>
+__device__ __forceinline__ void zstd_parse_queue(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q,
+                                                 const uint32_t* info, void* seq_out, uint32_t* seq_counts,
+                                                 uint32_t max_seq, const uint32_t* __restrict__ order,
+                                                 uint32_t (*ring_mem)[RING_BYTES / 4]) {
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    uint8_t* ring = (uint8_t*)ring_mem[wave];
+    const uint32_t n = uniform(__atomic_load_n(&q->n[GC_Q_PARSE], __ATOMIC_RELAXED));
+    uint32_t mine = 0;
+    for (uint32_t iter = 0;; iter++) {
+        uint32_t i = 0;
+        if (lane_id() == 0) i = atomicAdd(&q->next[GC_Q_PARSE], 1u);
+        i = uniform(i);
+        if (iter > n) {
+            if (lane_id() == 0) atomicOr(&q->err, 1u);
+            break;
+        }
+        if (i >= n) break;
+        zstd_parse_one<FMT, LITS>(arena, items, order ? order[i] : i, info, seq_out, nullptr, seq_counts, nullptr,
+                                  max_seq, ring);
+        mine++;
+    }
+    if (lane_id() == 0) queue_leave(q, GC_Q_PARSE, n, mine, gridDim.x * PARSE_WAVES);
+}
+
+template <uint32_t FMT, bool LITS>
+__device__ __forceinline__ void zstd_parse_one(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t k,
+                                               const uint32_t* info, void* seq_out, uint8_t* literals,
+                                               uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq,
+                                               uint8_t* ring) {
+    {
 #if PARSE_UNIFORM_ITEM
         const gc_item it = load_item(items, k);
 #else
@@ -1356,6 +1494,14 @@ GC_PARSE_KERNEL(gc_zstd_parse_kernel, GC_SEQ_ZSTD, false)
 GC_PARSE_KERNEL(gc_zstd_parse_lits_kernel, GC_SEQ_ZSTD, true)
 GC_PARSE_KERNEL(gc_zstd_parse_ref_kernel, GC_SEQ_REF, false)
 GC_PARSE_KERNEL(gc_zstd_parse_ref_lits_kernel, GC_SEQ_REF, true)
+
+// Persistent form (the batched and served forms' record layout, no literals).
+extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE, 8)
+gc_zstd_parse_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint32_t* info,
+                       void* seq_out, uint32_t* seq_counts, uint32_t max_seq, const uint32_t* __restrict__ order) {
+    __shared__ uint32_t ring_mem[PARSE_WAVES][RING_BYTES / 4];
+    zstd_parse_queue<GC_SEQ_ZSTD, false>(arena, items, q, info, seq_out, seq_counts, max_seq, order, ring_mem);
+}
 
 #endif  // GC_BUILD_PARSE
 

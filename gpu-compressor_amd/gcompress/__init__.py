@@ -89,6 +89,7 @@ SIGNATURES = [
     ("gc_batch_destroy", C.c_int, [_vp]),
     ("gc_batch_run", C.c_int, [_vp]),
     ("gc_batch_set_mode", C.c_int, [_vp, C.c_uint32]),
+    ("gc_batch_check", C.c_int, [_vp]),
     ("gc_batch_run_timed", C.c_int, [_vp, _vp]),
     ("gc_batch_set_tune", C.c_int, [_vp, _vp]),
     ("gc_batch_get_tune", C.c_int, [_vp, _vp]),
@@ -116,6 +117,7 @@ ALGO_ZSTD = 0
 ALGO_LZMA2 = 1
 BATCH_STATIC = 0
 BATCH_QUEUE_GRAPH = 1
+BATCH_PERSISTENT = 2
 
 
 class BatchInput(C.Structure):
@@ -316,6 +318,10 @@ class Batch:
 
     def set_mode(self, mode):
         _check(lib().gc_batch_set_mode(self.h, mode), "gc_batch_set_mode")
+
+    def check(self):
+        """Waits for the queued passes; raises if a persistent pass's work queue reported an error."""
+        _check(lib().gc_batch_check(self.h), "gc_batch_check")
 
     def run_timed(self):
         """Per-kernel ms: [candidates, LZMA2 lengths, Zstd verify, Zstd parse]."""

@@ -316,12 +316,21 @@ typedef struct {
     uint32_t info_strip;      /* positions per thread in the Zstd verify pass: 4 or 8 */
 } gc_tune;
 
-/* Batch execution modes.  GC_BATCH_QUEUE_GRAPH (a device work queue drained
- * by persistent workgroups, replayed from a captured hipGraph) is switched
- * off: it hung on its first replay on MI355X and the cause is not found;
- * gc_batch_set_mode refuses it with GC_EINVAL. */
+/* Batch execution modes.
+ *   GC_BATCH_STATIC: one launch per kernel, grid = items (or the tuned
+ *     workgroups per CU), items assigned grid-stride.
+ *   GC_BATCH_PERSISTENT: resident workgroups of each kernel drain a device
+ *     work queue; the whole pass (one stream, kernel nodes only) is captured
+ *     in a hipGraph once and replayed by gc_batch_run.  Queue loops are
+ *     bounded; gc_batch_check reports a loop that passed its bound or a
+ *     kernel that did not take every item (and the copy functions check it).
+ *   GC_BATCH_QUEUE_GRAPH: the first queue + hipGraph design (LZMA2 on a
+ *     forked stream inside the capture, queues reset by a captured kernel);
+ *     it hung on its first replay on MI355X with no cause found and is
+ *     refused with GC_EINVAL. */
 #define GC_BATCH_STATIC 0u
 #define GC_BATCH_QUEUE_GRAPH 1u
+#define GC_BATCH_PERSISTENT 2u
 
 typedef struct gc_batch gc_batch;
 
@@ -335,6 +344,9 @@ int gc_batch_destroy(gc_batch* b);
 /* Enqueue one pass on the context stream (asynchronous). */
 int gc_batch_run(gc_batch* b);
 int gc_batch_set_mode(gc_batch* b, uint32_t mode);
+/* Waits for the passes queued so far; GC_EHIP if a persistent pass's work
+ * queue reported an error since the last check (then cleared), else 0. */
+int gc_batch_check(gc_batch* b);
 /* One pass with events between the kernels, synchronous: ms[0] candidate
  * kernel, ms[1] LZMA2 length kernel, ms[2] Zstd verify kernel, ms[3] Zstd
  * parse kernel. */

@@ -43,6 +43,46 @@ def test_batch_mixed_equals_plain(gctx):
     _check_batch(gctx, b, datas, inputs, p)
 
 
+def _check_all(b, datas, inputs):
+    b.check()
+    for i, (o, s, algo, lv) in enumerate(inputs):
+        if algo == gcompress.ALGO_LZMA2:
+            np.testing.assert_array_equal(b.lzma2(i), oracle.lzma2_packed(datas[i]))
+        else:
+            seqs, counts = b.zstd(i)
+            oseq, osc, _, _ = oracle.zstd_match_find(datas[i], depth={1: 16, 2: 64}[lv])
+            np.testing.assert_array_equal(counts, osc)
+            for k, z in enumerate(oracle.zstd_as_zstd_sequences(oseq, osc)):
+                np.testing.assert_array_equal(seqs[k, : counts[k]], z)
+
+
+def test_batch_persistent_eager_equals_oracle(gctx):
+    # the persistent kernels launched one by one (no graph), twice: the second launch finds the queue
+    # counters the first one's last workgroups set back to zero
+    datas, inputs, p, size = _mixed_batch(gctx)
+    b = gcompress.Batch(gctx, p, size, inputs)
+    try:
+        b.set_mode(gcompress.BATCH_PERSISTENT)
+        for rep in range(2):
+            print(f"[batch persistent eager] pass {rep}", flush=True)
+            ms = b.run_timed()
+            assert len(ms) == 4
+            _check_all(b, datas, inputs)
+    finally:
+        b.close()
+        gctx.free(p)
+
+
+def test_batch_persistent_equals_oracle(gctx):
+    # the persistent form: resident workgroups drain the device work queue, the pass replayed from one
+    # captured hipGraph; every replay (with other work and synchronisations of the program between
+    # them) is checked against the oracle
+    datas, inputs, p, size = _mixed_batch(gctx)
+    b = gcompress.Batch(gctx, p, size, inputs)
+    b.set_mode(gcompress.BATCH_PERSISTENT)
+    _check_batch(gctx, b, datas, inputs, p, label="persistent", between=lambda: gctx.lzma2_packed(datas[8]))
+
+
 def test_batch_queue_graph_mode_is_refused(gctx):
     p = gctx.alloc(4096)
     b = gcompress.Batch(gctx, p, 4096, [(0, 100, gcompress.ALGO_ZSTD, 1)])
@@ -60,12 +100,15 @@ def test_batch_static_equals_plain(gctx):
     _check_batch(gctx, b, datas, inputs, p)
 
 
-def _check_batch(gctx, b, datas, inputs, p):
+def _check_batch(gctx, b, datas, inputs, p, label="static", between=None):
     try:
-        for rep in range(3):  # replays of one graph give the same results
-            print(f"[batch static] pass {rep} tune={b.tune()}", flush=True)
+        for rep in range(3):  # passes (in the persistent form, replays of one graph) give the same results
+            print(f"[batch {label}] pass {rep} tune={b.tune()}", flush=True)
             b.run()
             gctx.sync()
+            b.check()
+            if between:
+                between()
             for i, (o, s, algo, lv) in enumerate(inputs):
                 if algo == gcompress.ALGO_LZMA2:
                     got = b.lzma2(i)
@@ -78,9 +121,16 @@ def _check_batch(gctx, b, datas, inputs, p):
                         np.testing.assert_array_equal(seqs[k, : counts[k]], z)
         ms = b.run_timed()
         assert len(ms) == 4 and all(m >= 0 for m in ms)
-        b.set_tune(1, 2, 1, 4)  # fewer workgroups than items: grid-stride assignment; narrow strips
+        b.set_tune(1, 2, 1, 4)  # fewer workgroups than items: grid-stride or queue assignment; narrow strips
         b.run()
         gctx.sync()
+        b.check()
+        for i, (o, s, algo, lv) in enumerate(inputs):
+            if algo == gcompress.ALGO_LZMA2:
+                np.testing.assert_array_equal(b.lzma2(i), oracle.lzma2_packed(datas[i]))
+            else:
+                np.testing.assert_array_equal(b.zstd(i, with_seqs=False)[1],
+                                              oracle.zstd_match_find(datas[i], depth={1: 16, 2: 64}[lv])[1])
         np.testing.assert_array_equal(b.lzma2(2), oracle.lzma2_packed(datas[2]))
         seqs, counts = b.zstd(0)
         np.testing.assert_array_equal(counts, oracle.zstd_match_find(datas[0], depth=16)[1])
