@@ -70,8 +70,8 @@ int enqueue_persistent(gc_batch* b, bool timed) {
     gc_ctx* ctx = b->ctx;
     hipStream_t st = ctx->stream;
     gc_queue* q = (gc_queue*)b->queue.p;
+    // every kernel gets the whole item list: the queue holds each one's first item and count
     const gc_item* all = (const gc_item*)b->items.p;
-    const gc_item* zit = all + b->n_l;
     const uint32_t n_all = b->n_l + b->n_z;
     const uint16_t* cand = (const uint16_t*)b->cand.p;
     if (timed) (void)hipEventRecord(b->ev[0], st);
@@ -86,14 +86,14 @@ int enqueue_persistent(gc_batch* b, bool timed) {
     }
     if (timed) (void)hipEventRecord(b->ev[2], st);
     if (b->n_z) {
-        rc = launch_zstd_info_q(ctx, st, b->arena, zit, q, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
+        rc = launch_zstd_info_q(ctx, st, b->arena, all, q, grid_for(b, b->tune.elem_wg_per_cu, b->n_z), cand,
                                 (uint32_t*)b->info.p, b->tune.info_strip, (uint32_t*)b->cost.p);
         if (!rc) rc = launch_order(ctx, st, (const uint32_t*)b->cost.p, b->n_z, (uint32_t*)b->order.p, q);
         if (rc) return rc;
     }
     if (timed) (void)hipEventRecord(b->ev[3], st);
     if (b->n_z) {
-        rc = launch_zstd_parse_q(ctx, st, b->arena, zit, q, grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
+        rc = launch_zstd_parse_q(ctx, st, b->arena, all, q, grid_for(b, b->tune.parse_wg_per_cu, (b->n_z + 3) / 4),
                                  (const uint32_t*)b->info.p, b->seq.p, (uint32_t*)b->counts.p, kMaxSeq,
                                  (const uint32_t*)b->order.p);
         if (rc) return rc;
@@ -276,6 +276,7 @@ gc_batch* gc_batch_create(gc_ctx* ctx, const uint8_t* d_arena, uint64_t arena_si
         q0.n[GC_Q_LZMA2] = b->n_l;
         q0.n[GC_Q_INFO] = b->n_z;
         q0.n[GC_Q_PARSE] = b->n_z;
+        q0.base[GC_Q_INFO] = q0.base[GC_Q_PARSE] = b->n_l;  // Zstd items follow the LZMA2 ones
         rc = upload(ctx, b->queue, &q0, sizeof(q0));
     }
     for (int k = 0; !rc && k < 6; k++) {

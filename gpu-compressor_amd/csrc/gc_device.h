@@ -35,7 +35,9 @@ typedef struct {
 // (phase).  A workgroup (the parse: a wavefront) takes item next[ph]++ until
 // the value reaches n[ph]; the last one to leave checks that n[ph] items were
 // taken and sets next, taken and done back to 0, so the queue is ready for
-// the next launch without a reset kernel or memset.  err (never cleared by
+// the next launch without a reset kernel or memset.  The phase's items are
+// items[base[ph] .. base[ph] + n[ph]) of the launch's item array, so a
+// captured launch serves any split of a new item list.  err (never cleared by
 // the device): bit 0 a loop passed its trip bound, bit 1 a phase did not take
 // exactly n[ph] items.
 #define GC_Q_CAND 0u
@@ -47,6 +49,7 @@ typedef struct {
     uint32_t done[4];
     uint32_t taken[4];
     uint32_t n[4];
+    uint32_t base[4];
     uint32_t err;
-    uint32_t pad[15];
+    uint32_t pad[11];
 } gc_queue;

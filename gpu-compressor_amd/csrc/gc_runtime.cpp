@@ -182,6 +182,12 @@ int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
                   "launch gc_zstd_info_q_kernel");
 }
 
+int launch_tick_setup(gc_ctx* ctx, hipStream_t stream, const gc_queue* hdr, const gc_item* src, gc_queue* q,
+                      gc_item* items, uint32_t cap) {
+    void* args[] = {&hdr, &src, &q, &items, &cap};
+    return launch(stream, ctx->k.tick_setup, 64, 256, args, "launch gc_tick_setup");
+}
+
 int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
                         uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
                         const uint32_t* order) {
@@ -266,6 +272,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info4_q, ctx->k.module, "gc_zstd_info4_q_kernel");
     if (e == hipSuccess)
         e = hipModuleGetFunction(&ctx->k.zstd_parse_q, ctx->k.module_parse, "gc_zstd_parse_q_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.tick_setup, ctx->k.module, "gc_tick_setup");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);

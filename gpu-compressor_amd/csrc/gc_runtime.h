@@ -52,6 +52,7 @@ struct Kernels {
     hipFunction_t zstd_info_q = nullptr;
     hipFunction_t zstd_info4_q = nullptr;
     hipFunction_t zstd_parse_q = nullptr;
+    hipFunction_t tick_setup = nullptr;
 };
 
 }  // namespace gc
@@ -100,6 +101,10 @@ int launch_lzma2_len_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
                        uint32_t grid, const uint16_t* cand16, uint32_t* packed);
 int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
                        uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost);
+// Served form: copy a tick's counts and items from pinned memory into the
+// slot's queue and item array.
+int launch_tick_setup(gc_ctx* ctx, hipStream_t stream, const gc_queue* hdr, const gc_item* src, gc_queue* q,
+                      gc_item* items, uint32_t cap);
 int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
                         uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
                         const uint32_t* order);

@@ -1,27 +1,33 @@
-// gc_serve.cpp -- served form: a request loop with continuous batching.
+// gc_serve.cpp -- served form: match-finding requests admitted under an HBM
+// budget and batched continuously into the persistent kernels.
 //
-// Clients submit match-finding requests (Zstd L1/L2 or LZMA2, any size) from
-// any thread.  One server thread:
+// Clients submit requests (Zstd L1/L2 or LZMA2, any size) from any thread:
+// from host memory (the input is copied in, the results go back to host
+// memory) or already in device memory (gc_server_submit_dev: the input is
+// read where it lies and the results stay on the device).  One server
+// thread:
 //   1. admits pending requests in arrival order while their device memory
-//      (input, candidates, Zstd info, outputs) fits the HBM budget,
-//   2. copies each admitted input to the device on the copy stream,
-//   3. whenever no batch is running, launches every request whose input has
-//      arrived as one batch (one launch per kernel over all their sub-blocks),
-//   4. when a batch ends, starts the next one at once and meanwhile brings the
-//      finished requests' results back on a second copy stream (sequences
-//      packed by gc_gather_seqs first), straight into the caller's buffers
-//      when it gave some (gc_server_submit_into), frees their memory and marks
-//      them done.
-// Nothing on the compute stream uses a DMA engine (those carry the inputs in):
-// the batch's work items are read from pinned memory by a copy kernel, and
-// the parse writes its sequence counts straight into pinned memory, so the
-// host reads them when the batch's event completes.  The server thread only polls events: a batch that ends
-// while results of an earlier one are still being copied waits in `finished`
-// and the device goes on with the next batch.
-// So requests that arrive while others run are admitted as memory frees up
-// and join the next batch; a result does not depend on what it was batched
-// with (sub-blocks are independent).  No reference counterpart: the
-// reference processes one chunk at a time (pipeline.rs:138-175).
+//      (candidates, Zstd info words and sequence slots or LZMA2 matches, and
+//      a copy of a host input) fits the budget; a host input is copied in on
+//      the context's second stream;
+//   2. whenever one of kSlots tick slots is free and requests are ready,
+//      starts a tick with every ready request (up to max_batch_items
+//      sub-blocks): it writes the tick's work list into the slot's pinned
+//      staging memory and replays the slot's captured pass -- a setup kernel
+//      that loads the list into the slot's device work queue, then the
+//      persistent kernels (candidates, LZMA2 lengths, Zstd info, parse
+//      order, parse), which take the tick's sub-blocks from the queue.  Each
+//      slot has its own stream, so consecutive ticks overlap on the device:
+//      a request that arrives while others run starts at the next free slot,
+//      beside them, instead of waiting for them to end;
+//   3. when a tick ends: device-result requests are done at once (their
+//      sequence counts were written by the parse into pinned memory); the
+//      others have their results brought back on a fetch stream, straight
+//      into the caller's buffers when it gave pinned ones, and are done when
+//      that copy ends.
+// A request's results do not depend on what it was batched with (sub-blocks
+// are independent).  No reference counterpart: the reference processes one
+// chunk at a time (pipeline.rs:138-175).
 #include <string.h>
 
 #include <algorithm>
@@ -43,6 +49,7 @@ namespace {
 
 const uint32_t kMaxSeq = GC_SUB_BLOCK / 3;
 const uint64_t kAlign = 256;
+const int kSlots = 3;
 
 uint64_t up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
@@ -59,7 +66,7 @@ double now_s() {
     return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// First-fit allocator over [0, size) of one device pool.
+// First-fit allocator over [0, size).
 struct PoolAlloc {
     uint64_t size = 0;
     std::map<uint64_t, uint64_t> free_;  // offset -> length
@@ -67,7 +74,7 @@ struct PoolAlloc {
     void reset(uint64_t s) {
         size = s;
         free_.clear();
-        free_[0] = s;
+        if (s) free_[0] = s;
         used = 0;
     }
     bool alloc(uint64_t len, uint64_t* off) {
@@ -84,6 +91,7 @@ struct PoolAlloc {
         return false;
     }
     void release(uint64_t off, uint64_t len) {
+        if (!len) return;
         used -= len;
         auto it = free_.emplace(off, len).first;
         auto nx = std::next(it);
@@ -104,16 +112,19 @@ struct PoolAlloc {
 enum State { PENDING, COPYING, READY, RUNNING, FETCHING, DONE, FAILED };
 
 struct Req {
-    uint64_t id;
-    const uint8_t* data;
-    uint64_t n;
-    uint32_t algo, level, nsub;
-    uint64_t dev_off = 0, dev_len = 0;            // the request's region of the pool
+    uint64_t id = 0;
+    const uint8_t* data = nullptr;  // host input
+    const uint8_t* d_in = nullptr;  // device input (the caller's, or the copy of a host input)
+    uint64_t n = 0;
+    uint32_t algo = 0, level = 0, nsub = 0;
+    bool dev_result = false;  // results stay on the device (gc_server_submit_dev)
+    uint64_t dev_off = 0, dev_len = 0;  // the request's region of the pool
     uint64_t in_off = 0, cand_off = 0, info_off = 0, out_off = 0;  // byte offsets in the region
-    uint32_t item0 = 0;        // first Zstd count index of the request in its batch
-    void* dst = nullptr;       // caller's result buffer (gc_server_submit_into), else results kept here
-    void* dst_dev = nullptr;   // the same buffer as the device sees it, when it is pinned host memory
-    uint64_t dst_cap = 0;      // its size in bytes
+    uint64_t cnt_off = 0;  // first entry of the request's sequence counts in the counts zone
+    uint32_t cnt_len = 0;
+    void* dst = nullptr;      // caller's result buffer (gc_server_submit_into), else results kept here
+    void* dst_dev = nullptr;  // the same buffer as the device sees it, when it is pinned host memory
+    uint64_t dst_cap = 0;
     uint32_t* dst_counts = nullptr;
     hipEvent_t copied = nullptr;
     State state = PENDING;
@@ -122,30 +133,47 @@ struct Req {
     std::vector<uint32_t> counts;
     std::vector<gc_zstd_sequence> seqs;
     std::vector<uint32_t> packed;
-    uint64_t fetch_total = 0;  // sequences packed for the copy back
+    uint64_t fetch_total = 0;  // Zstd: sequences found
 };
 
 // Device bytes a request needs (every part 256-byte aligned).  For Zstd the
 // candidate and info regions are contiguous and together hold at least
-// 21845 x 16 bytes per sub-block: the packed sequences go there at the end.
+// 21845 x 16 bytes per sub-block: packed sequences for a pageable
+// destination are gathered there at the end.
 void layout(Req& r) {
     const uint64_t ns = r.nsub, slots = ns * GC_SUB_BLOCK;
     uint64_t o = 0;
-    r.in_off = o;
-    o += up(r.n + 16, kAlign);
+    if (!r.d_in) {  // a host input is copied here (with room past its end for the kernels' reads)
+        r.in_off = o;
+        o += up(r.n + 64, kAlign);
+    }
     r.cand_off = o;
     o += up(slots * 2, kAlign);
     if (r.algo == GC_ALGO_ZSTD) {
         r.info_off = o;
         o += up(slots * 4, kAlign);
-        r.out_off = o;  // sequence slots, later packed in place
+        r.out_off = o;
         o += up(ns * kMaxSeq * sizeof(gc_zstd_sequence), kAlign);
     } else {
-        r.out_off = o;  // one packed match per position
+        r.out_off = o;
         o += up(slots * 4, kAlign);
     }
     r.dev_len = o;
 }
+
+// One tick slot: its stream, pinned work list, device queue and captured pass.
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev_start = nullptr, ev_done = nullptr;
+    gc::DevBuf items, cost, order, queue;
+    gc_queue* hdr = nullptr;    // pinned: the tick's counts and first items
+    gc_item* stage = nullptr;   // pinned: the tick's items
+    uint32_t* err = nullptr;    // pinned: the queue's error word, copied after each tick
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<Req*> reqs;
+    bool busy = false;
+};
 
 }  // namespace
 
@@ -154,224 +182,277 @@ struct gc_server {
     uint8_t* pool = nullptr;
     uint64_t budget = 0;
     PoolAlloc alloc;
+    uint32_t* counts_zone = nullptr;  // pinned: Zstd sequence counts, written by the parse
+    uint64_t zone_len = 0;
+    PoolAlloc zone;
     std::thread th;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
     uint64_t next_id = 1;
-    std::deque<uint64_t> pending;                        // arrival order
+    std::deque<uint64_t> pending;  // arrival order
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
-    // batch state (server thread only)
-    // Requests of the running, finished and fetching batches.  Held by address: `reqs` is
-    // changed by client threads (under `mu`), and a request in a batch cannot be released
-    // until it is done.
-    std::vector<Req*> running, fetching;
-    std::deque<std::vector<Req*>> finished;  // batches done on the device, results not yet copied
-    hipStream_t fetch_stream = nullptr;  // results back to the host (the context's stream2 brings inputs in)
-    hipEvent_t batch_start = nullptr, batch_done = nullptr, fetch_start = nullptr, fetch_done = nullptr;
-    DevBuf items;                     // the running batch's work items
-    DevBuf cost, order;               // Zstd items: parse cost and the parse's order (costliest first)
-    uint32_t* counts_host = nullptr;  // pinned: per-batch sequence counts, written by the parse
-    size_t counts_host_cap = 0;       // in u32
-    gc_item* items_host = nullptr;    // pinned staging of `items`
-    size_t items_host_cap = 0;        // in items
-    uint8_t* meta_host = nullptr; // pinned: counts and offsets for the gathers
+    // server thread only (requests held by address: a request in flight is
+    // not released)
+    std::vector<Req*> copying;    // host inputs on their way in
+    std::deque<Req*> ready;       // admitted, input on the device, in arrival order
+    Slot slot[kSlots];
+    std::vector<Req*> fetching;
+    std::deque<std::vector<Req*>> finished;  // ticks whose results still go back to the host
+    hipStream_t fetch_stream = nullptr;
+    hipEvent_t fetch_start = nullptr, fetch_done = nullptr;
+    uint8_t* meta_host = nullptr;  // pinned: counts and offsets for the gathers
     size_t meta_cap = 0;
-    std::vector<gc_item> host_items;
+    uint32_t* zero_host = nullptr;  // pinned 0, to clear a queue's error word
     gc_server_stats stats{};
     uint32_t max_batch_items = 0;
 
+    int setup_slot(Slot& sl);
     void loop();
     void admit_locked();
-    bool launch_batch();
-    void batch_ended();
-    void finish_batch(std::vector<Req*>& done);
+    bool launch_tick(Slot& sl);
+    void tick_ended(Slot& sl);
+    void finish_fetch(std::vector<Req*>& done);
     void complete_fetch();
+    void fail_req_locked(Req& r, int code);
 };
+
+// The slot's pass, captured once on its stream: the setup kernel and the
+// persistent kernels with resident grids (4 candidate workgroups per CU, 2
+// of the 1024-thread per-position kernels, 8 four-wavefront parse
+// workgroups).  Inputs are addressed absolutely (a null arena), everything
+// else in the pool; the queue gives each kernel its items.
+int gc_server::setup_slot(Slot& sl) {
+    const uint32_t cap = max_batch_items;
+    hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&sl.ev_start);
+    if (e == hipSuccess) e = hipEventCreate(&sl.ev_done);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sl.hdr, sizeof(gc_queue), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sl.stage, (size_t)cap * sizeof(gc_item), 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&sl.err, 256, 0);
+    if (e != hipSuccess) return hip_fail(e, "gc_server_start: slot");
+    memset(sl.hdr, 0, sizeof(gc_queue));
+    *sl.err = 0;
+    int rc = sl.items.ensure((size_t)cap * sizeof(gc_item));
+    if (!rc) rc = sl.cost.ensure((size_t)cap * 4);
+    if (!rc) rc = sl.order.ensure((size_t)cap * 4);
+    if (!rc) {
+        gc_queue q0{};
+        rc = upload(ctx, sl.queue, &q0, sizeof(q0));
+    }
+    if (rc) return rc;
+    gc_queue* q = (gc_queue*)sl.queue.p;
+    gc_item* items = (gc_item*)sl.items.p;
+    const uint32_t cus = (uint32_t)ctx->cu_count;
+    e = hipStreamBeginCapture(sl.st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    rc = launch_tick_setup(ctx, sl.st, dev_addr(sl.hdr), dev_addr(sl.stage), q, items, cap);
+    if (!rc) rc = launch_cand_q(ctx, sl.st, nullptr, items, q, 4 * cus, (uint16_t*)pool);
+    if (!rc)
+        rc = launch_lzma2_len_q(ctx, sl.st, nullptr, items, q, 2 * cus, (const uint16_t*)pool, (uint32_t*)pool);
+    if (!rc)
+        rc = launch_zstd_info_q(ctx, sl.st, nullptr, items, q, 2 * cus, (const uint16_t*)pool, (uint32_t*)pool, 8,
+                                (uint32_t*)sl.cost.p);
+    if (!rc) rc = launch_order(ctx, sl.st, (const uint32_t*)sl.cost.p, 0, (uint32_t*)sl.order.p, q);
+    if (!rc)
+        rc = launch_zstd_parse_q(ctx, sl.st, nullptr, items, q, 8 * cus, (const uint32_t*)pool, pool,
+                                 dev_addr(counts_zone), kMaxSeq, (const uint32_t*)sl.order.p);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(sl.st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    sl.graph = g;
+    e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    return GC_OK;
+}
+
+void gc_server::fail_req_locked(Req& r, int code) {
+    r.state = FAILED;
+    r.error = code;
+    alloc.release(r.dev_off, r.dev_len);
+    r.dev_len = 0;
+    zone.release(r.cnt_off, r.cnt_len);
+    r.cnt_len = 0;
+    if (r.copied) (void)hipEventDestroy(r.copied);
+    r.copied = nullptr;
+    r.t_done = now_s();
+    cv_done.notify_all();
+}
 
 void gc_server::admit_locked() {
     while (!pending.empty()) {
         Req& r = *reqs[pending.front()];
         if (!alloc.alloc(r.dev_len, &r.dev_off)) break;  // in arrival order: wait for memory
+        if (r.algo == GC_ALGO_ZSTD) {
+            uint64_t z = 0;
+            if (!zone.alloc(r.nsub, &z)) {
+                alloc.release(r.dev_off, r.dev_len);
+                break;
+            }
+            r.cnt_off = z;
+            r.cnt_len = r.nsub;
+        }
         pending.pop_front();
         r.t_admit = now_s();
+        if (r.d_in) {
+            r.state = READY;
+            ready.push_back(&r);
+            continue;
+        }
         r.state = COPYING;
-        hipError_t e = hipMemcpyAsync(pool + r.dev_off + r.in_off, r.data, r.n, hipMemcpyHostToDevice, ctx->stream2);
+        uint8_t* dst = pool + r.dev_off + r.in_off;
+        r.d_in = dst;
+        hipError_t e = hipMemcpyAsync(dst, r.data, r.n, hipMemcpyHostToDevice, ctx->stream2);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&r.copied, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventRecord(r.copied, ctx->stream2);
         if (e != hipSuccess) {
-            r.state = FAILED;
-            r.error = GC_EHIP;
-            alloc.release(r.dev_off, r.dev_len);
-            r.t_done = now_s();
-            cv_done.notify_all();
+            fail_req_locked(r, GC_EHIP);
+            continue;
         }
+        copying.push_back(&r);
     }
 }
 
-bool gc_server::launch_batch() {
-    // every request whose input has arrived
+bool gc_server::launch_tick(Slot& sl) {
     std::vector<Req*> batch;
     {
         std::lock_guard<std::mutex> g(mu);
-        for (auto& kv : reqs) {
-            Req& r = *kv.second;
-            if (r.state == COPYING && hipEventQuery(r.copied) == hipSuccess) r.state = READY;
-        }
         uint32_t n_items = 0;
-        for (auto& kv : reqs) {
-            Req& r = *kv.second;
-            if (r.state != READY) continue;
-            if (n_items + r.nsub > max_batch_items && !batch.empty()) break;
-            batch.push_back(&r);
-            n_items += r.nsub;
+        while (!ready.empty()) {
+            Req* r = ready.front();
+            if (n_items + r->nsub > max_batch_items && !batch.empty()) break;
+            ready.pop_front();
+            batch.push_back(r);
+            n_items += r->nsub;
         }
-    }
-    if (batch.empty()) return false;
-    // items: LZMA2 first, then Zstd (each kernel gets a contiguous slice)
-    host_items.clear();
-    uint32_t n_l = 0;
-    for (int pass = 0; pass < 2; pass++)
-        for (Req* r : batch) {
-            if ((pass == 0) != (r->algo == GC_ALGO_LZMA2)) continue;
-            const uint64_t base = r->dev_off;
-            for (uint32_t s = 0; s < r->nsub; s++) {
-                gc_item it{};
-                const uint64_t start = (uint64_t)s * GC_SUB_BLOCK;
-                it.src_off = base + r->in_off + start;
-                it.avail = budget + 64 - it.src_off;  // the pool is readable to its end
-                it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, r->n - start);
-                it.cpos = (base + r->cand_off) / 2 + start;
-                it.out_span = GC_SUB_BLOCK;
-                if (r->algo == GC_ALGO_ZSTD) {
-                    it.opos = (base + r->info_off) / 4 + start;
-                    it.seq_pos = (base + r->out_off) / sizeof(gc_zstd_sequence) + (uint64_t)s * kMaxSeq;
-                    it.count_pos = 0;  // set below: counts are per request in the pool
-                    it.depth = zstd_depth_for_level(r->level);
-                } else {
-                    it.opos = (base + r->out_off) / 4 + start;
-                    it.depth = GC_LZMA2_DEPTH;
-                }
-                host_items.push_back(it);
-            }
-            if (pass == 0) n_l += r->nsub;
-        }
-    // Zstd counts: one u32 per sub-block in the server's counts array, in
-    // batch order
-    {
-        uint32_t k = n_l, c = 0;
-        for (Req* r : batch) {
-            if (r->algo != GC_ALGO_ZSTD) continue;
-            r->item0 = c;
-            for (uint32_t s2 = 0; s2 < r->nsub; s2++) host_items[k++].count_pos = c++;
-        }
-    }
-    const uint32_t n_all = (uint32_t)host_items.size(), n_z = n_all - n_l;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    int rc = items.ensure(n_all * sizeof(gc_item));
-    if (!rc) rc = cost.ensure((size_t)n_z * 4 + 4);
-    if (!rc) rc = order.ensure((size_t)n_z * 4 + 4);
-    // pinned staging (no batch runs now, so the last batch's reads and writes are done)
-    if (!rc && n_z > counts_host_cap) {
-        (void)hipHostFree(counts_host);
-        counts_host_cap = 0;
-        if (hipHostMalloc((void**)&counts_host, (size_t)n_z * 4, 0) != hipSuccess) {
-            counts_host = nullptr;
-            rc = fail(GC_ENOMEM, "gc_server: pinned counts buffer");
-        } else {
-            counts_host_cap = n_z;
-        }
-    }
-    if (!rc && n_all > items_host_cap) {
-        (void)hipHostFree(items_host);
-        items_host_cap = 0;
-        if (hipHostMalloc((void**)&items_host, (size_t)n_all * sizeof(gc_item), 0) != hipSuccess) {
-            items_host = nullptr;
-            rc = fail(GC_ENOMEM, "gc_server: pinned item buffer");
-        } else {
-            items_host_cap = n_all;
-        }
-    }
-    hipError_t e = hipSuccess;
-    if (!rc) {
-        memcpy(items_host, host_items.data(), (size_t)n_all * sizeof(gc_item));
-        e = hipEventRecord(batch_start, ctx->stream);
-        if (e == hipSuccess)
-            rc = launch_copy_u32(ctx, ctx->stream, (const uint32_t*)dev_addr(items_host), (uint32_t*)items.p,
-                                 (uint64_t)n_all * sizeof(gc_item) / 4);
-    }
-    const gc_item* all = (const gc_item*)items.p;
-    {
-        std::lock_guard<std::mutex> g2(mu);  // clients read the state in gc_server_wait
         const double t = now_s();
         for (Req* r : batch) {
             r->state = RUNNING;
             r->t_start = t;
         }
     }
-    if (!rc && e == hipSuccess) {
-        rc = launch_cand(ctx, ctx->stream, pool, all, n_all, n_all, (uint16_t*)pool);
-        if (!rc && n_l)
-            rc = launch_lzma2_len(ctx, ctx->stream, pool, all, n_l, n_l, (const uint16_t*)pool, (uint32_t*)pool,
-                                  nullptr, nullptr, 1);
-        if (!rc && n_z)
-            rc = launch_zstd_info(ctx, ctx->stream, pool, all + n_l, n_z, n_z, (const uint16_t*)pool, (uint32_t*)pool,
-                                  4, (uint32_t*)cost.p);
-        if (!rc && n_z) rc = launch_order(ctx, ctx->stream, (const uint32_t*)cost.p, n_z, (uint32_t*)order.p);
-        if (!rc && n_z)
-            rc = launch_zstd_parse(ctx, ctx->stream, pool, all + n_l, n_z, (n_z + 3) / 4, (const uint32_t*)pool,
-                                   pool, GC_SEQ_FORMAT_ZSTD, nullptr, dev_addr(counts_host), nullptr, kMaxSeq,
-                                   (const uint32_t*)order.p);
-        if (!rc && e == hipSuccess) e = hipEventRecord(batch_done, ctx->stream);
-    }
-    running = batch;
-    if (rc || e != hipSuccess) {
-        std::lock_guard<std::mutex> g2(mu);
+    if (batch.empty()) return false;
+    // items: LZMA2 first, then Zstd (the queue gives each kernel its range)
+    uint32_t k = 0, n_l = 0;
+    for (int pass = 0; pass < 2; pass++)
         for (Req* r : batch) {
-            r->state = FAILED;
-            r->error = rc ? rc : GC_EHIP;
-            r->t_done = now_s();
-            alloc.release(r->dev_off, r->dev_len);
-            if (r->copied) (void)hipEventDestroy(r->copied);
-            r->copied = nullptr;
+            if ((pass == 0) != (r->algo == GC_ALGO_LZMA2)) continue;
+            const uint64_t base = r->dev_off;
+            const uint64_t readable = r->data ? up(r->n + 64, kAlign) : r->n;  // a copy has room past its end
+            for (uint32_t s = 0; s < r->nsub; s++) {
+                gc_item& it = sl.stage[k++];
+                const uint64_t start = (uint64_t)s * GC_SUB_BLOCK;
+                it.src_off = (uint64_t)(uintptr_t)(r->d_in + start);  // absolute (the pass's arena is null)
+                it.avail = readable - start;
+                it.cpos = (base + r->cand_off) / 2 + start;
+                it.lit_pos = 0;
+                it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, r->n - start);
+                it.out_span = GC_SUB_BLOCK;
+                if (r->algo == GC_ALGO_ZSTD) {
+                    it.opos = (base + r->info_off) / 4 + start;
+                    it.seq_pos = (base + r->out_off) / sizeof(gc_zstd_sequence) + (uint64_t)s * kMaxSeq;
+                    it.count_pos = (uint32_t)(r->cnt_off + s);
+                    it.depth = zstd_depth_for_level(r->level);
+                } else {
+                    it.opos = (base + r->out_off) / 4 + start;
+                    it.seq_pos = 0;
+                    it.count_pos = 0;
+                    it.depth = GC_LZMA2_DEPTH;
+                }
+            }
+            if (pass == 0) n_l += r->nsub;
         }
-        running.clear();
-        cv_done.notify_all();
+    const uint32_t n_all = k, n_z = n_all - n_l;
+    gc_queue* h = sl.hdr;
+    h->n[GC_Q_CAND] = n_all;
+    h->n[GC_Q_LZMA2] = n_l;
+    h->n[GC_Q_INFO] = n_z;
+    h->n[GC_Q_PARSE] = n_z;
+    h->base[GC_Q_CAND] = h->base[GC_Q_LZMA2] = 0;
+    h->base[GC_Q_INFO] = h->base[GC_Q_PARSE] = n_l;
+    sl.reqs = batch;
+    hipError_t e = hipEventRecord(sl.ev_start, sl.st);
+    if (e == hipSuccess) e = hipGraphLaunch(sl.exec, sl.st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sl.err, (uint8_t*)sl.queue.p + offsetof(gc_queue, err), 4, hipMemcpyDeviceToHost, sl.st);
+    if (e == hipSuccess) e = hipEventRecord(sl.ev_done, sl.st);
+    if (e != hipSuccess) {
+        (void)hip_fail(e, "gc_server: tick launch");
+        (void)hipStreamSynchronize(sl.st);
+        std::lock_guard<std::mutex> g(mu);
+        for (Req* r : batch) fail_req_locked(*r, GC_EHIP);
+        sl.reqs.clear();
         return false;
     }
+    sl.busy = true;
     stats.batches++;
     stats.items += n_all;
     return true;
 }
 
-// The running batch ended on the device: take its sequence counts (already
-// in pinned memory) and queue it for the copy back.
-void gc_server::batch_ended() {
+// A tick ended on the device: device-result requests are done; the others
+// wait for their results to be brought back.
+void gc_server::tick_ended(Slot& sl) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, batch_start, batch_done) == hipSuccess) stats.t_batches += ms * 1e-3;
-    for (Req* rp : running) {
-        Req& r = *rp;
-        if (r.algo == GC_ALGO_ZSTD) r.counts.assign(counts_host + r.item0, counts_host + r.item0 + r.nsub);
+    if (hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done) == hipSuccess) stats.t_batches += ms * 1e-3;
+    const uint32_t err = *sl.err;
+    if (err) {  // the queue protocol reported an error: this tick's results are not trusted
+        *sl.err = 0;
+        (void)hipMemcpyAsync((uint8_t*)sl.queue.p + offsetof(gc_queue, err), zero_host, 4, hipMemcpyHostToDevice,
+                             sl.st);
+        (void)hipStreamSynchronize(sl.st);
     }
-    finished.emplace_back();
-    finished.back().swap(running);
+    std::vector<Req*> back;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        const double t = now_s();
+        for (Req* rp : sl.reqs) {
+            Req& r = *rp;
+            if (err) {
+                fail_req_locked(r, GC_EHIP);
+                continue;
+            }
+            if (r.algo == GC_ALGO_ZSTD) {
+                r.counts.assign(counts_zone + r.cnt_off, counts_zone + r.cnt_off + r.nsub);
+                uint64_t tot = 0;
+                for (uint32_t c : r.counts) tot += c;
+                r.fetch_total = tot;
+                zone.release(r.cnt_off, r.cnt_len);
+                r.cnt_len = 0;
+            }
+            if (r.dev_result) {
+                r.state = DONE;
+                r.t_done = t;
+                stats.completed++;
+                stats.bytes += r.n;
+            } else {
+                back.push_back(&r);
+            }
+        }
+        cv_done.notify_all();
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+    if (!back.empty()) finished.push_back(std::move(back));
 }
 
-// Bring a finished batch's results back on the fetch stream (no other fetch
-// is in flight).  Zstd: each request's used sequence slots are packed by
-// gc_gather_seqs (counts and offsets staged in `meta`) and copied.
-void gc_server::finish_batch(std::vector<Req*>& done) {
+// Bring a finished tick's host-bound results back on the fetch stream (no
+// other fetch is in flight).  Zstd: each request's used sequence slots are
+// packed by gc_gather_seqs (counts and offsets staged in pinned `meta`).
+void gc_server::finish_fetch(std::vector<Req*>& done) {
     fetching.swap(done);
     done.clear();
-    std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t cs = fetch_stream;
     (void)hipEventRecord(fetch_start, cs);
     uint32_t n_z = 0;
     for (Req* r : fetching)
         if (r->algo == GC_ALGO_ZSTD) n_z += r->nsub;
-    // meta: counts (u32) for every Zstd item, then offsets (u64)
     const size_t off_bytes = up((size_t)n_z * 4, 8);
-    if (off_bytes + (size_t)n_z * 8 > meta_cap) {  // a batch larger than any before
+    if (off_bytes + (size_t)n_z * 8 > meta_cap) {
         (void)hipHostFree(meta_host);
         meta_cap = off_bytes + (size_t)n_z * 8;
         if (hipHostMalloc((void**)&meta_host, meta_cap, 0) != hipSuccess) {
@@ -383,38 +464,38 @@ void gc_server::finish_batch(std::vector<Req*>& done) {
     uint32_t* mc = (uint32_t*)meta_host;
     uint64_t* mo = (uint64_t*)(meta_host + off_bytes);
     {
-        std::lock_guard<std::mutex> g2(mu);
+        std::lock_guard<std::mutex> g(mu);
         for (Req* r : fetching) r->state = FETCHING;
     }
-    for (Req* rp : fetching) {
-        Req& r = *rp;
+    uint32_t at = 0;
+    std::vector<uint32_t> first(fetching.size(), 0);
+    for (size_t j = 0; j < fetching.size(); j++) {
+        Req& r = *fetching[j];
         if (r.algo != GC_ALGO_ZSTD || !meta_host) continue;
+        first[j] = at;
         uint64_t tot = 0;
         for (uint32_t s2 = 0; s2 < r.nsub; s2++) {
-            mc[r.item0 + s2] = r.counts[s2];
-            mo[r.item0 + s2] = tot;
+            mc[at + s2] = r.counts[s2];
+            mo[at + s2] = tot;
             tot += r.counts[s2];
         }
-        r.fetch_total = tot;
+        at += r.nsub;
         if (r.dst) {
             memcpy(r.dst_counts, r.counts.data(), (size_t)r.nsub * 4);
             if (tot * sizeof(gc_zstd_sequence) > r.dst_cap) r.error = GC_ESIZE;
         }
     }
-    // the gathers read the counts and offsets from pinned memory (no earlier fetch is still reading it)
     uint8_t* meta_dev = dev_addr(meta_host);
-    for (Req* rp : fetching) {
-        Req& r = *rp;
+    for (size_t j = 0; j < fetching.size(); j++) {
+        Req& r = *fetching[j];
         if (r.error) continue;
         uint8_t* base = pool + r.dev_off;
-        // pinned caller memory is written by the gather / copy kernels directly (over PCIe, on a
-        // few hundred wavefronts); anything else is packed on the device and copied by DMA
         if (r.algo == GC_ALGO_ZSTD) {
             if (!r.dst) r.seqs.resize(r.fetch_total);
             if (!r.fetch_total) continue;
-            const uint32_t* mcounts = (const uint32_t*)meta_dev + r.item0;
-            const uint64_t* moffs = (const uint64_t*)(meta_dev + off_bytes) + r.item0;
-            if (r.dst_dev) {
+            const uint32_t* mcounts = (const uint32_t*)meta_dev + first[j];
+            const uint64_t* moffs = (const uint64_t*)(meta_dev + off_bytes) + first[j];
+            if (r.dst_dev) {  // pinned caller memory: written by the gather kernel over PCIe
                 (void)launch_gather(ctx, cs, base + r.out_off, mcounts, moffs, r.dst_dev, r.nsub, kMaxSeq);
                 continue;
             }
@@ -445,6 +526,7 @@ void gc_server::complete_fetch() {
         r.state = r.error ? FAILED : DONE;
         r.t_done = t;
         alloc.release(r.dev_off, r.dev_len);
+        r.dev_len = 0;
         if (r.copied) (void)hipEventDestroy(r.copied);
         r.copied = nullptr;
         stats.completed++;
@@ -457,32 +539,50 @@ void gc_server::complete_fetch() {
 void gc_server::loop() {
     (void)hipSetDevice(ctx->device);
     for (;;) {
+        bool any_busy = false;
+        for (auto& sl : slot) any_busy |= sl.busy;
         {
             std::unique_lock<std::mutex> lk(mu);
             admit_locked();
-            bool idle = running.empty() && fetching.empty() && finished.empty() && pending.empty();
-            bool any_copying = false;
-            for (auto& kv : reqs)
-                if (kv.second->state == COPYING || kv.second->state == READY) any_copying = true;
-            if (stop && idle && !any_copying) break;
-            if (idle && !any_copying) {
+            const bool idle = !any_busy && fetching.empty() && finished.empty() && pending.empty() &&
+                              copying.empty() && ready.empty();
+            if (stop && idle) break;
+            if (idle) {
                 cv_work.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
         }
         bool progressed = false;
-        double t0 = now_s();
-        if (!running.empty() && hipEventQuery(batch_done) == hipSuccess) {
-            batch_ended();
-            progressed = true;
+        const double t0 = now_s();
+        // host inputs that have arrived on the device
+        if (!copying.empty()) {
+            std::lock_guard<std::mutex> g(mu);
+            for (size_t j = 0; j < copying.size();) {
+                Req* r = copying[j];
+                if (hipEventQuery(r->copied) == hipSuccess) {
+                    r->state = READY;
+                    ready.push_back(r);
+                    copying[j] = copying.back();
+                    copying.pop_back();
+                    progressed = true;
+                } else {
+                    j++;
+                }
+            }
         }
-        if (running.empty()) progressed |= launch_batch();  // the device goes on at once
+        for (auto& sl : slot)
+            if (sl.busy && hipEventQuery(sl.ev_done) == hipSuccess) {
+                tick_ended(sl);
+                progressed = true;
+            }
+        for (auto& sl : slot)
+            if (!sl.busy) progressed |= launch_tick(sl);
         if (!fetching.empty() && hipEventQuery(fetch_done) == hipSuccess) {
             complete_fetch();
             progressed = true;
         }
         if (fetching.empty() && !finished.empty()) {
-            finish_batch(finished.front());
+            finish_fetch(finished.front());
             finished.pop_front();
             progressed = true;
         }
@@ -492,6 +592,65 @@ void gc_server::loop() {
             std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
+
+namespace {
+
+void destroy_server(gc_server* s) {
+    for (auto& sl : s->slot) {
+        if (sl.st) (void)hipStreamSynchronize(sl.st);
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
+        if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
+        if (sl.hdr) (void)hipHostFree(sl.hdr);
+        if (sl.stage) (void)hipHostFree(sl.stage);
+        if (sl.err) (void)hipHostFree(sl.err);
+        sl.items.release();
+        sl.cost.release();
+        sl.order.release();
+        sl.queue.release();
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
+    if (s->fetch_stream) {
+        (void)hipStreamSynchronize(s->fetch_stream);
+        (void)hipStreamDestroy(s->fetch_stream);
+    }
+    (void)hipStreamSynchronize(s->ctx->stream2);
+    for (auto& kv : s->reqs)
+        if (kv.second->copied) (void)hipEventDestroy(kv.second->copied);
+    if (s->fetch_start) (void)hipEventDestroy(s->fetch_start);
+    if (s->fetch_done) (void)hipEventDestroy(s->fetch_done);
+    if (s->meta_host) (void)hipHostFree(s->meta_host);
+    if (s->zero_host) (void)hipHostFree(s->zero_host);
+    if (s->counts_zone) (void)hipHostFree(s->counts_zone);
+    if (s->pool) (void)hipFree(s->pool);
+    delete s;
+}
+
+int64_t submit(gc_server* s, std::unique_ptr<Req> r) {
+    r->nsub = (uint32_t)((r->n + GC_SUB_BLOCK - 1) / GC_SUB_BLOCK);
+    layout(*r);
+    if (r->dev_len > s->budget) return fail(GC_EBUDGET, "request needs more device memory than the whole HBM budget");
+    if (r->nsub > s->max_batch_items) return fail(GC_EBUDGET, "request has more sub-blocks than a tick holds");
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->stop) return fail(GC_EINVAL, "server stopped");
+    r->id = s->next_id++;
+    const uint64_t id = r->id;
+    if (r->n == 0) {
+        r->state = DONE;
+        r->t_admit = r->t_start = r->t_done = r->t_submit;
+        s->reqs[id] = std::move(r);
+        s->cv_done.notify_all();
+        return (int64_t)id;
+    }
+    s->pending.push_back(id);
+    s->reqs[id] = std::move(r);
+    s->cv_work.notify_one();
+    return (int64_t)id;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -505,9 +664,11 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
     s->ctx = ctx;
     s->budget = up(hbm_budget, kAlign);
     s->max_batch_items = max_batch_items ? max_batch_items : 65536;
+    // every Zstd sub-block in flight holds more than 6 bytes per position of the budget
+    s->zone_len = s->budget / (6ull * GC_SUB_BLOCK) + s->max_batch_items;
     hipError_t e = hipMalloc((void**)&s->pool, s->budget + 64);
-    if (e == hipSuccess) e = hipEventCreate(&s->batch_start);
-    if (e == hipSuccess) e = hipEventCreate(&s->batch_done);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->counts_zone, s->zone_len * 4, 0);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->zero_host, 256, 0);
     if (e == hipSuccess) e = hipEventCreate(&s->fetch_start);
     if (e == hipSuccess) e = hipEventCreate(&s->fetch_done);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->fetch_stream, hipStreamNonBlocking);
@@ -516,12 +677,22 @@ gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_
     if (e == hipSuccess) e = hipHostMalloc((void**)&s->meta_host, meta_bytes, 0);
     if (e != hipSuccess) {
         hip_fail(e, "gc_server_start");
-        if (s->pool) (void)hipFree(s->pool);
-        if (s->fetch_stream) (void)hipStreamDestroy(s->fetch_stream);
-        delete s;
+        destroy_server(s);
         return nullptr;
     }
+    memset(s->zero_host, 0, 256);
+    memset(s->counts_zone, 0, s->zone_len * 4);
     s->alloc.reset(s->budget);
+    s->zone.reset(s->zone_len);
+    for (auto& sl : s->slot) {
+        int rc = s->setup_slot(sl);
+        if (rc) {
+            std::string keep = gc_last_error();
+            destroy_server(s);
+            set_error(keep);
+            return nullptr;
+        }
+    }
     s->th = std::thread([s] { s->loop(); });
     return s;
 }
@@ -554,26 +725,19 @@ int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uin
     r->n = n;
     r->algo = algo;
     r->level = level;
-    r->nsub = (uint32_t)((n + GC_SUB_BLOCK - 1) / GC_SUB_BLOCK);
-    layout(*r);
-    if (r->dev_len > s->budget)
-        return fail(GC_EBUDGET, "request needs more device memory than the whole HBM budget");
-    r->t_submit = now_s();
-    std::lock_guard<std::mutex> g(s->mu);
-    if (s->stop) return fail(GC_EINVAL, "server stopped");
-    r->id = s->next_id++;
-    const uint64_t id = r->id;
-    if (n == 0) {
-        r->state = DONE;
-        r->t_admit = r->t_start = r->t_done = r->t_submit;
-        s->reqs[id] = std::move(r);
-        s->cv_done.notify_all();
-        return (int64_t)id;
-    }
-    s->pending.push_back(id);
-    s->reqs[id] = std::move(r);
-    s->cv_work.notify_one();
-    return (int64_t)id;
+    return submit(s, std::move(r));
+}
+
+int64_t gc_server_submit_dev(gc_server* s, const uint8_t* d_input, uint64_t n, uint32_t algo, uint32_t level) {
+    if (!s || (!d_input && n) || algo > GC_ALGO_LZMA2) return fail(GC_EINVAL, "gc_server_submit_dev: bad argument");
+    if ((uintptr_t)d_input & 3u) return fail(GC_EINVAL, "gc_server_submit_dev: d_input must be 4-byte aligned");
+    auto r = std::make_unique<Req>();
+    r->d_in = d_input;
+    r->dev_result = true;
+    r->n = n;
+    r->algo = algo;
+    r->level = level;
+    return submit(s, std::move(r));
 }
 
 int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info) {
@@ -611,6 +775,7 @@ int gc_server_result_zstd(gc_server* s, uint64_t id, gc_zstd_sequence* seqs, uin
         return fail(GC_EINVAL, "no finished Zstd request with that id");
     Req& r = *it->second;
     if (r.dst) return fail(GC_EINVAL, "the request's results went to the buffers given at submit");
+    if (r.dev_result) return fail(GC_EINVAL, "the request's results stay on the device (gc_server_result_dev)");
     if (r.seqs.size() > cap) return fail(GC_ESIZE, "sequence capacity below the request's sequence count");
     memcpy(seq_counts, r.counts.data(), r.counts.size() * 4);
     if (!r.seqs.empty()) memcpy(seqs, r.seqs.data(), r.seqs.size() * sizeof(gc_zstd_sequence));
@@ -624,7 +789,21 @@ int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed) {
     if (it == s->reqs.end() || it->second->state != DONE || it->second->algo != GC_ALGO_LZMA2)
         return fail(GC_EINVAL, "no finished LZMA2 request with that id");
     if (it->second->dst) return fail(GC_EINVAL, "the request's results went to the buffers given at submit");
-    memcpy(packed, it->second->packed.data(), it->second->n * 4);
+    if (it->second->dev_result)
+        return fail(GC_EINVAL, "the request's results stay on the device (gc_server_result_dev)");
+    if (it->second->n) memcpy(packed, it->second->packed.data(), it->second->n * 4);
+    return GC_OK;
+}
+
+int gc_server_result_dev(gc_server* s, uint64_t id, void** d_results, const uint32_t** seq_counts) {
+    if (!s || !d_results) return fail(GC_EINVAL, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end() || it->second->state != DONE || !it->second->dev_result)
+        return fail(GC_EINVAL, "no finished device-result request with that id");
+    Req& r = *it->second;
+    *d_results = r.n ? (void*)(s->pool + r.dev_off + r.out_off) : nullptr;
+    if (seq_counts) *seq_counts = r.algo == GC_ALGO_ZSTD ? r.counts.data() : nullptr;
     return GC_OK;
 }
 
@@ -633,7 +812,13 @@ int gc_server_release(gc_server* s, uint64_t id) {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->reqs.find(id);
     if (it == s->reqs.end()) return fail(GC_EINVAL, "unknown request id");
-    if (it->second->state != DONE && it->second->state != FAILED) return fail(GC_EINVAL, "request still running");
+    Req& r = *it->second;
+    if (r.state != DONE && r.state != FAILED) return fail(GC_EINVAL, "request still running");
+    if (r.dev_len) {  // a device-result request holds its region until now
+        s->alloc.release(r.dev_off, r.dev_len);
+        r.dev_len = 0;
+        s->cv_work.notify_one();
+    }
     s->reqs.erase(it);
     return GC_OK;
 }
@@ -655,24 +840,7 @@ int gc_server_stop(gc_server* s) {
     }
     s->cv_work.notify_all();
     if (s->th.joinable()) s->th.join();
-    (void)hipStreamSynchronize(s->ctx->stream);
-    (void)hipStreamSynchronize(s->ctx->stream2);
-    (void)hipStreamSynchronize(s->fetch_stream);
-    (void)hipStreamDestroy(s->fetch_stream);
-    if (s->counts_host) (void)hipHostFree(s->counts_host);
-    if (s->items_host) (void)hipHostFree(s->items_host);
-    if (s->batch_start) (void)hipEventDestroy(s->batch_start);
-    if (s->fetch_start) (void)hipEventDestroy(s->fetch_start);
-    for (auto& kv : s->reqs)
-        if (kv.second->copied) (void)hipEventDestroy(kv.second->copied);
-    if (s->batch_done) (void)hipEventDestroy(s->batch_done);
-    if (s->fetch_done) (void)hipEventDestroy(s->fetch_done);
-    s->items.release();
-    s->cost.release();
-    s->order.release();
-    if (s->meta_host) (void)hipHostFree(s->meta_host);
-    if (s->pool) (void)hipFree(s->pool);
-    delete s;
+    destroy_server(s);
     return GC_OK;
 }
 

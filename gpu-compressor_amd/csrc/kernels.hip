@@ -77,10 +77,18 @@ struct SrcView {
     uint64_t avail;
 };
 
+// The byte at arena + off, as an address computed in integers: an item's
+// offset is relative to its launch's arena or, with a null arena, the
+// absolute address of an input that lives in another allocation.
+__device__ __forceinline__ const uint8_t* at(const uint8_t* arena, uint64_t off) {
+    return (const uint8_t*)((uintptr_t)arena + off);
+}
+
 __device__ __forceinline__ SrcView src_view(const uint8_t* arena, uint64_t src_off, uint64_t avail) {
     SrcView v;
-    v.mis = (uint32_t)(((uintptr_t)arena + src_off) & 3);
-    v.base = arena + (src_off - v.mis);
+    const uintptr_t a = (uintptr_t)arena + src_off;
+    v.mis = (uint32_t)(a & 3);
+    v.base = (const uint8_t*)(a - v.mis);
     v.avail = avail;
     return v;
 }
@@ -267,8 +275,13 @@ __device__ __forceinline__ void queue_leave(gc_queue* q, uint32_t ph, uint32_t n
     }
 }
 
-// Workgroup-level: body(i) for every item i the workgroup takes; all threads
-// call this, body may use barriers.
+// Workgroup-level: body(i) for every item i the workgroup takes (i counts
+// from the phase's first item, q->base[ph]); all threads call this, body may
+// use barriers.
+__device__ __forceinline__ uint32_t queue_base(const gc_queue* q, uint32_t ph) {
+    return uniform(__atomic_load_n(&q->base[ph], __ATOMIC_RELAXED));
+}
+
 template <class F>
 __device__ __forceinline__ void wg_queue_items(gc_queue* q, uint32_t ph, F&& body) {
     __shared__ uint32_t s_take;
@@ -596,6 +609,7 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
 extern "C" __global__ void __launch_bounds__(CAND_THREADS)
 gc_cand_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, uint16_t* cand16) {
     __shared__ CandLds s;
+    items += queue_base(q, GC_Q_CAND);
     wg_queue_items(q, GC_Q_CAND, [&](uint32_t i) {
         const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
@@ -766,8 +780,10 @@ __device__ __forceinline__ void lzma2_len_items(const uint8_t* arena, const gc_i
             if (threadIdx.x == 0) triage[it.count_pos] = wg_any ? 1 : 0;
         }
     };
-    if (q)
+    if (q) {
+        items += queue_base(q, GC_Q_LZMA2);
         wg_queue_items(q, GC_Q_LZMA2, body);
+    }
     else
         for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) body(i);
 }
@@ -902,8 +918,10 @@ __device__ __forceinline__ void zstd_info_items(const uint8_t* arena, const gc_i
             if (threadIdx.x == 0) cost[i] = verified;
         }
     };
-    if (q)
+    if (q) {
+        items += queue_base(q, GC_Q_INFO);
         wg_queue_items(q, GC_Q_INFO, body);
+    }
     else
         for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) body(i);
 }
@@ -1424,6 +1442,7 @@ __device__ __forceinline__ void zstd_parse_queue(const uint8_t* arena, const gc_
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     uint8_t* ring = (uint8_t*)ring_mem[wave];
     const uint32_t n = uniform(__atomic_load_n(&q->n[GC_Q_PARSE], __ATOMIC_RELAXED));
+    items += queue_base(q, GC_Q_PARSE);
     uint32_t mine = 0;
     for (uint32_t iter = 0;; iter++) {
         uint32_t i = 0;
@@ -1789,7 +1808,7 @@ __device__ void ze_frame(ZeLds& s, uint32_t* bits, const gc_item& it, const uint
         if (raw) __threadfence();  // the raw block below overwrites the bytes written so far
     }
     if (raw) {
-        ze_copy_bytes(dst + start, arena + it.src_off, n);
+        ze_copy_bytes(dst + start, at(arena, it.src_off), n);
         end = start + n;
     }
     if (lane < 3) {
@@ -1855,6 +1874,28 @@ gc_make_items(gc_item* items, uint64_t total, uint32_t sub, uint32_t nsub, uint3
     it.depth = depth;
     it.out_span = sub;
     items[s] = it;
+}
+
+// Served form: a tick's work list, written by the host into pinned memory
+// (hdr: item counts and first items per kernel; src: the items), copied into
+// the slot's device queue and item array by the first kernel of the slot's
+// captured pass.  The work counters of q are not touched (they are zero
+// between launches).  A count above the slot's capacity sets error bit 2 and
+// runs nothing.
+extern "C" __global__ void __launch_bounds__(256)
+gc_tick_setup(const gc_queue* hdr, const gc_item* src, gc_queue* q, gc_item* items, uint32_t cap) {
+    const uint32_t n_all = hdr->n[GC_Q_CAND];
+    const bool ok = n_all <= cap;
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        q->n[threadIdx.x] = ok ? hdr->n[threadIdx.x] : 0u;
+        q->base[threadIdx.x] = hdr->base[threadIdx.x];
+        if (!ok && threadIdx.x == 0) atomicOr(&q->err, 4u);
+    }
+    if (!ok) return;
+    const uint4* s4 = (const uint4*)src;
+    uint4* d4 = (uint4*)items;
+    const uint32_t words = n_all * (uint32_t)(sizeof(gc_item) / 16);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < words; k += gridDim.x * blockDim.x) d4[k] = s4[k];
 }
 
 // Small fill kernel.

@@ -103,7 +103,9 @@ SIGNATURES = [
     ("gc_lzma2_triage_dev", C.c_int, [_vp, _vp, C.c_uint64, _vp]),
     ("gc_server_submit", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_server_submit_into", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp, C.c_uint64, _vp]),
+    ("gc_server_submit_dev", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_server_wait", C.c_int, [_vp, C.c_uint64, C.c_int, _vp]),
+    ("gc_server_result_dev", C.c_int, [_vp, C.c_uint64, _vp, _vp]),
     ("gc_server_result_zstd", C.c_int, [_vp, C.c_uint64, _vp, C.c_uint64, _vp]),
     ("gc_server_result_lzma2", C.c_int, [_vp, C.c_uint64, _vp]),
     ("gc_server_release", C.c_int, [_vp, C.c_uint64]),
@@ -587,6 +589,21 @@ class Server:
             _check(int(rid), "gc_server_submit_into")
         self._keep[rid] = (data, out, counts)
         return rid
+
+    def submit_dev(self, d_input: int, n: int, algo: int, level: int = 1) -> int:
+        """Submit a request whose input is already in device memory; its results stay there."""
+        rid = lib().gc_server_submit_dev(self.h, d_input, n, algo, level)
+        if rid < 0:
+            _check(int(rid), "gc_server_submit_dev")
+        return rid
+
+    def result_dev(self, rid: int, n_sub: int):
+        """(device pointer of the results, Zstd sequence counts as an array or None)."""
+        p = C.c_void_p()
+        cp = C.POINTER(C.c_uint32)()
+        _check(lib().gc_server_result_dev(self.h, rid, C.byref(p), C.byref(cp)), "gc_server_result_dev")
+        counts = np.ctypeslib.as_array(cp, (n_sub,)).copy() if cp else None
+        return p.value, counts
 
     def wait(self, rid: int, timeout_ms: int = -1):
         info = RequestInfo()

@@ -367,12 +367,14 @@ int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed);
 /* ---- served form ------------------------------------------------------- */
 /* No reference counterpart (the reference compresses one chunk at a time,
  * pipeline.rs:138-175).  A server thread admits requests in arrival order
- * while their device memory fits hbm_budget bytes, copies them to the device
- * as they are admitted, and whenever the GPU is free launches every request
- * whose input has arrived as one batch; requests arriving meanwhile join the
- * next batch.  A request's results equal the plain form's for its input
- * alone.  Request memory (caller's `data`) must stay valid until the request
- * is done. */
+ * while their device memory fits hbm_budget bytes (a host input is copied in
+ * as it is admitted) and batches them continuously into the persistent
+ * kernels: whenever one of its tick slots (each with its own stream and its
+ * own captured pass) is free, every admitted request whose input is on the
+ * device goes into a tick, so requests that arrive while others run start
+ * beside them at the next free slot.  A request's results equal the plain
+ * form's for its input alone.  Request memory (caller's `data` or
+ * `d_input`) must stay valid until the request is done. */
 typedef struct gc_server gc_server;
 
 typedef struct {
@@ -384,14 +386,15 @@ typedef struct {
 } gc_request_info;
 
 typedef struct {
-    uint64_t batches, items, completed, bytes;
+    uint64_t batches, items, completed, bytes; /* batches: ticks started */
     uint64_t hbm_in_use, pending;
-    double t_batches;    /* device seconds of the batches (events around each) */
-    double t_fetch;      /* seconds of result copies on the copy-back stream (events around each batch's) */
-    double t_host;       /* server-thread seconds spent building batches and staging result copies */
+    double t_batches;    /* device seconds of the ticks (events around each; ticks overlap, so the sum can exceed wall time) */
+    double t_fetch;      /* seconds of result copies on the copy-back stream (events around each tick's) */
+    double t_host;       /* server-thread seconds spent building ticks and staging result copies */
 } gc_server_stats;
 
-/* max_batch_items: sub-blocks per batch (0 = 65536). */
+/* max_batch_items: sub-blocks per tick (0 = 65536); a request with more is
+ * refused (GC_EBUDGET). */
 gc_server* gc_server_start(gc_ctx* ctx, uint64_t hbm_budget, uint32_t max_batch_items);
 /* Returns the request id (> 0), or a negative GC_E* code (GC_EBUDGET: the
  * request alone needs more than the budget). */
@@ -406,12 +409,22 @@ int64_t gc_server_submit(gc_server* s, const uint8_t* data, uint64_t n, uint32_t
  * not apply to such requests. */
 int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uint32_t algo, uint32_t level, void* out,
                               uint64_t out_bytes, uint32_t* seq_counts);
+/* A request whose input is already in device memory (4-byte aligned, n
+ * bytes readable): it is read in place and its results stay on the device
+ * (gc_server_result_dev) until gc_server_release. */
+int64_t gc_server_submit_dev(gc_server* s, const uint8_t* d_input, uint64_t n, uint32_t algo, uint32_t level);
 /* 0 done, 1 still running after timeout_ms (< 0 waits), < 0 failed. */
 int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info);
 /* Results of a finished request: Zstd packed sequences (cap records) and per
  * sub-block counts; LZMA2 packed matches, one u32 per input byte. */
 int gc_server_result_zstd(gc_server* s, uint64_t id, gc_zstd_sequence* seqs, uint64_t cap, uint32_t* seq_counts);
 int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed);
+/* Results of a finished gc_server_submit_dev request, valid until it is
+ * released: *d_results is a device pointer, Zstd: n_sub * 21845
+ * gc_zstd_sequence slots (sub-block k's at k * 21845, the first
+ * seq_counts[k] used), LZMA2: one packed u32 per input byte; *seq_counts
+ * (Zstd, host memory, n_sub entries; NULL for LZMA2). */
+int gc_server_result_dev(gc_server* s, uint64_t id, void** d_results, const uint32_t** seq_counts);
 int gc_server_release(gc_server* s, uint64_t id);
 int gc_server_get_stats(gc_server* s, gc_server_stats* out);
 /* Finishes every submitted request, then stops. */

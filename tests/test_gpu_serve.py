@@ -215,3 +215,97 @@ def test_concurrent_clients(gctx):
         assert srv.stats()["completed"] == 6 * 150
     finally:
         srv.stop()
+
+
+def _dev_result(gctx, srv, rid, n, algo):
+    """A device-result request's results copied to the host: (rows as ZSTD_Sequence, counts) or packed."""
+    info = srv.wait(rid)
+    nsub = -(-n // SUB)
+    p, counts = srv.result_dev(rid, nsub)
+    if algo == gcompress.ALGO_LZMA2:
+        out = np.empty(n, np.uint32)
+        gctx.dtoh(out, p)
+        return out, info
+    slots = np.empty((nsub, gcompress.MAX_SEQ, 4), np.uint32)
+    gctx.dtoh(slots, p)
+    rows = np.concatenate([slots[k, : counts[k]] for k in range(nsub)] + [np.zeros((0, 4), np.uint32)])
+    assert info.n_sequences == rows.shape[0]
+    return (rows, counts), info
+
+
+def _check_dev(gctx, srv, rid, data, algo, level):
+    got, info = _dev_result(gctx, srv, rid, data.size, algo)
+    if algo == gcompress.ALGO_LZMA2:
+        np.testing.assert_array_equal(got, oracle.lzma2_packed(data))
+    else:
+        rows, counts = got
+        oseq, osc, _, _ = oracle.zstd_match_find(data, depth={1: 16, 2: 64}[level])
+        np.testing.assert_array_equal(counts, osc)
+        want = oracle.zstd_as_zstd_sequences(oseq, osc)
+        np.testing.assert_array_equal(rows, np.concatenate(want) if want else np.zeros((0, 4), np.uint32))
+    assert info.t_submit <= info.t_admit <= info.t_start <= info.t_done
+    srv.release(rid)
+
+
+def test_device_resident_requests(gctx):
+    """Requests whose input is already in HBM (windows of one device buffer, read in place), results left
+    on the device: alone and in a mix, equal to the oracle; their memory is held until released."""
+    pool = gcompress.corpus(8 << 20, 91, [1, 1, 1, 1], 65536, False)
+    d = gctx.alloc(pool.size + 64)
+    gctx.htod(d, pool)
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(24):
+        n = int(np.exp(rng.uniform(np.log(1024), np.log(3 << 20))))
+        o = int(rng.integers(0, pool.size - n)) & ~3
+        reqs.append((o, n) + [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)][i % 3])
+    reqs.append((pool.size - 5000, 5000, gcompress.ALGO_ZSTD, 1))  # ends where the buffer ends
+    reqs.append((pool.size - 70000, 70000, gcompress.ALGO_LZMA2, 1))
+    srv = gcompress.Server(gctx, 2 << 30)
+    try:
+        print("[serve] device-resident alone", flush=True)
+        for o, n, algo, level in reqs[:4] + reqs[-2:]:
+            rid = srv.submit_dev(d + o, n, algo, level)
+            _check_dev(gctx, srv, rid, pool[o:o + n], algo, level)
+        print("[serve] device-resident mix", flush=True)
+        ids = [srv.submit_dev(d + o, n, algo, level) for o, n, algo, level in reqs]
+        for rid, (o, n, algo, level) in zip(ids, reqs):
+            _check_dev(gctx, srv, rid, pool[o:o + n], algo, level)
+        assert srv.stats()["hbm_in_use"] == 0
+        with pytest.raises(gcompress.GcError, match="aligned"):
+            srv.submit_dev(d + 2, 100, gcompress.ALGO_ZSTD, 1)
+    finally:
+        srv.stop()
+        gctx.free(d)
+
+
+def test_ticks_overlap(gctx):
+    """Continuous batching: a request that arrives while a long tick runs starts in the next tick slot
+    beside it (after that tick started, before it ended), and its result is the same as alone."""
+    import time
+
+    pool = gcompress.corpus(32 << 20, 92, [1, 1, 1, 1], 65536, False)
+    d = gctx.alloc(pool.size + 64)
+    gctx.htod(d, pool)
+    srv = gcompress.Server(gctx, 64 << 30)
+    try:
+        print("[serve] overlapping ticks", flush=True)
+        big = [srv.submit_dev(d, pool.size, gcompress.ALGO_ZSTD, 1) for _ in range(96)]
+        time.sleep(0.005)
+        small_data = pool[:50000]
+        small = srv.submit_dev(d, small_data.size, gcompress.ALGO_LZMA2, 1)
+        s_info = srv.wait(small)
+        infos = [srv.wait(r) for r in big]
+        first = min(i.t_start for i in infos)
+        last_done = max(i.t_done for i in infos)
+        print(f"[serve] big ticks {first:.4f}..{last_done:.4f}, small start {s_info.t_start:.4f} "
+              f"done {s_info.t_done:.4f}", flush=True)
+        assert first < s_info.t_start < last_done
+        _check_dev(gctx, srv, small, small_data, gcompress.ALGO_LZMA2, 1)
+        _check_dev(gctx, srv, big[0], pool, gcompress.ALGO_ZSTD, 1)
+        for r in big[1:]:
+            srv.release(r)
+        assert srv.stats()["hbm_in_use"] == 0
+    finally:
+        srv.stop()
+        gctx.free(d)
