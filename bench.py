@@ -1,18 +1,29 @@
-"""Match-find throughput on one MI355X: batched (one launch per kernel over all chunks) vs plain per-chunk.
+"""Match-find throughput and latency on one MI355X: the served and batched forms against the plain
+per-chunk GPU form, at identical results.
 
-Default workload: the four-kind synthetic mix at 1 GiB in 16 MiB chunks, match-found at Zstd L1
-(lazy, depth 16) and L2 (depth 64) on one stream.  A step is one pass over the whole GiB at both
-levels (2 GiB of match-find input).  Inputs are resident in HBM before the timed region.
+Default (the driver's line): the serving workload.  Requests arrive open-loop (Poisson at 500, 2000 and
+8000 req/s, then all at once for the saturation passes that `value` is measured on), sizes log-uniform
+1 KiB-64 MiB, content from the four-kind mix (a 1 GiB pool), 50% Zstd L1 / 25% Zstd L2 / 25% LZMA2 L1,
+under a 64 GiB HBM budget.  Inputs are resident in HBM before the timed region (each request is a window
+of the pool, read in place) and results stay in HBM until the client releases them; latency runs from
+each request's scheduled arrival.  A step is one saturation pass over the same request set (K timed after
+W warm-up).  The plain form is one synchronous call per request on the same device-resident inputs.
+Every served result of a verification pass is compared with the plain form's, and a sample with the CPU
+oracle.  The batched configurations (1 GiB Zstd L1+L2 in 16 MiB chunks; 640 MiB LZMA2 in 128 MiB chunks;
+4096 inputs, persistent device-queue form replayed from a hipGraph) are extra keys, each with its own
+roofline.
 
-    python bench.py [--steps K] [--warmup W] [--size-mib 1024] [--workload zstd|lzma2]
+    python bench.py [--steps K] [--warmup W] [--workload serve|zstd|lzma2|mixed] [--no-extras] [--no-cpu]
 
-Prints one JSON line (see the keys below); --gpus N > 1 exits with a message: this path runs on one GPU.
+Prints one JSON line; --gpus N > 1 exits with a message: this path runs on one GPU.
 """
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -21,11 +32,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "gpu-compressor_amd"))
 
 import gcompress  # noqa: E402
+from gcompress import workloads as W  # noqa: E402
 
-KiB = 1 << 10
-MiB = 1 << 20
-GiB = 1 << 30
+KiB, MiB, GiB = 1 << 10, 1 << 20, 1 << 30
 HBM_PEAK_GBS = 8000.0
+MAX_SEQ = 65536 // 3
+METRIC = "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency"
+ZSTD, LZMA2 = gcompress.ALGO_ZSTD, gcompress.ALGO_LZMA2
+LEVEL_DEPTH = {1: 16, 2: 64}
 
 
 class Hip:
@@ -37,6 +51,7 @@ class Hip:
         self.L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
         self.L.hipEventSynchronize.argtypes = [C.c_void_p]
         self.L.hipEventElapsedTime.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        self.L.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
 
     def event(self):
         e = C.c_void_p()
@@ -53,8 +68,16 @@ class Hip:
         return ms.value
 
 
+def host_cores():
+    return {"os_cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def threads_all():
+    return len(os.sched_getaffinity(0))
+
+
 def hbm_traffic_per_input_byte(workload: str, kernel: str):
-    """HBM bytes per input byte of one launch of `kernel`, from the committed rocprofv3 summary
+    """HBM bytes per input byte of one launch of `kernel`, from the committed rocprofv3 counter summary
     profiles/<workload>_hbm_traffic.md (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
     counter reads for a device copy of known size), or None when there is none."""
     path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
@@ -67,623 +90,499 @@ def hbm_traffic_per_input_byte(workload: str, kernel: str):
     return None
 
 
-def cpu_baseline(data: np.ndarray, inputs, sample_bytes: int):
-    """The oracle (CPU restatement of the reference kernels) on the first inputs up to `sample_bytes`
-    of match-find input, all host cores."""
+def digest(*arrays) -> str:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def zstd_digest_from_slots(slots, counts):
+    mask = np.arange(slots.shape[1])[None, :] < counts[:, None]
+    return digest(counts.astype(np.uint32), np.ascontiguousarray(slots[mask][:, :3]))
+
+
+def oracle_digest(x, algo, level, threads):
     sys.path.insert(0, ROOT)
     import oracle
 
-    cores = min(os.cpu_count() or 1, 16)
-    done = 0
-    picked = []
-    for (o, s, algo, lv) in inputs:
-        if done >= sample_bytes:
-            break
-        s = min(s, sample_bytes - done)
-        picked.append((o, s, algo, lv))
-        done += s
-    t = time.perf_counter()
-    for (o, s, algo, lv) in picked:
-        if algo == gcompress.ALGO_ZSTD:
-            oracle.zstd_match_find(data[o: o + s], depth={1: 16, 2: 64}.get(lv, 1), threads=cores)
-        else:
-            oracle.lzma2_match_find(data[o: o + s], threads=cores)
-    dt = time.perf_counter() - t
-    return {"value": round(done / dt / 1e9, 4), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"the first {len(picked)} of the inputs, {done // MiB} MiB of match-find input, "
-                      f"oracle on {cores} threads"}
+    if algo == LZMA2:
+        return digest(oracle.lzma2_packed(x, threads=threads))
+    seq, sc, _, _ = oracle.zstd_match_find(x, depth=LEVEL_DEPTH.get(level, 1), threads=threads)
+    mask = np.arange(seq.shape[1])[None, :] < sc[:, None]
+    r = seq[mask]
+    return digest(sc.astype(np.uint32), np.stack([r[:, 2], r[:, 0], r[:, 1] + 3], axis=1).astype(np.uint32))
 
 
-def cpu_baselines_more(data: np.ndarray, workload: str, sample_bytes: int):
-    """The other CPU baselines, on a bounded sample of the workload's data (first bytes):
-    Zstd: the oracle on 1 core, and oracle match finding (all cores) + ZSTD_compressSequences with 1 encode
-    thread (the reference's shape) and with all cores;  LZMA2: the oracle on 1 core, oracle + triage +
-    liblzma preset 9 on a pool of min(cores, 8) (the reference's encoder), and level 0 (liblzma raw encoder,
-    min(cores, 8) workers on dictionary chunks).  GB/s of input."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    sys.path.insert(0, ROOT)
-    import oracle
-
-    L = gcompress.lib()
-    cores = min(os.cpu_count() or 1, 16)
-    pool_n = min(cores, 8)
-    x = np.ascontiguousarray(data[:sample_bytes])
-    nsub = -(-x.size // 65536)
-    out = {"sample_mib": x.size // MiB, "cores": cores}
-
-    def rate(t):
-        return round(x.size / t / 1e9, 4)
-
-    if workload != "lzma2":
-        t = time.perf_counter()
-        oracle.zstd_match_find(x, depth=16, threads=1)
-        out["zstd_matchfind_1core_gbs"] = rate(time.perf_counter() - t)
-        t = time.perf_counter()
-        oseq, osc, _, _ = oracle.zstd_match_find(x, depth=16, threads=cores)
-        find_s = time.perf_counter() - t
-        z = oracle.zstd_as_zstd_sequences(oseq, osc)
-        seqs = np.ascontiguousarray(np.concatenate(z) if z else np.zeros((1, 4), np.uint32))
-        counts = np.ascontiguousarray(osc.astype(np.uint32))
-        cap = L.gc_zstd_custom_bound(x.size)
-        buf = np.empty(cap, np.uint8)
-        sizes = np.empty(nsub, np.uint64)
-        olen = C.c_size_t()
-        for threads in (1, cores):
-            t = time.perf_counter()
-            rc = L.gc_zstd_encode_sequences(x.ctypes.data, x.size, seqs.ctypes.data, counts.ctypes.data,
-                                            buf.ctypes.data, cap, sizes.ctypes.data, C.byref(olen), threads)
-            assert rc == 0, L.gc_last_error()
-            enc_s = time.perf_counter() - t
-            out[f"zstd_find_and_encode_{threads}_encode_threads_gbs"] = rate(find_s + enc_s)
-        out["zstd_ratio"] = round(x.size / olen.value, 4)
-    if workload != "zstd":
-        t = time.perf_counter()
-        oracle.lzma2_match_find(x, max_matches=1, threads=1)
-        out["lzma2_matchfind_1core_gbs"] = rate(time.perf_counter() - t)
-        t = time.perf_counter()
-        _, c = oracle.lzma2_match_find(x, max_matches=1, threads=cores)
-        counts = np.ascontiguousarray(c)
-
-        def enc(s):
-            st = s * 65536
-            ln = min(65536, x.size - st)
-            b = np.empty(2 * ln + 1024, np.uint8)
-            n_out = C.c_size_t()
-            rc = L.gc_encode_single_sub_block(x[st:].ctypes.data, ln, counts[st:].ctypes.data, b.ctypes.data,
-                                              b.size, C.byref(n_out))
-            assert rc == 0, L.gc_last_error()
-            return n_out.value
-
-        with ThreadPoolExecutor(pool_n) as ex:
-            total = sum(ex.map(enc, range(nsub)))
-        out["lzma2_find_triage_preset9_gbs"] = rate(time.perf_counter() - t)
-        out["lzma2_preset9_ratio"] = round(x.size / total, 4)
-        cap = L.gc_nvlz_l0_bound(x.size, 128)
-        buf = np.empty(cap, np.uint8)
-        olen = C.c_size_t()
-        t = time.perf_counter()
-        rc = L.gc_nvlz_compress_l0(x.ctypes.data, x.size, 128, 0, pool_n, buf.ctypes.data, cap, C.byref(olen))
-        assert rc == 0, L.gc_last_error()
-        out["lzma2_level0_gbs"] = rate(time.perf_counter() - t)
-    return out
+# ---------------------------------------------------------------------------------------------------
+# Serving workload
+# ---------------------------------------------------------------------------------------------------
+KINDS = [(ZSTD, 1), (ZSTD, 2), (LZMA2, 1)]
+KIND_P = [0.5, 0.25, 0.25]
 
 
-def equal_ratio_check(ctx, batch, data, inputs, d_in, sample_bytes):
-    """Batched results against the plain per-chunk form on a sample of inputs (first inputs of each
-    algorithm/level, up to sample_bytes each); the Zstd ratio from the GPU's own sequences through
-    ZSTD_compressSequences. Identical sequences and matches give identical compressed output."""
-    L = gcompress.lib()
-    max_seq = 65536 // 3
-    picked, seen = [], {}
-    for i, (o, s, algo, lv) in enumerate(inputs):
-        key = (algo, lv)
-        if seen.get(key, 0) < sample_bytes:
-            seen[key] = seen.get(key, 0) + s
-            picked.append(i)
-    cmp_bytes = 0
-    zin = zout = 0
-    identical = True
-    for i in picked:
-        o, s, algo, lv = inputs[i]
-        ns = -(-s // 65536)
-        if algo == gcompress.ALGO_ZSTD:
-            d_s, d_c = ctx.alloc(ns * max_seq * 16), ctx.alloc(ns * 4)
-            rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_s, 0, None, d_c, None, 65536, ns, s,
-                                          {1: 16, 2: 64}.get(lv, 1), max_seq)
-            assert rc == 0, L.gc_last_error()
-            ps = np.empty((ns, max_seq, 4), np.uint32)
-            pc = np.empty(ns, np.uint32)
-            ctx.dtoh(ps, d_s)
-            ctx.dtoh(pc, d_c)
-            ctx.free(d_s)
-            ctx.free(d_c)
-            bs, bc = batch.zstd(i)
-            same = np.array_equal(pc, bc) and all(np.array_equal(ps[b, :pc[b]], bs[b, :bc[b]]) for b in range(ns))
-            seqs = np.concatenate([bs[b, :bc[b]] for b in range(ns)] + [np.zeros((1, 4), np.uint32)])
-            _, sizes = gcompress.zstd_encode_sequences(data[o:o + s], seqs, bc, threads=min(os.cpu_count() or 1, 16))
-            zin += s
-            zout += sum(sizes)
-        else:
-            d_p = ctx.alloc(4 * s + 64)
-            rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_p, 65536, s, 32)
-            assert rc == 0, L.gc_last_error()
-            pp = np.empty(s, np.uint32)
-            ctx.dtoh(pp, d_p)
-            ctx.free(d_p)
-            same = np.array_equal(pp, batch.lzma2(i))
-        identical &= bool(same)
-        cmp_bytes += s
-    assert identical, "batched results differ from the plain per-chunk form"
-    out = {"inputs_compared": len(picked), "bytes_compared": cmp_bytes, "identical_to_plain": identical}
-    if zin:
-        out["zstd_ratio_from_gpu_sequences"] = round(zin / zout, 4)
-    return out
+def draw_requests(rng, nreq, rate, pool_size):
+    """(scheduled arrival offsets in s, [(offset, size, algo, level)]): Poisson arrivals (rate 0: all at
+    once), sizes log-uniform 1 KiB-64 MiB, windows of the pool at 4-byte aligned offsets."""
+    gaps = rng.exponential(1.0 / rate, nreq) if rate else np.zeros(nreq)
+    sizes = np.exp(rng.uniform(np.log(KiB), np.log(64 * MiB), nreq)).astype(np.int64)
+    offs = (rng.integers(0, pool_size - sizes + 1) // 4) * 4
+    kinds = rng.choice(3, nreq, p=KIND_P)
+    reqs = [(int(offs[i]), int(sizes[i]), *KINDS[kinds[i]]) for i in range(nreq)]
+    return np.cumsum(gaps), reqs
 
 
-def encode_from_gpu_matches(batch, data, inputs, sample_bytes):
-    """The LZMA2 consumer of the GPU's match lists (gc_cpu_encode_from_matches: one verified stream per
-    sub-block, min(cores, 8) threads) on the first LZMA2 input's first sample_bytes, with the batched
-    run's own packed matches; its ratio beside liblzma preset 9's on the first 4 MiB of the same bytes."""
-    L = gcompress.lib()
-    i = next(k for k, (_, _, al, _) in enumerate(inputs) if al == gcompress.ALGO_LZMA2)
-    o, s, _, _ = inputs[i]
-    m = min(s, sample_bytes)
-    x = np.ascontiguousarray(data[o:o + m])
-    pk = np.ascontiguousarray(batch.lzma2(i)[:m])
-    threads = min(os.cpu_count() or 1, 8)
-    nsub = -(-m // 65536)
-    cap = nsub * L.gc_lzma2_encode_bound(65536)
-    out = np.empty(cap, np.uint8)
-    sizes = np.zeros(nsub, np.uint64)
-    olen = C.c_size_t()
-    t = time.perf_counter()
-    rc = L.gc_cpu_encode_from_matches(x.ctypes.data, m, pk.ctypes.data, threads, out.ctypes.data, cap,
-                                      sizes.ctypes.data, C.byref(olen))
-    dt = time.perf_counter() - t
-    assert rc == 0, L.gc_last_error()
-    k = min(m, 4 * MiB)
-    p9 = 0
-    for st in range(0, k, 65536):
-        sub = x[st:st + 65536]
-        p9 += len(gcompress.encode_single_sub_block(sub, (pk[st:st + sub.size] != 0).astype(np.uint32)))
-    ours_k = int(sizes[: -(-k // 65536)].sum())
-    return {"sample_mib": m // MiB, "threads": threads, "gbs": round(m / dt / 1e9, 4),
-            "ratio": round(m / olen.value, 4), "ratio_first_4mib": round(k / ours_k, 4),
-            "preset9_ratio_first_4mib": round(k / p9, 4)}
+def serve_run(srv, d_pool, arrivals, reqs, fetch=None):
+    """One open-loop run: a producer submits at the scheduled times, a consumer waits in order, records
+    completion and releases (after fetch(i, rid) when given).  Latency from the scheduled arrival."""
+    n = len(reqs)
+    ids = [None] * n
+    sched = np.zeros(n)
+    done = np.zeros(n)
+    start_t = np.zeros(n)
+    failed = []
+    ready = threading.Semaphore(0)
+    t0 = time.monotonic() + 0.02
+
+    def producer():
+        for i, (o, s, algo, lv) in enumerate(reqs):
+            t = t0 + arrivals[i]
+            sched[i] = t
+            d = t - time.monotonic()
+            if d > 0:
+                time.sleep(d)
+            ids[i] = srv.submit_dev(d_pool + o, s, algo, lv)
+            ready.release()
+
+    def consumer():
+        for i in range(n):
+            ready.acquire()
+            try:
+                info = srv.wait(ids[i])
+                done[i] = info.t_done
+                start_t[i] = info.t_start
+                if fetch:
+                    fetch(i, ids[i])
+            except gcompress.GcError as e:
+                failed.append((i, str(e)))
+                done[i] = time.monotonic()
+            srv.release(ids[i])
+
+    tp, tc = threading.Thread(target=producer), threading.Thread(target=consumer)
+    tp.start()
+    tc.start()
+    tp.join()
+    tc.join()
+    return {"sched": sched, "done": done, "start": start_t, "t0": t0, "failed": failed}
 
 
-def gpu_frames(ctx, hip, stream, d_in, data, inputs, sample_bytes):
-    """Zstd frames coded on the GPU (gc_zstd_encode_block_dev) for the first Zstd L1 input's bytes, up to
-    sample_bytes from its start, as one buffer of 64 KiB sub-chunks: the match finder with literals first
-    (untimed), then the coding kernel timed with HIP events (3 runs).  Ratio of the frames beside
-    libzstd's (ZSTD_compressSequences on the same sequences, first 16 MiB)."""
-    L = gcompress.lib()
-    o, s, _, _ = next(x for x in inputs if x[2] == gcompress.ALGO_ZSTD)
-    n = min(sample_bytes, data.size - o)
+def served_digest(ctx, srv, rid, n, algo):
     nsub = -(-n // 65536)
-    max_seq = 65536 // 3
-    slot = 65536 + 16
-    d_s, d_l = ctx.alloc(nsub * max_seq * 16), ctx.alloc(nsub * 65536)
-    d_sc, d_lc = ctx.alloc(nsub * 4), ctx.alloc(nsub * 4)
-    d_o, d_os = ctx.alloc(nsub * slot), ctx.alloc(nsub * 4)
-    rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_s, 0, d_l, d_sc, d_lc, 65536, nsub, n, 16, max_seq)
-    assert rc == 0, L.gc_last_error()
-    ctx.sync()
-    ms = []
-    for _ in range(3):
-        e0, e1 = hip.event(), hip.event()
-        hip.record(e0, stream)
-        rc = L.gc_zstd_encode_block_dev(ctx.h, d_in + o, d_s, 0, d_l, d_sc, d_lc, d_o, d_os, 65536, nsub, n, max_seq,
-                                        slot)
-        assert rc == 0, L.gc_last_error()
-        hip.record(e1, stream)
-        ms.append(hip.elapsed_ms(e0, e1))
-    sizes = np.zeros(nsub, np.uint32)
-    ctx.dtoh(sizes, d_os)
-    k = min(nsub, 256)
-    seqs = np.zeros((k, max_seq, 4), np.uint32)
-    counts = np.zeros(k, np.uint32)
-    ctx.dtoh(seqs, d_s)
-    ctx.dtoh(counts, d_sc)
-    for p_ in (d_s, d_l, d_sc, d_lc, d_o, d_os):
-        ctx.free(p_)
-    x = np.ascontiguousarray(data[o:o + min(n, k * 65536)])
-    rows = np.concatenate([seqs[b, :counts[b]] for b in range(k)] + [np.zeros((1, 4), np.uint32)])
-    _, zs = gcompress.zstd_encode_sequences(x, rows, counts, threads=min(os.cpu_count() or 1, 16))
-    kms = float(np.median(ms))
-    return {"sample_mib": n // MiB, "kernel_ms": round(kms, 3), "encode_gbs": round(n / (kms / 1e3) / 1e9, 2),
-            "ratio": round(n / int(sizes.sum()), 4), "ratio_first_16mib": round(x.size / int(sizes[:k].sum()), 4),
-            "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4)}
+    p, counts = srv.result_dev(rid, nsub)
+    if algo == LZMA2:
+        out = np.empty(n, np.uint32)
+        ctx.dtoh(out, p)
+        return digest(out)
+    slots = np.empty((nsub, MAX_SEQ, 4), np.uint32)
+    ctx.dtoh(slots, p)
+    return zstd_digest_from_slots(slots, counts)
 
 
-def workload_inputs(a):
-    """(host arena, inputs [(offset, size, algo, level)], description) of the chosen workload."""
-    if a.workload == "zstd":
-        # 1 GiB four-kind mix in 16 MiB chunks, every chunk at Zstd L1 and L2
-        n, chunk = a.size_mib * MiB, a.chunk_mib * MiB
-        data = gcompress.corpus(n, 2, (1, 1, 1, 1), 65536, True)
-        chunks = [(o, min(chunk, n - o)) for o in range(0, n, chunk)]
-        inputs = [(o, s, gcompress.ALGO_ZSTD, lv) for lv in (1, 2) for (o, s) in chunks]
-        desc = f"drop-in Zstd L1+L2 match-find, {a.size_mib} MiB mix in {a.chunk_mib} MiB chunks"
-    elif a.workload == "lzma2":
-        # disk-image-like mix: 40% structured, 25% text, 20% back-references, 15% random, 1 MiB
-        # segments, 8 MiB zero gaps every 96 MiB; dictionary-size chunks
-        n, chunk = a.size_mib * MiB, a.chunk_mib * MiB
-        data = gcompress.corpus(n, 3, (15, 25, 20, 40), MiB, True, 96 * MiB, 8 * MiB)
-        inputs = [(o, min(chunk, n - o), gcompress.ALGO_LZMA2, 1) for o in range(0, n, chunk)]
-        desc = f"LZMA2 L1 match-find, {a.size_mib} MiB disk-image-like mix in {a.chunk_mib} MiB chunks"
-    else:
-        # many inputs: sizes log-uniform in [4 KiB, 32 MiB], per-input Dirichlet(1,1,1,1) kind weights,
-        # a third each at Zstd L1, Zstd L2 and LZMA2
-        rng = np.random.default_rng(4)
-        sizes = np.exp(rng.uniform(np.log(4 * KiB), np.log(32 * MiB), a.inputs)).astype(np.int64)
-        weights = rng.dirichlet([1, 1, 1, 1], a.inputs)
-        kinds = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)]
-        offs = np.concatenate([[0], np.cumsum((sizes + 255) // 256 * 256)])
-        data = np.empty(int(offs[-1]), np.uint8)
-        inputs = []
-        for i in range(a.inputs):
-            o, s = int(offs[i]), int(sizes[i])
-            data[o: o + s] = gcompress.corpus(s, 1000 + i, weights[i], 65536, False)
-            algo, lv = kinds[i % 3]
-            inputs.append((o, s, algo, lv))
-        desc = (f"{a.inputs} inputs, sizes log-uniform 4 KiB-32 MiB ({int(sizes.sum()) / GiB:.2f} GiB in all), "
-                f"Dirichlet kind mix per input, a third each Zstd L1 / Zstd L2 / LZMA2")
-    return data, inputs, desc
+class Plain:
+    """The plain per-request GPU form: one synchronous kernel-ABI call per request on its device-resident
+    window (gc_zstd_match_find_dev / gc_lzma2_match_find_packed_dev), results into device scratch."""
+
+    def __init__(self, ctx, max_size):
+        self.ctx, self.L = ctx, gcompress.lib()
+        nsub = -(-max_size // 65536)
+        self.d_seq = ctx.alloc(nsub * MAX_SEQ * 16)
+        self.d_cnt = ctx.alloc(nsub * 4)
+        self.d_pk = ctx.alloc(max_size * 4 + 64)
+
+    def run(self, d_in, s, algo, lv):
+        if algo == ZSTD:
+            rc = self.L.gc_zstd_match_find_dev(self.ctx.h, d_in, self.d_seq, 0, None, self.d_cnt, None, 65536,
+                                               -(-s // 65536), s, LEVEL_DEPTH.get(lv, 1), MAX_SEQ)
+        else:
+            rc = self.L.gc_lzma2_match_find_packed_dev(self.ctx.h, d_in, self.d_pk, 65536, s, 32)
+        assert rc == 0, self.L.gc_last_error()
+        self.ctx.sync()
+
+    def digest(self, s, algo):
+        if algo == LZMA2:
+            out = np.empty(s, np.uint32)
+            self.ctx.dtoh(out, self.d_pk)
+            return digest(out)
+        nsub = -(-s // 65536)
+        counts = np.empty(nsub, np.uint32)
+        self.ctx.dtoh(counts, self.d_cnt)
+        slots = np.empty((nsub, MAX_SEQ, 4), np.uint32)
+        self.ctx.dtoh(slots, self.d_seq)
+        return zstd_digest_from_slots(slots, counts)
+
+    def close(self):
+        for p in (self.d_seq, self.d_cnt, self.d_pk):
+            self.ctx.free(p)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size-mib", type=int, default=None, help="default: 1024 (zstd), 640 (lzma2)")
-    ap.add_argument("--chunk-mib", type=int, default=None, help="default: 16 (zstd), 128 (lzma2)")
-    ap.add_argument("--workload", choices=["zstd", "lzma2", "mixed", "serve"], default="zstd")
-    ap.add_argument("--inputs", type=int, default=4096, help="mixed workload: number of inputs")
-    ap.add_argument("--rates", default="500,2000,8000")
-    ap.add_argument("--requests", type=int, default=2000, help="requests per serving rate")
-    ap.add_argument("--budget-gib", type=int, default=64)
-    ap.add_argument("--ring-gib", type=int, default=12, help="pinned result memory for the serving run")
-    ap.add_argument("--plain-requests", type=int, default=300)
-    ap.add_argument("--plain-steps", type=int, default=1)
-    ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--cpu-sample-mib", type=int, default=1024)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--check-sample-mib", type=int, default=32,
-                    help="bytes per algorithm/level compared against the plain form and encoded for the ratio")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="one device-to-device copy of this size first, to scale HBM byte counters")
-    a = ap.parse_args()
-    if a.size_mib is None:
-        a.size_mib = 640 if a.workload == "lzma2" else 1024
-    if a.chunk_mib is None:
-        a.chunk_mib = 128 if a.workload == "lzma2" else 16
-    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
-              "(sub-blocks shard by range or by request with no exchange between GPUs)", file=sys.stderr)
-        sys.exit(2)
+def lat_summary(r, reqs):
+    lat = (r["done"] - r["sched"]) * 1e3
+    nbytes = float(sum(s for (_, s, _, _) in reqs))
+    span = r["done"].max() - r["t0"]
+    arrive = r["sched"][-1] - r["t0"]
+    return {"requests": len(reqs), "failed": len(r["failed"]), "input_bytes": int(nbytes),
+            "offered_gbs": round(nbytes / arrive / 1e9, 2) if arrive > 1e-3 else None,  # None: all at once
+            "sustained_gbs": round(nbytes / span / 1e9, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3)}
 
-    hip = Hip()
-    ctx = gcompress.context(0)
-    if a.workload == "serve":
-        return bench_serve(a, hip, ctx)
+
+def kernel_roofline(stats0, stats1, wall_s):
+    """Top-level roofline of the served form: the dominant kernel (most device time over the ticks) with
+    the algorithmic bytes it is responsible for (Zstd parse: N_in + 16 n_seq + 4 n_sub; LZMA2 lengths:
+    5 N_in; candidates: N_in; Zstd info: N_in), over its average launch (HIP events around every launch
+    of it in the served runs)."""
+    names = ["gc_tick_setup", "gc_cand_sq_kernel", "gc_lzma2_len_sq_kernel", "gc_zstd_info_sq_kernel",
+             "gc_order_items", "gc_zstd_parse_sq_kernel"]
+    d = {k: (np.array(stats1[k]) - np.array(stats0[k])) if isinstance(stats1[k], list) else stats1[k] - stats0[k]
+         for k in ("t_kernel", "launches", "zstd_bytes", "zstd_sub_blocks", "zstd_sequences", "lzma2_bytes")}
+    zb, lb = int(d["zstd_bytes"]), int(d["lzma2_bytes"])
+    share = {1: zb + lb, 2: 5 * lb, 3: zb, 5: zb + 16 * int(d["zstd_sequences"]) + 4 * int(d["zstd_sub_blocks"])}
+    inputs_of = {1: zb + lb, 2: lb, 3: zb, 5: zb}  # the input bytes each kernel reads
+    dom = max(share, key=lambda k: d["t_kernel"][k])
+    launches = int(d["launches"][dom])
+    avg_s = float(d["t_kernel"][dom]) / max(launches, 1)
+    per_launch = share[dom] / max(launches, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    per_byte = hbm_traffic_per_input_byte("serve", names[dom])
+    return {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": int(per_byte * inputs_of[dom] / max(launches, 1)) if per_byte is not None else None,
+            "algorithmic_bytes": int(per_launch), "launches": launches, "avg_launch_ms": round(avg_s * 1e3, 4),
+            "kernel_s": {names[k]: round(float(d["t_kernel"][k]), 4) for k in range(6)},
+            "kernel_launches": {names[k]: int(d["launches"][k]) for k in range(6)},
+            # every served kernel together: algorithmic bytes of the timed passes over their wall time
+            "path_algorithmic_gbs": round((share[1] + share[2] + share[5]) / max(wall_s, 1e-9) / 1e9, 2),
+            "device_busy_frac_of_wall": round(float(sum(d["t_kernel"])) / max(wall_s, 1e-9), 3)}
+
+
+def cpu_baseline_serve(pool, reqs, budget_s):
+    """The oracle (CPU restatement of the reference kernels) on the serving stream's first requests, all
+    host cores, until about budget_s of work; then one core on a smaller share.  Returns the baseline and
+    the oracle digests of the requests it covered."""
+    t_all = threads_all()
+    digs = {}
+    done = 0
+    t = time.perf_counter()
+    for i, (o, s, algo, lv) in enumerate(reqs):
+        digs[i] = oracle_digest(pool[o:o + s], algo, lv, t_all)
+        done += s
+        if time.perf_counter() - t > budget_s:
+            break
+    dt = time.perf_counter() - t
+    one, one_b = 0.0, 0
+    t = time.perf_counter()
+    for (o, s, algo, lv) in reqs:
+        if one_b + s > 64 * MiB and one_b:
+            continue
+        oracle_digest(pool[o:o + s], algo, lv, 1)
+        one_b += s
+        if time.perf_counter() - t > budget_s / 4:
+            break
+    one = one_b / (time.perf_counter() - t) / 1e9
+    base = {"value": round(done / dt / 1e9, 4), "unit": "GB/s", "cores": t_all, "kind": "port",
+            "sample": f"the serving stream's first {len(digs)} requests ({done / 1e9:.2f} GB of match-find input, "
+                      f"the same request mix), oracle on {t_all} threads",
+            "one_core_gbs": round(one, 4), "one_core_sample_bytes": int(one_b), "host": host_cores()}
+    return base, digs
+
+
+def bench_serve(a, hip, ctx, extras):
     L = gcompress.lib()
-    stream = L.gc_stream(ctx.h)
-    t0 = time.perf_counter()
-    data, inputs, desc = workload_inputs(a)
-    gen_s = time.perf_counter() - t0
-    n = data.size
-    d_in = ctx.alloc(n + 256)
-    ctx.htod(d_in, data)
-    if a.calib_copy_mib:
-        # a plain copy of known size (the runtime's copy kernel): FETCH_SIZE / WRITE_SIZE read for it
-        # against calib_copy_mib MiB give the counters' scale
-        cb = a.calib_copy_mib * MiB
-        c_src, c_dst = ctx.alloc(cb), ctx.alloc(cb)
-        hip.L.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
-        assert hip.L.hipMemcpyAsync(c_dst, c_src, cb, 3, stream) == 0
-        ctx.sync()
-        ctx.free(c_src)
-        ctx.free(c_dst)
-    batch = gcompress.Batch(ctx, d_in, n + 256, inputs)  # the allocation is readable to its end
-    work_bytes = sum(s for (_, s, _, _) in inputs)
-    zstd_bytes = sum(s for (_, s, al, _) in inputs if al == gcompress.ALGO_ZSTD)
-    lzma_bytes = work_bytes - zstd_bytes
+    print("[bench] serving: pool", file=sys.stderr, flush=True)
+    pool = W.config5_pool()
+    d_pool = ctx.alloc(pool.size + 256)
+    ctx.htod(d_pool, pool)
+    rng = np.random.default_rng(5)
+    budget = a.budget_gib << 30
+    # the saturation request set (all arrive at once); the rates' own draws
+    _, sat = draw_requests(rng, a.sat_requests, 0, pool.size)
+    counts = {500: 1500, 2000: 4000, 8000: 8000}  # about 3, 2 and 1 s of arrivals
+    rate_sets = {r: draw_requests(rng, counts.get(r, 4000), r, pool.size)
+                 for r in [int(x) for x in a.rates.split(",") if x]}
+    sat_bytes = sum(s for (_, s, _, _) in sat)
 
-    # ---- plain form: one launch sequence per input (chunk), synchronised per input ----
-    max_seq = 65536 // 3
-    big = max(s for (_, s, _, _) in inputs)
-    nsub_big = -(-big // 65536)
-    d_seq = ctx.alloc(nsub_big * max_seq * 16) if zstd_bytes else None
-    d_cnt = ctx.alloc(nsub_big * 4) if zstd_bytes else None
-    d_pk = ctx.alloc(big * 4) if lzma_bytes else None
+    # plain per-request form on the first requests of the saturation set
+    plain = Plain(ctx, 64 * MiB)
+    k = min(a.plain_requests, len(sat))
+    for (o, s, algo, lv) in sat[:8]:
+        plain.run(d_pool + o, s, algo, lv)  # warm-up
+    t = time.perf_counter()
+    for (o, s, algo, lv) in sat[:k]:
+        plain.run(d_pool + o, s, algo, lv)
+    plain_s = time.perf_counter() - t
+    plain_bytes = sum(s for (_, s, _, _) in sat[:k])
+    plain_gbs = plain_bytes / plain_s / 1e9
+    print(f"[bench] plain per request: {plain_gbs:.3f} GB/s over {k} requests", file=sys.stderr, flush=True)
 
-    def plain_pass():
-        for (o, s, algo, lv) in inputs:
-            if algo == gcompress.ALGO_ZSTD:
-                rc = L.gc_zstd_match_find_dev(ctx.h, d_in + o, d_seq, 0, None, d_cnt, None, 65536,
-                                              -(-s // 65536), s, {1: 16, 2: 64}.get(lv, 1), max_seq)
-            else:
-                rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in + o, d_pk, 65536, s, 32)
-            assert rc == 0, L.gc_last_error()
-            ctx.sync()
+    srv = gcompress.Server(ctx, budget)
+    # verification pass (untimed): every result of the first V requests against the plain form's
+    v = min(a.verify_requests, len(sat))
+    want = []
+    for (o, s, algo, lv) in sat[:v]:
+        plain.run(d_pool + o, s, algo, lv)
+        want.append(plain.digest(s, algo))
+    got = [None] * v
+    serve_run(srv, d_pool, np.zeros(v), sat[:v],
+              fetch=lambda i, rid: got.__setitem__(i, served_digest(ctx, srv, rid, sat[i][1], sat[i][2])))
+    plain.close()
+    mism = [i for i in range(v) if got[i] != want[i]]
+    verify = {"requests": v, "bytes": int(sum(s for (_, s, _, _) in sat[:v])), "identical_to_plain": not mism}
+    assert not mism, f"served results differ from the plain form for requests {mism[:8]}"
 
-    plain_gbs = None
-    if a.plain_steps > 0:
-        plain_pass()  # warm-up
-        t = time.perf_counter()
-        for _ in range(a.plain_steps):
-            plain_pass()
-        plain_s = (time.perf_counter() - t) / a.plain_steps
-        plain_gbs = work_bytes / plain_s / 1e9
+    # latency at the stated rates
+    rates = {}
+    for rate, (arr, reqs) in rate_sets.items():
+        print(f"[bench] serving at {rate} req/s", file=sys.stderr, flush=True)
+        r = serve_run(srv, d_pool, arr, reqs)
+        rates[str(rate)] = lat_summary(r, reqs)
+        print(f"[bench]   {json.dumps(rates[str(rate)])}", file=sys.stderr, flush=True)
 
-    # ---- batched form ----
-    tune = batch.tune()
-    if not a.no_autotune:
-        tune, _ = batch.autotune(reps=1)
+    # saturation passes: W warm-up, K timed (events on the host clock: the passes span three streams)
     for _ in range(a.warmup):
-        batch.run()
-    ctx.sync()
-    e0, e1 = hip.event(), hip.event()
+        serve_run(srv, d_pool, np.zeros(len(sat)), sat)
+    st0 = srv.stats()
     ctx.sync()
     t = time.perf_counter()
-    hip.record(e0, stream)
-    for _ in range(a.steps):
-        batch.run()
-    hip.record(e1, stream)
+    runs = [serve_run(srv, d_pool, np.zeros(len(sat)), sat) for _ in range(a.steps)]
     ctx.sync()
-    wall_s = time.perf_counter() - t
-    ev_ms = hip.elapsed_ms(e0, e1)
-    ms_step = ev_ms / a.steps
-    value = work_bytes / (ms_step / 1e3) / 1e9
+    wall = time.perf_counter() - t
+    st1 = srv.stats()
+    value = sat_bytes * a.steps / wall / 1e9
+    sat_lat = lat_summary(runs[-1], sat)
+    roof = kernel_roofline(st0, st1, wall)
+    srv.stop()
 
-    # per-kernel durations (kernel-by-kernel launch with events on the library stream)
-    kt = [batch.run_timed() for _ in range(3)]
-    k_ms = [float(np.mean([k[i] for k in kt])) for i in range(4)]  # cand, lzma2 len, zstd info, zstd parse
-    knames = ["gc_cand_kernel", "gc_lzma2_len_kernel", "gc_zstd_info_kernel", "gc_zstd_parse_kernel"]
+    cpu, odig = (None, {}) if a.no_cpu else cpu_baseline_serve(pool, sat, a.cpu_budget_s)
+    parity = None
+    if odig:
+        common = [i for i in odig if i < v]
+        parity = {"requests": len(common), "bytes": int(sum(sat[i][1] for i in common)),
+                  "identical": all(odig[i] == want[i] for i in common)}
+        assert parity["identical"], "served/plain results differ from the oracle"
+    ctx.free(d_pool)
 
-    # algorithmic bytes and results summary (outside the timed region)
-    n_sub = 0
-    n_seq = 0
-    for i, (_, s, algo, _) in enumerate(inputs):
-        if algo == gcompress.ALGO_ZSTD:
-            _, counts = batch.zstd(i, with_seqs=False)
-            n_sub += counts.size
-            n_seq += int(counts.sum())
-        else:
-            n_sub += batch.n_sub(i)
-    zstd_sub = sum(-(-s // 65536) for (_, s, al, _) in inputs if al == gcompress.ALGO_ZSTD)
-    # Algorithmic bytes (what the path must move): Zstd N_in + 16 n_seq + 4 n_sub, LZMA2 5 N_in.
-    # Per kernel, the share it is responsible for: the candidate pass reads every input once, the
-    # Zstd info pass the Zstd inputs; the parse writes the sequences and counts, the LZMA2 length pass
-    # the 4-byte match per position.
-    zstd_alg = zstd_bytes + 16 * n_seq + 4 * zstd_sub
-    share = {0: work_bytes, 1: 5 * lzma_bytes, 2: zstd_bytes, 3: zstd_alg}
-    dom = max((i for i in share if share[i]), key=lambda i: k_ms[i])
-    dom_name, dom_ms, dom_bytes = knames[dom], k_ms[dom], share[dom]
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    per_byte = hbm_traffic_per_input_byte(a.workload, dom_name)
-    traffic = int(per_byte * work_bytes) if per_byte is not None else None
-
-    equal = equal_ratio_check(ctx, batch, data, inputs, d_in, a.check_sample_mib * MiB)
-    enc_m = None if a.no_cpu or not lzma_bytes else encode_from_gpu_matches(batch, data, inputs, 16 * MiB)
-    zframes = None if a.no_cpu or not zstd_bytes else gpu_frames(ctx, hip, stream, d_in, data, inputs, 256 * MiB)
-    cpu = None if a.no_cpu else cpu_baseline(data, inputs, a.cpu_sample_mib * MiB)
-    cpu_more = None
-    if not a.no_cpu and a.workload in ("zstd", "lzma2"):
-        cpu_more = cpu_baselines_more(data, a.workload, (64 if a.workload == "zstd" else 16) * MiB)
-    levels = sorted({lv for (_, _, al, lv) in inputs if al == gcompress.ALGO_ZSTD})
+    sizes = np.array([s for (_, s, _, _) in sat])
     out = {
-        "metric": "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_step, 3),
+        "ms_per_step": round(wall * 1e3 / a.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded four-kind mix: random, word text, back-references, structured binary)",
+        "data": "synthetic (seeded four-kind mix: random, word text, back-references, structured binary; 1 GiB pool)",
         "config": {
-            "workload": desc,
-            "form": "batched: every sub-block of every input in one launch per kernel (grid-stride items)",
-            "inputs": len(inputs),
-            "input_bytes_per_step": work_bytes,
+            "workload": "serving: open-loop requests, sizes log-uniform 1 KiB-64 MiB, 50% Zstd L1 / 25% Zstd L2 / "
+                        "25% LZMA2 L1, 64 GiB HBM budget; inputs resident in HBM (windows of the pool), results "
+                        "left in HBM; value = saturation passes (all requests arrive at once)",
+            "form": "served: continuous batching into the persistent device-queue kernels, 3 tick slots on "
+                    "their own streams",
+            "requests_per_step": len(sat),
+            "input_bytes_per_step": int(sat_bytes),
+            "mean_request_bytes": int(sizes.mean()),
+            "hbm_budget_gib": a.budget_gib,
             "sub_block": 65536,
-            "zstd_levels": levels,
-            "lzma2_bytes": lzma_bytes,
-            "tune": {"cand_wg_per_cu": tune[0], "parse_wg_per_cu": tune[1], "elem_wg_per_cu": tune[2],
-                     "info_strip": tune[3]},
         },
-        "plain_per_chunk_gbs": round(plain_gbs, 3) if plain_gbs else None,
-        "batched_vs_plain": round(value / plain_gbs, 2) if plain_gbs else None,
-        "wall_ms_per_step": round(wall_s * 1e3 / a.steps, 3),
-        "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)},
-        "results": {"n_sub_blocks": n_sub, "n_sequences": n_seq},
-        # results moved per input byte, this layout against the reference's: Zstd, 16 B records +
-        # 4 B count per sub-block (the reference copies back all 21845 12-byte slots of every 64 KiB
-        # sub-chunk and its count, compress_zstd_custom.rs:82-84,115-116); LZMA2, one 4 B match per
-        # position (the reference: 8 slots of 8 B plus a 4 B count, compress_lzma2_custom.rs:511-512,540)
-        "output_bytes_per_input_byte": {
-            **({"zstd": {"ours": round((16 * n_seq + 4 * zstd_sub) / zstd_bytes, 4),
-                         "reference_layout": round((12 * 21845 + 4) / 65536, 4)}} if zstd_bytes else {}),
-            **({"lzma2": {"ours": 4.0, "reference_layout": 68.0}} if lzma_bytes else {}),
-        },
-        "equal_ratio": equal,
-        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                     "algorithmic_bytes": dom_bytes},
-        "cpu_baseline": cpu,
-        "cpu_baselines_more": cpu_more,
-        "lzma2_encode_from_gpu_matches": enc_m,
-        "zstd_gpu_frames": zframes,
-        "corpus_gen_s": round(gen_s, 2),
-    }
-    print(json.dumps(out), flush=True)
-    batch.close()
-
-
-def _pinned(hip, nbytes):
-    ptr = C.c_void_p()
-    hip.L.hipHostMalloc.argtypes = [C.c_void_p, C.c_size_t, C.c_uint]
-    assert hip.L.hipHostMalloc(C.byref(ptr), nbytes, 0) == 0, "hipHostMalloc"
-    return np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(ptr.value))
-
-
-class ResultRing:
-    """Pinned result memory handed out in request order and freed in the same order."""
-
-    def __init__(self, buf):
-        import threading
-
-        self.buf, self.size = buf, buf.size
-        self.head = self.tail = 0  # running byte counts
-        self.cv = threading.Condition()
-
-    def alloc(self, nbytes):
-        nbytes = (nbytes + 255) // 256 * 256
-        with self.cv:
-            start = self.head
-            if start % self.size + nbytes > self.size:  # no wrap inside one buffer
-                start += self.size - start % self.size
-            end = start + nbytes
-            self.cv.wait_for(lambda: end - self.tail <= self.size)
-            self.head = end
-        o = start % self.size
-        return self.buf[o:o + nbytes], end
-
-    def free(self, end):
-        with self.cv:
-            self.tail = end
-            self.cv.notify_all()
-
-
-def bench_serve(a, hip, ctx):
-    """Open-loop serving: Poisson arrivals, log-uniform sizes 1 KiB..64 MiB from the four-kind mix,
-    50% Zstd L1 / 25% Zstd L2 / 25% LZMA2, HBM budget; latency from the scheduled arrival.
-    Inputs come from pinned host memory and results go by DMA into a pinned ring (submit_into);
-    the plain per-request form gets the same pinned buffers."""
-    import threading
-
-    L = gcompress.lib()
-    pool_n = 1 << 30
-    content = _pinned(hip, pool_n)
-    w = np.asarray((1, 1, 1, 1), np.float64)
-    assert L.gc_corpus_generate(content.ctypes.data, pool_n, 5, w.ctypes.data_as(C.c_void_p), 65536, 1, 0, 0) == 0
-    ring = ResultRing(_pinned(hip, a.ring_gib << 30))
-    rng = np.random.default_rng(5)
-    lo, hi = np.log(1024), np.log(64 << 20)
-    algos = [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)]
-
-    def out_bytes(kind, n):
-        # LZMA2: one u32 per byte; Zstd: room for n/4 sequences (the mix finds about n/20)
-        return 4 * n if algos[kind][0] == gcompress.ALGO_LZMA2 else (n // 4 + 64) * 16
-
-    def draw(nreq, rate):
-        gaps = rng.exponential(1.0 / rate, nreq)
-        sizes = np.exp(rng.uniform(lo, hi, nreq)).astype(np.int64)
-        offs = (rng.integers(0, pool_n - (64 << 20), nreq) // 4) * 4
-        kinds = rng.choice(3, nreq, p=[0.5, 0.25, 0.25])
-        return gaps, sizes, offs, kinds
-
-    # plain per-request form: one synchronous call per request, same kinds of requests
-    _, psizes, poffs, pkinds = draw(a.plain_requests, 1)
-    tot = 0
-    t = time.perf_counter()
-    for i in range(a.plain_requests):
-        algo, lv = algos[pkinds[i]]
-        n = int(psizes[i])
-        view = content[poffs[i]: poffs[i] + n]
-        out, end = ring.alloc(out_bytes(pkinds[i], n))
-        if algo == gcompress.ALGO_LZMA2:
-            rc = L.gc_lzma2_find_matches_packed(ctx.h, view.ctypes.data, n, out.ctypes.data)
-        else:
-            cnt = np.empty(-(-n // 65536), np.uint32)
-            tt = C.c_size_t()
-            rc = L.gc_zstd_find_sequences_packed(ctx.h, view.ctypes.data, n, lv, out.ctypes.data, out.size // 16,
-                                                 cnt.ctypes.data, C.byref(tt))
-        assert rc == 0, gcompress.lib().gc_last_error()
-        ring.free(end)
-        tot += n
-    plain_gbs = tot / (time.perf_counter() - t) / 1e9
-    print(json.dumps({"plain_per_request_gbs": round(plain_gbs, 3), "requests": a.plain_requests,
-                      "bytes": tot}), file=sys.stderr, flush=True)
-
-    results = {}
-    for rate in [int(r) for r in a.rates.split(",")]:
-        nreq = a.requests
-        gaps, sizes, offs, kinds = draw(nreq, rate)
-        srv = gcompress.Server(ctx, a.budget_gib << 30)
-        ids = [None] * nreq
-        ends = [0] * nreq
-        sched = np.zeros(nreq)
-        done = np.zeros(nreq)
-        failed = [0]
-        ready = threading.Semaphore(0)
-
-        def producer():
-            t = time.monotonic() + 0.05
-            for i in range(nreq):
-                t += gaps[i]
-                sched[i] = t
-                d = t - time.monotonic()
-                if d > 0:
-                    time.sleep(d)
-                algo, lv = algos[kinds[i]]
-                n = int(sizes[i])
-                out, ends[i] = ring.alloc(out_bytes(kinds[i], n))
-                counts = np.empty(-(-n // 65536), np.uint32) if algo == gcompress.ALGO_ZSTD else None
-                ids[i] = srv.submit_into(content[offs[i]: offs[i] + n], algo, lv, out, counts)
-                ready.release()
-
-        def consumer():
-            for i in range(nreq):
-                ready.acquire()
-                try:
-                    done[i] = srv.wait(ids[i]).t_done
-                except gcompress.GcError:
-                    failed[0] += 1
-                    done[i] = time.monotonic()
-                srv.release(ids[i])
-                ring.free(ends[i])
-
-        tp = threading.Thread(target=producer)
-        tc = threading.Thread(target=consumer)
-        tp.start()
-        tc.start()
-        tp.join()
-        tc.join()
-        st = srv.stats()
-        srv.stop()
-        lat = (done - sched) * 1e3
-        t_first = sched[0] - gaps[0]
-        results[str(rate)] = {
-            "requests": nreq, "failed": failed[0],
-            "offered_gbs": round(float(sizes.sum()) / (sched[-1] - t_first) / 1e9, 2),
-            "sustained_gbs": round(float(sizes.sum()) / (done.max() - t_first) / 1e9, 3),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "batches": int(st["batches"]), "input_bytes": int(sizes.sum()),
-            "server_s": {"batches": round(st["t_batches"], 3), "fetch": round(st["t_fetch"], 3),
-                         "host": round(st["t_host"], 3)},
-        }
-        print(json.dumps({"rate": rate, **results[str(rate)]}), file=sys.stderr, flush=True)
-    top = max(results.values(), key=lambda r: r["sustained_gbs"])
-    out = {
-        "metric": "match-find input GB/s at equal ratio, batched/served vs plain; p50/p99 latency",
-        "value": top["sustained_gbs"], "unit": "GB/s", "n_gpus": 1, "steps": a.requests, "warmup": 0,
-        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic (seeded four-kind mix, pinned host memory)",
-        "config": {"workload": "serving: Poisson arrivals, sizes log-uniform 1 KiB-64 MiB, 50% Zstd L1 / 25% L2 / "
-                               "25% LZMA2; inputs from pinned host memory, results by DMA into pinned host memory",
-                   "hbm_budget_gib": a.budget_gib, "rates": results},
+        "served_vs_plain": round(value / plain_gbs, 2),
         "plain_per_request_gbs": round(plain_gbs, 3),
-        "served_vs_plain": round(top["sustained_gbs"] / plain_gbs, 2),
+        "plain": {"requests": k, "bytes": int(plain_bytes),
+                  "form": "one synchronous kernel-ABI call per request, same device-resident inputs"},
+        "latency_at_rates": rates,
+        "saturation": sat_lat,
+        "verify_vs_plain": verify,
+        "parity_vs_oracle": parity,
+        "server": {"ticks": int(st1["batches"] - st0["batches"]),
+                   "sub_blocks": int(st1["items"] - st0["items"]),
+                   # every served byte of this run (verification, rates, warm-up and timed passes): what the
+                   # served kernels' HBM counters are divided by
+                   "totals": {"zstd_bytes": int(st1["zstd_bytes"]), "lzma2_bytes": int(st1["lzma2_bytes"]),
+                              "launches": [int(x) for x in st1["launches"]]}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "host": host_cores(),
     }
+    out.update(extras)
+    return out
+
+
+# ---------------------------------------------------------------------------------------------------
+# Batched configurations
+# ---------------------------------------------------------------------------------------------------
+def batched_config(a, hip, ctx, name, data, inputs, modes=("static",), plain_pass=True, steps=None):
+    """One batched configuration: K timed passes per mode (HIP events on the library stream), per-kernel
+    durations (one event pair per kernel), the plain per-chunk/per-input form, the dominant kernel's
+    roofline, and the batched results against the plain form on a sample."""
+    L = gcompress.lib()
+    stream = L.gc_stream(ctx.h)
+    steps = steps or a.steps
+    n = data.size
+    d_in = ctx.alloc(n + 256)
+    ctx.htod(d_in, data)
+    work = sum(s for (_, s, _, _) in inputs)
+    zbytes = sum(s for (_, s, al, _) in inputs if al == ZSTD)
+    lbytes = work - zbytes
+    out = {"inputs": len(inputs), "input_bytes_per_step": int(work), "zstd_bytes": int(zbytes),
+           "lzma2_bytes": int(lbytes)}
+    plain = None
+    if plain_pass:
+        big = max(s for (_, s, _, _) in inputs)
+        p = Plain(ctx, big)
+        for (o, s, algo, lv) in inputs[:2]:
+            p.run(d_in + o, s, algo, lv)
+        t = time.perf_counter()
+        for (o, s, algo, lv) in inputs:
+            p.run(d_in + o, s, algo, lv)
+        out["plain_gbs"] = round(work / (time.perf_counter() - t) / 1e9, 3)
+        plain = p
+    b = gcompress.Batch(ctx, d_in, n + 256, inputs)
+    knames = ["candidates", "lzma2_lengths", "zstd_info", "zstd_parse"]
+    for mode in modes:
+        b.set_mode(gcompress.BATCH_PERSISTENT if mode == "persistent" else gcompress.BATCH_STATIC)
+        for _ in range(max(a.warmup, 1)):
+            b.run()
+        ctx.sync()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        for _ in range(steps):
+            b.run()
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1) / steps
+        b.check()
+        kt = [b.run_timed() for _ in range(3)]
+        k_ms = [float(np.mean([x[i] for x in kt])) for i in range(4)]
+        out[mode] = {"gbs": round(work / (ms / 1e3) / 1e9, 3), "ms_per_pass": round(ms, 3),
+                     "kernel_ms": {knames[i]: round(k_ms[i], 3) for i in range(4)}}
+        if plain:
+            out[mode]["vs_plain"] = round(out[mode]["gbs"] / out["plain_gbs"], 2)
+    # results (outside the timed region): sequence totals, the roofline, equality with plain on a sample
+    n_seq = n_sub_z = 0
+    for i, (_, s, algo, _) in enumerate(inputs):
+        if algo == ZSTD:
+            _, counts = b.zstd(i, with_seqs=False)
+            n_seq += int(counts.sum())
+            n_sub_z += counts.size
+    share = {0: work, 1: 5 * lbytes, 2: zbytes, 3: zbytes + 16 * n_seq + 4 * n_sub_z}
+    names = {"static": ["gc_cand_kernel", "gc_lzma2_len_kernel", "gc_zstd_info_kernel", "gc_zstd_parse_kernel"],
+             "persistent": ["gc_cand_q_kernel", "gc_lzma2_len_q_kernel", "gc_zstd_info_q_kernel",
+                            "gc_zstd_parse_q_kernel"]}
+    for mode in modes:
+        k_ms = [out[mode]["kernel_ms"][kn] for kn in knames]
+        dom = max((i for i in share if share[i]), key=lambda i: k_ms[i])
+        achieved = share[dom] / (k_ms[dom] / 1e3) / 1e9
+        kname = names[mode][dom]
+        per_byte = hbm_traffic_per_input_byte(name, kname)
+        out[mode]["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
+                                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                                 "traffic": int(per_byte * work) if per_byte is not None else None,
+                                 "algorithmic_bytes": int(share[dom])}
+    out["results"] = {"n_sequences": n_seq, "zstd_sub_blocks": n_sub_z}
+    if plain:
+        same = True
+        picked = list(range(0, len(inputs), max(1, len(inputs) // 16)))
+        cmp_bytes = 0
+        for i in picked:
+            o, s, algo, lv = inputs[i]
+            plain.run(d_in + o, s, algo, lv)
+            if algo == LZMA2:
+                same &= digest(b.lzma2(i)) == plain.digest(s, algo)
+            else:
+                seqs, counts = b.zstd(i)
+                same &= zstd_digest_from_slots(seqs, counts) == plain.digest(s, algo)
+            cmp_bytes += s
+        out["identical_to_plain"] = {"inputs": len(picked), "bytes": int(cmp_bytes), "identical": bool(same)}
+        assert same, f"{name}: batched results differ from the plain form"
+        plain.close()
+    b.close()
+    ctx.free(d_in)
+    return out
+
+
+def extras_all(a, hip, ctx):
+    ex = {}
+    print("[bench] config 2 (1 GiB Zstd L1+L2)", file=sys.stderr, flush=True)
+    data, inputs = W.config2()
+    ex["config2_zstd_1GiB_L1L2"] = batched_config(a, hip, ctx, "zstd", data, inputs)
+    del data
+    print("[bench] config 3 (640 MiB LZMA2)", file=sys.stderr, flush=True)
+    data, inputs = W.config3()
+    ex["config3_lzma2_640MiB"] = batched_config(a, hip, ctx, "lzma2", data, inputs)
+    del data
+    print("[bench] config 4 (4096 inputs)", file=sys.stderr, flush=True)
+    data, inputs = W.config4(threads=min(threads_all(), 16))
+    ex["config4_4096_inputs"] = batched_config(a, hip, ctx, "mixed", data, inputs, modes=("persistent", "static"),
+                                               steps=max(1, a.steps // 2))
+    del data
+    return ex
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed"], default="serve")
+    ap.add_argument("--no-extras", action="store_true", help="serve: only the serving workload")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rates", default="500,2000,8000", help="serve: Poisson arrival rates (req/s), comma-separated")
+    ap.add_argument("--sat-requests", type=int, default=6000)
+    ap.add_argument("--plain-requests", type=int, default=1000)
+    ap.add_argument("--verify-requests", type=int, default=256)
+    ap.add_argument("--budget-gib", type=int, default=64)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="one device-to-device copy of this size first, to scale HBM byte counters")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this match-finding path runs on one GPU; multi-GPU runs are not part of it "
+              "(sub-blocks shard by range or by request with no exchange between GPUs)", file=sys.stderr)
+        sys.exit(2)
+    hip = Hip()
+    ctx = gcompress.context(0)
+    if a.calib_copy_mib:
+        # a plain copy of known size (the runtime's copy kernel): FETCH_SIZE / WRITE_SIZE read for it
+        # against calib_copy_mib MiB give the counters' scale
+        cb = a.calib_copy_mib * MiB
+        c_src, c_dst = ctx.alloc(cb), ctx.alloc(cb)
+        assert hip.L.hipMemcpyAsync(c_dst, c_src, cb, 3, gcompress.lib().gc_stream(ctx.h)) == 0
+        ctx.sync()
+        ctx.free(c_src)
+        ctx.free(c_dst)
+    if a.workload == "serve":
+        extras = {} if a.no_extras else extras_all(a, hip, ctx)
+        out = bench_serve(a, hip, ctx, extras)
+    else:
+        if a.workload == "zstd":
+            data, inputs = W.config2()
+            modes, desc = ("static", "persistent"), "drop-in Zstd L1+L2 match-find, 1 GiB mix in 16 MiB chunks"
+        elif a.workload == "lzma2":
+            data, inputs = W.config3()
+            modes, desc = ("static", "persistent"), "LZMA2 L1 match-find, 640 MiB disk-image-like mix, 128 MiB chunks"
+        else:
+            data, inputs = W.config4(threads=min(threads_all(), 16))
+            modes, desc = ("persistent", "static"), "4096 inputs, 4 KiB-32 MiB log-uniform, a third each Zstd L1/L2/LZMA2"
+        r = batched_config(a, hip, ctx, a.workload, data, inputs, modes=modes)
+        top = r[modes[0]]
+        out = {"metric": METRIC, "value": top["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+               "warmup": a.warmup, "ms_per_step": top["ms_per_pass"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded four-kind mix)",
+               "config": {"workload": desc, "form": modes[0]}, "roofline": top["roofline"], "cpu_baseline": None,
+               "detail": r, "host": host_cores()}
     print(json.dumps(out), flush=True)
+
 
 if __name__ == "__main__":
     main()

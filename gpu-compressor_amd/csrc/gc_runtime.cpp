@@ -164,22 +164,25 @@ int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t
 }
 
 int launch_cand_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                  uint32_t grid, uint16_t* cand16) {
+                  uint32_t grid, uint16_t* cand16, bool served) {
     void* args[] = {&arena, &items, &q, &cand16};
-    return launch(stream, ctx->k.cand_q, grid, GC_CAND_THREADS, args, "launch gc_cand_q_kernel");
+    return launch(stream, served ? ctx->k.cand_sq : ctx->k.cand_q, grid, GC_CAND_THREADS, args,
+                  "launch gc_cand_q_kernel");
 }
 
 int launch_lzma2_len_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                       uint32_t grid, const uint16_t* cand16, uint32_t* packed) {
+                       uint32_t grid, const uint16_t* cand16, uint32_t* packed, bool served) {
     void* args[] = {&arena, &items, &q, &cand16, &packed};
-    return launch(stream, ctx->k.lzma2_len_q, grid, 1024, args, "launch gc_lzma2_len_q_kernel");
+    return launch(stream, served ? ctx->k.lzma2_len_sq : ctx->k.lzma2_len_q, grid, 1024, args,
+                  "launch gc_lzma2_len_q_kernel");
 }
 
 int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost) {
+                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost,
+                       bool served) {
     void* args[] = {&arena, &items, &q, &cand16, &info, &cost};
-    return launch(stream, strip == 4 ? ctx->k.zstd_info4_q : ctx->k.zstd_info_q, grid, 1024, args,
-                  "launch gc_zstd_info_q_kernel");
+    hipFunction_t f = served ? ctx->k.zstd_info_sq : (strip == 4 ? ctx->k.zstd_info4_q : ctx->k.zstd_info_q);
+    return launch(stream, f, grid, 1024, args, "launch gc_zstd_info_q_kernel");
 }
 
 int launch_tick_setup(gc_ctx* ctx, hipStream_t stream, const gc_queue* hdr, const gc_item* src, gc_queue* q,
@@ -190,9 +193,10 @@ int launch_tick_setup(gc_ctx* ctx, hipStream_t stream, const gc_queue* hdr, cons
 
 int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
                         uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
-                        const uint32_t* order) {
+                        const uint32_t* order, bool served) {
     void* args[] = {&arena, &items, &q, &info, &seq, &seq_counts, &max_seq, &order};
-    return launch(stream, ctx->k.zstd_parse_q, grid, 256, args, "launch gc_zstd_parse_q_kernel");
+    return launch(stream, served ? ctx->k.zstd_parse_sq : ctx->k.zstd_parse_q, grid, 256, args,
+                  "launch gc_zstd_parse_q_kernel");
 }
 
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n) {
@@ -273,6 +277,11 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess)
         e = hipModuleGetFunction(&ctx->k.zstd_parse_q, ctx->k.module_parse, "gc_zstd_parse_q_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.tick_setup, ctx->k.module, "gc_tick_setup");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand_sq, ctx->k.module, "gc_cand_sq_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_sq, ctx->k.module, "gc_lzma2_len_sq_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_info_sq, ctx->k.module, "gc_zstd_info_sq_kernel");
+    if (e == hipSuccess)
+        e = hipModuleGetFunction(&ctx->k.zstd_parse_sq, ctx->k.module_parse, "gc_zstd_parse_sq_kernel");
     if (e != hipSuccess) {
         hip_fail(e, "gc_open (stream / hipModuleLoadData / hipModuleGetFunction)");
         gc_close(ctx);

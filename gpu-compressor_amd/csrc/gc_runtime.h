@@ -53,6 +53,11 @@ struct Kernels {
     hipFunction_t zstd_info4_q = nullptr;
     hipFunction_t zstd_parse_q = nullptr;
     hipFunction_t tick_setup = nullptr;
+    // the served form's launches of the same persistent kernels (names of their own)
+    hipFunction_t cand_sq = nullptr;
+    hipFunction_t lzma2_len_sq = nullptr;
+    hipFunction_t zstd_info_sq = nullptr;
+    hipFunction_t zstd_parse_sq = nullptr;
 };
 
 }  // namespace gc
@@ -95,19 +100,21 @@ int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t
                  const gc_queue* q = nullptr);
 // Persistent kernels: grid workgroups take items from the queue q (item
 // counts in q->n); nothing else differs from the static launches.
+// served: the served form's kernel names.
 int launch_cand_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                  uint32_t grid, uint16_t* cand16);
+                  uint32_t grid, uint16_t* cand16, bool served = false);
 int launch_lzma2_len_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                       uint32_t grid, const uint16_t* cand16, uint32_t* packed);
+                       uint32_t grid, const uint16_t* cand16, uint32_t* packed, bool served = false);
 int launch_zstd_info_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
-                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost);
+                       uint32_t grid, const uint16_t* cand16, uint32_t* info, uint32_t strip, uint32_t* cost,
+                       bool served = false);
 // Served form: copy a tick's counts and items from pinned memory into the
 // slot's queue and item array.
 int launch_tick_setup(gc_ctx* ctx, hipStream_t stream, const gc_queue* hdr, const gc_item* src, gc_queue* q,
                       gc_item* items, uint32_t cap);
 int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,
                         uint32_t grid, const uint32_t* info, void* seq, uint32_t* seq_counts, uint32_t max_seq,
-                        const uint32_t* order);
+                        const uint32_t* order, bool served = false);
 // Zstd frames from sequences and literals (one wavefront per item), and the
 // packing of frames written in fixed slots.
 int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,

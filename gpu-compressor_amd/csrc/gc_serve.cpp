@@ -13,13 +13,15 @@
 //   2. whenever one of kSlots tick slots is free and requests are ready,
 //      starts a tick with every ready request (up to max_batch_items
 //      sub-blocks): it writes the tick's work list into the slot's pinned
-//      staging memory and replays the slot's captured pass -- a setup kernel
-//      that loads the list into the slot's device work queue, then the
-//      persistent kernels (candidates, LZMA2 lengths, Zstd info, parse
-//      order, parse), which take the tick's sub-blocks from the queue.  Each
-//      slot has its own stream, so consecutive ticks overlap on the device:
-//      a request that arrives while others run starts at the next free slot,
-//      beside them, instead of waiting for them to end;
+//      staging memory and launches the slot's pass on its stream -- a setup
+//      kernel that loads the list into the slot's device work queue, then
+//      the persistent kernels (candidates, LZMA2 lengths, Zstd info, parse
+//      order, parse), which take the tick's sub-blocks from the queue, with
+//      an event between every two kernels (per-kernel device time is part of
+//      the server's statistics).  Each slot has its own stream, so
+//      consecutive ticks overlap on the device: a request that arrives while
+//      others run starts at the next free slot, beside them, instead of
+//      waiting for them to end;
 //   3. when a tick ends: device-result requests are done at once (their
 //      sequence counts were written by the parse into pinned memory); the
 //      others have their results brought back on a fetch stream, straight
@@ -161,18 +163,21 @@ void layout(Req& r) {
     r.dev_len = o;
 }
 
-// One tick slot: its stream, pinned work list, device queue and captured pass.
+// Kernels of a tick, in launch order (events between them).
+enum { K_SETUP, K_CAND, K_LZMA2, K_INFO, K_ORDER, K_PARSE, K_COUNT };
+
+// One tick slot: its stream, pinned work list and device queue.
 struct Slot {
     hipStream_t st = nullptr;
-    hipEvent_t ev_start = nullptr, ev_done = nullptr;
+    hipEvent_t ev[K_COUNT + 1] = {};  // before each kernel, and after the last
+    hipEvent_t ev_done = nullptr;
     gc::DevBuf items, cost, order, queue;
     gc_queue* hdr = nullptr;    // pinned: the tick's counts and first items
     gc_item* stage = nullptr;   // pinned: the tick's items
     uint32_t* err = nullptr;    // pinned: the queue's error word, copied after each tick
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
     std::vector<Req*> reqs;
     bool busy = false;
+    uint32_t n_l = 0, n_z = 0;  // the tick's LZMA2 and Zstd items
 };
 
 }  // namespace
@@ -208,6 +213,7 @@ struct gc_server {
     uint32_t max_batch_items = 0;
 
     int setup_slot(Slot& sl);
+    int launch_pass(Slot& sl, uint32_t n_all);
     void loop();
     void admit_locked();
     bool launch_tick(Slot& sl);
@@ -217,15 +223,11 @@ struct gc_server {
     void fail_req_locked(Req& r, int code);
 };
 
-// The slot's pass, captured once on its stream: the setup kernel and the
-// persistent kernels with resident grids (4 candidate workgroups per CU, 2
-// of the 1024-thread per-position kernels, 8 four-wavefront parse
-// workgroups).  Inputs are addressed absolutely (a null arena), everything
-// else in the pool; the queue gives each kernel its items.
 int gc_server::setup_slot(Slot& sl) {
     const uint32_t cap = max_batch_items;
     hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&sl.ev_start);
+    for (auto& ev : sl.ev)
+        if (e == hipSuccess) e = hipEventCreate(&ev);
     if (e == hipSuccess) e = hipEventCreate(&sl.ev_done);
     if (e == hipSuccess) e = hipHostMalloc((void**)&sl.hdr, sizeof(gc_queue), 0);
     if (e == hipSuccess) e = hipHostMalloc((void**)&sl.stage, (size_t)cap * sizeof(gc_item), 0);
@@ -240,34 +242,46 @@ int gc_server::setup_slot(Slot& sl) {
         gc_queue q0{};
         rc = upload(ctx, sl.queue, &q0, sizeof(q0));
     }
-    if (rc) return rc;
+    return rc;
+}
+
+// The tick's pass on the slot's stream: the setup kernel and the persistent
+// kernels with resident grids (4 candidate workgroups per CU, 2 of the
+// 1024-thread per-position kernels, 8 four-wavefront parse workgroups; fewer
+// when the tick has fewer items).  Inputs are addressed absolutely (a null
+// arena), everything else in the pool; the queue gives each kernel its items.
+int gc_server::launch_pass(Slot& sl, uint32_t n_all) {
     gc_queue* q = (gc_queue*)sl.queue.p;
     gc_item* items = (gc_item*)sl.items.p;
     const uint32_t cus = (uint32_t)ctx->cu_count;
-    e = hipStreamBeginCapture(sl.st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    rc = launch_tick_setup(ctx, sl.st, dev_addr(sl.hdr), dev_addr(sl.stage), q, items, cap);
-    if (!rc) rc = launch_cand_q(ctx, sl.st, nullptr, items, q, 4 * cus, (uint16_t*)pool);
-    if (!rc)
-        rc = launch_lzma2_len_q(ctx, sl.st, nullptr, items, q, 2 * cus, (const uint16_t*)pool, (uint32_t*)pool);
-    if (!rc)
-        rc = launch_zstd_info_q(ctx, sl.st, nullptr, items, q, 2 * cus, (const uint16_t*)pool, (uint32_t*)pool, 8,
-                                (uint32_t*)sl.cost.p);
-    if (!rc) rc = launch_order(ctx, sl.st, (const uint32_t*)sl.cost.p, 0, (uint32_t*)sl.order.p, q);
-    if (!rc)
-        rc = launch_zstd_parse_q(ctx, sl.st, nullptr, items, q, 8 * cus, (const uint32_t*)pool, pool,
-                                 dev_addr(counts_zone), kMaxSeq, (const uint32_t*)sl.order.p);
-    hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(sl.st, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-    sl.graph = g;
-    e = hipGraphInstantiate(&sl.exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
-    return GC_OK;
+    auto grid = [&](uint32_t per_cu, uint32_t n) { return std::max(1u, std::min(per_cu * cus, n)); };
+    hipStream_t st = sl.st;
+    int rc = GC_OK;
+    hipError_t e = hipEventRecord(sl.ev[K_SETUP], st);
+    if (e == hipSuccess) rc = launch_tick_setup(ctx, st, dev_addr(sl.hdr), dev_addr(sl.stage), q, items, max_batch_items);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_CAND], st);
+    if (!rc && e == hipSuccess) rc = launch_cand_q(ctx, st, nullptr, items, q, grid(4, n_all), (uint16_t*)pool, true);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_LZMA2], st);
+    if (!rc && e == hipSuccess && sl.n_l)
+        rc = launch_lzma2_len_q(ctx, st, nullptr, items, q, grid(2, sl.n_l), (const uint16_t*)pool, (uint32_t*)pool,
+                                true);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_INFO], st);
+    if (!rc && e == hipSuccess && sl.n_z)
+        rc = launch_zstd_info_q(ctx, st, nullptr, items, q, grid(2, sl.n_z), (const uint16_t*)pool, (uint32_t*)pool, 8,
+                                (uint32_t*)sl.cost.p, true);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_ORDER], st);
+    if (!rc && e == hipSuccess && sl.n_z)
+        rc = launch_order(ctx, st, (const uint32_t*)sl.cost.p, 0, (uint32_t*)sl.order.p, q);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_PARSE], st);
+    if (!rc && e == hipSuccess && sl.n_z)
+        rc = launch_zstd_parse_q(ctx, st, nullptr, items, q, grid(8, (sl.n_z + 3) / 4), (const uint32_t*)pool, pool,
+                                 dev_addr(counts_zone), kMaxSeq, (const uint32_t*)sl.order.p, true);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev[K_COUNT], st);
+    if (!rc && e == hipSuccess)
+        e = hipMemcpyAsync(sl.err, (uint8_t*)q + offsetof(gc_queue, err), 4, hipMemcpyDeviceToHost, st);
+    if (!rc && e == hipSuccess) e = hipEventRecord(sl.ev_done, st);
+    if (rc) return rc;
+    return e == hipSuccess ? GC_OK : hip_fail(e, "gc_server: tick launch");
 }
 
 void gc_server::fail_req_locked(Req& r, int code) {
@@ -375,13 +389,9 @@ bool gc_server::launch_tick(Slot& sl) {
     h->base[GC_Q_CAND] = h->base[GC_Q_LZMA2] = 0;
     h->base[GC_Q_INFO] = h->base[GC_Q_PARSE] = n_l;
     sl.reqs = batch;
-    hipError_t e = hipEventRecord(sl.ev_start, sl.st);
-    if (e == hipSuccess) e = hipGraphLaunch(sl.exec, sl.st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(sl.err, (uint8_t*)sl.queue.p + offsetof(gc_queue, err), 4, hipMemcpyDeviceToHost, sl.st);
-    if (e == hipSuccess) e = hipEventRecord(sl.ev_done, sl.st);
-    if (e != hipSuccess) {
-        (void)hip_fail(e, "gc_server: tick launch");
+    sl.n_l = n_l;
+    sl.n_z = n_z;
+    if (launch_pass(sl, n_all) != GC_OK) {
         (void)hipStreamSynchronize(sl.st);
         std::lock_guard<std::mutex> g(mu);
         for (Req* r : batch) fail_req_locked(*r, GC_EHIP);
@@ -398,7 +408,13 @@ bool gc_server::launch_tick(Slot& sl) {
 // wait for their results to be brought back.
 void gc_server::tick_ended(Slot& sl) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, sl.ev_start, sl.ev_done) == hipSuccess) stats.t_batches += ms * 1e-3;
+    if (hipEventElapsedTime(&ms, sl.ev[K_SETUP], sl.ev[K_COUNT]) == hipSuccess) stats.t_batches += ms * 1e-3;
+    const bool ran[K_COUNT] = {true, true, sl.n_l > 0, sl.n_z > 0, sl.n_z > 0, sl.n_z > 0};
+    for (int k = 0; k < K_COUNT; k++)
+        if (ran[k] && hipEventElapsedTime(&ms, sl.ev[k], sl.ev[k + 1]) == hipSuccess) {
+            stats.t_kernel[k] += ms * 1e-3;
+            stats.launches[k]++;
+        }
     const uint32_t err = *sl.err;
     if (err) {  // the queue protocol reported an error: this tick's results are not trusted
         *sl.err = 0;
@@ -423,6 +439,11 @@ void gc_server::tick_ended(Slot& sl) {
                 r.fetch_total = tot;
                 zone.release(r.cnt_off, r.cnt_len);
                 r.cnt_len = 0;
+                stats.zstd_bytes += r.n;
+                stats.zstd_sub_blocks += r.nsub;
+                stats.zstd_sequences += tot;
+            } else {
+                stats.lzma2_bytes += r.n;
             }
             if (r.dev_result) {
                 r.state = DONE;
@@ -598,9 +619,8 @@ namespace {
 void destroy_server(gc_server* s) {
     for (auto& sl : s->slot) {
         if (sl.st) (void)hipStreamSynchronize(sl.st);
-        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-        if (sl.graph) (void)hipGraphDestroy(sl.graph);
-        if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
+        for (auto& ev : sl.ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
         if (sl.hdr) (void)hipHostFree(sl.hdr);
         if (sl.stage) (void)hipHostFree(sl.stage);

@@ -605,9 +605,11 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
     }
 }
 
-// Persistent form: items from the device work queue (phase GC_Q_CAND).
-extern "C" __global__ void __launch_bounds__(CAND_THREADS)
-gc_cand_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, uint16_t* cand16) {
+// Persistent form: items from the device work queue (phase GC_Q_CAND).  The
+// served form launches the same code under a name of its own (the `_sq`
+// kernels), so that a kernel trace tells its ticks from batched passes.
+__device__ __forceinline__ void cand_queue(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q,
+                                           uint16_t* cand16) {
     __shared__ CandLds s;
     items += queue_base(q, GC_Q_CAND);
     wg_queue_items(q, GC_Q_CAND, [&](uint32_t i) {
@@ -618,6 +620,16 @@ gc_cand_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_que
             cand_item<false>(s, it, arena, cand16);
         lds_barrier();
     });
+}
+
+extern "C" __global__ void __launch_bounds__(CAND_THREADS)
+gc_cand_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, uint16_t* cand16) {
+    cand_queue(arena, items, q, cand16);
+}
+
+extern "C" __global__ void __launch_bounds__(CAND_THREADS)
+gc_cand_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, uint16_t* cand16) {
+    cand_queue(arena, items, q, cand16);
 }
 
 // ---------------------------------------------------------------------------
@@ -806,6 +818,12 @@ gc_lzma2_len_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, g
     lzma2_len_items<true, false>(arena, items, 0, cand16, packed, nullptr, nullptr, 1, nullptr, q);
 }
 
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
+gc_lzma2_len_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
+                       uint32_t* packed) {
+    lzma2_len_items<true, false>(arena, items, 0, cand16, packed, nullptr, nullptr, 1, nullptr, q);
+}
+
 // ---------------------------------------------------------------------------
 // Zstd per position, given cand(p): the candidate as the reference's table
 // returns it (an empty slot reads as position 0, zstd_compress.cu:198-200),
@@ -950,6 +968,12 @@ extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
 gc_zstd_info4_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
                        uint32_t* info, uint32_t* cost) {
     zstd_info_items<4>(arena, items, 0, cand16, info, cost, q);
+}
+
+extern "C" __global__ void __launch_bounds__(INFO_THREADS, 8)
+gc_zstd_info_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint16_t* cand16,
+                       uint32_t* info, uint32_t* cost) {
+    zstd_info_items<8>(arena, items, 0, cand16, info, cost, q);
 }
 
 // The parse's item order, costliest first (counting sort of cost >> 9 into
@@ -1518,6 +1542,13 @@ GC_PARSE_KERNEL(gc_zstd_parse_ref_lits_kernel, GC_SEQ_REF, true)
 extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE, 8)
 gc_zstd_parse_q_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint32_t* info,
                        void* seq_out, uint32_t* seq_counts, uint32_t max_seq, const uint32_t* __restrict__ order) {
+    __shared__ uint32_t ring_mem[PARSE_WAVES][RING_BYTES / 4];
+    zstd_parse_queue<GC_SEQ_ZSTD, false>(arena, items, q, info, seq_out, seq_counts, max_seq, order, ring_mem);
+}
+
+extern "C" __global__ void __launch_bounds__(PARSE_WAVES* WAVE, 8)
+gc_zstd_parse_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q, const uint32_t* info,
+                        void* seq_out, uint32_t* seq_counts, uint32_t max_seq, const uint32_t* __restrict__ order) {
     __shared__ uint32_t ring_mem[PARSE_WAVES][RING_BYTES / 4];
     zstd_parse_queue<GC_SEQ_ZSTD, false>(arena, items, q, info, seq_out, seq_counts, max_seq, order, ring_mem);
 }

@@ -139,7 +139,9 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("batches", C.c_uint64), ("items", C.c_uint64), ("completed", C.c_uint64), ("bytes", C.c_uint64),
                 ("hbm_in_use", C.c_uint64), ("pending", C.c_uint64), ("t_batches", C.c_double),
-                ("t_fetch", C.c_double), ("t_host", C.c_double)]
+                ("t_fetch", C.c_double), ("t_host", C.c_double), ("t_kernel", C.c_double * 6),
+                ("launches", C.c_uint64 * 6), ("zstd_bytes", C.c_uint64), ("zstd_sub_blocks", C.c_uint64),
+                ("zstd_sequences", C.c_uint64), ("lzma2_bytes", C.c_uint64)]
 
 
 class GcError(RuntimeError):
@@ -632,7 +634,11 @@ class Server:
     def stats(self):
         st = ServerStats()
         _check(lib().gc_server_get_stats(self.h, C.byref(st)), "gc_server_get_stats")
-        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+        out = {}
+        for k, _ in ServerStats._fields_:
+            v = getattr(st, k)
+            out[k] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def stop(self):
         if self.h:

@@ -369,9 +369,9 @@ int gc_batch_copy_lzma2(gc_batch* b, uint32_t i, uint32_t* packed);
  * pipeline.rs:138-175).  A server thread admits requests in arrival order
  * while their device memory fits hbm_budget bytes (a host input is copied in
  * as it is admitted) and batches them continuously into the persistent
- * kernels: whenever one of its tick slots (each with its own stream and its
- * own captured pass) is free, every admitted request whose input is on the
- * device goes into a tick, so requests that arrive while others run start
+ * kernels: whenever one of its tick slots (each with its own stream, work
+ * queue and staging memory) is free, every admitted request whose input is on
+ * the device goes into a tick, so requests that arrive while others run start
  * beside them at the next free slot.  A request's results equal the plain
  * form's for its input alone.  Request memory (caller's `data` or
  * `d_input`) must stay valid until the request is done. */
@@ -391,6 +391,13 @@ typedef struct {
     double t_batches;    /* device seconds of the ticks (events around each; ticks overlap, so the sum can exceed wall time) */
     double t_fetch;      /* seconds of result copies on the copy-back stream (events around each tick's) */
     double t_host;       /* server-thread seconds spent building ticks and staging result copies */
+    /* per kernel of a tick (setup, candidates, LZMA2 lengths, Zstd info, parse
+     * order, Zstd parse): device seconds between the events around it, summed
+     * over the ticks, and its launches */
+    double t_kernel[6];
+    uint64_t launches[6];
+    /* finished ticks: Zstd input bytes, sub-blocks and sequences, LZMA2 input bytes */
+    uint64_t zstd_bytes, zstd_sub_blocks, zstd_sequences, lzma2_bytes;
 } gc_server_stats;
 
 /* max_batch_items: sub-blocks per tick (0 = 65536); a request with more is
