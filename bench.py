@@ -261,7 +261,7 @@ def kernel_roofline(stats0, stats1, wall_s):
             "kernel_s": {names[k]: round(float(d["t_kernel"][k]), 4) for k in range(6)},
             "kernel_launches": {names[k]: int(d["launches"][k]) for k in range(6)},
             # every served kernel together: algorithmic bytes of the timed passes over their wall time
-            "path_algorithmic_gbs": round((share[1] + share[2] + share[5]) / max(wall_s, 1e-9) / 1e9, 2),
+            "path_algorithmic_gbs": round((share[2] + share[5]) / max(wall_s, 1e-9) / 1e9, 2),
             "device_busy_frac_of_wall": round(float(sum(d["t_kernel"])) / max(wall_s, 1e-9), 3)}
 
 
@@ -511,8 +511,50 @@ def batched_config(a, hip, ctx, name, data, inputs, modes=("static",), plain_pas
     return out
 
 
+def lzma2_end_to_end(a):
+    """LZMA2 L1 end to end on configuration 3 (the whole 640 MiB): the reference's shape -- GPU match
+    finding in 128 MiB chunks, device-side triage, liblzma preset 9 per 64 KiB sub-block on a pool of
+    min(cores, 8), NVLZ -- beside the range coder driven by the GPU's matches and level 0 (liblzma on whole
+    chunks, the reference's CPU path); every container decoded back and compared with the input."""
+    data, chunks = W.config3()
+    pool_n = min(threads_all(), 8)
+
+    def by_chunk(fn, name):
+        # the container of the whole input, made chunk by chunk (sub-blocks never cross a 128 MiB chunk,
+        # so the entries are the same) with a line after each chunk
+        sizes, payload = [], []
+        for k, (o, s, _, _) in enumerate(chunks):
+            blob = fn(data[o:o + s])
+            nch = int.from_bytes(blob[20:28], "little")
+            sizes += [int.from_bytes(blob[28 + 8 * j: 36 + 8 * j], "little") for j in range(nch)]
+            payload.append(blob[28 + 8 * nch:])
+            print(f"[bench]   {name}: chunk {k + 1}/{len(chunks)}", file=sys.stderr, flush=True)
+        head = b"NVLZ" + data.size.to_bytes(8, "little") + (65536).to_bytes(8, "little") + len(sizes).to_bytes(8, "little")
+        return head + b"".join(z.to_bytes(8, "little") for z in sizes) + b"".join(payload)
+
+    out = {"input_bytes": int(data.size), "encoder_threads": pool_n, "host": host_cores(),
+           "reference_readme": {"input": "613 MB Windows Server 2003 ISO, RTX 3090 / 3090 Ti, host unstated",
+                                "l1_s": 164.5, "l1_size_pct": 91.4, "l0_s": 12.3, "l0_size_pct": 91.2}}
+    legs = (("reference_shape_triage_preset9",
+             lambda: by_chunk(lambda x: gcompress.nvlz_compress(x, 1, 128, threads=pool_n), "preset 9")),
+            ("gpu_matches_range_coder", lambda: gcompress.nvlz_compress_matches(data, 128, threads=pool_n)),
+            ("level0_liblzma", lambda: gcompress.nvlz_compress_l0(data, 128, 0, threads=pool_n)))
+    for key, fn in legs:
+        print(f"[bench] LZMA2 end to end: {key}", file=sys.stderr, flush=True)
+        t = time.perf_counter()
+        blob = fn()
+        dt = time.perf_counter() - t
+        same = gcompress.decompress(blob) == data.tobytes()
+        assert same, f"{key}: NVLZ does not round-trip"
+        out[key] = {"s": round(dt, 2), "gbs": round(data.size / dt / 1e9, 4), "ratio": round(data.size / len(blob), 4),
+                    "size_pct": round(100.0 * len(blob) / data.size, 2), "round_trip": same}
+    return out
+
+
 def extras_all(a, hip, ctx):
     ex = {}
+    if not a.no_lzma2_e2e:
+        ex["lzma2_l1_end_to_end_config3"] = lzma2_end_to_end(a)
     print("[bench] config 2 (1 GiB Zstd L1+L2)", file=sys.stderr, flush=True)
     data, inputs = W.config2()
     ex["config2_zstd_1GiB_L1L2"] = batched_config(a, hip, ctx, "zstd", data, inputs)
@@ -537,6 +579,7 @@ def main():
     ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed"], default="serve")
     ap.add_argument("--no-extras", action="store_true", help="serve: only the serving workload")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lzma2-e2e", action="store_true", help="serve: skip the LZMA2 end-to-end leg")
     ap.add_argument("--rates", default="500,2000,8000", help="serve: Poisson arrival rates (req/s), comma-separated")
     ap.add_argument("--sat-requests", type=int, default=6000)
     ap.add_argument("--plain-requests", type=int, default=1000)

@@ -562,22 +562,37 @@ int gc_decompress(const uint8_t* in, size_t n, uint8_t* out, size_t cap, size_t*
     int rc = gc_container_info(in, n, &algo, &orig, &ch, &nch);
     if (rc) return rc;
     if (cap < orig) return fail(GC_ESIZE, "output capacity below the original size");
-    size_t pos = 28 + 8 * nch, o = 0;
+    // every entry's place in the payload and in the output first (decompress.rs:65-73), then the entries
+    // decoded by min(cores, 8) threads (they are independent)
+    std::vector<uint64_t> pos(nch), dsz(nch);
+    size_t p = 28 + 8 * nch;
     for (uint64_t c = 0; c < nch; c++) {
-        uint64_t csize = get_u64(in + 28 + 8 * c);
-        if (csize > n - pos) return fail(GC_EFORMAT, "truncated payload");  // read_exact, decompress.rs:81
-        uint64_t dsize = (c + 1 < nch) ? ch : orig - (nch - 1) * ch;     // decompress.rs:70-73
-        size_t got = 0;
-        if (algo == GC_ALGO_LZMA2)
-            rc = gc_lzma2_raw_decode(in + pos, csize, out + o, dsize, &got);
-        else
-            rc = gc_zstd_frame_decode(in + pos, csize, out + o, dsize, &got);
-        if (rc) return rc;
-        if (got != dsize) return fail(GC_EFORMAT, "an entry decoded to the wrong size");
-        pos += csize;
-        o += got;
+        const uint64_t csize = get_u64(in + 28 + 8 * c);
+        if (csize > n - p) return fail(GC_EFORMAT, "truncated payload");  // read_exact, decompress.rs:81
+        pos[c] = p;
+        dsz[c] = (c + 1 < nch) ? ch : orig - (nch - 1) * ch;  // decompress.rs:70-73
+        p += csize;
     }
-    *out_len = o;
+    std::vector<int> errs(nch, 0);
+    std::vector<std::string> msgs(nch);
+    parallel_for(nch, default_threads(), [&](size_t c) {
+        const uint64_t csize = (c + 1 < nch ? pos[c + 1] : p) - pos[c];
+        size_t got = 0;
+        uint8_t* dst = out + (size_t)c * ch;
+        int r = algo == GC_ALGO_LZMA2 ? gc_lzma2_raw_decode(in + pos[c], csize, dst, dsz[c], &got)
+                                      : gc_zstd_frame_decode(in + pos[c], csize, dst, dsz[c], &got);
+        if (!r && got != dsz[c]) {
+            r = GC_EFORMAT;
+            set_error("an entry decoded to the wrong size");
+        }
+        if (r) {
+            errs[c] = r;
+            msgs[c] = gc_last_error();  // (the message is per thread)
+        }
+    });
+    for (uint64_t c = 0; c < nch; c++)
+        if (errs[c]) return fail(errs[c], msgs[c]);
+    *out_len = (size_t)orig;
     return GC_OK;
 }
 
