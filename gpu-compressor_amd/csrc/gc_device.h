@@ -10,6 +10,14 @@
 #define GC_CAND_THREADS 512
 #endif
 
+// Segmented Zstd parse: wavefronts (segments) per sub-block, and per
+// workgroup two record buffers of GC_SEG_REC_CAP 8-byte records per segment
+// (a segment is at most 65536 / GC_SEG_WAVES = 4096 bytes, so it holds at
+// most 1366 sequences of 3 bytes or more).
+#define GC_SEG_WAVES 16u
+#define GC_SEG_REC_CAP 1376u
+#define GC_SEG_SCRATCH ((uint64_t)GC_SEG_WAVES * 2u * GC_SEG_REC_CAP * 8u)
+
 // Sequence output formats of the Zstd parse kernel.
 #define GC_SEQ_ZSTD 0u  // ZSTD_Sequence {offset, litLength, matchLength, rep}
 #define GC_SEQ_REF 1u   // reference device record {literal_length, match_length - 3, offset}

@@ -78,6 +78,9 @@ int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, u
         rc = launch_zstd_info(ctx, ctx->stream, d_in, d_items, nsub, nsub, (const uint16_t*)ctx->cand.p,
                               (uint32_t*)ctx->info.p);
     if (rc) return rc;
+    if (nsub <= ctx->seg_max_sub)  // too few sub-blocks to fill the GPU with one walk each
+        return launch_zstd_parse_seg(ctx, ctx->stream, d_in, d_items, nsub, (const uint32_t*)ctx->info.p, d_seq, fmt,
+                                     d_lit, d_sc, d_lc, max_seq, depth);
     return launch_zstd_parse(ctx, ctx->stream, d_in, d_items, nsub, (uint32_t)div_up(nsub, 4),
                              (const uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq);
 }

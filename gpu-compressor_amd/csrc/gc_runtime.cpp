@@ -121,6 +121,23 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
     return launch(stream, ctx->k.zstd_parse[ref][lits], grid, 256, args, "launch gc_zstd_parse_kernel");
 }
 
+int launch_zstd_parse_seg(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items,
+                          uint32_t n_items, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
+                          uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq, uint32_t depth) {
+    // two workgroups per CU are resident at most (66 KiB of LDS each); the
+    // grid takes items grid-stride, so the record buffers stay bounded
+    const uint32_t cap = 2u * (uint32_t)ctx->cu_count;
+    uint32_t grid = n_items < cap ? n_items : cap;
+    if (grid == 0) return GC_OK;
+    int rc = ctx->seg.ensure((size_t)grid * GC_SEG_SCRATCH);
+    if (rc) return rc;
+    void* scratch = ctx->seg.p;
+    void* args[] = {&arena, &items, &n_items, &info, &seq, &seq_fmt, &lit, &seq_counts, &lit_counts, &max_seq,
+                    &scratch};
+    return launch(stream, ctx->k.zstd_parse_seg[depth > 1 ? 1 : 0], grid, GC_SEG_WAVES * 64, args,
+                  "launch gc_zstd_parse_seg_kernel");
+}
+
 int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                        const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
                        const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
@@ -263,6 +280,10 @@ gc_ctx* gc_open(int device) {
     for (int r = 0; r < 2; r++)
         for (int l = 0; l < 2; l++)
             if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_parse[r][l], ctx->k.module_parse, parse_names[r][l]);
+    if (e == hipSuccess)
+        e = hipModuleGetFunction(&ctx->k.zstd_parse_seg[0], ctx->k.module_parse, "gc_zstd_parse_seg1_kernel");
+    if (e == hipSuccess)
+        e = hipModuleGetFunction(&ctx->k.zstd_parse_seg[1], ctx->k.module_parse, "gc_zstd_parse_seg_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.fill_u32, ctx->k.module, "gc_fill_u32");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_seqs, ctx->k.module, "gc_gather_seqs");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.copy_u32, ctx->k.module, "gc_copy_u32");
@@ -287,10 +308,17 @@ gc_ctx* gc_open(int device) {
         gc_close(ctx);
         return nullptr;
     }
+    ctx->seg_max_sub = GC_SEG_DEFAULT_MAX_SUB;
     // the allocation-free entry point's scratch on this device (a failure is
     // reported by that entry point; every other entry point works without it)
     (void)noalloc_attach(device);
     return ctx;
+}
+
+int gc_set_segmented_parse(gc_ctx* ctx, uint32_t max_sub_blocks) {
+    if (!ctx) return fail(GC_EINVAL, "null context");
+    ctx->seg_max_sub = max_sub_blocks;
+    return GC_OK;
 }
 
 void gc_close(gc_ctx* ctx) {
@@ -307,6 +335,7 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_d.release();
     ctx->out_e.release();
     ctx->out_f.release();
+    ctx->seg.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
     if (ctx->k.module_parse) (void)hipModuleUnload(ctx->k.module_parse);
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);

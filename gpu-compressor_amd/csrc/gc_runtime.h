@@ -39,6 +39,7 @@ struct Kernels {
     hipFunction_t zstd_info = nullptr;
     hipFunction_t zstd_info4 = nullptr;
     hipFunction_t zstd_parse[2][2] = {};  // [reference layout][literals]
+    hipFunction_t zstd_parse_seg[2] = {};  // segmented parse [lazy step]
     hipFunction_t fill_u32 = nullptr;
     hipFunction_t gather_seqs = nullptr;
     hipFunction_t copy_u32 = nullptr;
@@ -74,6 +75,10 @@ struct gc_ctx {
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
     gc::DevBuf out_a, out_b, out_c, out_d, out_e, out_f;
+    gc::DevBuf seg;        // record buffers of the segmented parse
+    // launches of at most this many sub-blocks parse with the segmented
+    // kernel (0: never); larger ones fill the GPU with one walk per wavefront
+    uint32_t seg_max_sub = 0;
     std::mutex mu;
 };
 
@@ -93,6 +98,11 @@ int launch_zstd_parse(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, con
                       uint32_t grid, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
                       uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq,
                       const uint32_t* order = nullptr);
+// Segmented parse (one workgroup of GC_SEG_WAVES wavefronts per item); the
+// context's record buffers are sized for the grid here.
+int launch_zstd_parse_seg(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items,
+                          uint32_t n_items, const uint32_t* info, void* seq, uint32_t seq_fmt, uint8_t* lit,
+                          uint32_t* seq_counts, uint32_t* lit_counts, uint32_t max_seq, uint32_t depth);
 int launch_fill(gc_ctx* ctx, hipStream_t stream, uint32_t* p, uint32_t v, uint64_t n);
 int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint32_t* d_counts, const uint64_t* d_offsets,
                   void* d_out, uint32_t nsub, uint32_t max_seq);

@@ -1058,16 +1058,84 @@ __device__ __forceinline__ uint32_t ring_w24(const uint8_t* ring, uint32_t p) {
     return __builtin_amdgcn_alignbyte(r[1], r[0], i & 3) & 0xFFFFFFu;
 }
 
-template <bool SAFE, uint32_t FMT, bool LITS, bool LAZY>
+// A segment of a sub-block's parse (the segmented form, gc_zstd_parse_seg_kernel):
+// the walk starts at `start` with the given repeat offsets, makes every decision
+// at a position below `stop`, writes its sequences as (head, offset | length
+// << 16) records and reports how it ended: the end of its last sequence when
+// that passes `stop`, else `stop` (a literal run reached it), and the offsets.
+//
+// A fix-up walk (SEG_FIX) also has the records of an earlier walk of the same
+// segment from another start state (xs, x0..x2): its states are that start and
+// the (end, offsets) after each of its records.  The parse's future from a
+// state depends on nothing else, so once the fix-up walk reaches one of those
+// states the earlier walk's records from there on are the fix-up walk's own:
+// it stops and reports which (conv: the index of the first such record).
+enum { SEG_NONE = 0, SEG_SPEC = 1, SEG_FIX = 2 };
+struct SegWalk {
+    uint32_t start, r0, r1, r2, stop;  // in
+    uint2* rec;                        // in: room for (stop - start) / 3 + 2 records
+    const uint2* srec;                 // in (SEG_FIX): the earlier walk's records
+    uint32_t scnt, xs, x0, x1, x2;     // in (SEG_FIX): their count and the earlier walk's start state
+    uint32_t count, ypos, y0, y1, y2;  // out (wavefront-uniform)
+    uint32_t conv;                     // out (SEG_FIX): first record of the earlier walk that follows, or ~0u
+};
+
+template <bool SAFE, uint32_t FMT, bool LITS, bool LAZY, uint32_t SEG = SEG_NONE>
 __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t* arena, const uint32_t* info,
                                                 void* seq_out, uint8_t* literals, uint32_t* seq_counts,
-                                                uint32_t* lit_counts, uint32_t max_seq, uint8_t* ring) {
+                                                uint32_t* lit_counts, uint32_t max_seq, uint8_t* ring,
+                                                SegWalk* sw = nullptr) {
     const SrcView src = src_view(arena, it.src_off, it.avail);
     const uint32_t n = it.n;
     const uint32_t lane = lane_id();
     const uint8_t* inf = (const uint8_t*)(info + it.opos);
     uint32_t rep0 = 1, rep1 = 4, rep2 = 8;  // zstd_compress.cu:225
     uint32_t pos = 0, nseq = 0, wbase = 0;  // the window is [wbase, wbase + WAVE)
+    uint32_t last_end = 0;                  // SEG: end of the last sequence (the start while there is none)
+    if constexpr (SEG != SEG_NONE) {
+        rep0 = sw->r0, rep1 = sw->r1, rep2 = sw->r2;
+        pos = sw->start;
+        wbase = pos & ~(WAVE - 1);
+        last_end = pos;
+    }
+    // SEG_FIX: the earlier walk's records 64 at a time from record cb: lane l
+    // holds record cb + l's end (~0u past the last) and offset; po1 and po2
+    // are the offsets of records cb - 1 and cb - 2 (before the first record,
+    // the earlier start's offsets).  States are compared at the ends of this
+    // walk's sequences, whose positions only grow.
+    uint32_t cb = 0, ce = 0, co = 0, po1 = 0, po2 = 0, conv = ~0u;
+    auto conv_load = [&]() {
+        const uint32_t i = cb + lane;
+        const uint2 r = i < sw->scnt ? sw->srec[i] : make_uint2(0u, 0u);
+        ce = i < sw->scnt ? r.x + (r.y >> 16) : ~0u;
+        co = r.y & 0xFFFFu;
+    };
+    // is (p, rep0, rep1, rep2) a state of the earlier walk? (sets conv)
+    auto conv_at = [&](uint32_t p) -> bool {
+        while (bcast(ce, WAVE - 1) < p) {  // (a lane past the last record reads ~0u: the loop ends)
+            po1 = bcast(co, WAVE - 1);
+            po2 = bcast(co, WAVE - 2);
+            cb += WAVE;
+            conv_load();
+        }
+        const uint64_t m = ballot(ce == p);
+        if (!m) return false;
+        const uint32_t k = (uint32_t)__builtin_ctzll(m);
+        const uint32_t o0 = bcast(co, k);
+        const uint32_t o1 = k >= 1 ? bcast(co, k - 1) : po1;
+        const uint32_t o2 = k >= 2 ? bcast(co, k - 2) : (k == 1 ? po1 : po2);
+        if (o0 != rep0 || o1 != rep1 || o2 != rep2) return false;
+        conv = cb + k + 1;
+        return true;
+    };
+    if constexpr (SEG == SEG_FIX) {
+        po1 = sw->x0, po2 = sw->x1;
+        conv_load();
+        if (pos == sw->xs && rep0 == sw->x0 && rep1 == sw->x1 && rep2 == sw->x2)
+            conv = 0;  // the earlier walk's own start
+        else
+            conv_at(pos);
+    }
     // literal bookkeeping for the literal copy, in vector registers (the same
     // in every lane): the scalar register count limits this kernel's occupancy
     uint32_t lit_start = in_vgpr(0u), nlit = in_vgpr(0u);
@@ -1083,6 +1151,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     uint32_t batch_end = in_vgpr(0u);  // end of the last sequence before the staged ones
     uint32_t matched = in_vgpr(0u);    // bytes covered by the stored sequences
     auto store_staged = [&](uint32_t first, uint32_t cnt) {  // sequences first .. first + cnt - 1
+        if constexpr (SEG != SEG_NONE) {
+            if (lane < cnt) sw->rec[first + lane] = make_uint2(q_h, q_ol);
+            return;
+        }
         const uint32_t len = q_ol >> 16;
         const uint32_t end = q_h + len;
         const uint32_t ll = q_h - wave_shr1(end, batch_end);
@@ -1213,9 +1285,23 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
         __asm__ volatile("" ::: "memory");  // ring writes before the reads
     };
-    // heads satisfy pos + 3 < n; the walk ends by lowering lim
+    // heads satisfy pos + 3 < n; the walk ends by lowering lim (SEG: and below stop)
     uint32_t lim = (n > 3 && max_seq > 0) ? n - 3 : 0;
-    if (lim) {
+    if constexpr (SEG != SEG_NONE) {
+        lim = min(lim, sw->stop);
+        if (conv != ~0u) lim = 0;  // (SEG_FIX) nothing to walk
+        if (pos < lim) {  // the window holding the start, the ring filled as far back as it reaches
+            xv = wbase + lane;
+            peek(xv, e, e1, w);
+            finish(xv);
+            enter(wbase, wbase + WAVE < PARSE_RING ? 0u : wbase + WAVE - PARSE_RING);
+            const bool far = any_far();
+            f = flags(xv, w, false, far);
+            __asm__ volatile("" ::: "memory");
+            load_shadows(far);
+            peek(wbase + WAVE + lane, ae, ae1, aw);
+        }
+    } else if (lim) {
         peek(lane, e, e1, w);
         finish(lane);
         enter(0, 0);
@@ -1323,6 +1409,12 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         } else {
             const uint32_t k = (uint32_t)__builtin_ctzll(m);
             uint32_t head = wbase + k;
+            if constexpr (SEG != SEG_NONE) {
+                if (head >= lim) {  // the next head is the next segment's: a literal run reaches stop
+                    pos = lim;
+                    continue;
+                }
+            }
             const uint32_t ek = bcast(e, k);
             uint32_t len, off, ol;  // ol: the staged record word, off | len << 16
             if (ek > 0xFFFFu) {
@@ -1397,6 +1489,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             rep1 = rep0;
             rep0 = off;
             pos = head + len;
+            if constexpr (SEG != SEG_NONE) last_end = pos;
+            if constexpr (SEG == SEG_FIX) {
+                if (conv_at(pos)) break;  // the earlier walk's records follow from here
+            }
             if constexpr (LITS) lit_start = in_vgpr(pos);
             if (pos >= wbase + WAVE) {
                 // the walk left the window: move to the window holding pos
@@ -1424,6 +1520,13 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
     }
     if (nseq & (WAVE - 1)) store_staged(nseq & ~(uint32_t)(WAVE - 1), nseq & (WAVE - 1));
+    if constexpr (SEG != SEG_NONE) {
+        sw->count = nseq;
+        sw->conv = conv;
+        sw->ypos = last_end >= sw->stop ? last_end : sw->stop;
+        sw->y0 = rep0, sw->y1 = rep1, sw->y2 = rep2;
+        return;
+    }
     // trailing literals (zstd_compress.cu:341-344): every byte not covered
     // by a sequence is a literal
     const uint32_t lit_end = nseq ? uniform(batch_end) : 0u;
@@ -1552,6 +1655,214 @@ gc_zstd_parse_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items,
     __shared__ uint32_t ring_mem[PARSE_WAVES][RING_BYTES / 4];
     zstd_parse_queue<GC_SEQ_ZSTD, false>(arena, items, q, info, seq_out, seq_counts, max_seq, order, ring_mem);
 }
+
+// ---------------------------------------------------------------------------
+// Segmented parse: one workgroup per sub-block, its walk split over
+// GC_SEG_WAVES wavefronts.  Used where a launch has too few sub-blocks to
+// fill the GPU with one walk per wavefront (one chunk of the plain drop-in):
+// the serial walk's length, not the GPU's throughput, then sets the time.
+//
+// The parse's decisions from a state (a position where it looks for the next
+// sequence, and the three repeat offsets) depend on nothing else: a
+// position's hash candidate is a pure function of the sub-block.  So:
+//   1. every wavefront j walks segment j (a 64-aligned range of positions)
+//      from a guessed state: its first position with the initial offsets
+//      (segment 0's guess is the parse's true start);
+//   2. rounds of fix-ups: segment j, when the end state of segment j - 1
+//      differs from the state its own walk started from, walks again from
+//      that end state, until it reaches a state of its earlier walk (then
+//      the earlier walk's records from there on stand: usually within a few
+//      sequences) or its end (then the new walk replaces the earlier one, and
+//      a changed end state sends segment j + 1 into the next round).  All
+//      fix-ups of a round run at once; once segment j - 1 is final, segment j
+//      is after its next fix-up, so there are at most GC_SEG_WAVES - 1 rounds;
+//   3. each segment's records (the fix-up's, then the earlier walk's from the
+//      state where they met) are written out in order after a prefix sum of
+//      the segments' counts, with literal lengths, up to max_seq sequences
+//      (the serial parse stops emitting there), and the literals if asked.
+// The result is the serial parse's, record for record.
+//
+// Scratch: per workgroup, GC_SEG_WAVES x 2 record buffers of
+// GC_SEG_REC_CAP (head, offset | length << 16) records (GC_SEG_SCRATCH
+// bytes), the walk and its fix-up alternating between the two.
+// ---------------------------------------------------------------------------
+template <bool SAFE, bool LAZY, uint32_t MODE>
+__device__ __forceinline__ void seg_walk(const gc_item& it, const uint8_t* arena, const uint32_t* info, uint8_t* ring,
+                                         SegWalk* sw) {
+    zstd_parse_item<SAFE, GC_SEQ_ZSTD, false, LAZY, MODE>(it, arena, info, nullptr, nullptr, nullptr, nullptr, ~0u,
+                                                          ring, sw);
+}
+
+template <bool LAZY, uint32_t MODE>
+__device__ __forceinline__ void seg_walk_any(bool safe, const gc_item& it, const uint8_t* arena, const uint32_t* info,
+                                             uint8_t* ring, SegWalk* sw) {
+    if (safe)
+        seg_walk<true, LAZY, MODE>(it, arena, info, ring, sw);
+    else
+        seg_walk<false, LAZY, MODE>(it, arena, info, ring, sw);
+}
+
+template <bool LAZY>
+__device__ __forceinline__ void zstd_parse_seg(const uint8_t* arena, const gc_item* __restrict__ items,
+                                               uint32_t n_items, const uint32_t* info, void* seq_out, uint32_t fmt,
+                                               uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,
+                                               uint32_t max_seq, uint2* scratch) {
+    __shared__ uint32_t ring_mem[GC_SEG_WAVES][RING_BYTES / 4];
+    __shared__ uint32_t s_y[GC_SEG_WAVES][4];  // each segment's end state (position, offsets)
+    __shared__ uint32_t s_changed[GC_SEG_WAVES];
+    __shared__ uint32_t s_cnt[GC_SEG_WAVES], s_end[GC_SEG_WAVES], s_len[GC_SEG_WAVES];
+    __shared__ uint32_t s_out[2];  // sequences written, bytes they cover
+    __shared__ uint32_t s_last;    // end of the last sequence written
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const uint32_t lane = lane_id();
+    uint8_t* ring = (uint8_t*)ring_mem[wave];
+    uint2* bufs = scratch + ((uint64_t)blockIdx.x * GC_SEG_WAVES + wave) * 2 * GC_SEG_REC_CAP;
+
+    for (uint32_t k = blockIdx.x; k < n_items; k += gridDim.x) {
+        const gc_item it = load_item(items, k);
+        const uint32_t n = it.n;
+        const bool safe = item_safe(it.avail, n);
+        // segment bounds: 64-aligned, at most GC_SEG_LEN bytes (the last ones may be empty)
+        const uint32_t seg = (((n + GC_SEG_WAVES - 1) / GC_SEG_WAVES) + WAVE - 1) & ~(WAVE - 1);
+        const uint32_t start = min(wave * seg, n), stop = min(start + seg, n);
+
+        // 1. the walk from the guessed state
+        SegWalk sw;
+        sw.start = start, sw.r0 = 1, sw.r1 = 4, sw.r2 = 8, sw.stop = stop;
+        sw.rec = bufs;
+        seg_walk_any<LAZY, SEG_SPEC>(safe, it, arena, info, ring, &sw);
+        uint32_t xs = start, x0 = 1, x1 = 4, x2 = 8;  // the state the records in spec_buf start from
+        uint32_t spec_buf = 0, scnt = sw.count, fcnt = 0, tail = 0;
+        uint32_t yp = sw.ypos, y0 = sw.y0, y1 = sw.y1, y2 = sw.y2;
+        if (lane == 0) s_y[wave][0] = yp, s_y[wave][1] = y0, s_y[wave][2] = y1, s_y[wave][3] = y2;
+        __syncthreads();
+
+        // 2. fix-up rounds
+        const uint32_t pw = wave ? wave - 1 : 0;  // the segment before (indexes stay in bounds for wave 0)
+        bool need = wave > 0 && (s_y[pw][0] != xs || s_y[pw][1] != x0 || s_y[pw][2] != x1 || s_y[pw][3] != x2);
+        for (uint32_t round = 0; __syncthreads_or(need) && round < GC_SEG_WAVES; round++) {
+            uint32_t pp = 0, p0 = 0, p1 = 0, p2 = 0;
+            if (need) pp = s_y[pw][0], p0 = s_y[pw][1], p1 = s_y[pw][2], p2 = s_y[pw][3];
+            __syncthreads();  // every end state read before any is replaced
+            uint32_t changed = 0;
+            if (need) {
+                SegWalk fw;
+                fw.start = pp, fw.r0 = p0, fw.r1 = p1, fw.r2 = p2, fw.stop = stop;
+                fw.rec = bufs + (spec_buf ^ 1) * GC_SEG_REC_CAP;
+                fw.srec = bufs + spec_buf * GC_SEG_REC_CAP;
+                fw.scnt = scnt, fw.xs = xs, fw.x0 = x0, fw.x1 = x1, fw.x2 = x2;
+                seg_walk_any<LAZY, SEG_FIX>(safe, it, arena, info, ring, &fw);
+                if (fw.conv != ~0u) {  // met the earlier walk: its end state stands
+                    fcnt = fw.count;
+                    tail = fw.conv;
+                } else {  // a whole new walk of the segment
+                    spec_buf ^= 1;
+                    scnt = fw.count, fcnt = 0, tail = 0;
+                    xs = pp, x0 = p0, x1 = p1, x2 = p2;
+                    changed = (fw.ypos != yp || fw.y0 != y0 || fw.y1 != y1 || fw.y2 != y2) ? 1u : 0u;
+                    yp = fw.ypos, y0 = fw.y0, y1 = fw.y1, y2 = fw.y2;
+                    if (lane == 0) s_y[wave][0] = yp, s_y[wave][1] = y0, s_y[wave][2] = y1, s_y[wave][3] = y2;
+                }
+            }
+            if (lane == 0) s_changed[wave] = changed;
+            __syncthreads();
+            need = wave > 0 && s_changed[pw] != 0;
+        }
+
+        // 3. the segment's records: fix[0, fcnt) then spec[tail, scnt)
+        const uint2* fix = bufs + (spec_buf ^ 1) * GC_SEG_REC_CAP;
+        const uint2* spec = bufs + spec_buf * GC_SEG_REC_CAP;
+        const uint32_t cnt = fcnt + scnt - tail;
+        auto rec_at = [&](uint32_t i) -> uint2 { return i < fcnt ? fix[i] : spec[tail + i - fcnt]; };
+        uint32_t lens = 0;
+        for (uint32_t i0 = 0; i0 < cnt; i0 += WAVE) {
+            const uint32_t i = i0 + lane;
+            lens += i < cnt ? rec_at(i).y >> 16 : 0u;
+        }
+        lens = wave_sum(lens);
+        uint32_t last = 0;
+        if (cnt) {
+            const uint2 r = rec_at(cnt - 1);
+            last = r.x + (r.y >> 16);
+        }
+        if (lane == 0) {
+            s_cnt[wave] = cnt, s_end[wave] = last, s_len[wave] = lens;
+            if (wave == 0) s_out[0] = 0, s_out[1] = 0, s_last = 0;
+        }
+        __syncthreads();
+        uint32_t base = 0, mbase = 0, pend = 0;  // sequences, matched bytes and the last end before this segment
+        for (uint32_t j = 0; j < wave; j++) {
+            base += s_cnt[j];
+            mbase += s_len[j];
+            pend = s_cnt[j] ? s_end[j] : pend;
+        }
+        uint32_t emitted = 0, covered = 0;
+        uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
+        const SrcView sv = src_view(arena, it.src_off, it.avail);
+        const uint8_t* src_bytes = sv.base + sv.mis;
+        for (uint32_t i0 = 0; i0 < cnt && base + i0 < max_seq; i0 += WAVE) {
+            const uint32_t i = i0 + lane;
+            const bool valid = i < cnt && base + i < max_seq;
+            const uint2 r = valid ? rec_at(i) : make_uint2(0u, 0u);
+            const uint32_t len = r.y >> 16, off = r.y & 0xFFFFu;
+            const uint32_t end = r.x + len;
+            const uint32_t prev = wave_shr1(end, pend);
+            const uint32_t mcum = wave_scan(valid ? len : 0u);  // this segment's matched bytes up to record i
+            if (valid) {
+                const uint64_t si = it.seq_pos + base + i;
+                if (fmt == GC_SEQ_REF) {
+                    gc_ref_sequence* o = (gc_ref_sequence*)seq_out + si;
+                    o->literal_length = r.x - prev;
+                    o->match_length_minus3 = len - 3;
+                    o->offset = off;
+                } else {
+                    *(uint4*)((gc_zstd_sequence*)seq_out + si) = make_uint4(off, r.x - prev, len, 0);
+                }
+            }
+            const uint32_t nv = (uint32_t)__builtin_popcountll(ballot(valid));
+            if (lit_dst) {
+                // record i's literals [prev, head) go where the bytes before prev that no
+                // sequence covers end: prev - (matched bytes before record i)
+                const uint32_t dst = prev - (mbase + covered + mcum - len);
+                for (uint32_t l = 0; l < nv; l++) {
+                    const uint32_t a = bcast(prev, l), h = bcast(r.x, l), d = bcast(dst, l);
+                    if (h > a && d + (h - a) <= n) copy_wave(lit_dst + d, src_bytes + a, h - a);
+                }
+            }
+            covered += bcast(mcum, nv - 1);
+            emitted += nv;
+            pend = bcast(end, nv - 1);  // (valid lanes are a prefix, lane 0 among them)
+        }
+        if (lane == 0 && emitted) {
+            atomicAdd(&s_out[0], emitted);
+            atomicAdd(&s_out[1], covered);
+            atomicMax(&s_last, pend);
+        }
+        __syncthreads();
+        const uint32_t nseq = s_out[0], matched = s_out[1], lend = s_last;
+        if (threadIdx.x == 0) {
+            seq_counts[it.count_pos] = nseq;
+            if (lit_counts) lit_counts[it.count_pos] = n - matched;
+        }
+        // trailing literals: every byte after the last sequence written
+        if (lit_dst && n > lend && lend >= matched) {
+            uint8_t* d = lit_dst + (lend - matched);
+            for (uint32_t i = threadIdx.x; i < n - lend; i += blockDim.x) d[i] = src_bytes[lend + i];
+        }
+        __syncthreads();  // LDS reused by the next sub-block
+    }
+}
+
+#define GC_PARSE_SEG_KERNEL(NAME, LAZY)                                                                          \
+    extern "C" __global__ void __launch_bounds__(GC_SEG_WAVES* WAVE, GC_SEG_WAVES / 4)                                          \
+        NAME(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* info,   \
+             void* seq_out, uint32_t seq_fmt, uint8_t* literals, uint32_t* seq_counts, uint32_t* lit_counts,    \
+             uint32_t max_seq, uint2* scratch) {                                                                \
+        zstd_parse_seg<LAZY>(arena, items, n_items, info, seq_out, seq_fmt, literals, seq_counts, lit_counts,   \
+                             max_seq, scratch);                                                                 \
+    }
+GC_PARSE_SEG_KERNEL(gc_zstd_parse_seg_kernel, true)
+GC_PARSE_SEG_KERNEL(gc_zstd_parse_seg1_kernel, false)
 
 #endif  // GC_BUILD_PARSE
 

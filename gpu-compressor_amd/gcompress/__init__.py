@@ -50,6 +50,7 @@ SIGNATURES = [
     ("gc_dev_zero", C.c_int, [_vp, _vp, _sz]),
     ("gc_sync", C.c_int, [_vp]),
     ("gc_stream", _vp, [_vp]),
+    ("gc_set_segmented_parse", C.c_int, [_vp, C.c_uint32]),
     ("gc_zstd_match_find_dev", C.c_int,
      [_vp, _vp, _vp, C.c_uint32, _vp, _vp, _vp, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_zstd_match_find_dev_noalloc", C.c_int,
@@ -120,6 +121,7 @@ ALGO_LZMA2 = 1
 BATCH_STATIC = 0
 BATCH_QUEUE_GRAPH = 1
 BATCH_PERSISTENT = 2
+SEG_DEFAULT_MAX_SUB = 2048  # GC_SEG_DEFAULT_MAX_SUB
 
 
 class BatchInput(C.Structure):
@@ -231,6 +233,10 @@ class Context:
 
     def sync(self):
         _check(lib().gc_sync(self.h), "gc_sync")
+
+    def set_segmented_parse(self, max_sub_blocks: int):
+        """Calls of at most max_sub_blocks sub-chunks use the segmented parse (0: never)."""
+        _check(lib().gc_set_segmented_parse(self.h, max_sub_blocks), "gc_set_segmented_parse")
 
     # -- host-level finders ----------------------------------------------
     def lzma2_packed(self, chunk) -> np.ndarray:

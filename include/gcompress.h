@@ -96,6 +96,14 @@ int gc_sync(gc_ctx* ctx);
 /* The context's HIP stream (hipStream_t), for callers that order their own
  * work against it. */
 void* gc_stream(gc_ctx* ctx);
+/* No counterpart in the reference (one parse kernel there): the parse form
+ * of the per-call Zstd entry points.  A call of at most max_sub_blocks
+ * sub-chunks splits each sub-chunk's parse over 16 wavefronts that meet in
+ * fix-up rounds (the segmented parse, the same sequences); a larger call
+ * fills the GPU with one serial walk per wavefront.  0: never segmented.
+ * Default GC_SEG_DEFAULT_MAX_SUB. */
+#define GC_SEG_DEFAULT_MAX_SUB 2048u
+int gc_set_segmented_parse(gc_ctx* ctx, uint32_t max_sub_blocks);
 
 /* ---- kernel-level entry points (device pointers) ---------------------- */
 

@@ -30,3 +30,9 @@ def test_python_signatures_cover_header():
 
 def test_last_error_is_a_string():
     assert isinstance(gcompress.lib().gc_last_error(), bytes)
+
+
+def test_constants_match_header():
+    txt = open(os.path.join(ROOT, "include", "gcompress.h")).read()
+    m = re.search(r"#define GC_SEG_DEFAULT_MAX_SUB (\d+)u", txt)
+    assert m and int(m.group(1)) == gcompress.SEG_DEFAULT_MAX_SUB
