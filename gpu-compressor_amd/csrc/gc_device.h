@@ -9,13 +9,17 @@
 #ifndef GC_CAND_THREADS
 #define GC_CAND_THREADS 512
 #endif
+// The wide form of the candidate kernel (launches of few sub-blocks).
+#define GC_CAND_WIDE_THREADS 1024
 
 // Segmented Zstd parse: wavefronts (segments) per sub-block, and per
 // workgroup two record buffers of GC_SEG_REC_CAP 8-byte records per segment
 // (a segment is at most 65536 / GC_SEG_WAVES = 4096 bytes, so it holds at
 // most 1366 sequences of 3 bytes or more).
+#ifndef GC_SEG_WAVES
 #define GC_SEG_WAVES 16u
 #define GC_SEG_REC_CAP 1376u
+#endif
 #define GC_SEG_SCRATCH ((uint64_t)GC_SEG_WAVES * 2u * GC_SEG_REC_CAP * 8u)
 
 // Sequence output formats of the Zstd parse kernel.

@@ -94,6 +94,8 @@ static int launch(hipStream_t stream, hipFunction_t f, uint32_t grid, uint32_t b
 int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                 uint32_t grid, uint16_t* cand16) {
     void* args[] = {&arena, &items, &n_items, &cand16};
+    if (n_items <= 2u * (uint32_t)ctx->cu_count)
+        return launch(stream, ctx->k.cand_wide, grid, GC_CAND_WIDE_THREADS, args, "launch gc_cand_wide_kernel");
     return launch(stream, ctx->k.cand, grid, GC_CAND_THREADS, args, "launch gc_cand_kernel");
 }
 
@@ -269,6 +271,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipModuleLoadData(&ctx->k.module, gc_code_object);
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand, ctx->k.module, "gc_cand_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand_wide, ctx->k.module, "gc_cand_wide_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len, ctx->k.module, "gc_lzma2_len_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_ref, ctx->k.module, "gc_lzma2_len_ref_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_triage, ctx->k.module, "gc_lzma2_triage_kernel");

@@ -33,6 +33,7 @@ struct Kernels {
     hipModule_t module = nullptr;        // every kernel but the parse
     hipModule_t module_parse = nullptr;  // the Zstd parse kernels
     hipFunction_t cand = nullptr;
+    hipFunction_t cand_wide = nullptr;  // 1024 threads (launches of few sub-blocks)
     hipFunction_t lzma2_len = nullptr;      // packed words
     hipFunction_t lzma2_len_ref = nullptr;  // reference layout
     hipFunction_t lzma2_triage = nullptr;   // triage byte only
@@ -85,6 +86,8 @@ struct gc_ctx {
 namespace gc {
 
 // Launch helpers.  grid = workgroups; items are taken grid-stride.
+// (the wide form for launches of at most 2 sub-blocks per CU: each
+// workgroup's latency sets their time)
 int launch_cand(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                 uint32_t grid, uint16_t* cand16);
 int launch_lzma2_len(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,

@@ -56,6 +56,7 @@
 #define WAVE 64
 #define CAND_AWAVES (GC_CAND_THREADS / WAVE - 1)  // candidate kernel: analysis wavefronts (+ 1 table)
 #define CAND_THREADS GC_CAND_THREADS
+#define CAND_WIDE_AWAVES (GC_CAND_WIDE_THREADS / WAVE - 1)  // the wide form (launches of few sub-blocks)
 #define INFO_THREADS 1024                          // per-position kernels: one workgroup per sub-block
 #define STAGE_WORDS ((GC_SUB_BLOCK + 64) / 4)      // a sub-block staged in LDS, with 64 bytes to spare
 #define PARSE_WAVES 4
@@ -416,7 +417,8 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
 // Candidate kernel: cand16[it.cpos + p] = cand(p) (GC_NONE when there is none
 // or p + 3 >= n) for every position of every item.
 //
-// One workgroup per sub-block, tiles of CAND_TILE positions, two roles:
+// One workgroup per sub-block, tiles of 64 x (analysis wavefronts) positions,
+// two roles:
 //  * CAND_AWAVES analysis wavefronts, 64 positions each: hash, and group the
 //    lanes of the wavefront that share a hash, so that each position knows
 //    its predecessor inside its 64 (if any) and whether a later lane of the
@@ -429,12 +431,12 @@ __device__ __forceinline__ uint32_t lcp_wave(const SrcView& src, uint32_t a, uin
 // The analysis of tile t + 1 runs while the table wavefront works on tile t:
 // one barrier per tile.
 // ---------------------------------------------------------------------------
-#define CAND_TILE (CAND_AWAVES * WAVE)
-
+// AW analysis wavefronts: tiles of AW x 64 positions.
+template <uint32_t AW>
 struct CandLds {
-    uint16_t table[GC_TABLE];               // the reference's 14-bit table
-    uint32_t ana[2][CAND_TILE];             // analysis words, double buffered
-    uint8_t dup[CAND_AWAVES][GC_DUP_SIZE + 4];  // write/read-back duplicate test (+ a junk slot)
+    uint16_t table[GC_TABLE];          // the reference's 14-bit table
+    uint32_t ana[2][AW * WAVE];        // analysis words, double buffered
+    uint8_t dup[AW][GC_DUP_SIZE + 4];  // write/read-back duplicate test (+ a junk slot)
 };
 
 // Analysis word of a position, decoded as far as the table wavefront's
@@ -450,8 +452,9 @@ struct CandLds {
 
 // ALL: every lane's position is valid (p + 3 < n), as in all but a
 // sub-block's last tiles: the validity masks and selects drop out.
-template <bool ALL>
-__device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint32_t word, uint32_t p, uint32_t n) {
+template <bool ALL, uint32_t AW>
+__device__ __forceinline__ uint32_t cand_analyse(CandLds<AW>& s, uint32_t wave, uint32_t word, uint32_t p,
+                                                 uint32_t n) {
     const uint32_t lane = lane_id();
     const uint32_t h = hash4(word);
     const bool valid = ALL || p + 3 < n;
@@ -514,13 +517,15 @@ __device__ __forceinline__ uint32_t cand_analyse(CandLds& s, uint32_t wave, uint
 }
 
 // The table wavefront's work on one tile at t0 (analysis words in `ana`).
-__device__ __forceinline__ void cand_table(CandLds& s, const uint32_t* ana, uint32_t t0, uint32_t n, uint16_t* out) {
+template <uint32_t AW>
+__device__ __forceinline__ void cand_table(CandLds<AW>& s, const uint32_t* ana, uint32_t t0, uint32_t n,
+                                           uint16_t* out) {
     const uint32_t lane = lane_id();
-    uint32_t a[CAND_AWAVES], v[CAND_AWAVES];
+    uint32_t a[AW], v[AW];
 #pragma unroll
-    for (int j = 0; j < CAND_AWAVES; j++) a[j] = ana[j * WAVE + lane];
+    for (int j = 0; j < (int)AW; j++) a[j] = ana[j * WAVE + lane];
 #pragma unroll
-    for (int j = 0; j < CAND_AWAVES; j++) {
+    for (int j = 0; j < (int)AW; j++) {
         // every lane reads its entry (lanes whose candidate is decided already
         // ignore it); the last lane of each hash writes its position
         const uint16_t* e = (const uint16_t*)((const uint8_t*)s.table + (a[j] & (2 * GC_TABLE - 2)));
@@ -531,23 +536,24 @@ __device__ __forceinline__ void cand_table(CandLds& s, const uint32_t* ana, uint
     // tested per lane only in a sub-block's last tile)
     uint16_t* o = out + t0 + lane;
     const int32_t left = (int32_t)(n - t0);
-    if (left >= (int32_t)CAND_TILE) {
+    if (left >= (int32_t)(AW * WAVE)) {
 #pragma unroll
-        for (int j = 0; j < CAND_AWAVES; j++) {
+        for (int j = 0; j < (int)AW; j++) {
             const uint32_t c = (a[j] >> 15) & 0xFFFFu;
             o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < CAND_AWAVES; j++) {
+        for (int j = 0; j < (int)AW; j++) {
             const uint32_t c = (a[j] >> 15) & 0xFFFFu;
             if ((int32_t)lane < left - j * (int32_t)WAVE) o[j * WAVE] = (uint16_t)(c == ANA_FROM_TABLE ? v[j] : c);
         }
     }
 }
 
-template <bool SAFE>
-__device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const uint8_t* arena, uint16_t* cand16) {
+template <bool SAFE, uint32_t AW>
+__device__ __forceinline__ void cand_item(CandLds<AW>& s, const gc_item& it, const uint8_t* arena, uint16_t* cand16) {
+    constexpr uint32_t TILE = AW * WAVE, THREADS = TILE + WAVE;
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid / WAVE));
     const uint32_t lane = lane_id();
@@ -555,32 +561,32 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
     const uint32_t n = it.n;
     uint16_t* out = cand16 + it.cpos;
     uint32_t* t = (uint32_t*)s.table;
-    for (uint32_t k = tid; k < GC_TABLE / 2; k += CAND_THREADS) t[k] = 0xFFFFFFFFu;
+    for (uint32_t k = tid; k < GC_TABLE / 2; k += THREADS) t[k] = 0xFFFFFFFFu;
     // (lane 0's position decides for the wavefront whether all 64 are valid)
     auto analyse = [&](uint32_t word, uint32_t p) {
-        return uniform(p) + (WAVE - 1) + 3 < n ? cand_analyse<true>(s, wave, word, p, n)
-                                               : cand_analyse<false>(s, wave, word, p, n);
+        return uniform(p) + (WAVE - 1) + 3 < n ? cand_analyse<true, AW>(s, wave, word, p, n)
+                                               : cand_analyse<false, AW>(s, wave, word, p, n);
     };
     // analysis of tile 0; words two tiles ahead.  The loads are unconditional
     // (positions past the end read position 0, masked when used): a load
     // under a per-lane condition is waited for where the branches join.
     Raw32 next{0u, 0u, 0u};
-    if (wave < CAND_AWAVES) {
+    if (wave < AW) {
         const uint32_t p = wave * WAVE + lane;
         const uint32_t word = ld32<SAFE>(src, p + 3 < n ? p : 0u);
-        const uint32_t q = CAND_TILE + p;
+        const uint32_t q = TILE + p;
         next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
         s.ana[0][p] = analyse(word, p);
     }
     lds_barrier();
     uint32_t buf = 0;
-    for (uint32_t t0 = 0; t0 < n; t0 += CAND_TILE) {
-        if (wave < CAND_AWAVES) {
-            const uint32_t t1 = t0 + CAND_TILE;
+    for (uint32_t t0 = 0; t0 < n; t0 += TILE) {
+        if (wave < AW) {
+            const uint32_t t1 = t0 + TILE;
             if (t1 < n) {
                 const uint32_t p = t1 + wave * WAVE + lane;
                 const uint32_t word = next.word();
-                const uint32_t q = p + CAND_TILE;
+                const uint32_t q = p + TILE;
                 next = ld32_raw<SAFE>(src, q + 3 < n ? q : 0u);
                 s.ana[buf ^ 1][wave * WAVE + lane] = analyse(word, p);
             }
@@ -592,17 +598,32 @@ __device__ __forceinline__ void cand_item(CandLds& s, const gc_item& it, const u
     }
 }
 
-extern "C" __global__ void __launch_bounds__(CAND_THREADS)
-gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, uint16_t* cand16) {
-    __shared__ CandLds s;
+template <uint32_t AW>
+__device__ __forceinline__ void cand_items(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
+                                           uint16_t* cand16) {
+    __shared__ CandLds<AW> s;
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
         const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
-            cand_item<true>(s, it, arena, cand16);
+            cand_item<true, AW>(s, it, arena, cand16);
         else
-            cand_item<false>(s, it, arena, cand16);
+            cand_item<false, AW>(s, it, arena, cand16);
         lds_barrier();  // the table is reset for the next item
     }
+}
+
+extern "C" __global__ void __launch_bounds__(CAND_THREADS)
+gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, uint16_t* cand16) {
+    cand_items<CAND_AWAVES>(arena, items, n_items, cand16);
+}
+
+// The same with twice the analysis wavefronts per workgroup: half the tiles,
+// so half the serial steps per sub-block.  For launches of few sub-blocks,
+// where each workgroup's latency sets the time; with many sub-blocks the
+// narrower form keeps more workgroups resident per CU and is faster.
+extern "C" __global__ void __launch_bounds__(GC_CAND_WIDE_THREADS)
+gc_cand_wide_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, uint16_t* cand16) {
+    cand_items<CAND_WIDE_AWAVES>(arena, items, n_items, cand16);
 }
 
 // Persistent form: items from the device work queue (phase GC_Q_CAND).  The
@@ -610,14 +631,14 @@ gc_cand_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t
 // kernels), so that a kernel trace tells its ticks from batched passes.
 __device__ __forceinline__ void cand_queue(const uint8_t* arena, const gc_item* __restrict__ items, gc_queue* q,
                                            uint16_t* cand16) {
-    __shared__ CandLds s;
+    __shared__ CandLds<CAND_AWAVES> s;
     items += queue_base(q, GC_Q_CAND);
     wg_queue_items(q, GC_Q_CAND, [&](uint32_t i) {
         const gc_item it = load_item(items, i);
         if (item_safe(it.avail, it.n))
-            cand_item<true>(s, it, arena, cand16);
+            cand_item<true, CAND_AWAVES>(s, it, arena, cand16);
         else
-            cand_item<false>(s, it, arena, cand16);
+            cand_item<false, CAND_AWAVES>(s, it, arena, cand16);
         lds_barrier();
     });
 }
@@ -1668,14 +1689,20 @@ gc_zstd_parse_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items,
 //   1. every wavefront j walks segment j (a 64-aligned range of positions)
 //      from a guessed state: its first position with the initial offsets
 //      (segment 0's guess is the parse's true start);
-//   2. rounds of fix-ups: segment j, when the end state of segment j - 1
-//      differs from the state its own walk started from, walks again from
-//      that end state, until it reaches a state of its earlier walk (then
-//      the earlier walk's records from there on stand: usually within a few
-//      sequences) or its end (then the new walk replaces the earlier one, and
-//      a changed end state sends segment j + 1 into the next round).  All
-//      fix-ups of a round run at once; once segment j - 1 is final, segment j
-//      is after its next fix-up, so there are at most GC_SEG_WAVES - 1 rounds;
+//   2. rounds of fix-ups.  A walk that finds no sequence passes its start's
+//      offsets through to its end state.  Each round, segment j expects to
+//      start from the end state of the nearest earlier segment that has
+//      sequences (or segment 0's), moved on to j's first position; when its
+//      records start from another state, it walks again from the expected
+//      one until it reaches a state of its earlier walk (then that walk's
+//      records from there on stand: usually within a few sequences) or its
+//      end (then the new walk replaces the earlier one).  All fix-ups of a
+//      round run at once.  When no segment has to walk, the walks chain from
+//      segment 0's true start, so they are the serial walk; once the segments
+//      before j are final, j's expected start is its true one and j is final
+//      after its next fix-up, so there are at most GC_SEG_WAVES - 1 rounds.
+//      (Incompressible data with one rare sequence: every later segment must
+//      carry its offsets, and does so in one round, not one round each.)
 //   3. each segment's records (the fix-up's, then the earlier walk's from the
 //      state where they met) are written out in order after a prefix sum of
 //      the segments' counts, with literal lengths, up to max_seq sequences
@@ -1709,7 +1736,7 @@ __device__ __forceinline__ void zstd_parse_seg(const uint8_t* arena, const gc_it
                                                uint32_t max_seq, uint2* scratch) {
     __shared__ uint32_t ring_mem[GC_SEG_WAVES][RING_BYTES / 4];
     __shared__ uint32_t s_y[GC_SEG_WAVES][4];  // each segment's end state (position, offsets)
-    __shared__ uint32_t s_changed[GC_SEG_WAVES];
+    __shared__ uint32_t s_tr[GC_SEG_WAVES];  // 1: the segment's records are none (its end state passes its start's offsets)
     __shared__ uint32_t s_cnt[GC_SEG_WAVES], s_end[GC_SEG_WAVES], s_len[GC_SEG_WAVES];
     __shared__ uint32_t s_out[2];  // sequences written, bytes they cover
     __shared__ uint32_t s_last;    // end of the last sequence written
@@ -1722,7 +1749,7 @@ __device__ __forceinline__ void zstd_parse_seg(const uint8_t* arena, const gc_it
         const gc_item it = load_item(items, k);
         const uint32_t n = it.n;
         const bool safe = item_safe(it.avail, n);
-        // segment bounds: 64-aligned, at most GC_SEG_LEN bytes (the last ones may be empty)
+        // segment bounds: 64-aligned, at most 65536 / GC_SEG_WAVES bytes (the last ones may be empty)
         const uint32_t seg = (((n + GC_SEG_WAVES - 1) / GC_SEG_WAVES) + WAVE - 1) & ~(WAVE - 1);
         const uint32_t start = min(wave * seg, n), stop = min(start + seg, n);
 
@@ -1735,38 +1762,44 @@ __device__ __forceinline__ void zstd_parse_seg(const uint8_t* arena, const gc_it
         uint32_t spec_buf = 0, scnt = sw.count, fcnt = 0, tail = 0;
         uint32_t yp = sw.ypos, y0 = sw.y0, y1 = sw.y1, y2 = sw.y2;
         if (lane == 0) s_y[wave][0] = yp, s_y[wave][1] = y0, s_y[wave][2] = y1, s_y[wave][3] = y2;
-        __syncthreads();
 
-        // 2. fix-up rounds
-        const uint32_t pw = wave ? wave - 1 : 0;  // the segment before (indexes stay in bounds for wave 0)
-        bool need = wave > 0 && (s_y[pw][0] != xs || s_y[pw][1] != x0 || s_y[pw][2] != x1 || s_y[pw][3] != x2);
-        for (uint32_t round = 0; __syncthreads_or(need) && round < GC_SEG_WAVES; round++) {
-            uint32_t pp = 0, p0 = 0, p1 = 0, p2 = 0;
-            if (need) pp = s_y[pw][0], p0 = s_y[pw][1], p1 = s_y[pw][2], p2 = s_y[pw][3];
-            __syncthreads();  // every end state read before any is replaced
-            uint32_t changed = 0;
+        // 2. fix-up rounds.  Each segment's expected start: the end state of the
+        // nearest earlier segment that has sequences (or segment 0), moved on to
+        // this segment's start -- a segment without sequences passes its start's
+        // offsets through, so its successors need not wait for it to be redone.
+        // A segment whose current records start elsewhere walks again from there.
+        uint32_t wp = start, w0 = 1, w1 = 4, w2 = 8;  // the state the segment's records start from
+        for (uint32_t round = 0; round < GC_SEG_WAVES; round++) {
+            if (lane == 0) s_tr[wave] = (fcnt + scnt - tail) == 0 ? 1u : 0u;
+            __syncthreads();  // end states and record counts of the last round
+            uint32_t ep = 0, e0 = 0, e1 = 0, e2 = 0;
+            bool need = false;
+            if (wave > 0) {
+                uint32_t m = wave - 1;
+                while (m > 0 && s_tr[m]) m--;
+                ep = max(s_y[m][0], start), e0 = s_y[m][1], e1 = s_y[m][2], e2 = s_y[m][3];
+                need = ep != wp || e0 != w0 || e1 != w1 || e2 != w2;
+            }
+            if (!__syncthreads_or(need)) break;  // (also: every expected start read before any end state changes)
             if (need) {
                 SegWalk fw;
-                fw.start = pp, fw.r0 = p0, fw.r1 = p1, fw.r2 = p2, fw.stop = stop;
+                fw.start = ep, fw.r0 = e0, fw.r1 = e1, fw.r2 = e2, fw.stop = stop;
                 fw.rec = bufs + (spec_buf ^ 1) * GC_SEG_REC_CAP;
                 fw.srec = bufs + spec_buf * GC_SEG_REC_CAP;
                 fw.scnt = scnt, fw.xs = xs, fw.x0 = x0, fw.x1 = x1, fw.x2 = x2;
                 seg_walk_any<LAZY, SEG_FIX>(safe, it, arena, info, ring, &fw);
+                wp = ep, w0 = e0, w1 = e1, w2 = e2;
                 if (fw.conv != ~0u) {  // met the earlier walk: its end state stands
                     fcnt = fw.count;
                     tail = fw.conv;
                 } else {  // a whole new walk of the segment
                     spec_buf ^= 1;
                     scnt = fw.count, fcnt = 0, tail = 0;
-                    xs = pp, x0 = p0, x1 = p1, x2 = p2;
-                    changed = (fw.ypos != yp || fw.y0 != y0 || fw.y1 != y1 || fw.y2 != y2) ? 1u : 0u;
+                    xs = ep, x0 = e0, x1 = e1, x2 = e2;
                     yp = fw.ypos, y0 = fw.y0, y1 = fw.y1, y2 = fw.y2;
                     if (lane == 0) s_y[wave][0] = yp, s_y[wave][1] = y0, s_y[wave][2] = y1, s_y[wave][3] = y2;
                 }
             }
-            if (lane == 0) s_changed[wave] = changed;
-            __syncthreads();
-            need = wave > 0 && s_changed[pw] != 0;
         }
 
         // 3. the segment's records: fix[0, fcnt) then spec[tail, scnt)

@@ -3,16 +3,23 @@ zstd_compress.cu:230-338 split into segments that start from a guessed state and
 
 The parse's future from a decision point depends only on the position and the three repeat offsets
 (the hash candidate of a position is a pure function of the sub-block).  So:
-  * every segment j walks from a start state X_j (first: its first byte and offsets {1,4,8}; segment
+  * every segment j walks from a start state W_j (first: its first byte and offsets {1,4,8}; segment
     0's is the true start) up to its end e_j: it makes every decision at a position below e_j and stops
-    with Y_j = (the end of its last sequence when that passes e_j, else e_j; its offsets);
-  * in each fix-up round, every segment j whose predecessor's end state Y_{j-1} differs from the state
-    its records were walked from walks again from Y_{j-1}, and stops as soon as it reaches one of its
-    earlier walk's states (its start, or the end of one of its sequences with the same offsets): from
-    there on the earlier records are the true ones and Y_j stands.  A walk that reaches e_j without
-    meeting them replaces the earlier walk, and a new Y_j sends segment j + 1 into the next round;
-  * all fix-ups of a round start from the end states as they were when the round began; a segment is
-    final once its predecessor is, after its next fix-up, so there are at most (segments - 1) rounds.
+    with Y_j = (the end of its last sequence when that passes e_j, else e_j; its offsets).  A walk with
+    no sequence passes its start's offsets through: Y_j = (max(W_j's position, e_j), W_j's offsets);
+    such a segment is "transparent";
+  * each fix-up round first derives every segment's expected start S_j from the segments before it:
+    the end state of the nearest earlier segment k that is not transparent (or segment 0), moved on to
+    e_{j-1} -- what it would be if the transparent ones between stayed so for the new offsets.  Every
+    segment with W_j != S_j walks again from S_j, and stops as soon as it reaches one of its earlier
+    walk's states (that walk's start, or the end of one of its sequences with the same offsets): from
+    there on the earlier records are the true ones and the earlier Y_j stands.  A walk that reaches
+    e_j without meeting them replaces the earlier walk;
+  * once every S_j == W_j the walks are consistent and, from segment 0's true start, true.  When the
+    segments before j are final, S_j is j's true start, so j is final after its next fix-up: there are
+    at most (segments - 1) rounds.  A rare sequence in incompressible data, whose offsets every later
+    segment must carry (there is no sequence there to meet an earlier walk at), costs one round, not one
+    round per segment.
 The records that remain, in order, are the serial parse's sequences; the sequence cap keeps the first
 max_seq of them (the serial parse stops emitting there).  Used by tests/test_segment_model.py."""
 import numpy as np
@@ -111,35 +118,41 @@ def segmented(src, segments=16, depth=16, max_seq=None, stats=None):
     cand = candidates(src)
     starts, ends = bounds(src.size, segments)
     S = len(ends)
-    x = [(starts[j], (1, 4, 8)) for j in range(S)]
+    x = [(starts[j], (1, 4, 8)) for j in range(S)]  # the state the records in spec[j] start from
     spec, y = [], []
     for j in range(S):
         r, yy, _, _ = walk(src, cand, x[j][0], x[j][1], ends[j], depth)
         spec.append(r)
         y.append(yy)
+    w = list(x)  # the state the segment's current records start from
     fix, tail = [[] for _ in range(S)], [0] * S
-    need = [False] + [x[j] != y[j - 1] for j in range(1, S)]
     rounds = 0
-    while any(need):
-        assert rounds < S
-        ys = list(y)
-        changed = [False] * S
+    while True:
+        trans = [len(fix[j]) + len(spec[j]) - tail[j] == 0 for j in range(S)]
+        expect = [w[0]]
+        for j in range(1, S):
+            k = j - 1
+            while k > 0 and trans[k]:
+                k -= 1
+            expect.append((max(y[k][0], starts[j]), y[k][1]))
+        need = [j > 0 and expect[j] != w[j] for j in range(S)]
+        if not any(need):
+            break
+        assert rounds < S - 1
         for j in range(1, S):
             if not need[j]:
                 continue
             earlier = {}
             for t, st in enumerate(states(x[j], spec[j])):
                 earlier.setdefault(st, t)
-            r, yy, t, wk = walk(src, cand, ys[j - 1][0], ys[j - 1][1], ends[j], depth, earlier)
+            r, yy, t, wk = walk(src, cand, expect[j][0], expect[j][1], ends[j], depth, earlier)
             if stats is not None:
                 stats.append((rounds, j, wk, t is not None))
+            w[j] = expect[j]
             if t is not None:
                 fix[j], tail[j] = r, t
             else:
-                spec[j], fix[j], tail[j], x[j] = r, [], 0, ys[j - 1]
-                changed[j] = yy != y[j]
-                y[j] = yy
-        need = [False] + [changed[j - 1] for j in range(1, S)]
+                spec[j], fix[j], tail[j], x[j], y[j] = r, [], 0, expect[j], yy
         rounds += 1
     out = []
     for j in range(S):

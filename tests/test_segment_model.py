@@ -75,11 +75,29 @@ def test_long_match_spans_segments():
     assert got == serial(data, 16)
 
 
-def test_chain_of_fixups():
+def test_offsets_carried_through_transparent_segments():
     # random bytes with a rare planted repeat: the offsets it makes current travel through every later
-    # segment (no sequence there to resynchronise on), one fix-up round each
+    # segment (no sequence there to meet an earlier walk at); they are carried through the segments
+    # without sequences at once, so one fix-up round redoes every later segment together
     data = datagen.random_bytes(65536, 17).copy()
     data[3000:3010] = data[1000:1010]
-    got, rounds = M.segmented(data, 16, 16)
+    stats = []
+    got, rounds = M.segmented(data, 16, 16, stats=stats)
     assert got == serial(data, 16)
-    assert rounds == 15
+    assert rounds == 1 and sorted(j for (_, j, _, _) in stats) == list(range(1, 16))
+
+
+def test_transparent_guess_that_fails():
+    # 3-byte repeats at an offset that only the carried offsets hold (their 4th bytes differ, so no hash
+    # candidate finds them): segments that found nothing with the guessed offsets find sequences with
+    # the carried ones, the speculation past each is redone in a further round, and the result is exact
+    data = datagen.random_bytes(65536, 18).copy()
+    data[3000:3010] = data[1000:1010]  # makes offset 2000 current
+    for at in (30000, 45000, 52000):
+        data[at:at + 3] = data[at - 2000:at - 1997]
+        data[at + 3] = data[at - 1997] ^ 0x55
+    want = serial(data, 16)
+    assert [h for (h, ln, off) in want if off == 2000] == [3000, 30000, 45000, 52000]
+    got, rounds = M.segmented(data, 16, 16)
+    assert got == want
+    assert 2 <= rounds <= 15
