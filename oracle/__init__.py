@@ -30,8 +30,8 @@ def lib():
         L.orc_hash4.argtypes = [vp]
         L.orc_zstd_match_find.restype = C.c_int
         L.orc_zstd_match_find.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.c_int]
-        L.orc_zstd_frame_predef.restype = C.c_size_t
-        L.orc_zstd_frame_predef.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, C.c_size_t]
+        L.orc_zstd_frame.restype = C.c_size_t
+        L.orc_zstd_frame.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, C.c_size_t]
         L.orc_lzma2_match_find.restype = C.c_int
         L.orc_lzma2_match_find.argtypes = [vp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, C.c_int]
         _lib = L
@@ -105,16 +105,16 @@ def zstd_as_zstd_sequences(seq: np.ndarray, counts: np.ndarray):
     return out
 
 
-def zstd_frame_predef(sub, rows) -> bytes:
+def zstd_frame(sub, rows) -> bytes:
     """One Zstd frame for a sub-block (<= 65536 bytes) from its ZSTD_Sequence rows {offset, litLength,
-    matchLength, rep}, with the format's predefined FSE tables and raw literals: the checker of the GPU
-    sequence encoder."""
+    matchLength, rep}: per dimension the cheapest of predefined, RLE and FSE_Compressed tables, repeat
+    offset codes, Huffman or RLE literals when smaller than raw -- the checker of the GPU frame coder."""
     src = _u8(sub)
     r = np.ascontiguousarray(rows, dtype=np.uint32).reshape(-1, 4)
     cap = src.size + 64
     out = np.empty(cap, np.uint8)
-    n = lib().orc_zstd_frame_predef(src.ctypes.data, src.size, r.ctypes.data if r.size else None, r.shape[0],
-                                    out.ctypes.data, cap)
+    n = lib().orc_zstd_frame(src.ctypes.data, src.size, r.ctypes.data if r.size else None, r.shape[0],
+                             out.ctypes.data, cap)
     if n == 0:
         raise RuntimeError("frame did not fit")
     return out[:n].tobytes()
