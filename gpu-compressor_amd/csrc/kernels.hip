@@ -1901,30 +1901,36 @@ GC_PARSE_SEG_KERNEL(gc_zstd_parse_seg1_kernel, false)
 
 #if GC_BUILD_MAIN
 // ---------------------------------------------------------------------------
-// Zstd frames on the GPU (the capability of zstd_compress.cu:756-1038): one
-// frame per sub-block from its sequences and literals, the sequences coded
-// with the format's predefined FSE distributions (RFC 8878 3.1.1.3.2.2) and
-// the literals raw, or a raw block when that is not smaller.
+// Zstd frames on the GPU (the capability of zstd_compress.cu:756-1038, where
+// the coder builds per dimension an RLE or FSE_Compressed table from the
+// sub-block's own code histogram, :529-740, 835-903), widened: per dimension
+// the cheapest of the predefined, RLE and FSE_Compressed tables (estimated
+// bits; an accuracy log from the sequence count), offsets as repeat codes
+// where the format's offset history allows (RFC 8878 3.1.2.5), and literals
+// Huffman-coded in four streams (weights written directly, bytes below 129)
+// or RLE when smaller than raw; a raw block when nothing is smaller.
 //
-// One wavefront per sub-block.  The literal section is copied by the whole
-// wavefront.  Sequences are taken 64 at a time from the last: each lane
-// computes its sequence's three codes and extra-bit fields, then the FSE
-// state chain (serial by nature: each state follows from the next) runs on
-// the scalar unit, reading the lanes' fields with readlane and one LDS entry
-// per symbol: for (symbol, next state) the cell to move to and the bits to
-// write, tabulated per workgroup from the decoder's tables.  The bits go into
-// a 64-bit scalar accumulator; whole dwords are staged one per lane and
-// stored 256 bytes at a time.  Offsets are never coded as repeats
-// (Offset_Value = offset + 3).
+// One wavefront per sub-block.  Work that is small and serial -- ordering
+// the literal bytes by count, the Huffman merge, code lengths cut to 11 bits,
+// normalising a histogram, writing a table description, spreading a table's
+// cells -- runs in one lane over the wavefront's LDS workspace, step for step
+// as the oracle's coder (oracle/gc_oracle.c orc_zstd_frame) does it.  The
+// parallel work is the wavefront's: histograms, ranks, literal streams and
+// the sequences' bits, 64 at a time: every lane computes its field, a prefix
+// sum over the lanes places it, the lanes OR their bits into an LDS buffer
+// whose whole dwords then go to the frame.  Sequences are taken 64 at a time
+// from the last: the FSE state chain (each state follows from the next
+// sequence's) runs on the scalar unit with one LDS read per table and
+// sequence; the offset history runs forward, once to count the offset codes
+// (its state kept at every 64th sequence) and once per 64 from that state.
 // ---------------------------------------------------------------------------
 #define ZE_THREADS 256
 #define ZE_WAVES (ZE_THREADS / WAVE)
-#define ZE_ENC_ML (36 * 64)
-#define ZE_ENC_OF (36 * 64 + 53 * 64)
-#define ZE_ENC_SIZE (36 * 64 + 53 * 64 + 29 * 32)
-// per-wavefront bitstream buffer: carried bits (< 32) + 64 sequences of at
-// most 65 bits, + the 2 dwords past the last that a lane's 3-dword OR reaches
-#define ZE_BITBUF 136
+// per-wavefront bitstream buffer: carried bits (< 32) + 64 fields of at most
+// 74 bits (9 + 9 + 8 state bits, 16 + 16 + 16 extra bits), + the dwords past
+// the last that a lane's 4-dword OR reaches
+#define ZE_BITBUF 160
+#define ZE_MAXCHUNK ((GC_SUB_BLOCK / 3 + WAVE - 1) / WAVE)  // 64-sequence chunks of a sub-block
 
 __constant__ uint32_t ze_ll_base[36] = {0,  1,  2,   3,   4,   5,   6,    7,    8,    9,     10,    11,
                                         12, 13, 14,  15,  16,  18,  20,   22,   24,   28,    32,    40,
@@ -1945,71 +1951,348 @@ __constant__ int8_t ze_norm[36 + 53 + 29] = {
     1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1, -1, -1,
     1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, -1};
 
-struct ZeLds {
-    uint16_t enc[ZE_ENC_SIZE];  // per (symbol, next state): cell | bits << 6 | value << 9
-    uint8_t sym[160];           // decoder cells: LL 0..63, ML 64..127, OF 128..159
-    uint8_t nb[160];
-    uint16_t base[160];
+// Tables and symbol ranges: 0 LL (36 codes, table log <= 9), 1 ML (53, 9), 2 OF (32 slots, 29 predefined, 8).
+#define ZE_SYM0(t) ((t) == 0 ? 0u : ((t) == 1 ? 36u : 89u))  // first per-symbol slot
+#define ZE_ST0(t) ((t) == 0 ? 0u : ((t) == 1 ? 512u : 1024u))  // first next-state slot (dynamic tables)
+#define ZE_NSYM 121u                                           // 36 + 53 + 32
+
+// FSE encoding table seen from the encoder: st (next states of each
+// symbol's cells in cell order, state = table size + cell), per symbol dnb
+// (bits out = (state + dnb) >> 16) and dfs (first next-state index - the
+// symbol's normalized count).
+struct ZeShared {  // the predefined tables, built once per workgroup
+    uint16_t st[64 + 64 + 32];
+    uint32_t dnb[ZE_NSYM];
+    int32_t dfs[ZE_NSYM];
 };
 
-// table t (0 LL, 1 ML, 2 OF): first cell, accuracy log, symbols, first norm
-__device__ __forceinline__ void ze_table(uint32_t t, uint32_t& cell0, uint32_t& log, uint32_t& nsym, uint32_t& norm0) {
-    cell0 = t == 0 ? 0u : (t == 1 ? 64u : 128u);
-    log = t == 2 ? 5u : 6u;
-    nsym = t == 0 ? 36u : (t == 1 ? 53u : 29u);
-    norm0 = t == 0 ? 0u : (t == 1 ? 36u : 89u);
-}
+struct ZeWave {  // one wavefront's workspace
+    uint32_t bits[ZE_BITBUF];
+    uint32_t reps[ZE_MAXCHUNK][3];  // the offset history at each 64th sequence
+    union {
+        struct {  // literals
+            uint32_t hist[256];
+            uint32_t sc[256];  // counts in (count, byte) order
+            uint32_t nw[256];  // Huffman internal nodes' weights
+            uint16_t srt[256];
+            uint16_t code[256];
+            uint8_t lp[256], np[256], dn[256], len[256];
+        } h;
+        struct {  // sequences
+            uint32_t hist[ZE_NSYM];
+            uint32_t dnb[ZE_NSYM];
+            int32_t dfs[ZE_NSYM];
+            int16_t nc[ZE_NSYM];
+            uint16_t st[512 + 512 + 256];
+            uint8_t cell[512];
+            uint8_t desc[3][128];  // table descriptions as written
+        } f;
+    } u;
+};
 
-__device__ void ze_build(ZeLds& s) {
-    const uint32_t tid = threadIdx.x;
-    if (tid < 3) {  // the cells' symbols: "less than 1" symbols at the top, the rest spread
-        uint32_t c0, log, nsym, n0;
-        ze_table(tid, c0, log, nsym, n0);
-        const uint32_t size = 1u << log, mask = size - 1;
-        uint32_t high = size - 1;
-        for (uint32_t k = 0; k < nsym; k++)
-            if (ze_norm[n0 + k] == -1) s.sym[c0 + high--] = (uint8_t)k;
-        const uint32_t step = (size >> 1) + (size >> 3) + 3;
-        uint32_t pos = 0;
-        for (uint32_t k = 0; k < nsym; k++)
-            for (int r = 0; r < ze_norm[n0 + k]; r++) {
-                s.sym[c0 + pos] = (uint8_t)k;
-                do pos = (pos + step) & mask;
-                while (pos > high);
-            }
-    }
-    __syncthreads();
-    for (uint32_t c = tid; c < 160; c += blockDim.x) {  // bits and baseline of each cell
-        const uint32_t t = c < 64 ? 0u : (c < 128 ? 1u : 2u);
-        uint32_t c0, log, nsym, n0;
-        ze_table(t, c0, log, nsym, n0);
-        const uint32_t k = s.sym[c];
-        uint32_t x = ze_norm[n0 + k] == -1 ? 1u : (uint32_t)ze_norm[n0 + k];
-        for (uint32_t c2 = c0; c2 < c; c2++) x += s.sym[c2] == k ? 1u : 0u;
-        const uint32_t bits = log - (31u - (uint32_t)__builtin_clz(x));
-        s.nb[c] = (uint8_t)bits;
-        s.base[c] = (uint16_t)((x << bits) - (1u << log));
-    }
-    __syncthreads();
-    for (uint32_t e = tid; e < ZE_ENC_SIZE; e += blockDim.x) {  // the encoder's view
-        const uint32_t t = e < ZE_ENC_ML ? 0u : (e < ZE_ENC_OF ? 1u : 2u);
-        uint32_t c0, log, nsym, n0;
-        ze_table(t, c0, log, nsym, n0);
-        const uint32_t r = e - (t == 0 ? 0u : (t == 1 ? ZE_ENC_ML : ZE_ENC_OF));
-        const uint32_t k = r >> log, target = r & ((1u << log) - 1);
-        uint32_t v = 0;
-        for (uint32_t c = c0; c < c0 + (1u << log); c++)
-            if (s.sym[c] == k && s.base[c] <= target && target < (uint32_t)s.base[c] + (1u << s.nb[c]))
-                v = (c - c0) | ((uint32_t)s.nb[c] << 6) | ((target - s.base[c]) << 9);
-        s.enc[e] = (uint16_t)v;
-    }
-    __syncthreads();
+__device__ __forceinline__ uint32_t ze_highbit(uint32_t v) { return 31u - (uint32_t)__builtin_clz(v); }  // v >= 1
+
+// 256 x log2(x), linear between powers of two (the cost estimates)
+__device__ __forceinline__ uint32_t ze_log256(uint32_t x) {
+    const uint32_t h = ze_highbit(x);
+    return (h << 8) + (((x << 8) >> h) - 256u);
 }
 
 __device__ __forceinline__ uint32_t ze_code(const uint32_t* base, uint32_t ncode, uint32_t v) {
     uint32_t c = 0;
     for (uint32_t k = 1; k < ncode; k++) c = base[k] <= v ? k : c;
     return c;
+}
+
+// One table's encoder view from normalized counts nc[0, nsym) with accuracy
+// log `log` (run by one thread): cells spread as the decoder spreads them
+// ("less than 1" symbols at the top), next states in cell order per symbol.
+__device__ void ze_ctab_serial(const int16_t* nc, uint32_t nsym, uint32_t log, uint16_t* st, uint32_t* dnb,
+                               int32_t* dfs, uint8_t* cell) {
+    const uint32_t size = 1u << log, mask = size - 1;
+    uint32_t high = size - 1;
+    for (uint32_t s = 0; s < nsym; s++)
+        if (nc[s] == -1) cell[high--] = (uint8_t)s;
+    const uint32_t step = (size >> 1) + (size >> 3) + 3;
+    uint32_t pos = 0;
+    for (uint32_t s = 0; s < nsym; s++)
+        for (int k = 0; k < nc[s]; k++) {
+            cell[pos] = (uint8_t)s;
+            do pos = (pos + step) & mask;
+            while (pos > high);
+        }
+    // dfs first holds each symbol's first next-state index (a running cursor)
+    int32_t total = 0;
+    for (uint32_t s = 0; s < nsym; s++) {
+        dfs[s] = total;
+        total += nc[s] == -1 ? 1 : (nc[s] > 0 ? nc[s] : 0);
+    }
+    for (uint32_t u = 0; u < size; u++) st[dfs[cell[u]]++] = (uint16_t)(size + u);
+    total = 0;
+    for (uint32_t s = 0; s < nsym; s++) {
+        if (nc[s] == 0) {
+            dnb[s] = ((log + 1) << 16) - size, dfs[s] = 0;
+        } else if (nc[s] == -1 || nc[s] == 1) {
+            dnb[s] = (log << 16) - size, dfs[s] = total - 1, total += 1;
+        } else {
+            const uint32_t mb = log - ze_highbit((uint32_t)nc[s] - 1);
+            dnb[s] = (mb << 16) - ((uint32_t)nc[s] << mb), dfs[s] = total - nc[s], total += nc[s];
+        }
+    }
+}
+
+__device__ void ze_build(ZeShared& s, int16_t* tmp_nc, uint8_t* tmp_cell) {
+    const uint32_t tid = threadIdx.x;
+    if (tid < 3) {  // one thread per table
+        const uint32_t nsym = tid == 0 ? 36u : (tid == 1 ? 53u : 29u);
+        const uint32_t n0 = tid == 0 ? 0u : (tid == 1 ? 36u : 89u);
+        int16_t* nc = tmp_nc + ZE_SYM0(tid);
+        for (uint32_t k = 0; k < nsym; k++) nc[k] = ze_norm[n0 + k];
+        ze_ctab_serial(nc, nsym, tid == 2 ? 5u : 6u, s.st + (tid == 0 ? 0u : (tid == 1 ? 64u : 128u)),
+                       s.dnb + ZE_SYM0(tid), s.dfs + ZE_SYM0(tid), tmp_cell + 64 * tid);
+    }
+    __syncthreads();
+}
+
+// Bit writer of one wavefront: each lane's field (lo: bits 0-63, hi: 64-95,
+// nbit in all) is appended in lane order (rev: from the highest lane down);
+// whole dwords go to dst from byte wpos.  Fields past `limit` set `over`.
+struct ZeBits {
+    uint32_t* buf;
+    uint8_t* dst;
+    uint32_t carry, wpos, limit;
+    bool over;
+
+    __device__ void put(uint64_t lo, uint32_t hi, uint32_t nbit, bool rev) {
+        const uint32_t incl = wave_scan(nbit);
+        const uint32_t batch = bcast(incl, WAVE - 1);
+        const uint32_t at = carry + (rev ? batch - incl : incl - nbit);
+        const uint32_t d = at >> 5, sh = at & 31;
+        const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+        const uint32_t w0 = l0 << sh;
+        const uint32_t w1 = sh ? __builtin_amdgcn_alignbit(l1, l0, 32 - sh) : l1;
+        const uint32_t w2 = sh ? __builtin_amdgcn_alignbit(hi, l1, 32 - sh) : hi;
+        const uint32_t w3 = sh ? hi >> (32 - sh) : 0u;
+        if (nbit) {
+            atomicOr(&buf[d], w0);
+            atomicOr(&buf[d + 1], w1);
+            atomicOr(&buf[d + 2], w2);
+            atomicOr(&buf[d + 3], w3);
+        }
+        __asm__ volatile("" ::: "memory");
+        const uint32_t total = carry + batch, full = total >> 5;
+        if (wpos + 4 * full >= limit) {
+            over = true;
+        } else {
+            for (uint32_t k = lane_id(); k < full; k += WAVE) {
+                const uint32_t v = buf[k];
+                uint8_t* q = dst + wpos + 4 * k;
+                q[0] = (uint8_t)v, q[1] = (uint8_t)(v >> 8), q[2] = (uint8_t)(v >> 16), q[3] = (uint8_t)(v >> 24);
+            }
+        }
+        const uint32_t part = buf[full];
+        __asm__ volatile("" ::: "memory");
+        for (uint32_t k = lane_id(); k <= full + 3; k += WAVE) buf[k] = k == 0 ? part : 0u;
+        __asm__ volatile("" ::: "memory");
+        carry = total & 31;
+        wpos += 4 * full;
+    }
+    // a field of lane 0 only (the other lanes add nothing)
+    __device__ void put1(uint32_t v, uint32_t nb) {
+        const bool me = lane_id() == 0;
+        put(me ? (uint64_t)(nb < 32 ? v & ((1u << nb) - 1) : v) : 0ull, 0u, me ? nb : 0u, false);
+    }
+    // the end: the partial bits to the next byte boundary; returns the end position
+    __device__ uint32_t finish() {
+        const uint32_t tail = (carry + 7) / 8;
+        if (wpos + tail >= limit) over = true;
+        if (!over && lane_id() < tail) dst[wpos + lane_id()] = (uint8_t)(buf[lane_id() >> 2] >> (8 * (lane_id() & 3)));
+        return wpos + tail;
+    }
+    __device__ void reset(uint32_t at) {
+        for (uint32_t k = lane_id(); k < ZE_BITBUF; k += WAVE) buf[k] = 0;
+        __asm__ volatile("" ::: "memory");
+        carry = 0;
+        wpos = at;
+    }
+};
+
+// Huffman code lengths (at most 11 bits) of the present bytes, in one lane
+// (w.h.sc and w.h.srt already hold the n present bytes by (count, byte));
+// returns the longest.  As the oracle: two-queue merge, a leaf before a node
+// of equal weight; depths over 11 cut to 11, the code made complete again
+// one symbol at a time; lengths to the symbols in sorted order, the longest
+// to the least frequent.
+__device__ uint32_t ze_huff_lengths_serial(ZeWave& w, uint32_t n) {
+    uint32_t li = 0, ni = 0, nn = 0;
+    for (uint32_t k = 0; k + 1 < n; k++) {
+        uint32_t wt = 0;
+        for (int pick = 0; pick < 2; pick++) {
+            if (li < n && (ni >= nn || w.u.h.sc[li] <= w.u.h.nw[ni])) {
+                wt += w.u.h.sc[li];
+                w.u.h.lp[li++] = (uint8_t)nn;
+            } else {
+                wt += w.u.h.nw[ni];
+                w.u.h.np[ni++] = (uint8_t)nn;
+            }
+        }
+        w.u.h.nw[nn++] = wt;
+    }
+    w.u.h.dn[nn - 1] = 0;
+    for (int32_t j = (int32_t)nn - 2; j >= 0; j--) w.u.h.dn[j] = (uint8_t)min(w.u.h.dn[w.u.h.np[j]] + 1u, 200u);
+    uint32_t blc[12];
+    for (uint32_t l = 0; l < 12; l++) blc[l] = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t d = w.u.h.dn[w.u.h.lp[i]] + 1u;
+        blc[d < 11 ? d : 11]++;
+    }
+    uint32_t k2 = 0;
+    for (uint32_t l = 1; l <= 11; l++) k2 += blc[l] << (11 - l);
+    while (k2 > 2048) {
+        uint32_t l = 10;
+        while (blc[l] == 0) l--;
+        blc[l]--, blc[l + 1]++, k2 -= 1u << (10 - l);
+    }
+    while (k2 < 2048) {
+        uint32_t l = 11;
+        while (blc[l] == 0) l--;
+        blc[l]--, blc[l - 1]++, k2 += 1u << (11 - l);
+    }
+    uint32_t maxlen = 11;
+    while (blc[maxlen] == 0) maxlen--;
+    uint32_t i = 0;
+    for (uint32_t l = maxlen; l >= 1; l--)
+        for (uint32_t c = 0; c < blc[l]; c++) w.u.h.len[w.u.h.srt[i++]] = (uint8_t)l;
+    return maxlen;
+}
+
+// FSE table description of nc[0, nsym) (RFC 8878 4.1.1), in one lane, into
+// out (byte-aligned); returns its bytes.
+__device__ uint32_t ze_ncount_serial(const int16_t* nc, uint32_t nsym, uint32_t log, uint8_t* out) {
+    uint64_t acc = log - 5;
+    uint32_t bits = 4, o = 0;
+    auto put = [&](uint32_t v, uint32_t nb) {
+        acc |= (uint64_t)(v & ((1u << nb) - 1)) << bits;
+        bits += nb;
+        while (bits >= 8) out[o++] = (uint8_t)acc, acc >>= 8, bits -= 8;
+    };
+    int32_t remaining = (1 << log) + 1, threshold = 1 << log;
+    uint32_t nb = log + 1, s = 0;
+    bool prev0 = false;
+    while (s < nsym && remaining > 1) {
+        if (prev0) {
+            uint32_t z = 0;
+            while (s < nsym && nc[s] == 0) s++, z++;
+            while (z >= 3) put(3, 2), z -= 3;
+            put(z, 2);
+        }
+        const int32_t count = nc[s++];
+        const int32_t max = (2 * threshold - 1) - remaining;
+        remaining -= count < 0 ? -count : count;
+        int32_t v = count + 1;
+        if (v >= threshold) v += max;
+        put((uint32_t)v, nb - (v < max ? 1u : 0u));
+        prev0 = count == 0;
+        while (remaining < threshold) nb--, threshold >>= 1;
+    }
+    if (bits) out[o++] = (uint8_t)acc;
+    return o;
+}
+
+// Per-dimension choice for table t (in one lane): 0 predefined, 1 RLE
+// (*rle = the symbol), 2 FSE_Compressed (normalized counts in w.u.f.nc,
+// description in w.u.f.desc[t], *dlen bytes, accuracy log *log).
+__device__ uint32_t ze_choose_serial(ZeWave& w, uint32_t t, uint32_t nseq, uint32_t* log, uint32_t* rle,
+                                     uint32_t* dlen) {
+    const uint32_t s0 = ZE_SYM0(t), nsym = t == 0 ? 36u : (t == 1 ? 53u : 29u);
+    const uint32_t n0 = t == 0 ? 0u : (t == 1 ? 36u : 89u), plog = t == 2 ? 5u : 6u, maxlog = t == 2 ? 8u : 9u;
+    const uint32_t* hist = w.u.f.hist + s0;
+    int16_t* nc = w.u.f.nc + s0;
+    uint32_t npresent = 0, maxs = 0;
+    for (uint32_t s = 0; s < nsym; s++)
+        if (hist[s]) npresent++, maxs = s;
+    if (npresent == 1) {
+        *rle = maxs;
+        return 1;
+    }
+    uint64_t cp = 0;
+    for (uint32_t s = 0; s <= maxs; s++)
+        if (hist[s]) {
+            const int32_t pn = ze_norm[n0 + s];
+            cp += (uint64_t)hist[s] * ((plog << 8) - ze_log256(pn < 0 ? 1u : (uint32_t)pn));
+        }
+    uint32_t lg = ze_highbit(nseq);
+    lg = lg > 7 ? lg - 2 : 5;
+    const uint32_t need = ze_highbit(npresent - 1) + 2;
+    if (lg < need) lg = need;
+    if (lg > maxlog) lg = maxlog;
+    // normalize (as the oracle: rounded shares, at least 1, the difference from the first largest count)
+    const uint32_t size = 1u << lg;
+    int32_t dist = 0;
+    for (uint32_t s = 0; s <= maxs; s++) {
+        if (!hist[s]) {
+            nc[s] = 0;
+            continue;
+        }
+        uint32_t v = (uint32_t)(((uint64_t)hist[s] * size + nseq / 2) / nseq);
+        if (!v) v = 1;
+        nc[s] = (int16_t)v;
+        dist += (int32_t)v;
+    }
+    while (dist != (int32_t)size) {
+        const bool up = dist < (int32_t)size;
+        uint32_t best = 0, bc = 0;
+        for (uint32_t s = 0; s <= maxs; s++)
+            if (hist[s] > bc && (up || nc[s] > 1)) bc = hist[s], best = s;
+        nc[best] += up ? 1 : -1;
+        dist += up ? 1 : -1;
+    }
+    for (uint32_t s = maxs + 1; s < nsym; s++) nc[s] = 0;
+    *dlen = ze_ncount_serial(nc, maxs + 1, lg, w.u.f.desc[t]);
+    uint64_t cc = (uint64_t)*dlen * 8 * 256;
+    for (uint32_t s = 0; s <= maxs; s++)
+        if (hist[s]) cc += (uint64_t)hist[s] * ((lg << 8) - ze_log256((uint32_t)nc[s]));
+    *log = lg;
+    return cc < cp ? 2u : 0u;
+}
+
+// The offset history (RFC 8878 3.1.2.5) over lane j's sequence (uniform
+// off, ll): its Offset_Value, and the history after it.
+__device__ __forceinline__ uint32_t ze_rep_step(uint32_t off, uint32_t ll, uint32_t& r0, uint32_t& r1, uint32_t& r2) {
+    uint32_t v;
+    if (ll > 0) {
+        if (off == r0) {
+            v = 1;
+        } else if (off == r1) {
+            v = 2, r1 = r0, r0 = off;
+        } else if (off == r2) {
+            v = 3, r2 = r1, r1 = r0, r0 = off;
+        } else {
+            v = off + 3, r2 = r1, r1 = r0, r0 = off;
+        }
+    } else {
+        if (off == r1) {
+            v = 1, r1 = r0, r0 = off;
+        } else if (off == r2) {
+            v = 2, r2 = r1, r1 = r0, r0 = off;
+        } else if (off == r0 - 1) {
+            v = 3, r2 = r1, r1 = r0, r0 = off;
+        } else {
+            v = off + 3, r2 = r1, r1 = r0, r0 = off;
+        }
+    }
+    return v;
+}
+
+template <uint32_t FMT>
+__device__ __forceinline__ void ze_load_seq(const void* seqs, uint64_t r, uint32_t& off, uint32_t& ll, uint32_t& ml) {
+    if constexpr (FMT == GC_SEQ_REF) {
+        const gc_ref_sequence q = ((const gc_ref_sequence*)seqs)[r];
+        ll = q.literal_length, ml = q.match_length_minus3 + 3, off = q.offset;
+    } else {
+        const uint4 q = ((const uint4*)seqs)[r];
+        off = q.x, ll = q.y, ml = q.z;
+    }
 }
 
 __device__ __forceinline__ void ze_copy_bytes(uint8_t* dst, const uint8_t* src, uint32_t len) {
@@ -2022,8 +2305,127 @@ __device__ __forceinline__ void ze_copy_bytes(uint8_t* dst, const uint8_t* src, 
     for (; k < len; k += WAVE) dst[k] = src[k];
 }
 
+// The literals section at dst[o, limit) for lit[0, nlit): returns its end, or
+// `limit` when it does not fit (the block then goes raw).
+__device__ uint32_t ze_literals(ZeWave& w, ZeBits& bw, uint8_t* dst, uint32_t o, uint32_t limit, const uint8_t* lit,
+                                uint32_t nlit) {
+    const uint32_t lane = lane_id();
+    const uint32_t rh = nlit < 32 ? 1u : (nlit < 4096 ? 2u : 3u);
+    for (uint32_t k = lane; k < 256; k += WAVE) w.u.h.hist[k] = 0;
+    __asm__ volatile("" ::: "memory");
+    for (uint32_t k = lane; k < nlit; k += WAVE) atomicAdd(&w.u.h.hist[lit[k]], 1u);
+    __asm__ volatile("" ::: "memory");
+    uint32_t nz = 0, maxs = 0;
+    for (uint32_t k0 = 0; k0 < 256; k0 += WAVE) {
+        const uint64_t m = ballot(w.u.h.hist[k0 + lane] != 0);
+        nz += (uint32_t)__builtin_popcountll(m);
+        if (m) maxs = k0 + 63u - (uint32_t)__builtin_clzll(m);
+    }
+    uint32_t type = (nlit > 0 && nz == 1) ? 1u : 0u;
+    if (nlit >= 64 && nz >= 2 && maxs <= 128) {
+        for (uint32_t k = lane; k < 256; k += WAVE) w.u.h.len[k] = 0;
+        // rank of every present byte by (count, byte)
+        for (uint32_t k0 = 0; k0 < 256; k0 += WAVE) {
+            const uint32_t s = k0 + lane, hs = w.u.h.hist[s];
+            uint32_t rank = 0;
+            for (uint32_t t = 0; t < 256; t++) {
+                const uint32_t ht = w.u.h.hist[t];
+                rank += (ht != 0 && (ht < hs || (ht == hs && t < s))) ? 1u : 0u;
+            }
+            if (hs) {
+                w.u.h.srt[rank] = (uint16_t)s;
+                w.u.h.sc[rank] = hs;
+            }
+        }
+        __asm__ volatile("" ::: "memory");
+        uint32_t maxlen = 0;
+        if (lane == 0) maxlen = ze_huff_lengths_serial(w, nz);
+        maxlen = uniform(maxlen);
+        __asm__ volatile("" ::: "memory");
+        // canonical codes (the decoder's: weights from 1 up take consecutive
+        // table ranges, bytes in order within a weight)
+        if (lane == 0) {
+            uint32_t start[13], cntw[13];
+            for (uint32_t k = 0; k < 13; k++) cntw[k] = 0;
+            for (uint32_t s = 0; s < 256; s++)
+                if (w.u.h.len[s]) cntw[maxlen + 1 - w.u.h.len[s]]++;
+            uint32_t pos = 0;
+            for (uint32_t k = 1; k <= maxlen; k++) start[k] = pos, pos += cntw[k] << (k - 1);
+            for (uint32_t s = 0; s < 256; s++) {
+                const uint32_t l = w.u.h.len[s];
+                if (l) {
+                    const uint32_t wt = maxlen + 1 - l;
+                    w.u.h.code[s] = (uint16_t)(start[wt] >> (wt - 1));
+                    start[wt] += 1u << (wt - 1);
+                }
+            }
+        }
+        __asm__ volatile("" ::: "memory");
+        // stream sizes first: whether the section is smaller than raw, and its header size
+        const uint32_t seg = (nlit + 3) / 4;
+        uint32_t sbits[4];
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t a = k * seg, e = k == 3 ? nlit : (k + 1) * seg;
+            uint32_t b = 0;
+            for (uint32_t i = a + lane; i < e; i += WAVE) b += w.u.h.len[lit[i]];
+            sbits[k] = wave_sum(b) + 1;  // + the end mark
+        }
+        const uint32_t tree = 1 + (maxs + 1) / 2;
+        uint32_t cs = tree + 6;
+        for (uint32_t k = 0; k < 4; k++) cs += (sbits[k] + 7) / 8;
+        const uint32_t hh = (nlit <= 1023 && cs <= 1023) ? 3u : ((nlit <= 16383 && cs <= 16383) ? 4u : 5u);
+        if (hh + cs < rh + nlit) {
+            if (o + hh + cs >= limit) return limit;
+            const uint64_t h = hh == 3 ? (2ull | (1ull << 2) | ((uint64_t)nlit << 4) | ((uint64_t)cs << 14))
+                                       : (hh == 4 ? (2ull | (2ull << 2) | ((uint64_t)nlit << 4) | ((uint64_t)cs << 18))
+                                                  : (2ull | (3ull << 2) | ((uint64_t)nlit << 4) | ((uint64_t)cs << 22)));
+            if (lane < hh) dst[o + lane] = (uint8_t)(h >> (8 * lane));
+            o += hh;
+            // weights of bytes 0 .. maxs - 1, 4 bits each, the first in the high nibble
+            if (lane == 0) dst[o] = (uint8_t)(127 + maxs);
+            for (uint32_t s = 2 * lane; s < maxs; s += 2 * WAVE) {
+                const uint32_t l0 = w.u.h.len[s], l1 = s + 1 < maxs ? w.u.h.len[s + 1] : 0u;
+                const uint32_t w0 = l0 ? maxlen + 1 - l0 : 0u, w1 = l1 ? maxlen + 1 - l1 : 0u;
+                dst[o + 1 + s / 2] = (uint8_t)((w0 << 4) | w1);
+            }
+            o += tree;
+            const uint32_t jt = o;
+            o += 6;
+            for (uint32_t k = 0; k < 4; k++) {
+                const uint32_t a = k * seg, e = k == 3 ? nlit : (k + 1) * seg;
+                bw.reset(o);
+                // the stream codes the bytes from the last one back (the decoder reads it backward)
+                for (uint32_t q0 = 0; q0 < e - a; q0 += WAVE) {
+                    const uint32_t q = q0 + lane;
+                    uint32_t v = 0, nb = 0;
+                    if (q < e - a) {
+                        const uint32_t b = lit[e - 1 - q];
+                        v = w.u.h.code[b], nb = w.u.h.len[b];
+                    }
+                    bw.put(v, 0u, nb, false);
+                }
+                bw.put1(1, 1);  // end mark
+                const uint32_t end = bw.finish();
+                if (k < 3 && lane < 2) dst[jt + 2 * k + lane] = (uint8_t)((end - o) >> (8 * lane));
+                o = end;
+            }
+            return o;
+        }
+    }
+    const uint32_t body = type == 1 ? 1u : nlit;
+    if (o + rh + body >= limit) return limit;
+    const uint32_t v = rh == 1 ? (type | (nlit << 3)) : (rh == 2 ? (type | 4u | (nlit << 4)) : (type | 12u | (nlit << 4)));
+    if (lane < rh) dst[o + lane] = (uint8_t)(v >> (8 * lane));
+    if (type == 1) {
+        if (lane == 0) dst[o + rh] = lit[0];
+    } else {
+        ze_copy_bytes(dst + o + rh, lit, nlit);
+    }
+    return o + rh + body;
+}
+
 template <uint32_t FMT>
-__device__ void ze_frame(ZeLds& s, uint32_t* bits, const gc_item& it, const uint8_t* arena, const void* seqs,
+__device__ void ze_frame(const ZeShared& sh, ZeWave& w, const gc_item& it, const uint8_t* arena, const void* seqs,
                          const uint8_t* literals, const uint32_t* seq_counts, const uint32_t* lit_counts, uint8_t* out,
                          uint32_t* sizes, uint32_t max_frame, uint32_t max_seq) {
     const uint32_t lane = lane_id();
@@ -2039,147 +2441,177 @@ __device__ void ze_frame(ZeLds& s, uint32_t* bits, const gc_item& it, const uint
                                     : (lane == 4 ? (n >= 256 ? 0x60u : 0x20u) : ((fcs >> (8 * (lane - 5))) & 0xFF));
         dst[lane] = (uint8_t)b;
     }
-    const uint32_t start = hdr + 3;  // after the block header
+    const uint32_t start = hdr + 3;    // after the block header
     const uint32_t limit = start + n;  // a compressed block must end before this
-    bool raw = nlit + 2 >= n;         // literals alone would not be smaller
+    // (raw also when the counts cannot be a sub-block's: literals alone would not be smaller, or more
+    // sequences than 3-byte matches fit)
+    bool raw = nlit + 2 >= n || nseq > ZE_MAXCHUNK * WAVE;
     uint32_t end = 0;
+    ZeBits bw{w.bits, dst, 0, 0, limit, false};
     if (!raw) {
-        const uint32_t lh = nlit < 32 ? 1u : (nlit < 4096 ? 2u : 3u);
-        const uint32_t sh = nseq < 128 ? 1u : 2u;
-        // literal section header (raw literals, 5-, 12- or 20-bit size)
-        if (lane < lh) {
-            const uint32_t v = lh == 1 ? (nlit << 3) : (lh == 2 ? ((nlit << 4) | 4u) : ((nlit << 4) | 12u));
-            dst[start + lane] = (uint8_t)(v >> (8 * lane));
+        uint32_t o = ze_literals(w, bw, dst, start, limit, literals + it.lit_pos, nlit);
+        raw = bw.over || o + 4 > limit;
+        if (!raw) {
+            const uint32_t shd = nseq < 128 ? 1u : 2u;
+            if (lane < shd) {
+                const uint32_t v = shd == 1 ? nseq : (lane == 0 ? (nseq >> 8) + 0x80 : nseq & 0xFF);
+                dst[o + lane] = (uint8_t)v;
+            }
+            o += shd;
+            end = o;
         }
-        ze_copy_bytes(dst + start + lh, literals + it.lit_pos, nlit);
-        uint32_t o = start + lh + nlit;
-        if (lane < sh) {
-            const uint32_t v = sh == 1 ? nseq : (lane == 0 ? (nseq >> 8) + 0x80 : nseq & 0xFF);
-            dst[o + lane] = (uint8_t)v;
-        }
-        o += sh;
-        if (nseq) {
-            if (lane == 0) dst[o] = 0;  // predefined modes for LL, OF and ML
+        if (!raw && nseq) {
+            // forward: the offset history (kept at every 64th sequence) and the codes' histograms
+            for (uint32_t k = lane; k < ZE_NSYM; k += WAVE) w.u.f.hist[k] = 0;
+            __asm__ volatile("" ::: "memory");
+            uint32_t r0 = 1, r1 = 4, r2 = 8;
+            for (uint32_t i0 = 0; i0 < nseq; i0 += WAVE) {
+                const uint32_t cnt = min(nseq - i0, (uint32_t)WAVE);
+                if (lane == 0) w.reps[i0 / WAVE][0] = r0, w.reps[i0 / WAVE][1] = r1, w.reps[i0 / WAVE][2] = r2;
+                uint32_t off = 0, ll = 0, ml = 3;
+                if (lane < cnt) ze_load_seq<FMT>(seqs, it.seq_pos + i0 + lane, off, ll, ml);
+                uint32_t ov = 0;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    const uint32_t v = ze_rep_step(bcast(off, j), bcast(ll, j), r0, r1, r2);
+                    ov = lane == j ? v : ov;
+                }
+                if (lane < cnt) {
+                    atomicAdd(&w.u.f.hist[ze_code(ze_ll_base, 36, ll)], 1u);
+                    atomicAdd(&w.u.f.hist[36 + ze_code(ze_ml_base, 53, ml)], 1u);
+                    atomicAdd(&w.u.f.hist[89 + ze_highbit(ov)], 1u);
+                }
+            }
+            __asm__ volatile("" ::: "memory");
+            // per-dimension modes and tables (one lane; the descriptions into LDS)
+            uint32_t mode[3] = {0, 0, 0}, lg[3] = {6, 6, 5}, rs[3] = {0, 0, 0}, dl[3] = {0, 0, 0};
+            if (lane == 0) {
+                for (uint32_t t = 0; t < 3; t++) {
+                    mode[t] = ze_choose_serial(w, t, nseq, &lg[t], &rs[t], &dl[t]);
+                    if (mode[t] == 2)
+                        ze_ctab_serial(w.u.f.nc + ZE_SYM0(t), t == 0 ? 36u : (t == 1 ? 53u : 29u), lg[t],
+                                       w.u.f.st + ZE_ST0(t), w.u.f.dnb + ZE_SYM0(t), w.u.f.dfs + ZE_SYM0(t),
+                                       w.u.f.cell);
+                }
+            }
+            for (uint32_t t = 0; t < 3; t++) {
+                mode[t] = uniform(mode[t]), rs[t] = uniform(rs[t]), dl[t] = uniform(dl[t]);
+                lg[t] = mode[t] == 2 ? uniform(lg[t]) : (t == 2 ? 5u : 6u);
+            }
+            __asm__ volatile("" ::: "memory");
+            // modes byte, then LL, OF, ML: the RLE byte or the table description
+            if (lane == 0) dst[o] = (uint8_t)((mode[0] << 6) | (mode[2] << 4) | (mode[1] << 2));
             o += 1;
-        }
-        // The bitstream, 64 sequences at a time from the last: the FSE state
-        // chain (each state follows from the next sequence's) runs serially
-        // and records each sequence's three table entries in the sequence's
-        // lane; then every lane assembles its sequence's fields (state
-        // updates OF, ML, LL, extra bits LL, ML, OF: at most 65 bits), a
-        // prefix sum over the lanes gives each its bit offset (the last
-        // sequence first), and the lanes OR their bits into the wavefront's
-        // LDS buffer, whose whole dwords then go to the frame.  bits[0] holds
-        // the `carry` bits not yet written, which start at frame byte wpos.
-        uint32_t carry = 0, wpos = o;
-        bool over = false;
-        for (uint32_t k = lane; k < ZE_BITBUF; k += WAVE) bits[k] = 0;
-        __asm__ volatile("" ::: "memory");
-        uint32_t sl = 0, sm = 0, so = 0;
-        for (int32_t top = (int32_t)nseq; top > 0 && !over; top -= WAVE) {
-            const int32_t i0 = top > WAVE ? top - WAVE : 0;
-            const uint32_t cnt = (uint32_t)(top - i0);
-            uint32_t codes = 0, xl = 0, xm = 0, xo = 0;
-            if (lane < cnt) {
-                uint32_t off, ll, ml;
-                const uint64_t r = it.seq_pos + (uint64_t)i0 + lane;
-                if constexpr (FMT == GC_SEQ_REF) {
-                    const gc_ref_sequence q = ((const gc_ref_sequence*)seqs)[r];
-                    ll = q.literal_length, ml = q.match_length_minus3 + 3, off = q.offset;
-                } else {
-                    const uint4 q = ((const uint4*)seqs)[r];
-                    off = q.x, ll = q.y, ml = q.z;
+            const uint32_t order[3] = {0, 2, 1};
+            for (uint32_t q = 0; q < 3; q++) {
+                const uint32_t t = order[q];
+                if (mode[t] == 1) {
+                    if (lane == 0 && o < limit) dst[o] = (uint8_t)rs[t];
+                    o += 1;
+                } else if (mode[t] == 2) {
+                    if (o + dl[t] < limit)
+                        for (uint32_t k = lane; k < dl[t]; k += WAVE) dst[o + k] = w.u.f.desc[t][k];
+                    o += dl[t];
+                }
+            }
+            // each dimension's table: its own or the predefined one (LDS offsets)
+            const uint16_t* stp[3];
+            const uint32_t* dnbp[3];
+            const int32_t* dfsp[3];
+            for (uint32_t t = 0; t < 3; t++) {
+                const bool own = mode[t] == 2;
+                stp[t] = own ? w.u.f.st + ZE_ST0(t) : sh.st + (t == 0 ? 0u : (t == 1 ? 64u : 128u));
+                dnbp[t] = own ? w.u.f.dnb + ZE_SYM0(t) : sh.dnb + ZE_SYM0(t);
+                dfsp[t] = own ? w.u.f.dfs + ZE_SYM0(t) : sh.dfs + ZE_SYM0(t);
+            }
+            // the bitstream, 64 sequences at a time from the last
+            bw.reset(o);
+            uint32_t sl = 0, sm = 0, so = 0;
+            const uint32_t nchunk = (nseq + WAVE - 1) / WAVE;
+            for (int32_t c = (int32_t)nchunk - 1; c >= 0 && !raw && !bw.over; c--) {
+                const uint32_t i0 = (uint32_t)c * WAVE, cnt = min(nseq - i0, (uint32_t)WAVE);
+                uint32_t off = 0, ll = 0, ml = 3;
+                if (lane < cnt) ze_load_seq<FMT>(seqs, it.seq_pos + i0 + lane, off, ll, ml);
+                uint32_t q0 = w.reps[c][0], q1 = w.reps[c][1], q2 = w.reps[c][2];
+                q0 = uniform(q0), q1 = uniform(q1), q2 = uniform(q2);
+                uint32_t ov = 0;
+                for (uint32_t j = 0; j < cnt; j++) {
+                    const uint32_t v = ze_rep_step(bcast(off, j), bcast(ll, j), q0, q1, q2);
+                    ov = lane == j ? v : ov;
                 }
                 const uint32_t lc = ze_code(ze_ll_base, 36, ll), mc = ze_code(ze_ml_base, 53, ml);
-                const uint32_t ofv = off + 3, oc = min(31u - (uint32_t)__builtin_clz(ofv), 28u);  // 16 at most here
-                codes = lc | (mc << 8) | (oc << 16);
-                xl = (ll - ze_ll_base[lc]) | ((uint32_t)ze_ll_bits[lc] << 16);
-                xm = (ml - ze_ml_base[mc]) | ((uint32_t)ze_ml_bits[mc] << 16);
-                xo = (ofv - (1u << oc)) | (oc << 16);
-            }
-            // the state chain; lane j gets LL | ML << 16 and OF table entries
-            // (none, so no bits, for the frame's last sequence, whose states
-            // are the cells reached from state 0)
-            uint32_t rlm = 0, ro = 0;
-            for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
-                const uint32_t cd = bcast(codes, (uint32_t)j);
-                const uint32_t lc = cd & 0xFF, mc = (cd >> 8) & 0xFF, oc = cd >> 16;
-                if (i0 + j == (int32_t)nseq - 1) {
-                    sl = uniform(s.enc[lc * 64]) & 63;
-                    sm = uniform(s.enc[ZE_ENC_ML + mc * 64]) & 63;
-                    so = uniform(s.enc[ZE_ENC_OF + oc * 32]) & 63;
-                } else {
-                    const uint32_t el = uniform(s.enc[lc * 64 + sl]);
-                    const uint32_t em = uniform(s.enc[ZE_ENC_ML + mc * 64 + sm]);
-                    const uint32_t eo = uniform(s.enc[ZE_ENC_OF + oc * 32 + so]);
-                    write_lane2(rlm, el | (em << 16), ro, eo, (uint32_t)j);
-                    sl = el & 63, sm = em & 63, so = eo & 63;
+                const uint32_t oc = lane < cnt ? ze_highbit(ov) : 0u;
+                // each lane's symbols' dnb and dfs, read in parallel before the chain
+                const uint32_t dl_ = dnbp[0][lc], dm_ = dnbp[1][mc], do_ = dnbp[2][oc];
+                const int32_t fl = dfsp[0][lc], fm = dfsp[1][mc], fo = dfsp[2][oc];
+                // the state chain: lane j gets its state bits (value | bits << 12) per table
+                uint32_t bl = 0, bm = 0, bo = 0;
+                for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
+                    const uint32_t nl = bcast(dl_, (uint32_t)j), nm = bcast(dm_, (uint32_t)j), no = bcast(do_, (uint32_t)j);
+                    const int32_t gl = (int32_t)bcast((uint32_t)fl, (uint32_t)j), gm = (int32_t)bcast((uint32_t)fm, (uint32_t)j),
+                                  go = (int32_t)bcast((uint32_t)fo, (uint32_t)j);
+                    if (i0 + (uint32_t)j == nseq - 1) {  // the last sequence: initial states, no bits
+                        if (mode[0] != 1) {
+                            const uint32_t nb = (nl + (1u << 15)) >> 16;
+                            sl = uniform(stp[0][(int32_t)((((nb << 16) - nl)) >> nb) + gl]);
+                        }
+                        if (mode[1] != 1) {
+                            const uint32_t nb = (nm + (1u << 15)) >> 16;
+                            sm = uniform(stp[1][(int32_t)((((nb << 16) - nm)) >> nb) + gm]);
+                        }
+                        if (mode[2] != 1) {
+                            const uint32_t nb = (no + (1u << 15)) >> 16;
+                            so = uniform(stp[2][(int32_t)((((nb << 16) - no)) >> nb) + go]);
+                        }
+                    } else {
+                        uint32_t el = 0, em = 0, eo = 0;
+                        if (mode[0] != 1) {
+                            const uint32_t nb = (sl + nl) >> 16;
+                            el = (sl & ((1u << nb) - 1)) | (nb << 12);
+                            sl = uniform(stp[0][(int32_t)(sl >> nb) + gl]);
+                        }
+                        if (mode[1] != 1) {
+                            const uint32_t nb = (sm + nm) >> 16;
+                            em = (sm & ((1u << nb) - 1)) | (nb << 12);
+                            sm = uniform(stp[1][(int32_t)(sm >> nb) + gm]);
+                        }
+                        if (mode[2] != 1) {
+                            const uint32_t nb = (so + no) >> 16;
+                            eo = (so & ((1u << nb) - 1)) | (nb << 12);
+                            so = uniform(stp[2][(int32_t)(so >> nb) + go]);
+                        }
+                        write_lane2(bl, uniform(el), bm, uniform(em), (uint32_t)j);
+                        bo = lane == (uint32_t)j ? uniform(eo) : bo;
+                    }
                 }
-            }
-            // this lane's fields, lowest bits first (fields 1-5 fill at most 49
-            // bits, so only the last can cross into the high word)
-            const uint32_t el = rlm & 0xFFFFu, em = rlm >> 16;
-            uint64_t lo = 0, hi = 0;
-            uint32_t nbit = 0;
-            auto field = [&](uint32_t v, uint32_t k) {
-                lo |= (uint64_t)v << nbit;
-                if (nbit + k > 64) hi = (uint64_t)(v >> (64 - nbit));
-                nbit += k;
-            };
-            field(ro >> 9, (ro >> 6) & 7);
-            field(em >> 9, (em >> 6) & 7);
-            field(el >> 9, (el >> 6) & 7);
-            field(xl & 0xFFFFu, xl >> 16);
-            field(xm & 0xFFFFu, xm >> 16);
-            field(xo & 0xFFFFu, xo >> 16);
-            const uint32_t incl = wave_scan(nbit);
-            const uint32_t batch = bcast(incl, WAVE - 1);
-            const uint32_t at = carry + (batch - incl);  // the lanes above come first
-            const uint32_t d = at >> 5, sh = at & 31;
-            const uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32), l2 = (uint32_t)hi;
-            const uint32_t w0 = l0 << sh;
-            const uint32_t w1 = sh ? __builtin_amdgcn_alignbit(l1, l0, 32 - sh) : l1;
-            const uint32_t w2 = sh ? __builtin_amdgcn_alignbit(l2, l1, 32 - sh) : l2;
-            atomicOr(&bits[d], w0);
-            atomicOr(&bits[d + 1], w1);
-            atomicOr(&bits[d + 2], w2);
-            __asm__ volatile("" ::: "memory");
-            // the whole dwords to the frame (bytes: the frame is not aligned)
-            const uint32_t total = carry + batch, full = total >> 5;
-            if (wpos + 4 * full >= limit) {
-                over = true;
-            } else {
-                for (uint32_t k = lane; k < full; k += WAVE) {
-                    const uint32_t v = bits[k];
-                    uint8_t* q = dst + wpos + 4 * k;
-                    q[0] = (uint8_t)v, q[1] = (uint8_t)(v >> 8), q[2] = (uint8_t)(v >> 16), q[3] = (uint8_t)(v >> 24);
+                // this lane's fields in write order: state bits OF, ML, LL, extra bits LL, ML, OF
+                uint64_t lo = 0;
+                uint32_t hi = 0, nbit = 0;
+                auto field = [&](uint32_t v, uint32_t k) {
+                    if (nbit < 64) lo |= (uint64_t)v << nbit;
+                    if (nbit + k > 64) hi |= nbit >= 64 ? v << (nbit - 64) : v >> (64 - nbit);
+                    nbit += k;
+                };
+                if (lane < cnt) {
+                    field(bo & 0xFFFu, bo >> 12);
+                    field(bm & 0xFFFu, bm >> 12);
+                    field(bl & 0xFFFu, bl >> 12);
+                    field(ll - ze_ll_base[lc], ze_ll_bits[lc]);
+                    field(ml - ze_ml_base[mc], ze_ml_bits[mc]);
+                    field(ov - (1u << oc), oc);
                 }
+                bw.put(lo, hi, nbit, true);  // the last sequence first
             }
-            const uint32_t part = bits[full];
-            __asm__ volatile("" ::: "memory");
-            for (uint32_t k = lane; k <= full + 2; k += WAVE) bits[k] = k == 0 ? part : 0u;
-            __asm__ volatile("" ::: "memory");
-            carry = total & 31;
-            wpos += 4 * full;
-        }
-        uint32_t total = carry;
-        if (nseq && !over) {
-            // initial states, read as LL, OF, ML; then the end mark
-            const uint64_t v = (uint64_t)(sm | (so << 6) | (sl << 11) | (1u << 17)) << carry;
-            if (lane == 0) {
-                bits[0] |= (uint32_t)v;
-                bits[1] |= (uint32_t)(v >> 32);
+            if (!raw && !bw.over) {
+                // initial states, read as LL, OF, ML; then the end mark
+                if (mode[1] != 1) bw.put1(sm & ((1u << lg[1]) - 1), lg[1]);
+                if (mode[2] != 1) bw.put1(so & ((1u << lg[2]) - 1), lg[2]);
+                if (mode[0] != 1) bw.put1(sl & ((1u << lg[0]) - 1), lg[0]);
+                bw.put1(1, 1);
+                end = bw.finish();
             }
-            __asm__ volatile("" ::: "memory");
-            total = carry + 18;
+            if (bw.over || end >= limit) raw = true;
         }
-        const uint32_t tail = (total + 7) / 8;
-        end = wpos + tail;
-        if (over || end >= limit) {
-            raw = true;
-        } else {
-            if (lane < tail) dst[wpos + lane] = (uint8_t)(bits[lane >> 2] >> (8 * (lane & 3)));
-        }
+        if (!raw && end >= limit) raw = true;
         if (raw) __threadfence();  // the raw block below overwrites the bytes written so far
     }
     if (raw) {
@@ -2200,17 +2632,18 @@ gc_zstd_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, u
                       uint32_t seq_fmt, const uint8_t* literals, const uint32_t* seq_counts,
                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
                       uint32_t max_seq) {
-    __shared__ ZeLds s;
-    __shared__ uint32_t bitbuf[ZE_WAVES][ZE_BITBUF];
-    ze_build(s);
+    __shared__ ZeShared sh;
+    __shared__ ZeWave ws[ZE_WAVES];
+    // the predefined tables, built with the first wavefront's workspace as scratch
+    ze_build(sh, ws[0].u.f.nc, ws[0].u.f.cell);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     for (uint32_t i = blockIdx.x * ZE_WAVES + wave; i < n_items; i += gridDim.x * ZE_WAVES) {
         const gc_item it = load_item(items, i);
         if (seq_fmt == GC_SEQ_REF)
-            ze_frame<GC_SEQ_REF>(s, bitbuf[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
+            ze_frame<GC_SEQ_REF>(sh, ws[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
                                  max_frame, max_seq);
         else
-            ze_frame<GC_SEQ_ZSTD>(s, bitbuf[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
+            ze_frame<GC_SEQ_ZSTD>(sh, ws[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
                                   max_frame, max_seq);
     }
 }

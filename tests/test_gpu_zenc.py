@@ -1,6 +1,8 @@
 """GPU Zstd frame coding (gc_zstd_encode_block_dev, gc_zstd_frames_gpu; the capability of
-zstd_compress.cu:756-1038): every frame equals the oracle's frame for the same sequences byte for byte
-(oracle/gc_oracle.c orc_zstd_frame_predef, itself checked by libzstd) and decodes with libzstd."""
+zstd_compress.cu:756-1038, widened: per-dimension predefined/RLE/FSE_Compressed tables, repeat-offset
+codes, Huffman and RLE literals): every frame equals the oracle's frame for the same sequences byte for
+byte (oracle/gc_oracle.c orc_zstd_frame, itself checked by libzstd) and decodes with libzstd; the ratio
+on the Zstd mix is at least 2.25."""
 import numpy as np
 import pytest
 
@@ -33,7 +35,7 @@ def oracle_frames(data, level):
     depth = {1: 16, 2: 64}.get(level, 1)
     seq, sc, _, _ = oracle.zstd_match_find(data, depth=depth)
     rows = oracle.zstd_as_zstd_sequences(seq, sc)
-    return [oracle.zstd_frame_predef(data[s * SUB:(s + 1) * SUB], rows[s]) for s in range(len(rows))]
+    return [oracle.zstd_frame(data[s * SUB:(s + 1) * SUB], rows[s]) for s in range(len(rows))]
 
 
 @pytest.mark.parametrize("level", [1, 2, 3])
@@ -89,7 +91,7 @@ def test_encode_block_dev_reference_abi(gctx, fmt, chunk):
     for s in range(nsub):
         sub = data[s * chunk:(s + 1) * chunk]
         f = frames[s * slot: s * slot + sizes[s]].tobytes()
-        assert f == oracle.zstd_frame_predef(sub, rows[s])
+        assert f == oracle.zstd_frame(sub, rows[s])
         assert gcompress.zstd_frame_decode(f, sub.size) == sub.tobytes()
     # a frame slot smaller than the raw frame is refused
     assert L.gc_zstd_encode_block_dev(gctx.h, 8, 8, fmt, 8, 8, 8, 8, 8, chunk, nsub, n, max_seq, chunk) != 0
@@ -103,3 +105,30 @@ def test_nvzs_gpu_container(gctx, n):
     algo, orig, ch, nch = gcompress.container_info(blob)
     assert (algo, orig, ch, nch) == (gcompress.ALGO_ZSTD, n, SUB, -(-n // SUB))
     assert gcompress.decompress(blob) == data.tobytes()
+
+
+# the oracle coder's own corpus, on which it makes every choice it can (tests/test_zstd_frame_oracle.py)
+from test_zstd_frame_oracle import CORPUS as CHOICES  # noqa: E402
+
+
+@pytest.mark.parametrize("name,data", CHOICES, ids=[c[0] for c in CHOICES])
+def test_every_choice_equals_oracle(gctx, name, data):
+    print(f"[zenc choices] {name} n={data.size}", flush=True)
+    frames, _ = gcompress.zstd_frames_gpu(data, 1)
+    want = oracle_frames(data, 1)
+    for s, (f, w) in enumerate(zip(frames, want)):
+        assert f == w, f"sub-block {s}: frame differs from the oracle's ({len(f)} vs {len(w)} bytes)"
+
+
+def test_ratio_on_the_zstd_mix(gctx):
+    from gcompress import workloads as W
+
+    data, inputs = W.config2()
+    o, _, _, _ = inputs[0]
+    x = data[o:o + 64 * SUB]
+    print(f"[zenc ratio] {x.size} bytes", flush=True)
+    frames, sizes = gcompress.zstd_frames_gpu(x, 1)
+    assert frames == oracle_frames(x, 1)
+    ratio = x.size / sum(sizes)
+    print(f"[zenc ratio] {ratio:.4f}", flush=True)
+    assert ratio >= 2.25
