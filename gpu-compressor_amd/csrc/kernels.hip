@@ -72,8 +72,17 @@
 // rounded down to 4 bytes and the misalignment, so that every read is a
 // 32-bit offset from a scalar base (no 64-bit address arithmetic per lane).
 // `avail` is the number of readable bytes from the sub-block start.
+// Input bytes are read through global-address-space pointers: a load through
+// a generic pointer is a FLAT load, which also counts against the LDS
+// counter, so the first LDS wait after it (a table or ring read) would wait
+// for it too -- a prefetch from HBM would be waited for at once.
+#define GC_GLOBAL __attribute__((address_space(1)))
+typedef const uint8_t GC_GLOBAL* gbytes;
+typedef const uint32_t GC_GLOBAL* gwords;
+typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
+
 struct SrcView {
-    const uint8_t* base;  // sub-block start - mis, 4-byte aligned
+    gbytes base;  // sub-block start - mis, 4-byte aligned
     uint32_t mis;         // sub-block start & 3
     uint64_t avail;
 };
@@ -89,14 +98,14 @@ __device__ __forceinline__ SrcView src_view(const uint8_t* arena, uint64_t src_o
     SrcView v;
     const uintptr_t a = (uintptr_t)arena + src_off;
     v.mis = (uint32_t)(a & 3);
-    v.base = (const uint8_t*)(a - v.mis);
+    v.base = (gbytes)(a - v.mis);
     v.avail = avail;
     return v;
 }
 
 // The aligned dword at byte offset a (a multiple of 4) of the view's base.
 __device__ __forceinline__ uint32_t dword_at(const SrcView& v, uint32_t a) {
-    return *(const uint32_t*)(v.base + a);
+    return *(gwords)(v.base + a);
 }
 
 // 4 bytes at sub-block offset `off`, little endian, built from the two
@@ -104,7 +113,7 @@ __device__ __forceinline__ uint32_t dword_at(const SrcView& v, uint32_t a) {
 // sub-block of n bytes, which covers every read below (at most 24 bytes from
 // an offset below n); otherwise a read that would pass the end is built byte
 // by byte from the bytes that exist.
-__device__ __attribute__((noinline)) uint32_t ld32_tail(const uint8_t* p, uint32_t off, uint64_t avail) {
+__device__ __attribute__((noinline)) uint32_t ld32_tail(gbytes p, uint32_t off, uint64_t avail) {
     uint32_t v = 0;
     for (uint32_t i = 0; i < 4; i++)
         if ((uint64_t)off + i < avail) v |= (uint32_t)p[i] << (8 * i);
@@ -115,8 +124,8 @@ template <bool SAFE>
 __device__ __forceinline__ uint32_t ld32(const SrcView& v, uint32_t off) {
     if (SAFE || (uint64_t)off + 8 <= v.avail) {
         const uint32_t t = off + v.mis;
-        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));  // q[0], q[1]: one 8-byte load
-        return __builtin_amdgcn_alignbyte(q[1], q[0], t & 3);
+        gwords q = (gwords)(v.base + (t & ~3u));  // q[0], q[1]: one 8-byte load
+        return __builtin_amdgcn_alignbyte(q[1], q[0], t);
     }
     return ld32_tail(v.base + v.mis + off, off, v.avail);  // the last bytes of the arena, byte by byte
 }
@@ -130,7 +139,7 @@ template <bool SAFE>
 __device__ __forceinline__ uint32_t ld32u(const SrcView& v, uint32_t off) {
     if (SAFE || (uint64_t)off + 8 <= v.avail) {
         uint32_t w;
-        __builtin_memcpy(&w, v.base + (off + v.mis), 4);  // scalar base + 32-bit lane offset
+        w = *(const GC_GLOBAL u32_unaligned*)(v.base + (off + v.mis));  // scalar base + 32-bit lane offset
         return w;
     }
     return ld32_tail(v.base + v.mis + off, off, v.avail);
@@ -147,8 +156,8 @@ template <bool SAFE>
 __device__ __forceinline__ Raw32 ld32_raw(const SrcView& v, uint32_t off) {
     if constexpr (SAFE) {
         const uint32_t t = off + v.mis;
-        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));
-        return Raw32{q[0], q[1], t & 3};
+        gwords q = (gwords)(v.base + (t & ~3u));
+        return Raw32{q[0], q[1], t};
     } else {
         return Raw32{ld32<false>(v, off), 0u, 0u};
     }
@@ -160,8 +169,8 @@ template <bool SAFE>
 __device__ __forceinline__ void ld128(const SrcView& v, uint32_t off, uint32_t o[4]) {
     if (SAFE || (uint64_t)off + 24 <= v.avail) {
         const uint32_t t = off + v.mis;
-        const uint32_t* q = (const uint32_t*)(v.base + (t & ~3u));
-        const uint32_t sh = t & 3;
+        gwords q = (gwords)(v.base + (t & ~3u));
+        const uint32_t sh = t;
         const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
         o[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
         o[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
@@ -171,6 +180,9 @@ __device__ __forceinline__ void ld128(const SrcView& v, uint32_t off, uint32_t o
     }
     for (int k = 0; k < 4; k++) o[k] = ld32<false>(v, off + 4 * k);
 }
+
+// (v_alignbyte_b32 shifts by 8 x the low two bits of its third operand, so
+// the byte offsets above and below are passed to it unmasked.)
 
 // Per-item reads may skip bounds checks when the arena extends 32 bytes past
 // the sub-block (every sub-block but the last of an arena).
@@ -315,12 +327,12 @@ struct LdsBytes {
 
 __device__ __forceinline__ uint32_t ld32(const LdsBytes& v, uint32_t off) {
     const uint32_t* q = v.w + (off >> 2);
-    return __builtin_amdgcn_alignbyte(q[1], q[0], off & 3);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], off);
 }
 
 __device__ __forceinline__ void ld128(const LdsBytes& v, uint32_t off, uint32_t o[4]) {
     const uint32_t* q = v.w + (off >> 2);
-    const uint32_t sh = off & 3;
+    const uint32_t sh = off;
     const uint32_t d0 = q[0], d1 = q[1], d2 = q[2], d3 = q[3], d4 = q[4];
     o[0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
     o[1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
@@ -849,7 +861,9 @@ gc_lzma2_len_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, 
 // Zstd per position, given cand(p): the candidate as the reference's table
 // returns it (an empty slot reads as position 0, zstd_compress.cu:198-200),
 // and whether it verifies 3 bytes (:241-245) with the common length capped
-// at GC_ZSTD_SHORT: info = len << 16 | cand, len 0 = no.  A length equal to
+// at GC_ZSTD_SHORT: info = len << 16 | (p - cand) & 0xFFFF, len 0 = no (the
+// low half then means nothing).  The low half of a verified word is the
+// match offset (1..65535), the form the parse works in.  A length equal to
 // the cap is re-measured by the parse.  Written for p + 3 < n.
 //
 // One workgroup of INFO_THREADS per sub-block, the sub-block staged in LDS
@@ -908,7 +922,7 @@ __device__ __forceinline__ uint32_t zstd_info_lds(const LdsBytes& L, uint32_t n,
             }
             next_len = len;
             verified += len ? 1u : 0u;
-            o[j] = (len << 16) | c[j];
+            o[j] = (len << 16) | ((p - c[j]) & 0xFFFFu);
         }
         uint4* q = (uint4*)(out + p0);
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
@@ -1045,7 +1059,7 @@ extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t
 // per sequence sets this kernel's speed, so per-lane work stays in vector
 // registers and loop-carried flags are plain integers.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void copy_wave(uint8_t* dst, const uint8_t* src, uint32_t len) {
+__device__ __forceinline__ void copy_wave(uint8_t* dst, gbytes src, uint32_t len) {
     for (uint32_t i = lane_id(); i < len; i += WAVE) dst[i] = src[i];
 }
 
@@ -1076,7 +1090,7 @@ __device__ __forceinline__ uint32_t rep_eq(const SrcView& src, uint32_t x, uint3
 __device__ __forceinline__ uint32_t ring_w24(const uint8_t* ring, uint32_t p) {
     const uint32_t i = p & (PARSE_RING - 1);
     const uint32_t* r = (const uint32_t*)(ring + (i & ~3u));
-    return __builtin_amdgcn_alignbyte(r[1], r[0], i & 3) & 0xFFFFFFu;
+    return __builtin_amdgcn_alignbyte(r[1], r[0], i) & 0xFFFFFFu;
 }
 
 // A segment of a sub-block's parse (the segmented form, gc_zstd_parse_seg_kernel):
@@ -1217,19 +1231,21 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         wv = ld32u<SAFE>(src, x < n ? x : 0u);
         if constexpr (LAZY) ev1 = ip[1];
     };
-    // finish also turns each word's candidate into its offset: the low half
-    // of e becomes x - cand (1..65535 where a candidate verified, the high
-    // half, the length, untouched), so that a head's offset is one scalar
-    // AND; e1's becomes x + 1 - cand.  The info array itself is unchanged.
-    auto finish = [&](uint32_t x) {  // e, e1, w of the window at x - lane, as peek left them
+    // finish masks what peek left for the window at nb (x = nb + lane).  The
+    // info words' low halves are already match offsets (x - cand), so a
+    // head's offset is one scalar AND.  A window whose every position has
+    // x + 4 < n (all but a sub-block's last window or two: a uniform branch)
+    // keeps its words as loaded, but for the bytes' 4th byte.
+    auto finish = [&](uint32_t nb) {
+        if (nb + WAVE + 4 <= n) {
+            w &= 0xFFFFFFu;
+            return;
+        }
+        const uint32_t x = nb + lane;
         const bool act = x + 3 < n;
         e = act ? e : 0;
-        e = (e & 0xFFFF0000u) | ((x - e) & 0xFFFFu);
         w = (x < n ? (w & 0xFFFFFFu) : 0) | (act ? 0 : 0x1000000u);
-        if constexpr (LAZY) {
-            e1 = x + 4 < n ? e1 : 0;
-            e1 = (e1 & 0xFFFF0000u) | ((x + 1 - e1) & 0xFFFFu);
-        }
+        if constexpr (LAZY) e1 = x + 4 < n ? e1 : 0;
     };
     // Repeat offsets beyond the ring's reach: at each window entry, for the
     // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
@@ -1276,6 +1292,10 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
             if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
         }
+        // (a window at or past every offset, as all but a sub-block's first
+        // windows are for the usual offsets, needs no x >= r masks)
+        if (wbase >= max(rep0, max(rep1, rep2)))
+            return (v0 == wv ? 1u : 0u) | (v1 == wv ? 2u : 0u) | (v2 == wv ? 4u : 0u);
         return (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
                (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
     };
@@ -1314,7 +1334,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         if (pos < lim) {  // the window holding the start, the ring filled as far back as it reaches
             xv = wbase + lane;
             peek(xv, e, e1, w);
-            finish(xv);
+            finish(wbase);
             enter(wbase, wbase + WAVE < PARSE_RING ? 0u : wbase + WAVE - PARSE_RING);
             const bool far = any_far();
             f = flags(xv, w, false, far);
@@ -1324,7 +1344,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
     } else if (lim) {
         peek(lane, e, e1, w);
-        finish(lane);
+        finish(0);
         enter(0, 0);
         f = flags(lane, w, false, false);  // (offsets 1, 4, 8)
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
@@ -1348,7 +1368,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         }
         wbase = nb;
         xv = nb + lane;
-        finish(xv);
+        finish(nb);
         enter(nb, from);
         const bool far = any_far();
         f = flags(xv, w, next, far);
@@ -1411,7 +1431,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             e = ae;
             e1 = ae1;
             w = aw;
-            finish(xv);
+            finish(wbase);
             enter(wbase, wbase);
             const bool far = any_far();
             f = flags(xv, w, true, far);
@@ -1527,7 +1547,12 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
                 uint32_t b0;
                 if (off <= RING_REACH) {
                     const uint32_t v = ring_w24(ring, xv - off);
-                    b0 = ((xv >= off) & (v == w)) ? 1u : 0u;  // (w has bit 24 set where xv + 3 >= n)
+                    // (w has bit 24 set where xv + 3 >= n; the x >= off mask
+                    // only where the window starts before the offset)
+                    if (off <= wbase)
+                        b0 = v == w ? 1u : 0u;
+                    else
+                        b0 = ((xv >= off) & (v == w)) ? 1u : 0u;
                 } else if (off == rep1) {  // a far offset equal to the old rep0 or rep1: its flags
                     b0 = f & 1u;
                 } else if (off == rep2) {
@@ -1832,7 +1857,7 @@ __device__ __forceinline__ void zstd_parse_seg(const uint8_t* arena, const gc_it
         uint32_t emitted = 0, covered = 0;
         uint8_t* lit_dst = literals ? literals + it.lit_pos : nullptr;
         const SrcView sv = src_view(arena, it.src_off, it.avail);
-        const uint8_t* src_bytes = sv.base + sv.mis;
+        gbytes src_bytes = sv.base + sv.mis;
         for (uint32_t i0 = 0; i0 < cnt && base + i0 < max_seq; i0 += WAVE) {
             const uint32_t i = i0 + lane;
             const bool valid = i < cnt && base + i < max_seq;

@@ -41,7 +41,8 @@ def v3(d, x):
 
 
 def zstd_info(d, strip=8):
-    """gc_zstd_info_kernel: info word per position (len << 16 | cand), strips walked backwards."""
+    """gc_zstd_info_kernel: info word per position (len << 16 | (p - cand) & 0xFFFF: the low half of a
+    verified word is the match offset), strips walked backwards."""
     n = len(d)
     cand = cands(d)
     info = [0] * n
@@ -65,7 +66,7 @@ def zstd_info(d, strip=8):
                 else:
                     ln = common(d, c[j], p, 3, lim)
             next_len = ln
-            out[j] = ln << 16 | c[j]
+            out[j] = ln << 16 | ((p - c[j]) & 0xFFFF)
         for j in range(strip):
             if p0 + j + 3 < n:
                 info[p0 + j] = out[j]
@@ -117,11 +118,10 @@ def zstd_parse(d, info, depth=16, max_seq=None):
         head = pos + k
         ek = e[k]
         if ek >> 16:
-            c = ek & 0xFFFF
-            off = head - c
+            off = ek & 0xFFFF
             ln = ek >> 16
             if ln >= SHORT:
-                ln = common(d, c, head, SHORT, n - head)
+                ln = common(d, head - off, head, SHORT, n - head)
         else:
             fk = f[k]
             off = rep[0] if fk & 1 else (rep[1] if fk & 2 else rep[2])
@@ -129,14 +129,14 @@ def zstd_parse(d, info, depth=16, max_seq=None):
         if depth > 1 and head + 4 < n:
             e2 = e[k + 1] if k + 1 < WAVE else ae[0]
             if e2 >> 16:
-                c2 = e2 & 0xFFFF
+                off2 = e2 & 0xFFFF
                 l2 = e2 >> 16
                 if l2 >= SHORT:
-                    l2 = common(d, c2, head + 1, SHORT, n - head - 1)
+                    l2 = common(d, head + 1 - off2, head + 1, SHORT, n - head - 1)
                 if l2 > ln + 1:
                     head += 1
                     ln = l2
-                    off = head - c2
+                    off = off2
         seqs.append((head - lit_start, ln - 3, off))
         o0, o1, o2 = rep
         rep = [off, o0, o1]
@@ -257,11 +257,10 @@ def zstd_parse_fixed(d, info, depth=16, max_seq=None, scan=False):
         head = wbase + k
         ek = e[k]
         if ek >> 16:
-            c = ek & 0xFFFF
-            off = head - c
+            off = ek & 0xFFFF
             ln = ek >> 16
             if ln >= SHORT:
-                ln = common(d, c, head, SHORT, n - head)
+                ln = common(d, head - off, head, SHORT, n - head)
         else:
             fk = f[k]
             off = rep[0] if fk & 1 else (rep[1] if fk & 2 else rep[2])
@@ -269,14 +268,14 @@ def zstd_parse_fixed(d, info, depth=16, max_seq=None, scan=False):
         if depth > 1 and head + 4 < n:
             e2 = e[k + 1] if k + 1 < WAVE else ae[0]
             if e2 >> 16:
-                c2 = e2 & 0xFFFF
+                off2 = e2 & 0xFFFF
                 l2 = e2 >> 16
                 if l2 >= SHORT:
-                    l2 = common(d, c2, head + 1, SHORT, n - head - 1)
+                    l2 = common(d, head + 1 - off2, head + 1, SHORT, n - head - 1)
                 if l2 > ln + 1:
                     head += 1
                     ln = l2
-                    off = head - c2
+                    off = off2
         seqs.append((head - lit_start, ln - 3, off))
         o0, o1, o2 = rep
         rep = [off, o0, o1]

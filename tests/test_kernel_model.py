@@ -49,7 +49,7 @@ def test_info_strips_equal_direct(name, data, strip):
     for p in range(n - 3):
         c = cand[p] if cand[p] is not None else 0
         ln = km.common(d, c, p, 3, min(n - p, km.SHORT)) if p > 0 and km.v3(d, c) == km.v3(d, p) else 0
-        assert info[p] == (ln << 16 | c), p
+        assert info[p] == (ln << 16 | ((p - c) & 0xFFFF)), p
 
 
 @pytest.mark.parametrize("name,data", CASES, ids=[c[0] for c in CASES])
