@@ -213,6 +213,14 @@ __device__ __forceinline__ uint32_t in_vgpr(uint32_t s) {
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
+// max of wavefront-uniform values on the scalar unit (the compiler otherwise
+// takes a max of three to the vector unit, with a move per scalar operand)
+__device__ __forceinline__ uint32_t smax(uint32_t a, uint32_t b) {
+    uint32_t r;
+    __asm__("s_max_u32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
 }
@@ -1088,9 +1096,8 @@ __device__ __forceinline__ uint32_t rep_eq(const SrcView& src, uint32_t x, uint3
 
 // The 3 bytes at p (little endian) from the ring.
 __device__ __forceinline__ uint32_t ring_w24(const uint8_t* ring, uint32_t p) {
-    const uint32_t i = p & (PARSE_RING - 1);
-    const uint32_t* r = (const uint32_t*)(ring + (i & ~3u));
-    return __builtin_amdgcn_alignbyte(r[1], r[0], i) & 0xFFFFFFu;
+    const uint32_t* r = (const uint32_t*)(ring + (p & (PARSE_RING - 4)));
+    return __builtin_amdgcn_alignbyte(r[1], r[0], p) & 0xFFFFFFu;
 }
 
 // A segment of a sub-block's parse (the segmented form, gc_zstd_parse_seg_kernel):
@@ -1247,54 +1254,31 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         w = (x < n ? (w & 0xFFFFFFu) : 0) | (act ? 0 : 0x1000000u);
         if constexpr (LAZY) e1 = x + 4 < n ? e1 : 0;
     };
-    // Repeat offsets beyond the ring's reach: at each window entry, for the
-    // offsets then in use, sh0..sh2 are loaded with the 3 bytes at y - sr0..2
-    // for the next window (y = wbase + WAVE + lane), like the info words.  They stay
-    // in place (a register whose load is in flight is not copied) and serve
-    // the next window's flags for whichever offsets are still in use.
-    // (The tags sr0..sr2 are uniform but live in vector registers: scalar
-    // registers bound the occupancy.)
-    uint32_t sh0 = 0, sh1 = 0, sh2 = 0, sr0 = in_vgpr(0u), sr1 = in_vgpr(0u), sr2 = in_vgpr(0u);
+    // Repeat offsets beyond the ring's reach are compared with a global read
+    // where they are needed.  (Prefetching their next window's words at each
+    // window entry cost more than it saved: every later wait for a direct
+    // read, which counts loads in order, waited for those loads too.)
     auto far_w24 = [&](uint32_t y, uint32_t r) -> uint32_t {  // the 3 bytes at y - r (clamped read)
         return ld32<SAFE>(src, (y >= r && y < n) ? y - r : 0u) & 0xFFFFFFu;
     };
     // (far: some current offset is beyond the ring's reach; one test per
-    // window change instead of one per offset in flags and load_shadows)
-    auto any_far = [&]() { return max(rep0, max(rep1, rep2)) > RING_REACH; };
-    auto load_shadows = [&](bool far) {  // for the window after the current one
-        if (!far) {  // (no tag may match a later far offset: the shadows are stale)
-            sr0 = sr1 = sr2 = in_vgpr(0u);
-            return;
-        }
-        sr0 = in_vgpr(rep0 > RING_REACH ? rep0 : 0u);
-        sr1 = in_vgpr(rep1 > RING_REACH ? rep1 : 0u);
-        sr2 = in_vgpr(rep2 > RING_REACH ? rep2 : 0u);
-        if (rep0 > RING_REACH) sh0 = far_w24(wbase + WAVE + lane, rep0);
-        if (rep1 > RING_REACH) sh1 = far_w24(wbase + WAVE + lane, rep1);
-        if (rep2 > RING_REACH) sh2 = far_w24(wbase + WAVE + lane, rep2);
-    };
-    auto far_flag_word = [&](uint32_t x, uint32_t r, bool shadows) -> uint32_t {
-        if (shadows && r == uniform(sr0)) return sh0;
-        if (shadows && r == uniform(sr1)) return sh1;
-        if (shadows && r == uniform(sr2)) return sh2;
-        return far_w24(x, r);
-    };
+    // window change instead of one per offset)
+    auto any_far = [&]() { return smax(rep0, smax(rep1, rep2)) > RING_REACH; };
     // the repeat flags of the window at x - lane: from the ring, or for a
-    // far offset from a shadow (`shadows`: the window is the one they were
-    // loaded for) or a global read
+    // far offset a global read
     // (wv comes from peek: bit 24 is set where x + 3 >= n, so no flag is set
     // there)
-    auto flags = [&](uint32_t x, uint32_t wv, bool shadows, bool far) -> uint32_t {
+    auto flags = [&](uint32_t x, uint32_t wv, bool far) -> uint32_t {
         // every lane reads (indexes wrap inside the ring), then masks
         uint32_t v0 = ring_w24(ring, x - rep0), v1 = ring_w24(ring, x - rep1), v2 = ring_w24(ring, x - rep2);
         if (far) {
-            if (rep0 > RING_REACH) v0 = far_flag_word(x, rep0, shadows);
-            if (rep1 > RING_REACH) v1 = far_flag_word(x, rep1, shadows);
-            if (rep2 > RING_REACH) v2 = far_flag_word(x, rep2, shadows);
+            if (rep0 > RING_REACH) v0 = far_w24(x, rep0);
+            if (rep1 > RING_REACH) v1 = far_w24(x, rep1);
+            if (rep2 > RING_REACH) v2 = far_w24(x, rep2);
         }
         // (a window at or past every offset, as all but a sub-block's first
         // windows are for the usual offsets, needs no x >= r masks)
-        if (wbase >= max(rep0, max(rep1, rep2)))
+        if (wbase >= smax(rep0, smax(rep1, rep2)))
             return (v0 == wv ? 1u : 0u) | (v1 == wv ? 2u : 0u) | (v2 == wv ? 4u : 0u);
         return (((x >= rep0) & (v0 == wv)) ? 1u : 0u) | (((x >= rep1) & (v1 == wv)) ? 2u : 0u) |
                (((x >= rep2) & (v2 == wv)) ? 4u : 0u);
@@ -1336,17 +1320,15 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             peek(xv, e, e1, w);
             finish(wbase);
             enter(wbase, wbase + WAVE < PARSE_RING ? 0u : wbase + WAVE - PARSE_RING);
-            const bool far = any_far();
-            f = flags(xv, w, false, far);
+            f = flags(xv, w, any_far());
             __asm__ volatile("" ::: "memory");
-            load_shadows(far);
             peek(wbase + WAVE + lane, ae, ae1, aw);
         }
     } else if (lim) {
         peek(lane, e, e1, w);
         finish(0);
         enter(0, 0);
-        f = flags(lane, w, false, false);  // (offsets 1, 4, 8)
+        f = flags(lane, w, false);  // (offsets 1, 4, 8)
         __asm__ volatile("" ::: "memory");  // the window first, then the prefetch
         peek(WAVE + lane, ae, ae1, aw);
     }
@@ -1355,7 +1337,7 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
     // that window
     auto move_to = [&](uint32_t nb, bool prefetched) {
         const uint32_t wend = wbase + WAVE;
-        const bool next = prefetched && nb == wend;  // the window the shadows are for
+        const bool next = prefetched && nb == wend;
         uint32_t from = nb;
         if (next) {
             e = ae;
@@ -1370,10 +1352,8 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
         xv = nb + lane;
         finish(nb);
         enter(nb, from);
-        const bool far = any_far();
-        f = flags(xv, w, next, far);
+        f = flags(xv, w, any_far());
         __asm__ volatile("" ::: "memory");
-        load_shadows(far);
         peek(nb + WAVE + lane, ae, ae1, aw);
     };
     // Look-ahead over a stretch of literals: from sb (a window start), 256
@@ -1433,10 +1413,8 @@ __device__ __forceinline__ void zstd_parse_item(const gc_item& it, const uint8_t
             w = aw;
             finish(wbase);
             enter(wbase, wbase);
-            const bool far = any_far();
-            f = flags(xv, w, true, far);
+            f = flags(xv, w, any_far());
             __asm__ volatile("" ::: "memory");
-            load_shadows(far);
             peek(wbase + WAVE + lane, ae, ae1, aw);
             if (pos < lim && ballot(((e >> 16) | f) != 0) == 0) {
                 const uint32_t nb = scan(wbase + WAVE);
