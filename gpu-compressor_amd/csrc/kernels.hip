@@ -718,7 +718,19 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
     // INNER: every position of the wavefront's strips lies in a full, wide
     // strip with p > 0 and p + GC_LZMA_MAX <= n (the end tests and the length
     // cap drop out); chosen per step for the wavefront (base: lane 0's strip)
-    auto strip = [&](uint32_t p0, auto inner) {
+    //
+    // Lengths: along a diagonal (the target of p + 1 is the target of p, plus
+    // 1) the length at p is one more than at p + 1, capped, so only the last
+    // position of a run of such positions (a run end) is measured.  A run can
+    // cross strips: each lane finds, from a ballot of the lanes whose whole
+    // strip continues, the first lane after it whose run ends inside its strip
+    // and takes that lane's length at its first position (ds_bpermute), plus 8
+    // per whole strip between.  Each wavefront walks its own region from the
+    // last step, so a run that crosses steps continues from the step before
+    // (carry_t, carry_len: the target and length at the first position of the
+    // step above): a long match costs one measurement per region, not one per
+    // strip.
+    auto strip = [&](uint32_t p0, auto inner, uint32_t& carry_t, uint32_t& carry_len) {
         constexpr bool INNER = decltype(inner)::value;
         uint32_t d[4];
         d[0] = src.w[p0 / 4];
@@ -743,38 +755,64 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
         }
 #pragma unroll
         for (uint32_t j = 0; j < STRIP; j++) vc[j] = tgt[j] != GC_NONE ? ld32(src, tgt[j]) : 0;
+        // the match target of each position (GC_NONE: no match)
+        uint32_t t[STRIP];
+#pragma unroll
+        for (int j = 0; j < (int)STRIP; j++) {
+            const uint32_t p = p0 + j;
+            t[j] = GC_NONE;
+            if (INNER || p + 1 < n) {
+                const uint32_t cur = strip_word(d, j);
+                if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
+                    t[j] = tgt[j];  // the probe reports a match only from 2 common bytes on
+                } else if (INNER || p > 0) {
+                    const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
+                    if (((prev ^ cur) & 0xFFFFu) == 0) t[j] = p - 1;  // src[p-1] == src[p] == src[p+1]
+                }
+            }
+            any |= t[j] != GC_NONE;
+        }
+        if constexpr (!LENGTHS) return;
+        auto maxlen = [&](uint32_t p) { return INNER ? GC_LZMA_MAX : (n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX); };
+        // cont bit j: position j's diagonal continues at j + 1 (for j = 7, the
+        // next lane's first position; for lane 63, the first of the step above)
+        const uint32_t tn = (uint32_t)__builtin_amdgcn_update_dpp((int)carry_t, (int)t[0], 0x130, 0xF, 0xF, false);
+        uint32_t cont = 0;
+#pragma unroll
+        for (int j = 0; j < (int)STRIP; j++) {
+            const uint32_t nt = j + 1 < (int)STRIP ? t[j + 1] : tn;
+            if (t[j] != GC_NONE && nt != GC_NONE && nt == t[j] + 1) cont |= 1u << j;
+        }
+        // run ends, measured
+        uint32_t L[STRIP];
+#pragma unroll
+        for (int j = 0; j < (int)STRIP; j++)
+            L[j] = (t[j] != GC_NONE && !((cont >> j) & 1)) ? lcp_lane<true>(src, t[j], p0 + j, 2, maxlen(p0 + j)) : 0;
+        // this lane's length at its first position where its run ends inside
+        // the strip (else unused)
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = STRIP - 1; j >= 0; j--)
+            v = t[j] == GC_NONE ? 0 : ((cont >> j) & 1) ? min(v + 1, GC_LZMA_MAX) : L[j];
+        // the length at p0 + STRIP: `run` whole strips continue after this
+        // lane's, then a lane (or the step above) whose run ends
+        const uint64_t whole = ballot(cont == (1u << STRIP) - 1);
+        const uint32_t lane = lane_id();
+        const uint64_t after = lane + 1 < WAVE ? whole >> (lane + 1) : 0;
+        const uint32_t run = (uint32_t)__builtin_ctzll(~after);
+        const uint32_t vm = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane + run + 1) << 2), (int)v);
+        const uint32_t in = min((lane + run + 1 < WAVE ? vm : carry_len) + STRIP * run, GC_LZMA_MAX);
         uint32_t out[STRIP];
-        uint32_t next_t = GC_NONE, next_len = 0;
+        uint32_t nl = in;
 #pragma unroll
         for (int j = STRIP - 1; j >= 0; j--) {
             const uint32_t p = p0 + j;
-            uint32_t o = 0, t = GC_NONE;
-            if (INNER || p + 1 < n) {
-                const uint32_t cur = strip_word(d, j);
-                const uint32_t maxlen = INNER ? GC_LZMA_MAX : (n - p < GC_LZMA_MAX ? n - p : GC_LZMA_MAX);
-                if (tgt[j] != GC_NONE && ((vc[j] ^ cur) & 0xFFFFu) == 0) {
-                    t = tgt[j];  // the probe reports a match only from 2 common bytes on
-                } else if (INNER || p > 0) {
-                    const uint32_t prev = j > 0 ? strip_word(d, j - 1) : prev0;
-                    if (((prev ^ cur) & 0xFFFFu) == 0) t = p - 1;  // src[p-1] == src[p] == src[p+1]
-                }
-                if (t != GC_NONE) {
-                    any = true;
-                    if constexpr (LENGTHS) {
-                        uint32_t len;
-                        if (next_t != GC_NONE && next_t == t + 1)
-                            len = next_len + 1 < maxlen ? next_len + 1 : maxlen;
-                        else
-                            len = lcp_lane<true>(src, t, p, 2, maxlen);
-                        o = ((p - t) << 16) | len;
-                        next_len = len;
-                    }
-                }
-            }
-            next_t = t;
-            out[j] = o;
+            const uint32_t len = t[j] == GC_NONE ? 0 : ((cont >> j) & 1) ? min(nl + 1, maxlen(p)) : L[j];
+            out[j] = len ? ((p - t[j]) << 16) | len : 0;
+            nl = len;
         }
-        if constexpr (!LENGTHS) return;
+        carry_t = bcast(t[0], 0);
+        carry_len = bcast(nl, 0);
         if (!REF && full) {
             uint4* q = (uint4*)(packed + it.opos + p0);
             q[0] = make_uint4(out[0], out[1], out[2], out[3]);
@@ -798,12 +836,20 @@ __device__ __forceinline__ bool lzma2_item(const gc_item& it, const LdsBytes& sr
             }
         }
     };
-    for (uint32_t p0 = threadIdx.x * STRIP; p0 < n; p0 += INFO_THREADS * STRIP) {
-        const uint32_t base = uniform(p0);
-        if (wide && base > 0 && base + WAVE * STRIP + GC_LZMA_MAX <= n)
-            strip(p0, std::true_type{});
+    // wavefront w's region: [w * REGION, (w + 1) * REGION), steps of WAVE
+    // strips from the last
+    constexpr uint32_t STEP_POS = WAVE * STRIP;
+    constexpr uint32_t REGION = GC_SUB_BLOCK / (INFO_THREADS / WAVE);
+    const uint32_t rbase = uniform((threadIdx.x / WAVE) * REGION);
+    uint32_t carry_t = GC_NONE, carry_len = 0;
+    for (int k = (int)(REGION / STEP_POS) - 1; k >= 0; k--) {
+        const uint32_t sbase = rbase + (uint32_t)k * STEP_POS;
+        if (sbase >= n) continue;
+        const uint32_t p0 = sbase + lane_id() * STRIP;
+        if (wide && sbase > 0 && sbase + STEP_POS + GC_LZMA_MAX <= n)
+            strip(p0, std::true_type{}, carry_t, carry_len);
         else
-            strip(p0, std::false_type{});
+            strip(p0, std::false_type{}, carry_t, carry_len);
     }
     // positions past the end of a short sub-block (lzma2_match_find.cu:188-190)
     if (LENGTHS && REF)
