@@ -551,6 +551,56 @@ def lzma2_end_to_end(a):
     return out
 
 
+def gpu_frames(hip, ctx, data, inputs, sample_bytes=256 * MiB):
+    """Zstd frames coded on the GPU (gc_zstd_encode_block_dev: per-dimension predefined/RLE/FSE tables,
+    repeat-offset codes, Huffman or RLE literals) for the first Zstd L1 input's bytes onwards, up to
+    sample_bytes, as one buffer of 64 KiB sub-chunks: the match finder with literals first (untimed), then
+    the coding kernel alone timed with HIP events on the library stream (median of 3).  Ratio beside
+    libzstd's (ZSTD_compressSequences on the same sequences, first 16 MiB); the frames themselves are
+    checked against the oracle coder and libzstd in tests/test_gpu_zenc.py."""
+    L = gcompress.lib()
+    stream = L.gc_stream(ctx.h)
+    o = next(x for x in inputs if x[2] == ZSTD and x[3] == 1)[0]
+    n = min(sample_bytes, data.size - o)
+    nsub = -(-n // 65536)
+    max_seq = 65536 // 3
+    slot = 65536 + 16
+    d_in = ctx.alloc(n + 256)
+    ctx.htod(d_in, np.ascontiguousarray(data[o:o + n]))
+    d_s, d_l = ctx.alloc(nsub * max_seq * 16), ctx.alloc(nsub * 65536)
+    d_sc, d_lc = ctx.alloc(nsub * 4), ctx.alloc(nsub * 4)
+    d_o, d_os = ctx.alloc(nsub * slot), ctx.alloc(nsub * 4)
+    rc = L.gc_zstd_match_find_dev(ctx.h, d_in, d_s, 0, d_l, d_sc, d_lc, 65536, nsub, n, 16, max_seq)
+    assert rc == 0, L.gc_last_error()
+    ctx.sync()
+    ms = []
+    for _ in range(3):
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        rc = L.gc_zstd_encode_block_dev(ctx.h, d_in, d_s, 0, d_l, d_sc, d_lc, d_o, d_os, 65536, nsub, n, max_seq,
+                                        slot)
+        assert rc == 0, L.gc_last_error()
+        hip.record(e1, stream)
+        ms.append(hip.elapsed_ms(e0, e1))
+    sizes = np.zeros(nsub, np.uint32)
+    ctx.dtoh(sizes, d_os)
+    k = min(nsub, 256)
+    seqs = np.zeros((k, max_seq, 4), np.uint32)
+    counts = np.zeros(k, np.uint32)
+    ctx.dtoh(seqs, d_s)
+    ctx.dtoh(counts, d_sc)
+    for p_ in (d_in, d_s, d_l, d_sc, d_lc, d_o, d_os):
+        ctx.free(p_)
+    x = np.ascontiguousarray(data[o:o + min(n, k * 65536)])
+    rows = np.concatenate([seqs[b, :counts[b]] for b in range(k)] + [np.zeros((1, 4), np.uint32)])
+    _, zs = gcompress.zstd_encode_sequences(x, rows, counts, threads=min(threads_all(), 16))
+    kms = float(np.median(ms))
+    return {"sample_bytes": int(n), "kernel": "gc_zstd_encode_kernel", "kernel_ms": round(kms, 3),
+            "encode_input_gbs": round(n / (kms / 1e3) / 1e9, 2), "ratio": round(n / int(sizes.sum()), 4),
+            "ratio_first_16mib": round(x.size / int(sizes[:k].sum()), 4),
+            "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4)}
+
+
 def extras_all(a, hip, ctx):
     ex = {}
     if not a.no_lzma2_e2e:
@@ -558,6 +608,8 @@ def extras_all(a, hip, ctx):
     print("[bench] config 2 (1 GiB Zstd L1+L2)", file=sys.stderr, flush=True)
     data, inputs = W.config2()
     ex["config2_zstd_1GiB_L1L2"] = batched_config(a, hip, ctx, "zstd", data, inputs)
+    print("[bench] GPU frame coder (256 MiB of the config 2 mix)", file=sys.stderr, flush=True)
+    ex["zstd_gpu_frames_config2"] = gpu_frames(hip, ctx, data, inputs)
     del data
     print("[bench] config 3 (640 MiB LZMA2)", file=sys.stderr, flush=True)
     data, inputs = W.config3()

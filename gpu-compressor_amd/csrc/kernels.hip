@@ -2018,6 +2018,9 @@ struct ZeShared {  // the predefined tables, built once per workgroup
 struct ZeWave {  // one wavefront's workspace
     uint32_t bits[ZE_BITBUF];
     uint32_t reps[ZE_MAXCHUNK][3];  // the offset history at each 64th sequence
+    uint2 sym[3][WAVE];             // per table, a chunk's sequences' (dnb, dfs)
+    uint32_t ebits[3][WAVE];        // per table, their state bits (value | count << 12)
+    uint32_t rq[4];                 // the offset history after a chunk
     union {
         struct {  // literals
             uint32_t hist[256];
@@ -2056,7 +2059,9 @@ __device__ __forceinline__ uint32_t ze_code(const uint32_t* base, uint32_t ncode
 // One table's encoder view from normalized counts nc[0, nsym) with accuracy
 // log `log` (run by one thread): cells spread as the decoder spreads them
 // ("less than 1" symbols at the top), next states in cell order per symbol.
-__device__ void ze_ctab_serial(const int16_t* nc, uint32_t nsym, uint32_t log, uint16_t* st, uint32_t* dnb,
+// (Not inlined: the coding kernel then needs 139 vector registers, which
+// leaves three waves per SIMD, as many as its LDS allows.)
+__device__ __attribute__((noinline)) void ze_ctab_serial(const int16_t* nc, uint32_t nsym, uint32_t log, uint16_t* st, uint32_t* dnb,
                                int32_t* dfs, uint8_t* cell) {
     const uint32_t size = 1u << log, mask = size - 1;
     uint32_t high = size - 1;
@@ -2305,32 +2310,91 @@ __device__ uint32_t ze_choose_serial(ZeWave& w, uint32_t t, uint32_t nseq, uint3
     return cc < cp ? 2u : 0u;
 }
 
-// The offset history (RFC 8878 3.1.2.5) over lane j's sequence (uniform
-// off, ll): its Offset_Value, and the history after it.
-__device__ __forceinline__ uint32_t ze_rep_step(uint32_t off, uint32_t ll, uint32_t& r0, uint32_t& r1, uint32_t& r2) {
-    uint32_t v;
-    if (ll > 0) {
-        if (off == r0) {
-            v = 1;
-        } else if (off == r1) {
-            v = 2, r1 = r0, r0 = off;
-        } else if (off == r2) {
-            v = 3, r2 = r1, r1 = r0, r0 = off;
-        } else {
-            v = off + 3, r2 = r1, r1 = r0, r0 = off;
-        }
-    } else {
-        if (off == r1) {
-            v = 1, r1 = r0, r0 = off;
-        } else if (off == r2) {
-            v = 2, r2 = r1, r1 = r0, r0 = off;
-        } else if (off == r0 - 1) {
-            v = 3, r2 = r1, r1 = r0, r0 = off;
-        } else {
-            v = off + 3, r2 = r1, r1 = r0, r0 = off;
-        }
+// The offset history (RFC 8878 3.1.2.5) over lane j's sequence (off, ll),
+// for j < cnt in order: lane j gets its Offset_Value (1-3: repeat offset 1-3,
+// with ll == 0 meaning r1, r2, r0 - 1; else off + 3); r0..r2 (uniform) end as
+// the history after the chunk.  The new first offset is always the
+// sequence's own; the other two follow from two flags (a repeat of r0 with
+// literals keeps the history, a repeat of r1 swaps the first two), written
+// without branches: the scalar unit selects.
+// Without that one exception (a sequence with no literals whose offset is
+// the previous sequence's: the history then holds that offset twice), the
+// history is the three most recent distinct offsets, most recent first, and
+// the first is always the previous sequence's own.  Such lists of the
+// sequences before each lane come from a prefix scan over the lanes (merge:
+// the later list, then the earlier list's offsets not in it), six steps of
+// ds_bpermute; a chunk with the exception, or a history holding an offset
+// twice, is walked one sequence at a time in lane 0.
+__device__ __forceinline__ void ze_mru_merge(uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t a0, uint32_t a1,
+                                             uint32_t a2) {
+    // o (later, o0 != 0) then a (earlier); 0 = no offset
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t a = k == 0 ? a0 : (k == 1 ? a1 : a2);
+        const bool add = a != 0 && a != o0 && a != o1 && a != o2;
+        const bool at1 = add && o1 == 0, at2 = add && o1 != 0 && o2 == 0;
+        o1 = at1 ? a : o1;
+        o2 = at2 ? a : o2;
     }
+}
+
+__device__ __forceinline__ uint32_t ze_rep_value(uint32_t o, bool lit, uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t v = o + 3;
+    v = o == (lit ? c : a - 1) ? 3u : v;
+    v = o == (lit ? b : c) ? 2u : v;
+    v = o == (lit ? a : b) ? 1u : v;
     return v;
+}
+
+__device__ __forceinline__ uint32_t ze_rep_chunk(ZeWave& w, uint32_t off, uint32_t ll, uint32_t cnt, uint32_t& r0,
+                                                 uint32_t& r1, uint32_t& r2) {
+    const uint32_t lane = lane_id();
+    const bool valid = lane < cnt;
+    // the exception: no literals and the previous offset (r0 for lane 0)
+    const uint32_t prev = wave_shr1(off, r0);
+    // (an offset of 0, which no sequence should have, would read as "no
+    // offset" in the lists: walked one at a time too)
+    const bool odd = (valid && ((ll == 0 && off == prev) || off == 0)) || r0 == r1 || r1 == r2 || r0 == r2;
+    if (ballot(odd) == 0) {
+        // inclusive scan: lane i's list of the distinct offsets of lanes 0..i
+        uint32_t h0 = valid ? off : 0u, h1 = 0, h2 = 0;
+#pragma unroll
+        for (uint32_t d = 1; d < WAVE; d <<= 1) {
+            const int src = (int)((lane >= d ? lane - d : lane) << 2);
+            const uint32_t a0 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)h0);
+            const uint32_t a1 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)h1);
+            const uint32_t a2 = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)h2);
+            if (lane >= d) ze_mru_merge(h0, h1, h2, a0, a1, a2);
+        }
+        // then the history before the chunk, and the history before each lane
+        if (valid) ze_mru_merge(h0, h1, h2, r0, r1, r2);
+        const uint32_t a = wave_shr1(h0, r0), b = wave_shr1(h1, r1), c = wave_shr1(h2, r2);
+        const uint32_t v = ze_rep_value(off, ll > 0, a, b, c);
+        r0 = bcast(h0, cnt - 1), r1 = bcast(h1, cnt - 1), r2 = bcast(h2, cnt - 1);
+        return v;
+    }
+    // one sequence at a time: the chunk's (off, ll) staged in LDS, walked by
+    // lane 0 (the chains' buffers are free until the chains run)
+    uint2* q = (uint2*)&w.ebits[1][0];
+    uint32_t* res = w.ebits[0];
+    q[lane] = make_uint2(off, ll);
+    __asm__ volatile("" ::: "memory");  // (one wavefront: its LDS operations run in order)
+    if (lane == 0) {
+        uint32_t a = r0, b = r1, c = r2;
+        for (uint32_t j = 0; j < cnt; j++) {
+            const uint2 x = q[j];
+            const uint32_t o = x.x;
+            const bool lit = x.y > 0;
+            res[j] = ze_rep_value(o, lit, a, b, c);
+            const bool keep = lit && o == a, swap = !keep && o == b;
+            const uint32_t n1 = keep ? b : a, n2 = (keep || swap) ? c : b;
+            a = o, b = n1, c = n2;
+        }
+        w.rq[0] = a, w.rq[1] = b, w.rq[2] = c;
+    }
+    __asm__ volatile("" ::: "memory");
+    r0 = uniform(w.rq[0]), r1 = uniform(w.rq[1]), r2 = uniform(w.rq[2]);
+    return res[lane];
 }
 
 template <uint32_t FMT>
@@ -2355,8 +2419,9 @@ __device__ __forceinline__ void ze_copy_bytes(uint8_t* dst, const uint8_t* src, 
 }
 
 // The literals section at dst[o, limit) for lit[0, nlit): returns its end, or
-// `limit` when it does not fit (the block then goes raw).
-__device__ uint32_t ze_literals(ZeWave& w, ZeBits& bw, uint8_t* dst, uint32_t o, uint32_t limit, const uint8_t* lit,
+// `limit` when it does not fit (the block then goes raw).  Not inlined (as
+// ze_ctab_serial: fewer vector registers in the kernel).
+__device__ __attribute__((noinline)) uint32_t ze_literals(ZeWave& w, ZeBits& bw, uint8_t* dst, uint32_t o, uint32_t limit, const uint8_t* lit,
                                 uint32_t nlit) {
     const uint32_t lane = lane_id();
     const uint32_t rh = nlit < 32 ? 1u : (nlit < 4096 ? 2u : 3u);
@@ -2373,19 +2438,26 @@ __device__ uint32_t ze_literals(ZeWave& w, ZeBits& bw, uint8_t* dst, uint32_t o,
     uint32_t type = (nlit > 0 && nz == 1) ? 1u : 0u;
     if (nlit >= 64 && nz >= 2 && maxs <= 128) {
         for (uint32_t k = lane; k < 256; k += WAVE) w.u.h.len[k] = 0;
-        // rank of every present byte by (count, byte)
-        for (uint32_t k0 = 0; k0 < 256; k0 += WAVE) {
-            const uint32_t s = k0 + lane, hs = w.u.h.hist[s];
-            uint32_t rank = 0;
-            for (uint32_t t = 0; t < 256; t++) {
-                const uint32_t ht = w.u.h.hist[t];
-                rank += (ht != 0 && (ht < hs || (ht == hs && t < s))) ? 1u : 0u;
-            }
-            if (hs) {
-                w.u.h.srt[rank] = (uint16_t)s;
-                w.u.h.sc[rank] = hs;
-            }
+        // rank of every present byte by (count, byte): lane l ranks bytes l,
+        // l + 64, l + 128, l + 192, reading each count once (a count read per
+        // byte pair in the inner loop, hoisted out of an outer loop over the
+        // lane's bytes, held all 256 counts in registers)
+        uint32_t hs[4], rank[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++) hs[q] = w.u.h.hist[q * WAVE + lane], rank[q] = 0;
+#pragma unroll 4
+        for (uint32_t t = 0; t < 256; t++) {
+            const uint32_t ht = w.u.h.hist[t];
+#pragma unroll
+            for (uint32_t q = 0; q < 4; q++)
+                rank[q] += (ht != 0 && (ht < hs[q] || (ht == hs[q] && t < q * WAVE + lane))) ? 1u : 0u;
         }
+#pragma unroll
+        for (uint32_t q = 0; q < 4; q++)
+            if (hs[q]) {
+                w.u.h.srt[rank[q]] = (uint16_t)(q * WAVE + lane);
+                w.u.h.sc[rank[q]] = hs[q];
+            }
         __asm__ volatile("" ::: "memory");
         uint32_t maxlen = 0;
         if (lane == 0) maxlen = ze_huff_lengths_serial(w, nz);
@@ -2519,11 +2591,7 @@ __device__ void ze_frame(const ZeShared& sh, ZeWave& w, const gc_item& it, const
                 if (lane == 0) w.reps[i0 / WAVE][0] = r0, w.reps[i0 / WAVE][1] = r1, w.reps[i0 / WAVE][2] = r2;
                 uint32_t off = 0, ll = 0, ml = 3;
                 if (lane < cnt) ze_load_seq<FMT>(seqs, it.seq_pos + i0 + lane, off, ll, ml);
-                uint32_t ov = 0;
-                for (uint32_t j = 0; j < cnt; j++) {
-                    const uint32_t v = ze_rep_step(bcast(off, j), bcast(ll, j), r0, r1, r2);
-                    ov = lane == j ? v : ov;
-                }
+                const uint32_t ov = ze_rep_chunk(w, off, ll, cnt, r0, r1, r2);
                 if (lane < cnt) {
                     atomicAdd(&w.u.f.hist[ze_code(ze_ll_base, 36, ll)], 1u);
                     atomicAdd(&w.u.f.hist[36 + ze_code(ze_ml_base, 53, ml)], 1u);
@@ -2572,9 +2640,15 @@ __device__ void ze_frame(const ZeShared& sh, ZeWave& w, const gc_item& it, const
                 dnbp[t] = own ? w.u.f.dnb + ZE_SYM0(t) : sh.dnb + ZE_SYM0(t);
                 dfsp[t] = own ? w.u.f.dfs + ZE_SYM0(t) : sh.dfs + ZE_SYM0(t);
             }
-            // the bitstream, 64 sequences at a time from the last
+            // the bitstream, 64 sequences at a time from the last.  The three FSE
+            // state chains (each state follows from the next sequence's) run in
+            // lanes 0-2, table t in lane t, with its sequences' (dnb, dfs) staged
+            // in LDS by every lane: one LDS read of the next-state table per
+            // sequence and lane, the three chains side by side.
             bw.reset(o);
-            uint32_t sl = 0, sm = 0, so = 0;
+            const uint16_t* mst = lane == 0 ? stp[0] : (lane == 1 ? stp[1] : stp[2]);
+            const bool mchain = lane < 3 && mode[lane < 3 ? lane : 0] != 1;
+            uint32_t cst = 0;  // lane t: table t's state
             const uint32_t nchunk = (nseq + WAVE - 1) / WAVE;
             for (int32_t c = (int32_t)nchunk - 1; c >= 0 && !raw && !bw.over; c--) {
                 const uint32_t i0 = (uint32_t)c * WAVE, cnt = min(nseq - i0, (uint32_t)WAVE);
@@ -2582,56 +2656,36 @@ __device__ void ze_frame(const ZeShared& sh, ZeWave& w, const gc_item& it, const
                 if (lane < cnt) ze_load_seq<FMT>(seqs, it.seq_pos + i0 + lane, off, ll, ml);
                 uint32_t q0 = w.reps[c][0], q1 = w.reps[c][1], q2 = w.reps[c][2];
                 q0 = uniform(q0), q1 = uniform(q1), q2 = uniform(q2);
-                uint32_t ov = 0;
-                for (uint32_t j = 0; j < cnt; j++) {
-                    const uint32_t v = ze_rep_step(bcast(off, j), bcast(ll, j), q0, q1, q2);
-                    ov = lane == j ? v : ov;
-                }
+                const uint32_t ov = ze_rep_chunk(w, off, ll, cnt, q0, q1, q2);
                 const uint32_t lc = ze_code(ze_ll_base, 36, ll), mc = ze_code(ze_ml_base, 53, ml);
                 const uint32_t oc = lane < cnt ? ze_highbit(ov) : 0u;
-                // each lane's symbols' dnb and dfs, read in parallel before the chain
-                const uint32_t dl_ = dnbp[0][lc], dm_ = dnbp[1][mc], do_ = dnbp[2][oc];
-                const int32_t fl = dfsp[0][lc], fm = dfsp[1][mc], fo = dfsp[2][oc];
-                // the state chain: lane j gets its state bits (value | bits << 12) per table
-                uint32_t bl = 0, bm = 0, bo = 0;
-                for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
-                    const uint32_t nl = bcast(dl_, (uint32_t)j), nm = bcast(dm_, (uint32_t)j), no = bcast(do_, (uint32_t)j);
-                    const int32_t gl = (int32_t)bcast((uint32_t)fl, (uint32_t)j), gm = (int32_t)bcast((uint32_t)fm, (uint32_t)j),
-                                  go = (int32_t)bcast((uint32_t)fo, (uint32_t)j);
-                    if (i0 + (uint32_t)j == nseq - 1) {  // the last sequence: initial states, no bits
-                        if (mode[0] != 1) {
-                            const uint32_t nb = (nl + (1u << 15)) >> 16;
-                            sl = uniform(stp[0][(int32_t)((((nb << 16) - nl)) >> nb) + gl]);
+                // each lane's symbols' dnb and dfs, read in parallel before the chains
+                w.sym[0][lane] = make_uint2(dnbp[0][lc], (uint32_t)dfsp[0][lc]);
+                w.sym[1][lane] = make_uint2(dnbp[1][mc], (uint32_t)dfsp[1][mc]);
+                w.sym[2][lane] = make_uint2(dnbp[2][oc], (uint32_t)dfsp[2][oc]);
+                __asm__ volatile("" ::: "memory");  // (one wavefront: its LDS operations run in order)
+                if (mchain) {
+                    uint32_t st = cst;
+                    for (int32_t j = (int32_t)cnt - 1; j >= 0; j--) {
+                        const uint2 sd = w.sym[lane][j];
+                        uint32_t e = 0;
+                        if (i0 + (uint32_t)j == nseq - 1) {  // the last sequence: the initial state, no bits
+                            const uint32_t nb = (sd.x + (1u << 15)) >> 16;
+                            st = mst[(int32_t)(((nb << 16) - sd.x) >> nb) + (int32_t)sd.y];
+                        } else {
+                            const uint32_t nb = (st + sd.x) >> 16;
+                            e = (st & ((1u << nb) - 1)) | (nb << 12);
+                            st = mst[(int32_t)(st >> nb) + (int32_t)sd.y];
                         }
-                        if (mode[1] != 1) {
-                            const uint32_t nb = (nm + (1u << 15)) >> 16;
-                            sm = uniform(stp[1][(int32_t)((((nb << 16) - nm)) >> nb) + gm]);
-                        }
-                        if (mode[2] != 1) {
-                            const uint32_t nb = (no + (1u << 15)) >> 16;
-                            so = uniform(stp[2][(int32_t)((((nb << 16) - no)) >> nb) + go]);
-                        }
-                    } else {
-                        uint32_t el = 0, em = 0, eo = 0;
-                        if (mode[0] != 1) {
-                            const uint32_t nb = (sl + nl) >> 16;
-                            el = (sl & ((1u << nb) - 1)) | (nb << 12);
-                            sl = uniform(stp[0][(int32_t)(sl >> nb) + gl]);
-                        }
-                        if (mode[1] != 1) {
-                            const uint32_t nb = (sm + nm) >> 16;
-                            em = (sm & ((1u << nb) - 1)) | (nb << 12);
-                            sm = uniform(stp[1][(int32_t)(sm >> nb) + gm]);
-                        }
-                        if (mode[2] != 1) {
-                            const uint32_t nb = (so + no) >> 16;
-                            eo = (so & ((1u << nb) - 1)) | (nb << 12);
-                            so = uniform(stp[2][(int32_t)(so >> nb) + go]);
-                        }
-                        write_lane2(bl, uniform(el), bm, uniform(em), (uint32_t)j);
-                        bo = lane == (uint32_t)j ? uniform(eo) : bo;
+                        w.ebits[lane][j] = e;
                     }
+                    cst = st;
                 }
+                __asm__ volatile("" ::: "memory");
+                // lane j's state bits (value | count << 12) per table; none for an RLE table
+                const uint32_t bl = mode[0] != 1 ? w.ebits[0][lane] : 0u;
+                const uint32_t bm = mode[1] != 1 ? w.ebits[1][lane] : 0u;
+                const uint32_t bo = mode[2] != 1 ? w.ebits[2][lane] : 0u;
                 // this lane's fields in write order: state bits OF, ML, LL, extra bits LL, ML, OF
                 uint64_t lo = 0;
                 uint32_t hi = 0, nbit = 0;
@@ -2652,6 +2706,7 @@ __device__ void ze_frame(const ZeShared& sh, ZeWave& w, const gc_item& it, const
             }
             if (!raw && !bw.over) {
                 // initial states, read as LL, OF, ML; then the end mark
+                const uint32_t sl = bcast(cst, 0), sm = bcast(cst, 1), so = bcast(cst, 2);
                 if (mode[1] != 1) bw.put1(sm & ((1u << lg[1]) - 1), lg[1]);
                 if (mode[2] != 1) bw.put1(so & ((1u << lg[2]) - 1), lg[2]);
                 if (mode[0] != 1) bw.put1(sl & ((1u << lg[0]) - 1), lg[0]);

@@ -132,3 +132,59 @@ def test_ratio_on_the_zstd_mix(gctx):
     ratio = x.size / sum(sizes)
     print(f"[zenc ratio] {ratio:.4f}", flush=True)
     assert ratio >= 2.25
+
+
+def test_repeat_offsets_with_equal_offsets_in_a_row(gctx):
+    """Sequences a match finder does not make but a caller may hand over: every match of 6 bytes or more
+    split in two, the second half with no literals and the same offset (the offset history then holds
+    one offset twice, which the coder walks one sequence at a time).  Frames equal the oracle coder's
+    and decode."""
+    data = datagen.mix(3 * SUB + 1234, 21)
+    n = data.size
+    nsub = -(-n // SUB)
+    max_seq = SUB // 3
+    seq, sc, _, _ = oracle.zstd_match_find(data, SUB, 16, max_seq)
+    rows = oracle.zstd_as_zstd_sequences(seq, sc)
+    seqs = np.zeros((nsub, max_seq, 4), np.uint32)
+    lits = np.zeros(nsub * SUB, np.uint8)
+    counts = np.zeros(nsub, np.uint32)
+    lcounts = np.zeros(nsub, np.uint32)
+    split_rows = []
+    for s in range(nsub):
+        sub = data[s * SUB:(s + 1) * SUB]
+        out, pos, lp = [], 0, 0
+        for off, ll, ml, _ in rows[s]:
+            lits[s * SUB + lp: s * SUB + lp + ll] = sub[pos:pos + ll]
+            lp += ll
+            if ml >= 6:
+                out += [(off, ll, 3, 0), (off, 0, ml - 3, 0)]
+            else:
+                out.append((off, ll, ml, 0))
+            pos += ll + ml
+        lits[s * SUB + lp: s * SUB + lp + sub.size - pos] = sub[pos:]
+        lcounts[s] = lp + sub.size - pos
+        assert len(out) <= max_seq
+        counts[s] = len(out)
+        if out:
+            seqs[s, :len(out)] = np.array(out, np.uint32)
+        split_rows.append(np.array(out, np.uint32).reshape(-1, 4))
+    assert sum(int((r[1:, 1] == 0).sum() if len(r) > 1 else 0) for r in split_rows) > 100
+    print(f"[zenc split] {int(counts.sum())} sequences", flush=True)
+    slot = SUB + 16
+    L = gcompress.lib()
+    d_in, d_s, d_l = _dev(gctx, data), _dev(gctx, seqs), _dev(gctx, lits)
+    d_sc, d_lc = _dev(gctx, counts), _dev(gctx, lcounts)
+    d_o, d_os = gctx.alloc(nsub * slot), gctx.alloc(nsub * 4)
+    rc = L.gc_zstd_encode_block_dev(gctx.h, d_in, d_s, 0, d_l, d_sc, d_lc, d_o, d_os, SUB, nsub, n, max_seq, slot)
+    assert rc == 0, L.gc_last_error()
+    frames = np.zeros(nsub * slot, np.uint8)
+    sizes = np.zeros(nsub, np.uint32)
+    gctx.dtoh(frames, d_o)
+    gctx.dtoh(sizes, d_os)
+    for p in (d_in, d_s, d_l, d_sc, d_lc, d_o, d_os):
+        gctx.free(p)
+    for s in range(nsub):
+        sub = data[s * SUB:(s + 1) * SUB]
+        f = frames[s * slot: s * slot + sizes[s]].tobytes()
+        assert f == oracle.zstd_frame(sub, split_rows[s]), f"sub-block {s}"
+        assert gcompress.zstd_frame_decode(f, sub.size) == sub.tobytes()
