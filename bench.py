@@ -514,8 +514,9 @@ def batched_config(a, hip, ctx, name, data, inputs, modes=("static",), plain_pas
 def lzma2_end_to_end(a):
     """LZMA2 L1 end to end on configuration 3 (the whole 640 MiB): the reference's shape -- GPU match
     finding in 128 MiB chunks, device-side triage, liblzma preset 9 per 64 KiB sub-block on a pool of
-    min(cores, 8), NVLZ -- beside the range coder driven by the GPU's matches and level 0 (liblzma on whole
-    chunks, the reference's CPU path); every container decoded back and compared with the input."""
+    min(cores, 8), NVLZ -- beside the range coder driven by the GPU's matches (on the host, and on the GPU:
+    the same bytes) and level 0 (liblzma on whole chunks, the reference's CPU path); every container decoded
+    back and compared with the input."""
     data, chunks = W.config3()
     pool_n = min(threads_all(), 8)
 
@@ -538,6 +539,7 @@ def lzma2_end_to_end(a):
     legs = (("reference_shape_triage_preset9",
              lambda: by_chunk(lambda x: gcompress.nvlz_compress(x, 1, 128, threads=pool_n), "preset 9")),
             ("gpu_matches_range_coder", lambda: gcompress.nvlz_compress_matches(data, 128, threads=pool_n)),
+            ("gpu_matches_gpu_range_coder", lambda: gcompress.nvlz_compress_gpu(data, 128)),
             ("level0_liblzma", lambda: gcompress.nvlz_compress_l0(data, 128, 0, threads=pool_n)))
     for key, fn in legs:
         print(f"[bench] LZMA2 end to end: {key}", file=sys.stderr, flush=True)
@@ -601,6 +603,40 @@ def gpu_frames(hip, ctx, data, inputs, sample_bytes=256 * MiB):
             "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4)}
 
 
+def gpu_lzma2_coder(hip, ctx, data):
+    """LZMA2 streams coded on the GPU (gc_lzma2_encode_dev: one stream per 64 KiB sub-block from the packed
+    matches, the host coder's bytes) for the whole 640 MiB of configuration 3 as one buffer: the match finder
+    first (untimed), then the coding kernel alone timed with HIP events on the library stream (median of 3).
+    The streams are checked against the host coder in tests/test_gpu_lzma_enc.py."""
+    L = gcompress.lib()
+    stream = L.gc_stream(ctx.h)
+    n = int(data.size)
+    nsub = -(-n // 65536)
+    d_in, d_pk = ctx.alloc(n + 256), ctx.alloc(n * 4)
+    ctx.htod(d_in, data)
+    d_o, d_os = ctx.alloc(nsub * gcompress.LZMA2_GPU_SLOT), ctx.alloc(nsub * 4)
+    rc = L.gc_lzma2_match_find_packed_dev(ctx.h, d_in, d_pk, 65536, n, 32)
+    assert rc == 0, L.gc_last_error()
+    ctx.sync()
+    ms = []
+    for _ in range(3):
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        rc = L.gc_lzma2_encode_dev(ctx.h, d_in, d_pk, d_o, d_os, n)
+        assert rc == 0, L.gc_last_error()
+        hip.record(e1, stream)
+        ms.append(hip.elapsed_ms(e0, e1))
+    sizes = np.zeros(nsub, np.uint32)
+    ctx.dtoh(sizes, d_os)
+    for p_ in (d_in, d_pk, d_o, d_os):
+        ctx.free(p_)
+    kms = float(np.median(ms))
+    raw = int((sizes == np.minimum(65536, n - 65536 * np.arange(nsub)) + 4).sum())
+    return {"input_bytes": n, "kernel": "gc_lzma2_encode_kernel", "kernel_ms": round(kms, 3),
+            "encode_input_gbs": round(n / (kms / 1e3) / 1e9, 3), "ratio": round(n / int(sizes.sum()), 4),
+            "raw_sub_blocks": raw, "sub_blocks": nsub}
+
+
 def extras_all(a, hip, ctx):
     ex = {}
     if not a.no_lzma2_e2e:
@@ -614,6 +650,8 @@ def extras_all(a, hip, ctx):
     print("[bench] config 3 (640 MiB LZMA2)", file=sys.stderr, flush=True)
     data, inputs = W.config3()
     ex["config3_lzma2_640MiB"] = batched_config(a, hip, ctx, "lzma2", data, inputs)
+    print("[bench] GPU LZMA2 coder (the 640 MiB of config 3)", file=sys.stderr, flush=True)
+    ex["lzma2_gpu_coder_config3"] = gpu_lzma2_coder(hip, ctx, data)
     del data
     print("[bench] config 4 (4096 inputs)", file=sys.stderr, flush=True)
     data, inputs = W.config4(threads=min(threads_all(), 16))

@@ -265,10 +265,11 @@ struct Encoder {
 };
 
 // the GPU's match at p (0 length when there is none), clamped to the block
+// and to the format's longest match
 inline void gpu_match(const uint32_t* packed, uint32_t p, uint32_t n, uint32_t& dist, uint32_t& len) {
     const uint32_t w = packed[p];
     dist = w >> 16;
-    len = std::min<uint32_t>(w & 0xFFFFu, n - p);
+    len = std::min<uint32_t>(std::min<uint32_t>(w & 0xFFFFu, kMatchMax), n - p);
     if (len < kMatchMin || dist == 0 || dist > p) len = 0;
 }
 

@@ -378,6 +378,98 @@ int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t lev
     return GC_OK;
 }
 
+int gc_lzma2_encode_dev(gc_ctx* ctx, const uint8_t* d_input, const uint32_t* d_packed, uint8_t* d_output,
+                        uint32_t* d_output_sizes, uint64_t total_input_size) {
+    if (!ctx || !d_input || !d_packed || !d_output || !d_output_sizes) return fail(GC_EINVAL, "null argument");
+    const uint64_t nsub = div_up(total_input_size, GC_SUB_BLOCK);
+    if (nsub == 0) return GC_OK;
+    if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
+    std::vector<gc_item> items;
+    make_items(items, total_input_size, GC_SUB_BLOCK, (uint32_t)nsub, 0, 0, false);
+    int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
+    if (rc) return rc;
+    return launch_lzma2_encode(ctx, ctx->stream, d_input, (const gc_item*)ctx->items.p, (uint32_t)nsub, d_packed,
+                               d_output, d_output_sizes);
+}
+
+int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* out, size_t cap, uint64_t* sizes,
+                         size_t* out_len) {
+    if (!ctx || (!chunk && n) || !out || !sizes || !out_len) return fail(GC_EINVAL, "null argument");
+    *out_len = 0;
+    if (n == 0) return GC_OK;
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    int rc = stage_input(ctx, chunk, n);
+    if (!rc) rc = ctx->out_a.ensure(n * 4);                        // packed matches
+    if (!rc) rc = ctx->out_e.ensure((size_t)nsub * GC_LZMA2_GPU_SLOT);  // one slot per sub-block
+    if (!rc) rc = ctx->out_c.ensure(n + 4 * nsub + 16);            // the streams, back to back
+    if (!rc) rc = ctx->out_f.ensure(nsub * 12 + 16);
+    if (rc) return rc;
+    const uint8_t* d_in = (const uint8_t*)ctx->input.p;
+    uint32_t* d_packed = (uint32_t*)ctx->out_a.p;
+    uint32_t* d_sizes = (uint32_t*)ctx->out_f.p;
+    uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
+    rc = run_lzma2(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, d_packed, nullptr, nullptr, 1, GC_LZMA2_DEPTH);
+    if (!rc)  // the items run_lzma2 uploaded describe the same sub-blocks
+        rc = launch_lzma2_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, d_packed,
+                                 (uint8_t*)ctx->out_e.p, d_sizes);
+    if (rc) return rc;
+    std::vector<uint32_t> fs(nsub);
+    hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: stream sizes");
+    std::vector<uint64_t> offs(nsub);
+    uint64_t t = 0;
+    for (uint64_t s = 0; s < nsub; s++) {
+        offs[s] = t;
+        sizes[s] = fs[s];
+        t += fs[s];
+    }
+    if (t > cap) return fail(GC_ESIZE, "output capacity below the streams' total size");
+    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: offsets");
+    uint8_t* d_out = (uint8_t*)ctx->out_c.p;
+    rc = launch_gather_bytes(ctx, ctx->stream, (const uint8_t*)ctx->out_e.p + GC_LZMA2_GPU_HEAD, d_sizes, d_offs,
+                             d_out, GC_LZMA2_GPU_SLOT, (uint32_t)nsub);
+    if (rc) return rc;
+    e = hipMemcpyAsync(out, d_out, t, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: streams");
+    *out_len = (size_t)t;
+    return GC_OK;
+}
+
+int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
+                         size_t* out_len) {
+    if (!ctx || (!src && n) || !out || !out_len) return fail(GC_EINVAL, "null argument");
+    if (cap < gc_container_bound(n, GC_ALGO_LZMA2)) return fail(GC_ESIZE, "output capacity below gc_container_bound");
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    const size_t head = 28 + 8 * nsub;
+    memcpy(out, "NVLZ", 4);  // pipeline_lzma2.rs:344-351
+    auto put64 = [](uint8_t* p, uint64_t v) {
+        for (int i = 0; i < 8; i++) p[i] = (uint8_t)(v >> (8 * i));
+    };
+    put64(out + 4, n);
+    put64(out + 12, GC_SUB_BLOCK);
+    put64(out + 20, nsub);
+    if (dict_mb == 0) dict_mb = 128;
+    const size_t pchunk = (size_t)dict_mb << 20;  // pipeline_lzma2.rs:40
+    std::vector<uint64_t> sizes(nsub, 0);
+    size_t o = head;
+    for (size_t c0 = 0; c0 < n; c0 += pchunk) {
+        const size_t len = n - c0 < pchunk ? n - c0 : pchunk;
+        size_t w = 0;
+        int rc = gc_lzma2_streams_gpu(ctx, src + c0, len, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
+        if (rc) return rc;
+        o += w;
+    }
+    for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
+    *out_len = o;
+    return GC_OK;
+}
+
 size_t gc_nvzs_gpu_bound(size_t n) { return 28 + 8 * (size_t)div_up(n, GC_SUB_BLOCK) + n + 16 * (size_t)div_up(n, GC_SUB_BLOCK); }
 
 }  // extern "C"

@@ -154,6 +154,14 @@ int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
     return launch(stream, ctx->k.zstd_encode, grid, 256, args, "launch gc_zstd_encode_kernel");
 }
 
+int launch_lzma2_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                        const uint32_t* packed, uint8_t* out, uint32_t* sizes) {
+    void* args[] = {&arena, &items, &n_items, &packed, &out, &sizes};
+    // one wavefront and its 14.6 KB probability model per workgroup, one workgroup per sub-block:
+    // the dispatcher keeps ten per CU and hands out the next as one ends
+    return launch(stream, ctx->k.lzma2_encode, n_items, 64, args, "launch gc_lzma2_encode_kernel");
+}
+
 int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
                         const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub) {
     void* args[] = {&in, &sizes, &offsets, &out, &slot, &nsub};
@@ -293,6 +301,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.order_items, ctx->k.module, "gc_order_items");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_encode, ctx->k.module, "gc_zstd_encode_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_bytes, ctx->k.module, "gc_gather_bytes");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_encode, ctx->k.module, "gc_lzma2_encode_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.make_items, ctx->k.module, "gc_make_items");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand_q, ctx->k.module, "gc_cand_q_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_q, ctx->k.module, "gc_lzma2_len_q_kernel");

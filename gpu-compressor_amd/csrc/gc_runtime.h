@@ -47,6 +47,7 @@ struct Kernels {
     hipFunction_t order_items = nullptr;
     hipFunction_t zstd_encode = nullptr;
     hipFunction_t gather_bytes = nullptr;
+    hipFunction_t lzma2_encode = nullptr;
     hipFunction_t make_items = nullptr;
     // persistent forms: items from a device work queue (gc_queue)
     hipFunction_t cand_q = nullptr;
@@ -134,6 +135,11 @@ int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
                        const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
                        const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
                        uint32_t max_seq);
+// LZMA2 chunks coded from packed matches: one 64-thread workgroup per item,
+// chunk of item count_pos s in out[s * GC_LZMA2_GPU_SLOT + GC_LZMA2_GPU_HEAD ...],
+// its length in sizes[s].
+int launch_lzma2_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
+                        const uint32_t* packed, uint8_t* out, uint32_t* sizes);
 int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
                         const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub);
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);

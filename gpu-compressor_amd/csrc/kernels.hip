@@ -2845,3 +2845,395 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
     }
 }
 #endif  // GC_BUILD_MAIN
+
+#if GC_BUILD_MAIN
+// ---------------------------------------------------------------------------
+// LZMA2 chunks on the GPU: the coder of gc_lzma_enc.cpp (one LZMA chunk per
+// sub-block with properties lc 3, lp 0, pb 2; greedy parse with one lazy step
+// over the match finder's packed matches and the four repeat distances;
+// compress_lzma2_custom.rs:344-465 and :601-664 for what it replaces),
+// producing the same bytes.  One wavefront per sub-block, its probability
+// model (7319 11-bit probabilities, 14.6 KB) in LDS: ten wavefronts per CU.
+//
+// The parse walks 64-position windows.  Entering a window at p loads, one
+// position per lane, the byte at p + lane, the byte one repeat distance back
+// for each of the four distances and the packed match; a ballot per distance
+// is a 64-bit mask of the positions whose byte differs from the byte that
+// distance back (or that lie before the distance or past the sub-block), so
+// the repeat lengths at every position of the window are counted from the
+// masks; a length that reaches the window's end goes on 64 bytes per step.
+// Literals and repeat-0 matches keep the distances, so the window serves the
+// positions after them; a match or another repeat moves the distances and
+// the next window starts where that symbol ends.
+//
+// A symbol is coded as a list of at most 32 binary decisions, one per lane
+// (a probability index and the bit, or a direct bit): every lane reads its
+// probability and writes back its update at once (the decisions of one
+// symbol use distinct probabilities), then the range coder takes the
+// decisions in order with the wavefront's uniform state.  Finished bytes go
+// to a 256-byte LDS buffer that is written out 256 bytes at a time.
+//
+// Output: slot s (LE_SLOT bytes, item count_pos s) holds the chunk from byte
+// LE_HEAD (6 header bytes, the coded data from byte LE_HEAD + 6 = 8 so that
+// the buffer's dword stores are aligned, the end marker), or the raw chunk
+// (3 header bytes, the bytes, the end marker) when the coded chunk would not
+// be smaller; sizes[s] is the chunk's length.
+// ---------------------------------------------------------------------------
+#define LE_IS_MATCH 0u     // [state][pos state]
+#define LE_IS_REP 48u      // [state]
+#define LE_IS_REP_G0 60u
+#define LE_IS_REP_G1 72u
+#define LE_IS_REP_G2 84u
+#define LE_REP0_LONG 96u   // [state][pos state]
+#define LE_POS_SLOT 144u   // [length state][64]
+#define LE_POS_SPECIAL 400u
+#define LE_ALIGN 515u
+#define LE_LEN 531u        // choice, choice2, low[4][8], mid[4][8], high[256]
+#define LE_REP_LEN 853u
+#define LE_LIT 1175u       // [previous byte >> 5][0x300]
+#define LE_NPROBS 7320u    // 7319 and one to make whole dwords
+#define LE_DIRECT 0xFFFFu
+#define LE_HEAD 2u
+#define LE_SLOT 65552u
+
+struct LeRc {  // range coder state (wavefront-uniform)
+    uint64_t low;
+    uint32_t range, cache, pending, nout, cap;
+    bool full;
+};
+
+// Decisions of one symbol, one per lane: idx = probability index or
+// LE_DIRECT, bit; e = the count so far (uniform).
+struct LeList {
+    uint32_t lane, idx, bit, e;
+    __device__ __forceinline__ explicit LeList(uint32_t l) : lane(l), idx(LE_DIRECT), bit(0), e(0) {}
+    __device__ __forceinline__ void one(uint32_t i, uint32_t b) {
+        if (lane == e) idx = i, bit = b;
+        e++;
+    }
+    // most significant bit first, node index from 1
+    __device__ __forceinline__ void tree(uint32_t base, uint32_t nb, uint32_t v) {
+        const uint32_t i = lane - e;
+        if (i < nb) idx = base + ((1u << i) | (v >> (nb - i))), bit = (v >> (nb - 1 - i)) & 1u;
+        e += nb;
+    }
+    // least significant bit first: node i is 1 followed by bits 0 .. i-1
+    __device__ __forceinline__ void tree_reverse(uint32_t base, uint32_t nb, uint32_t v) {
+        const uint32_t i = lane - e;
+        if (i < nb)
+            idx = base + ((1u << i) | (i ? __builtin_bitreverse32(v) >> (32 - i) : 0u)), bit = (v >> i) & 1u;
+        e += nb;
+    }
+    __device__ __forceinline__ void direct(uint32_t v, uint32_t nb) {
+        const uint32_t i = lane - e;
+        if (i < nb) idx = LE_DIRECT, bit = (v >> (nb - 1 - i)) & 1u;
+        e += nb;
+    }
+    __device__ __forceinline__ void length(uint32_t base, uint32_t len, uint32_t ps) {
+        const uint32_t l = len - 2;
+        if (l < 8) {
+            one(base, 0);
+            tree(base + 2 + ps * 8, 3, l);
+        } else if (l < 16) {
+            one(base, 1);
+            one(base + 1, 0);
+            tree(base + 34 + ps * 8, 3, l - 8);
+        } else {
+            one(base, 1);
+            one(base + 1, 1);
+            tree(base + 66, 8, l - 16);
+        }
+    }
+};
+
+// The 256-byte output buffer to the slot as 64 aligned dwords.
+__device__ __forceinline__ void le_flush(const uint32_t* obuf, uint8_t* data, uint32_t upto, uint32_t lane) {
+    ((uint32_t*)(data + upto - 256))[lane] = obuf[lane];
+}
+
+__device__ __forceinline__ void le_put(LeRc& rc, uint32_t b, uint32_t* obuf, uint8_t* data, uint32_t lane) {
+    if (rc.nout < rc.cap) {
+        if (lane == 0) ((uint8_t*)obuf)[rc.nout & 255] = (uint8_t)b;
+        rc.nout++;
+        if ((rc.nout & 255) == 0) le_flush(obuf, data, rc.nout, lane);
+    } else {
+        rc.full = true;
+    }
+}
+
+__device__ __forceinline__ void le_shift_low(LeRc& rc, uint32_t* obuf, uint8_t* data, uint32_t lane) {
+    const uint32_t hi = (uint32_t)(rc.low >> 32);
+    if ((uint32_t)rc.low < 0xFF000000u || hi != 0) {
+        uint32_t b = rc.cache;
+        do {
+            le_put(rc, (b + hi) & 0xFFu, obuf, data, lane);
+            b = 0xFFu;
+        } while (--rc.pending != 0);
+        rc.cache = (uint32_t)(rc.low >> 24) & 0xFFu;
+    }
+    rc.pending++;
+    rc.low = (rc.low & 0x00FFFFFFu) << 8;
+}
+
+// Codes the list: probabilities read and updated by the lanes at once, the
+// range coder stepping through the decisions in order.
+__device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* model, uint32_t* obuf, uint8_t* data) {
+    const bool prob = L.idx != LE_DIRECT;
+    uint32_t pv = 0;
+    if (prob) {
+        pv = model[L.idx];
+        model[L.idx] = (uint16_t)(L.bit ? pv - (pv >> 5) : pv + ((2048u - pv) >> 5));
+    }
+    const uint64_t bits = ballot(L.bit != 0 && L.lane < L.e);
+    const uint64_t direct = ballot(!prob && L.lane < L.e);
+    for (uint32_t k = 0; k < L.e; k++) {
+        const uint32_t b = (uint32_t)(bits >> k) & 1u;
+        if ((direct >> k) & 1u) {
+            rc.range >>= 1;
+            if (b) rc.low += rc.range;
+        } else {
+            const uint32_t bound = (rc.range >> 11) * bcast(pv, k);
+            if (b) {
+                rc.low += bound;
+                rc.range -= bound;
+            } else {
+                rc.range = bound;
+            }
+        }
+        // one step always restores range >= 2^24: a probability stays within
+        // [31, 2017], so range >= 2^13 * 31 after a decision
+        if (rc.range < (1u << 24)) {
+            rc.range <<= 8;
+            le_shift_low(rc, obuf, data, L.lane);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t le_after_literal(uint32_t s) { return s < 4 ? 0 : (s < 10 ? s - 3 : s - 6); }
+
+// Repeat length at the window position k with the mismatch mask mm (at most
+// limit), continued past the window 64 bytes at a time.
+__device__ __forceinline__ uint32_t le_rep_len(uint64_t mm, uint32_t k, uint32_t limit, gbytes d, uint32_t wend,
+                                               uint32_t r, uint32_t end, uint32_t lane) {
+    const uint64_t x = mm >> k;
+    if (x) return min((uint32_t)__builtin_ctzll(x), limit);
+    uint32_t l = WAVE - k;
+    for (uint32_t q = wend; l < limit; q += WAVE) {
+        const uint32_t pos = q + lane;
+        const bool ok = pos < end && d[pos] == d[pos - r];
+        const uint64_t m = ballot(!ok);
+        if (m) return min(l + (uint32_t)__builtin_ctzll(m), limit);
+        l += WAVE;
+    }
+    return limit;
+}
+
+__device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t* packed, uint16_t* model,
+                        uint32_t* obuf, uint8_t* out, uint32_t* sizes) {
+    const uint32_t lane = lane_id();
+    const uint32_t n = it.n;
+    gbytes d = (gbytes)at(arena, it.src_off);
+    const uint32_t GC_GLOBAL* pk = (const uint32_t GC_GLOBAL*)(packed + it.opos);
+    uint8_t* slot = out + (uint64_t)it.count_pos * LE_SLOT;
+    uint8_t* data = slot + LE_HEAD + 6;
+    uint32_t* m32 = (uint32_t*)model;
+    for (uint32_t i = lane; i < LE_NPROBS / 2; i += WAVE) m32[i] = 0x04000400u;  // every probability 1024
+
+    LeRc rc;
+    rc.low = 0;
+    rc.range = 0xFFFFFFFFu;
+    rc.cache = 0;
+    rc.pending = 1;
+    rc.nout = 0;
+    rc.cap = min(n + 5, GC_SUB_BLOCK);  // the host coder's room; past it the raw chunk is smaller anyway
+    rc.full = false;
+    uint32_t state = 0, rep0 = 1, rep1 = 1, rep2 = 1, rep3 = 1;
+    uint32_t p = 0, wb = 0;
+    bool fresh = false;
+    uint32_t cur = 0, src0 = 0, pkw = 0, pk64 = 0, prev0 = 0;
+    uint64_t mm0 = 0, mm1 = 0, mm2 = 0, mm3 = 0;
+    while (p < n && !rc.full) {
+        if (!fresh || p >= wb + WAVE) {  // a window at p
+            wb = p;
+            const uint32_t pos = wb + lane;
+            const bool in = pos < n;
+            cur = in ? d[pos] : 0u;
+            const uint32_t s0 = in && rep0 <= pos ? d[pos - rep0] : 0u;
+            const uint32_t s1 = in && rep1 <= pos ? d[pos - rep1] : 0u;
+            const uint32_t s2 = in && rep2 <= pos ? d[pos - rep2] : 0u;
+            const uint32_t s3 = in && rep3 <= pos ? d[pos - rep3] : 0u;
+            pkw = in ? pk[pos] : 0u;
+            pk64 = wb + WAVE < n ? pk[wb + WAVE] : 0u;
+            prev0 = wb ? d[wb - 1] : 0u;
+            mm0 = ballot(!(in && rep0 <= pos && s0 == cur));
+            mm1 = ballot(!(in && rep1 <= pos && s1 == cur));
+            mm2 = ballot(!(in && rep2 <= pos && s2 == cur));
+            mm3 = ballot(!(in && rep3 <= pos && s3 == cur));
+            src0 = s0;
+            fresh = true;
+        }
+        const uint32_t k = p - wb;
+        const uint32_t avail = min(n - p, 273u);
+        const uint32_t ps = p & 3u;
+        // the packed match here (gc_lzma_enc.cpp gpu_match)
+        const uint32_t w = bcast(pkw, k);
+        const uint32_t mdist = w >> 16;
+        uint32_t mlen = min(min(w & 0xFFFFu, 273u), n - p);
+        if (mlen < 2 || mdist == 0 || mdist > p) mlen = 0;
+        uint32_t rlen = 0, ridx = 0;
+        if (avail >= 2) {
+            const uint32_t wend = wb + WAVE, end = p + avail;
+            uint32_t l = le_rep_len(mm0, k, avail, d, wend, rep0, end, lane);
+            if (l > rlen) rlen = l, ridx = 0;
+            l = le_rep_len(mm1, k, avail, d, wend, rep1, end, lane);
+            if (l > rlen) rlen = l, ridx = 1;
+            l = le_rep_len(mm2, k, avail, d, wend, rep2, end, lane);
+            if (l > rlen) rlen = l, ridx = 2;
+            l = le_rep_len(mm3, k, avail, d, wend, rep3, end, lane);
+            if (l > rlen) rlen = l, ridx = 3;
+        }
+        // a match at a repeat distance is coded as that repeat
+        if (mlen) {
+            const uint32_t i = rep0 == mdist ? 0u : rep1 == mdist ? 1u : rep2 == mdist ? 2u : rep3 == mdist ? 3u : 4u;
+            if (i < 4 && mlen >= rlen) rlen = mlen, ridx = i, mlen = 0;
+        }
+        bool take_rep = false, take_match = false;
+        if (rlen >= 2 && (rlen >= 64 || rlen + 1 >= mlen || (rlen + 2 >= mlen && mdist >= (1u << 9)) ||
+                          (rlen + 3 >= mlen && mdist >= (1u << 15))))
+            take_rep = true;
+        else if (mlen >= 2 && !(mlen == 2 && mdist >= 128))
+            take_match = true;
+        if (take_match && mlen < 64 && p + 1 < n) {  // one lazy step
+            const uint32_t w1 = k + 1 < WAVE ? bcast(pkw, k + 1) : pk64;
+            const uint32_t ndist = w1 >> 16;
+            uint32_t nlen = min(min(w1 & 0xFFFFu, 273u), n - p - 1);
+            if (nlen < 2 || ndist == 0 || ndist > p + 1) nlen = 0;
+            if (nlen >= mlen + 2 || (nlen == mlen + 1 && ndist < mdist)) take_match = false;
+        }
+        LeList L(lane);
+        if (take_rep) {
+            L.one(LE_IS_MATCH + state * 4 + ps, 1);
+            L.one(LE_IS_REP + state, 1);
+            if (ridx == 0) {
+                L.one(LE_IS_REP_G0 + state, 0);
+                L.one(LE_REP0_LONG + state * 4 + ps, 1);
+            } else {
+                L.one(LE_IS_REP_G0 + state, 1);
+                if (ridx == 1) {
+                    L.one(LE_IS_REP_G1 + state, 0);
+                } else {
+                    L.one(LE_IS_REP_G1 + state, 1);
+                    L.one(LE_IS_REP_G2 + state, ridx == 3 ? 1u : 0u);
+                }
+                const uint32_t dist = ridx == 1 ? rep1 : ridx == 2 ? rep2 : rep3;
+                if (ridx == 3) rep3 = rep2;
+                if (ridx >= 2) rep2 = rep1;
+                rep1 = rep0;
+                rep0 = dist;
+                fresh = false;
+            }
+            L.length(LE_REP_LEN, rlen, ps);
+            state = state < 7 ? 8u : 11u;
+            p += rlen;
+        } else if (take_match) {
+            L.one(LE_IS_MATCH + state * 4 + ps, 1);
+            L.one(LE_IS_REP + state, 0);
+            L.length(LE_LEN, mlen, ps);
+            const uint32_t dd = mdist - 1;
+            uint32_t slot_ = dd;
+            if (dd >= 4) {
+                const uint32_t top = 31u - (uint32_t)__builtin_clz(dd);
+                slot_ = 2 * top + ((dd >> (top - 1)) & 1u);
+            }
+            L.tree(LE_POS_SLOT + min(mlen - 2, 3u) * 64, 6, slot_);
+            if (slot_ >= 4) {
+                const uint32_t nbits = (slot_ >> 1) - 1;
+                const uint32_t base = (2u | (slot_ & 1u)) << nbits;
+                const uint32_t reduced = dd - base;
+                if (slot_ < 14) {
+                    L.tree_reverse(LE_POS_SPECIAL + base - slot_, nbits, reduced);
+                } else {
+                    L.direct(reduced >> 4, nbits - 4);
+                    L.tree_reverse(LE_ALIGN, 4, reduced & 15u);
+                }
+            }
+            rep3 = rep2;
+            rep2 = rep1;
+            rep1 = rep0;
+            rep0 = mdist;
+            fresh = false;
+            state = state < 7 ? 7u : 10u;
+            p += mlen;
+        } else if (state >= 7 && !((mm0 >> k) & 1u)) {
+            // the short repeat: the byte at distance rep0 (what a literal here
+            // would be coded against)
+            L.one(LE_IS_MATCH + state * 4 + ps, 1);
+            L.one(LE_IS_REP + state, 1);
+            L.one(LE_IS_REP_G0 + state, 0);
+            L.one(LE_REP0_LONG + state * 4 + ps, 0);
+            state = state < 7 ? 9u : 11u;
+            p += 1;
+        } else {
+            const uint32_t byte = bcast(cur, k);
+            const uint32_t prev = k ? bcast(cur, k - 1) : prev0;
+            const uint32_t base = LE_LIT + (prev >> 5) * 0x300u;
+            L.one(LE_IS_MATCH + state * 4 + ps, 0);
+            if (state < 7) {
+                L.tree(base, 8, byte);
+            } else {  // matched literal: the byte at distance rep0 steers the bits until they differ
+                const uint32_t mbyte = bcast(src0, k);
+                const uint32_t i = lane - L.e;
+                if (i < 8) {
+                    const uint32_t sym = (1u << i) | (byte >> (8 - i));
+                    const bool same = (byte >> (8 - i)) == (mbyte >> (8 - i));
+                    const uint32_t mb = (mbyte >> (7 - i)) & 1u;
+                    L.idx = base + (same ? ((1u + mb) << 8) + sym : sym);
+                    L.bit = (byte >> (7 - i)) & 1u;
+                }
+                L.e += 8;
+            }
+            state = le_after_literal(state);
+            p += 1;
+        }
+        le_code(rc, L, model, obuf, data);
+    }
+    for (int i = 0; i < 5; i++) le_shift_low(rc, obuf, data, lane);
+    const uint32_t c = rc.nout;
+    if (!rc.full && 7 + c < n + 4) {
+        // the buffer's last bytes, then the header and the end marker
+        const uint32_t tail = c & 255u, t0 = c - tail;
+        for (uint32_t j = lane; j < tail; j += WAVE) data[t0 + j] = ((const uint8_t*)obuf)[j];
+        if (lane < 6) {
+            const uint32_t h = lane == 0   ? 0xE0u | ((n - 1) >> 16)
+                               : lane == 1 ? ((n - 1) >> 8) & 0xFFu
+                               : lane == 2 ? (n - 1) & 0xFFu
+                               : lane == 3 ? ((c - 1) >> 8) & 0xFFu
+                               : lane == 4 ? (c - 1) & 0xFFu
+                                           : 2u * 5u * 9u + 3u;  // (pb * 5 + lp) * 9 + lc
+            slot[LE_HEAD + lane] = (uint8_t)h;
+        }
+        if (lane == 0) data[c] = 0;
+        if (lane == 0) sizes[it.count_pos] = 7 + c;
+    } else {
+        // one uncompressed chunk with a dictionary reset
+        uint8_t* r = slot + LE_HEAD;
+        for (uint32_t j = lane; j < n; j += WAVE) r[3 + j] = d[j];
+        if (lane == 0) {
+            r[0] = 0x01;
+            r[1] = (uint8_t)((n - 1) >> 8);
+            r[2] = (uint8_t)(n - 1);
+            r[3 + n] = 0;
+            sizes[it.count_pos] = n + 4;
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(WAVE)
+gc_lzma2_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
+                       const uint32_t* packed, uint8_t* out, uint32_t* sizes) {
+    __shared__ uint16_t model[LE_NPROBS];
+    __shared__ uint32_t obuf[WAVE];
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x)
+        le_item(load_item(items, i), arena, packed, model, obuf, out, sizes);
+}
+#endif  // GC_BUILD_MAIN

@@ -23,6 +23,8 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GCOMPRESS_LIB") or os.path.join(_PKG, "libgcompress.so")
 
 SUB_BLOCK = 65536            # constants.rs:5 / :11
+LZMA2_GPU_SLOT = 65552       # gc_lzma2_encode_dev: bytes per sub-block's output slot
+LZMA2_GPU_HEAD = 2           # ... and where in the slot the stream starts
 ZSTD_DEPTH_LAZY = 16         # constants.rs:6
 LZMA2_MAX_MATCHES = 8        # constants.rs:12
 LZMA2_DEPTH = 32             # constants.rs:13
@@ -79,6 +81,9 @@ SIGNATURES = [
     ("gc_zstd_frames_gpu", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp, _vp]),
     ("gc_nvzs_gpu_bound", _sz, [_sz]),
     ("gc_nvzs_compress_gpu", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp]),
+    ("gc_lzma2_encode_dev", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint64]),
+    ("gc_lzma2_streams_gpu", C.c_int, [_vp, _vp, _sz, _vp, _sz, _vp, _vp]),
+    ("gc_nvlz_compress_gpu", C.c_int, [_vp, _vp, _sz, C.c_uint32, _vp, _sz, _vp]),
     ("gc_container_bound", _sz, [_sz, C.c_uint32]),
     ("gc_nvzs_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_int, _vp, _sz, _vp]),
     ("gc_nvlz_compress", C.c_int, [_vp, _vp, _sz, C.c_uint32, C.c_uint32, C.c_int, _vp, _sz, _vp]),
@@ -475,6 +480,34 @@ def nvzs_compress_gpu(data, level: int = 1, device_id: int = 0) -> bytes:
     n = C.c_size_t()
     _check(lib().gc_nvzs_compress_gpu(context(device_id).h, _ptr(src), src.size, level, _ptr(out), cap, C.byref(n)),
            "gc_nvzs_compress_gpu")
+    return out[: n.value].tobytes()
+
+
+def lzma2_streams_gpu(data, device_id: int = 0):
+    """Match finding and LZMA2 coding on the GPU -> (streams, sizes), one stream per 64 KiB."""
+    src = _as_u8(data)
+    nsub = -(-src.size // SUB_BLOCK)
+    cap = src.size + 4 * max(nsub, 1)
+    out = np.empty(max(cap, 1), np.uint8)
+    sizes = np.zeros(max(nsub, 1), np.uint64)
+    n = C.c_size_t()
+    _check(lib().gc_lzma2_streams_gpu(context(device_id).h, _ptr(src), src.size, _ptr(out), cap, _ptr(sizes),
+                                      C.byref(n)), "gc_lzma2_streams_gpu")
+    streams, o = [], 0
+    for z in sizes[:nsub]:
+        streams.append(out[o:o + int(z)].tobytes())
+        o += int(z)
+    return streams, [int(z) for z in sizes[:nsub]]
+
+
+def nvlz_compress_gpu(data, dict_mb: int = 128, device_id: int = 0) -> bytes:
+    """NVLZ container whose sub-blocks are coded on the GPU from the GPU's matches."""
+    src = _as_u8(data)
+    cap = lib().gc_container_bound(src.size, ALGO_LZMA2)
+    out = np.empty(cap, np.uint8)
+    n = C.c_size_t()
+    _check(lib().gc_nvlz_compress_gpu(context(device_id).h, _ptr(src), src.size, dict_mb, _ptr(out), cap,
+                                      C.byref(n)), "gc_nvlz_compress_gpu")
     return out[: n.value].tobytes()
 
 

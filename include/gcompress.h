@@ -240,6 +240,32 @@ int gc_cpu_encode_from_matches(const uint8_t* src, size_t n, const uint32_t* pac
 int gc_nvlz_compress_matches(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, int threads, uint8_t* out,
                              size_t cap, size_t* out_len);
 
+/* The coder of gc_lzma2_encode_matches on the GPU (the capability of
+ * compress_lzma2_custom.rs:601-664 cpu_encode_from_matches, moved to the
+ * device): one stream per 64 KiB sub-block of d_input (total_input_size
+ * bytes) from d_packed (gc_lzma2_match_find_packed_dev layout, sub-block size
+ * 65536).  Stream s is written to d_output + s * GC_LZMA2_GPU_SLOT +
+ * GC_LZMA2_GPU_HEAD, its length to d_output_sizes[s]; d_output holds
+ * ceil(total_input_size / 65536) slots.  Each stream has the bytes
+ * gc_lzma2_encode_matches writes for the sub-block with verify = 0 (no
+ * decode-back check: the coder is pinned by its tests instead).  Device
+ * pointers, asynchronous on the context's stream. */
+#define GC_LZMA2_GPU_SLOT 65552u
+#define GC_LZMA2_GPU_HEAD 2u
+int gc_lzma2_encode_dev(gc_ctx* ctx, const uint8_t* d_input, const uint32_t* d_packed, uint8_t* d_output,
+                        uint32_t* d_output_sizes, uint64_t total_input_size);
+/* Host chunk -> match finding and LZMA2 coding on the GPU: one stream per
+ * 64 KiB sub-block, packed in order into out, sizes[s]; cap >= n + 4 per
+ * sub-block suffices.  The GPU counterpart of gc_cpu_encode_from_matches on
+ * gc_lzma2_find_matches_packed's matches. */
+int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* out, size_t cap, uint64_t* sizes,
+                         size_t* out_len);
+/* NVLZ container of GPU-coded streams (dictionary-size chunks as
+ * gc_nvlz_compress_matches; cap >= gc_container_bound(n, GC_ALGO_LZMA2));
+ * decoded by gc_decompress. */
+int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
+                         size_t* out_len);
+
 /* zstd_encode_block (zstd_compress.cu:756-1038, not called by the reference's
  * pipeline): one Zstd frame per sub-chunk from the match finder's sequences
  * (seq_format as gc_zstd_match_find_dev) and literals (d_literals at
