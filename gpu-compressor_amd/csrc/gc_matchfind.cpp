@@ -454,12 +454,17 @@ int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dic
     put64(out + 4, n);
     put64(out + 12, GC_SUB_BLOCK);
     put64(out + 20, nsub);
-    if (dict_mb == 0) dict_mb = 128;
-    const size_t pchunk = (size_t)dict_mb << 20;  // pipeline_lzma2.rs:40
+    // Matches never cross a 64 KiB sub-block, so the dictionary-size chunks of
+    // pipeline_lzma2.rs:40 change no byte of the output; the whole input goes
+    // through in pieces of up to 2 GiB instead (one coder launch per piece:
+    // a sub-block's stream is a long serial walk, so a launch needs many
+    // sub-blocks to keep the GPU busy).
+    (void)dict_mb;
+    const size_t piece = (size_t)2 << 30;
     std::vector<uint64_t> sizes(nsub, 0);
     size_t o = head;
-    for (size_t c0 = 0; c0 < n; c0 += pchunk) {
-        const size_t len = n - c0 < pchunk ? n - c0 : pchunk;
+    for (size_t c0 = 0; c0 < n; c0 += piece) {
+        const size_t len = n - c0 < piece ? n - c0 : piece;
         size_t w = 0;
         int rc = gc_lzma2_streams_gpu(ctx, src + c0, len, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
         if (rc) return rc;

@@ -2898,15 +2898,19 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
 
 struct LeRc {  // range coder state (wavefront-uniform)
     uint64_t low;
-    uint32_t range, cache, pending, nout, cap;
+    // add: what the decisions since the last byte step added to low (the
+    // additions never exceed the range at that step, so 32 bits hold them;
+    // the carry into low is taken once, at the next byte step)
+    uint32_t add, range, cache, pending, nout, cap, acc;
     bool full;
 };
 
 // Decisions of one symbol, one per lane: idx = probability index or
-// LE_DIRECT, bit; e = the count so far (uniform).
+// LE_DIRECT, bit; e = the count so far, d0 / dn = the run of direct bits
+// (at most one per symbol) -- all three uniform.
 struct LeList {
-    uint32_t lane, idx, bit, e;
-    __device__ __forceinline__ explicit LeList(uint32_t l) : lane(l), idx(LE_DIRECT), bit(0), e(0) {}
+    uint32_t lane, idx, bit, e, d0, dn;
+    __device__ __forceinline__ explicit LeList(uint32_t l) : lane(l), idx(LE_DIRECT), bit(0), e(0), d0(0), dn(0) {}
     __device__ __forceinline__ void one(uint32_t i, uint32_t b) {
         if (lane == e) idx = i, bit = b;
         e++;
@@ -2927,6 +2931,8 @@ struct LeList {
     __device__ __forceinline__ void direct(uint32_t v, uint32_t nb) {
         const uint32_t i = lane - e;
         if (i < nb) idx = LE_DIRECT, bit = (v >> (nb - 1 - i)) & 1u;
+        d0 = e;
+        dn = nb;
         e += nb;
     }
     __device__ __forceinline__ void length(uint32_t base, uint32_t len, uint32_t ps) {
@@ -2946,22 +2952,30 @@ struct LeList {
     }
 };
 
-// The 256-byte output buffer to the slot as 64 aligned dwords.
-__device__ __forceinline__ void le_flush(const uint32_t* obuf, uint8_t* data, uint32_t upto, uint32_t lane) {
-    ((uint32_t*)(data + upto - 256))[lane] = obuf[lane];
+// Finished bytes are gathered four to a word (rc.acc, little endian), the
+// words in the LDS buffer, which goes to the slot as 64 aligned dwords each
+// time 256 bytes are in it.
+__device__ __forceinline__ void le_put_word(LeRc& rc, uint32_t* obuf, uint8_t* data, uint32_t lane) {
+    const uint32_t w = (rc.nout >> 2) & (WAVE - 1);  // the word just completed (nout is past it)
+    if (lane == 0) obuf[(w - 1) & (WAVE - 1)] = rc.acc;
+    rc.acc = 0;
+    if ((rc.nout & 255) == 0) ((uint32_t*)(data + rc.nout - 256))[lane] = obuf[lane];
 }
 
 __device__ __forceinline__ void le_put(LeRc& rc, uint32_t b, uint32_t* obuf, uint8_t* data, uint32_t lane) {
     if (rc.nout < rc.cap) {
-        if (lane == 0) ((uint8_t*)obuf)[rc.nout & 255] = (uint8_t)b;
+        rc.acc |= b << (8 * (rc.nout & 3));
         rc.nout++;
-        if ((rc.nout & 255) == 0) le_flush(obuf, data, rc.nout, lane);
+        if ((rc.nout & 3) == 0) le_put_word(rc, obuf, data, lane);
     } else {
         rc.full = true;
     }
 }
 
+// The range coder's byte output step, taken about once per 8 decisions.
 __device__ __forceinline__ void le_shift_low(LeRc& rc, uint32_t* obuf, uint8_t* data, uint32_t lane) {
+    rc.low += rc.add;
+    rc.add = 0;
     const uint32_t hi = (uint32_t)(rc.low >> 32);
     if ((uint32_t)rc.low < 0xFF000000u || hi != 0) {
         uint32_t b = rc.cache;
@@ -2975,38 +2989,96 @@ __device__ __forceinline__ void le_shift_low(LeRc& rc, uint32_t* obuf, uint8_t* 
     rc.low = (rc.low & 0x00FFFFFFu) << 8;
 }
 
-// Codes the list: probabilities read and updated by the lanes at once, the
-// range coder stepping through the decisions in order.
-__device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* model, uint32_t* obuf, uint8_t* data) {
-    const bool prob = L.idx != LE_DIRECT;
-    uint32_t pv = 0;
-    if (prob) {
-        pv = model[L.idx];
-        model[L.idx] = (uint16_t)(L.bit ? pv - (pv >> 5) : pv + ((2048u - pv) >> 5));
+// One decision: probability pv, bit = bit k of v (all uniform).  bound =
+// (range >> 11) * pv; a 1 adds bound to low and leaves range - bound, a 0
+// leaves bound.  Written out because the compiler spends 11 scalar
+// instructions on it (a bit-field extract, two ANDs and a compare for the
+// bit) where a bit test and two selects do: 7, plus the normalisation test.
+// One normalisation step always restores range >= 2^24: a probability stays
+// within [31, 2017], so range >= 2^13 * 31 after a decision.
+__device__ __forceinline__ void le_bit(LeRc& rc, uint32_t pv, uint32_t v, uint32_t k, uint32_t* obuf, uint8_t* data,
+                                       uint32_t lane) {
+    uint32_t t, u;
+    __asm__("s_lshr_b32 %[t], %[r], 11\n\t"
+            "s_mul_i32 %[t], %[t], %[p]\n\t"
+            "s_sub_u32 %[u], %[r], %[t]\n\t"
+            "s_bitcmp1_b32 %[v], %[k]\n\t"
+            "s_cselect_b32 %[r], %[u], %[t]\n\t"
+            "s_cselect_b32 %[t], %[t], 0\n\t"
+            "s_add_u32 %[a], %[a], %[t]"
+            : [r] "+s"(rc.range), [a] "+s"(rc.add), [t] "=&s"(t), [u] "=&s"(u)
+            : [p] "s"(pv), [v] "s"(v), [k] "s"(k)
+            : "scc");
+    if (__builtin_expect(rc.range < (1u << 24), 0)) {
+        rc.range <<= 8;
+        le_shift_low(rc, obuf, data, lane);
     }
-    const uint64_t bits = ballot(L.bit != 0 && L.lane < L.e);
-    const uint64_t direct = ballot(!prob && L.lane < L.e);
-    for (uint32_t k = 0; k < L.e; k++) {
-        const uint32_t b = (uint32_t)(bits >> k) & 1u;
-        if ((direct >> k) & 1u) {
+}
+
+// The probability update of a decision's lane (branch-free).
+__device__ __forceinline__ uint32_t le_update(uint32_t pv, uint32_t bit) {
+    const uint32_t dn = pv - (pv >> 5), up = pv + ((2048u - pv) >> 5);
+    return bit ? dn : up;
+}
+
+// Decisions [k0, k1) of a list coded with probabilities pv (bit k of bits
+// = decision k's bit).
+__device__ __forceinline__ void le_run(LeRc& rc, uint32_t pv, uint32_t bits, uint32_t k0, uint32_t k1,
+                                       uint32_t* obuf, uint8_t* data, uint32_t lane) {
+    for (uint32_t k = k0; k < k1; k++) le_bit(rc, bcast(pv, k), bits, k, obuf, data, lane);
+}
+
+// Codes the list: probabilities read and updated by the lanes at once (lanes
+// without one use the spare slot), the range coder stepping through the
+// decisions in order (the direct bits, when there are any, as their own run).
+__device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* model, uint32_t* obuf, uint8_t* data) {
+    const uint32_t e = uniform(L.e), dn = uniform(L.dn);
+    const uint32_t d0 = dn ? uniform(L.d0) : e;
+    const uint32_t idx = L.idx != LE_DIRECT ? L.idx : LE_NPROBS - 1;  // lanes past the list hold LE_DIRECT
+    const uint32_t pv = model[idx];
+    model[idx] = (uint16_t)le_update(pv, L.bit);
+    const uint32_t bits = uniform((uint32_t)ballot(L.bit != 0));  // at most 32 decisions; lanes past the list hold 0
+    le_run(rc, pv, bits, 0, d0, obuf, data, L.lane);
+    if (dn) {
+        for (uint32_t k = d0; k < d0 + dn; k++) {
             rc.range >>= 1;
-            if (b) rc.low += rc.range;
-        } else {
-            const uint32_t bound = (rc.range >> 11) * bcast(pv, k);
-            if (b) {
-                rc.low += bound;
-                rc.range -= bound;
-            } else {
-                rc.range = bound;
+            rc.add += ((bits >> k) & 1u) ? rc.range : 0u;
+            if (__builtin_expect(rc.range < (1u << 24), 0)) {
+                rc.range <<= 8;
+                le_shift_low(rc, obuf, data, L.lane);
             }
         }
-        // one step always restores range >= 2^24: a probability stays within
-        // [31, 2017], so range >= 2^13 * 31 after a decision
-        if (rc.range < (1u << 24)) {
-            rc.range <<= 8;
-            le_shift_low(rc, obuf, data, L.lane);
-        }
+        le_run(rc, pv, bits, d0 + dn, e, obuf, data, L.lane);
     }
+}
+
+// A literal: is_match 0, then the byte's 8 bits msb first through the
+// literal coder of context prev >> 5, in matched mode (state >= 7) steered
+// by mbyte until the first bit where the two differ.  Lane 0 reads and
+// updates is_match, lanes 1-8 the tree nodes; the 9 decisions are unrolled.
+__device__ __forceinline__ void le_literal(LeRc& rc, uint32_t state, uint32_t p, uint32_t byte, uint32_t prev,
+                                           uint32_t mbyte, uint16_t* model, uint32_t* obuf, uint8_t* data,
+                                           uint32_t lane) {
+    const uint32_t i = lane - 1;
+    const uint32_t base = LE_LIT + (prev >> 5) * 0x300u;
+    const uint32_t sym = (1u << i) | (byte >> (8 - i));
+    uint32_t idx = base + sym;
+    if (state >= 7) {
+        const bool same = (byte >> (8 - i)) == (mbyte >> (8 - i));
+        idx += same ? (1u + ((mbyte >> (7 - i)) & 1u)) << 8 : 0u;
+    }
+    if (lane == 0) idx = LE_IS_MATCH + state * 4 + (p & 3u);
+    if (lane >= 9) idx = LE_NPROBS - 1;  // the spare slot: every lane reads and writes, none branches
+    const uint32_t bit = lane ? (byte >> (8 - lane)) & 1u : 0u;
+    const uint32_t pv = model[idx];
+    model[idx] = (uint16_t)le_update(pv, bit);
+    uint32_t q[9];  // every probability in a scalar register before the chain starts
+#pragma unroll
+    for (uint32_t k = 0; k < 9; k++) q[k] = bcast(pv, k);
+    const uint32_t v = uniform(byte);
+    le_bit(rc, q[0], 0u, 0, obuf, data, lane);
+#pragma unroll
+    for (uint32_t k = 1; k < 9; k++) le_bit(rc, q[k], v, 8 - k, obuf, data, lane);
 }
 
 __device__ __forceinline__ uint32_t le_after_literal(uint32_t s) { return s < 4 ? 0 : (s < 10 ? s - 3 : s - 6); }
@@ -3041,17 +3113,19 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
 
     LeRc rc;
     rc.low = 0;
+    rc.add = 0;
     rc.range = 0xFFFFFFFFu;
     rc.cache = 0;
     rc.pending = 1;
     rc.nout = 0;
+    rc.acc = 0;
     rc.cap = min(n + 5, GC_SUB_BLOCK);  // the host coder's room; past it the raw chunk is smaller anyway
     rc.full = false;
     uint32_t state = 0, rep0 = 1, rep1 = 1, rep2 = 1, rep3 = 1;
     uint32_t p = 0, wb = 0;
     bool fresh = false;
     uint32_t cur = 0, src0 = 0, pkw = 0, pk64 = 0, prev0 = 0;
-    uint64_t mm0 = 0, mm1 = 0, mm2 = 0, mm3 = 0;
+    uint64_t mm0 = 0, mm1 = 0, mm2 = 0, mm3 = 0, quiet = 0;
     while (p < n && !rc.full) {
         if (!fresh || p >= wb + WAVE) {  // a window at p
             wb = p;
@@ -3070,9 +3144,26 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
             mm2 = ballot(!(in && rep2 <= pos && s2 == cur));
             mm3 = ballot(!(in && rep3 <= pos && s3 == cur));
             src0 = s0;
+            // quiet positions: no packed match of 2 bytes or more and every
+            // repeat length below 2 (a mismatch here or at the next
+            // position; at the window's last position only a mismatch here
+            // counts), so the step is a literal or a short repeat
+            const uint32_t pl = pkw & 0xFFFFu, pd = pkw >> 16;
+            quiet = ~ballot(in && pl >= 2 && pd != 0 && pd <= pos) & (mm0 | (mm0 >> 1)) & (mm1 | (mm1 >> 1)) &
+                    (mm2 | (mm2 >> 1)) & (mm3 | (mm3 >> 1));
             fresh = true;
         }
-        const uint32_t k = p - wb;
+        uint32_t k = p - wb;
+        if (state < 7 && ((quiet >> k) & 1u)) {
+            // a run of plain literals: the walk's tests are settled by the masks
+            do {
+                le_literal(rc, state, p, bcast(cur, k), k ? bcast(cur, k - 1) : prev0, 0, model, obuf, data, lane);
+                state = le_after_literal(state);
+                p++;
+                k++;
+            } while (k < WAVE && p < n && !rc.full && ((quiet >> k) & 1u));
+            continue;
+        }
         const uint32_t avail = min(n - p, 273u);
         const uint32_t ps = p & 3u;
         // the packed match here (gc_lzma_enc.cpp gpu_match)
@@ -3174,34 +3265,21 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
             state = state < 7 ? 9u : 11u;
             p += 1;
         } else {
-            const uint32_t byte = bcast(cur, k);
-            const uint32_t prev = k ? bcast(cur, k - 1) : prev0;
-            const uint32_t base = LE_LIT + (prev >> 5) * 0x300u;
-            L.one(LE_IS_MATCH + state * 4 + ps, 0);
-            if (state < 7) {
-                L.tree(base, 8, byte);
-            } else {  // matched literal: the byte at distance rep0 steers the bits until they differ
-                const uint32_t mbyte = bcast(src0, k);
-                const uint32_t i = lane - L.e;
-                if (i < 8) {
-                    const uint32_t sym = (1u << i) | (byte >> (8 - i));
-                    const bool same = (byte >> (8 - i)) == (mbyte >> (8 - i));
-                    const uint32_t mb = (mbyte >> (7 - i)) & 1u;
-                    L.idx = base + (same ? ((1u + mb) << 8) + sym : sym);
-                    L.bit = (byte >> (7 - i)) & 1u;
-                }
-                L.e += 8;
-            }
+            le_literal(rc, state, p, bcast(cur, k), k ? bcast(cur, k - 1) : prev0, bcast(src0, k), model, obuf, data,
+                       lane);
             state = le_after_literal(state);
             p += 1;
+            continue;
         }
         le_code(rc, L, model, obuf, data);
     }
     for (int i = 0; i < 5; i++) le_shift_low(rc, obuf, data, lane);
     const uint32_t c = rc.nout;
     if (!rc.full && 7 + c < n + 4) {
-        // the buffer's last bytes, then the header and the end marker
+        // the buffer's last bytes (whole words in obuf, the last 0-3 in acc),
+        // then the header and the end marker
         const uint32_t tail = c & 255u, t0 = c - tail;
+        if (lane == 0 && (c & 3u)) obuf[(c >> 2) & (WAVE - 1)] = rc.acc;
         for (uint32_t j = lane; j < tail; j += WAVE) data[t0 + j] = ((const uint8_t*)obuf)[j];
         if (lane < 6) {
             const uint32_t h = lane == 0   ? 0xE0u | ((n - 1) >> 16)

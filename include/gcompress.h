@@ -260,9 +260,10 @@ int gc_lzma2_encode_dev(gc_ctx* ctx, const uint8_t* d_input, const uint32_t* d_p
  * gc_lzma2_find_matches_packed's matches. */
 int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* out, size_t cap, uint64_t* sizes,
                          size_t* out_len);
-/* NVLZ container of GPU-coded streams (dictionary-size chunks as
- * gc_nvlz_compress_matches; cap >= gc_container_bound(n, GC_ALGO_LZMA2));
- * decoded by gc_decompress. */
+/* NVLZ container of GPU-coded streams, byte-equal to gc_nvlz_compress_matches'
+ * for the same input (dict_mb is accepted for that interface and changes
+ * nothing: no match crosses a sub-block); cap >= gc_container_bound(n,
+ * GC_ALGO_LZMA2).  Decoded by gc_decompress. */
 int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
                          size_t* out_len);
 
