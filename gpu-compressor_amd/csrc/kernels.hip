@@ -2853,7 +2853,7 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
 // over the match finder's packed matches and the four repeat distances;
 // compress_lzma2_custom.rs:344-465 and :601-664 for what it replaces),
 // producing the same bytes.  One wavefront per sub-block, its probability
-// model (7319 11-bit probabilities, 14.6 KB) in LDS: ten wavefronts per CU.
+// model (7319 11-bit probabilities, 14.6 KB) in LDS: eleven wavefronts per CU.
 //
 // The parse walks 64-position windows.  Entering a window at p loads, one
 // position per lane, the byte at p + lane, the byte one repeat distance back
@@ -2870,8 +2870,9 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
 // (a probability index and the bit, or a direct bit): every lane reads its
 // probability and writes back its update at once (the decisions of one
 // symbol use distinct probabilities), then the range coder takes the
-// decisions in order with the wavefront's uniform state.  Finished bytes go
-// to a 256-byte LDS buffer that is written out 256 bytes at a time.
+// decisions in order with the wavefront's uniform state.  Finished bytes are
+// gathered four to a word, the words of a 256-byte block in a vector register
+// (one per lane), which is written out when the block is whole.
 //
 // Output: slot s (LE_SLOT bytes, item count_pos s) holds the chunk from byte
 // LE_HEAD (6 header bytes, the coded data from byte LE_HEAD + 6 = 8 so that
@@ -2896,13 +2897,17 @@ gc_gather_seqs(const uint4* in, const uint32_t* counts, const uint64_t* offsets,
 #define LE_HEAD 2u
 #define LE_SLOT 65552u
 
-struct LeRc {  // range coder state (wavefront-uniform)
+struct LeRc {  // range coder state (wavefront-uniform but for obv)
     uint64_t low;
     // add: what the decisions since the last byte step added to low (the
     // additions never exceed the range at that step, so 32 bits hold them;
     // the carry into low is taken once, at the next byte step)
-    uint32_t add, range, cache, pending, nout, cap, acc;
-    bool full;
+    uint32_t add, range, cache, pending;
+    // output: nout bytes so far (counted on past cap: more than cap means
+    // the coded chunk does not fit and the raw one is written); the bytes of
+    // the current word in acc from bit 0, sh = the next byte's shift; the
+    // words of the current 256-byte block in obv, word j in lane j
+    uint32_t nout, cap, acc, sh, obv;
 };
 
 // Decisions of one symbol, one per lane: idx = probability index or
@@ -2952,41 +2957,44 @@ struct LeList {
     }
 };
 
-// Finished bytes are gathered four to a word (rc.acc, little endian), the
-// words in the LDS buffer, which goes to the slot as 64 aligned dwords each
-// time 256 bytes are in it.
-__device__ __forceinline__ void le_put_word(LeRc& rc, uint32_t* obuf, uint8_t* data, uint32_t lane) {
-    const uint32_t w = (rc.nout >> 2) & (WAVE - 1);  // the word just completed (nout is past it)
-    if (lane == 0) obuf[(w - 1) & (WAVE - 1)] = rc.acc;
-    rc.acc = 0;
-    if ((rc.nout & 255) == 0) ((uint32_t*)(data + rc.nout - 256))[lane] = obuf[lane];
+// lane `lane` of v set to the uniform value x (the lane index in M0: two
+// scalar operands would exceed the constant bus)
+__device__ __forceinline__ void write_lane(uint32_t& v, uint32_t x, uint32_t lane) {
+    __asm__("v_writelane_b32 %0, %1, %2" : "+v"(v) : "s"(uniform(x)), "{m0}"(lane));
 }
 
-__device__ __forceinline__ void le_put(LeRc& rc, uint32_t b, uint32_t* obuf, uint8_t* data, uint32_t lane) {
-    if (rc.nout < rc.cap) {
-        rc.acc |= b << (8 * (rc.nout & 3));
-        rc.nout++;
-        if ((rc.nout & 3) == 0) le_put_word(rc, obuf, data, lane);
-    } else {
-        rc.full = true;
+// One finished byte: into acc; a whole word into its lane of obv; a whole
+// block of 256 bytes to the slot as 64 aligned dwords (only blocks inside
+// the largest chunk: past it the raw chunk is written anyway).
+__device__ __forceinline__ void le_put(LeRc& rc, uint32_t b, uint8_t* data) {
+    rc.acc |= (b & 0xFFu) << rc.sh;
+    rc.sh += 8;
+    rc.nout++;
+    if (rc.sh == 32) {
+        write_lane(rc.obv, rc.acc, ((rc.nout >> 2) - 1) & (WAVE - 1));
+        rc.acc = 0;
+        rc.sh = 0;
+        if ((rc.nout & 255) == 0 && rc.nout <= GC_SUB_BLOCK)
+            ((uint32_t*)(data + rc.nout - 256))[lane_id()] = rc.obv;
     }
 }
 
-// The range coder's byte output step, taken about once per 8 decisions.
-__device__ __forceinline__ void le_shift_low(LeRc& rc, uint32_t* obuf, uint8_t* data, uint32_t lane) {
+// The range coder's byte step (after a decision leaves range < 2^24): the
+// top byte of low is final unless a later carry can still reach it, which
+// is the case while it is 0xFF -- such bytes wait as a count (pending)
+// behind the last byte that can take a carry (cache).
+__device__ __forceinline__ void le_shift_low(LeRc& rc, uint8_t* data) {
     rc.low += rc.add;
     rc.add = 0;
-    const uint32_t hi = (uint32_t)(rc.low >> 32);
-    if ((uint32_t)rc.low < 0xFF000000u || hi != 0) {
-        uint32_t b = rc.cache;
-        do {
-            le_put(rc, (b + hi) & 0xFFu, obuf, data, lane);
-            b = 0xFFu;
-        } while (--rc.pending != 0);
-        rc.cache = (uint32_t)(rc.low >> 24) & 0xFFu;
+    const uint32_t lo = (uint32_t)rc.low, hi = (uint32_t)(rc.low >> 32);
+    if (lo < 0xFF000000u || hi != 0) {
+        le_put(rc, rc.cache + hi, data);
+        for (uint32_t j = 1; j < rc.pending; j++) le_put(rc, 0xFFu + hi, data);
+        rc.pending = 0;
+        rc.cache = lo >> 24;
     }
     rc.pending++;
-    rc.low = (rc.low & 0x00FFFFFFu) << 8;
+    rc.low = (uint64_t)((lo & 0x00FFFFFFu) << 8);
 }
 
 // One decision: probability pv, bit = bit k of v (all uniform).  bound =
@@ -2996,8 +3004,7 @@ __device__ __forceinline__ void le_shift_low(LeRc& rc, uint32_t* obuf, uint8_t* 
 // bit) where a bit test and two selects do: 7, plus the normalisation test.
 // One normalisation step always restores range >= 2^24: a probability stays
 // within [31, 2017], so range >= 2^13 * 31 after a decision.
-__device__ __forceinline__ void le_bit(LeRc& rc, uint32_t pv, uint32_t v, uint32_t k, uint32_t* obuf, uint8_t* data,
-                                       uint32_t lane) {
+__device__ __forceinline__ void le_bit(LeRc& rc, uint32_t pv, uint32_t v, uint32_t k, uint8_t* data) {
     uint32_t t, u;
     __asm__("s_lshr_b32 %[t], %[r], 11\n\t"
             "s_mul_i32 %[t], %[t], %[p]\n\t"
@@ -3011,7 +3018,7 @@ __device__ __forceinline__ void le_bit(LeRc& rc, uint32_t pv, uint32_t v, uint32
             : "scc");
     if (__builtin_expect(rc.range < (1u << 24), 0)) {
         rc.range <<= 8;
-        le_shift_low(rc, obuf, data, lane);
+        le_shift_low(rc, data);
     }
 }
 
@@ -3024,31 +3031,31 @@ __device__ __forceinline__ uint32_t le_update(uint32_t pv, uint32_t bit) {
 // Decisions [k0, k1) of a list coded with probabilities pv (bit k of bits
 // = decision k's bit).
 __device__ __forceinline__ void le_run(LeRc& rc, uint32_t pv, uint32_t bits, uint32_t k0, uint32_t k1,
-                                       uint32_t* obuf, uint8_t* data, uint32_t lane) {
-    for (uint32_t k = k0; k < k1; k++) le_bit(rc, bcast(pv, k), bits, k, obuf, data, lane);
+                                       uint8_t* data) {
+    for (uint32_t k = k0; k < k1; k++) le_bit(rc, bcast(pv, k), bits, k, data);
 }
 
 // Codes the list: probabilities read and updated by the lanes at once (lanes
 // without one use the spare slot), the range coder stepping through the
 // decisions in order (the direct bits, when there are any, as their own run).
-__device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* model, uint32_t* obuf, uint8_t* data) {
+__device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* model, uint8_t* data) {
     const uint32_t e = uniform(L.e), dn = uniform(L.dn);
     const uint32_t d0 = dn ? uniform(L.d0) : e;
     const uint32_t idx = L.idx != LE_DIRECT ? L.idx : LE_NPROBS - 1;  // lanes past the list hold LE_DIRECT
     const uint32_t pv = model[idx];
     model[idx] = (uint16_t)le_update(pv, L.bit);
     const uint32_t bits = uniform((uint32_t)ballot(L.bit != 0));  // at most 32 decisions; lanes past the list hold 0
-    le_run(rc, pv, bits, 0, d0, obuf, data, L.lane);
+    le_run(rc, pv, bits, 0, d0, data);
     if (dn) {
         for (uint32_t k = d0; k < d0 + dn; k++) {
             rc.range >>= 1;
             rc.add += ((bits >> k) & 1u) ? rc.range : 0u;
             if (__builtin_expect(rc.range < (1u << 24), 0)) {
                 rc.range <<= 8;
-                le_shift_low(rc, obuf, data, L.lane);
+                le_shift_low(rc, data);
             }
         }
-        le_run(rc, pv, bits, d0 + dn, e, obuf, data, L.lane);
+        le_run(rc, pv, bits, d0 + dn, e, data);
     }
 }
 
@@ -3057,8 +3064,7 @@ __device__ __forceinline__ void le_code(LeRc& rc, const LeList& L, uint16_t* mod
 // by mbyte until the first bit where the two differ.  Lane 0 reads and
 // updates is_match, lanes 1-8 the tree nodes; the 9 decisions are unrolled.
 __device__ __forceinline__ void le_literal(LeRc& rc, uint32_t state, uint32_t p, uint32_t byte, uint32_t prev,
-                                           uint32_t mbyte, uint16_t* model, uint32_t* obuf, uint8_t* data,
-                                           uint32_t lane) {
+                                           uint32_t mbyte, uint16_t* model, uint8_t* data, uint32_t lane) {
     const uint32_t i = lane - 1;
     const uint32_t base = LE_LIT + (prev >> 5) * 0x300u;
     const uint32_t sym = (1u << i) | (byte >> (8 - i));
@@ -3076,9 +3082,9 @@ __device__ __forceinline__ void le_literal(LeRc& rc, uint32_t state, uint32_t p,
 #pragma unroll
     for (uint32_t k = 0; k < 9; k++) q[k] = bcast(pv, k);
     const uint32_t v = uniform(byte);
-    le_bit(rc, q[0], 0u, 0, obuf, data, lane);
+    le_bit(rc, q[0], 0u, 0, data);
 #pragma unroll
-    for (uint32_t k = 1; k < 9; k++) le_bit(rc, q[k], v, 8 - k, obuf, data, lane);
+    for (uint32_t k = 1; k < 9; k++) le_bit(rc, q[k], v, 8 - k, data);
 }
 
 __device__ __forceinline__ uint32_t le_after_literal(uint32_t s) { return s < 4 ? 0 : (s < 10 ? s - 3 : s - 6); }
@@ -3101,7 +3107,7 @@ __device__ __forceinline__ uint32_t le_rep_len(uint64_t mm, uint32_t k, uint32_t
 }
 
 __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t* packed, uint16_t* model,
-                        uint32_t* obuf, uint8_t* out, uint32_t* sizes) {
+                        uint8_t* out, uint32_t* sizes) {
     const uint32_t lane = lane_id();
     const uint32_t n = it.n;
     gbytes d = (gbytes)at(arena, it.src_off);
@@ -3119,14 +3125,15 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
     rc.pending = 1;
     rc.nout = 0;
     rc.acc = 0;
+    rc.sh = 0;
+    rc.obv = 0;
     rc.cap = min(n + 5, GC_SUB_BLOCK);  // the host coder's room; past it the raw chunk is smaller anyway
-    rc.full = false;
     uint32_t state = 0, rep0 = 1, rep1 = 1, rep2 = 1, rep3 = 1;
     uint32_t p = 0, wb = 0;
     bool fresh = false;
     uint32_t cur = 0, src0 = 0, pkw = 0, pk64 = 0, prev0 = 0;
     uint64_t mm0 = 0, mm1 = 0, mm2 = 0, mm3 = 0, quiet = 0;
-    while (p < n && !rc.full) {
+    while (p < n && rc.nout <= rc.cap) {
         if (!fresh || p >= wb + WAVE) {  // a window at p
             wb = p;
             const uint32_t pos = wb + lane;
@@ -3155,13 +3162,19 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
         }
         uint32_t k = p - wb;
         if (state < 7 && ((quiet >> k) & 1u)) {
-            // a run of plain literals: the walk's tests are settled by the masks
-            do {
-                le_literal(rc, state, p, bcast(cur, k), k ? bcast(cur, k - 1) : prev0, 0, model, obuf, data, lane);
-                state = le_after_literal(state);
-                p++;
-                k++;
-            } while (k < WAVE && p < n && !rc.full && ((quiet >> k) & 1u));
+            // a run of plain literals: the walk's tests are settled by the
+            // masks (a full output buffer stops the walk after the run: the
+            // coder writes nothing past it)
+            const uint64_t x = ~(quiet >> k);
+            const uint32_t run = min(x ? (uint32_t)__builtin_ctzll(x) : (uint32_t)WAVE, n - p);
+            uint32_t prev = k ? bcast(cur, k - 1) : prev0;
+            for (uint32_t j = 0; j < run; j++) {
+                const uint32_t byte = bcast(cur, k + j);
+                le_literal(rc, state, p + j, byte, prev, 0, model, data, lane);
+                state = state < 4 ? 0u : state - 3;  // le_after_literal below 7
+                prev = byte;
+            }
+            p += run;
             continue;
         }
         const uint32_t avail = min(n - p, 273u);
@@ -3265,22 +3278,20 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
             state = state < 7 ? 9u : 11u;
             p += 1;
         } else {
-            le_literal(rc, state, p, bcast(cur, k), k ? bcast(cur, k - 1) : prev0, bcast(src0, k), model, obuf, data,
-                       lane);
+            le_literal(rc, state, p, bcast(cur, k), k ? bcast(cur, k - 1) : prev0, bcast(src0, k), model, data, lane);
             state = le_after_literal(state);
             p += 1;
             continue;
         }
-        le_code(rc, L, model, obuf, data);
+        le_code(rc, L, model, data);
     }
-    for (int i = 0; i < 5; i++) le_shift_low(rc, obuf, data, lane);
+    for (int i = 0; i < 5; i++) le_shift_low(rc, data);
     const uint32_t c = rc.nout;
-    if (!rc.full && 7 + c < n + 4) {
-        // the buffer's last bytes (whole words in obuf, the last 0-3 in acc),
-        // then the header and the end marker
-        const uint32_t tail = c & 255u, t0 = c - tail;
-        if (lane == 0 && (c & 3u)) obuf[(c >> 2) & (WAVE - 1)] = rc.acc;
-        for (uint32_t j = lane; j < tail; j += WAVE) data[t0 + j] = ((const uint8_t*)obuf)[j];
+    if (c <= rc.cap && 7 + c < n + 4) {
+        // the last block's whole words are in obv, its last 0-3 bytes in acc
+        // (the rest of acc is 0): acc's word also carries the end marker
+        write_lane(rc.obv, rc.acc, (c >> 2) & (WAVE - 1));
+        if (lane <= ((c & 255u) >> 2)) ((uint32_t*)(data + (c & ~255u)))[lane] = rc.obv;
         if (lane < 6) {
             const uint32_t h = lane == 0   ? 0xE0u | ((n - 1) >> 16)
                                : lane == 1 ? ((n - 1) >> 8) & 0xFFu
@@ -3290,7 +3301,6 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
                                            : 2u * 5u * 9u + 3u;  // (pb * 5 + lp) * 9 + lc
             slot[LE_HEAD + lane] = (uint8_t)h;
         }
-        if (lane == 0) data[c] = 0;
         if (lane == 0) sizes[it.count_pos] = 7 + c;
     } else {
         // one uncompressed chunk with a dictionary reset
@@ -3310,8 +3320,7 @@ extern "C" __global__ void __launch_bounds__(WAVE)
 gc_lzma2_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
                        const uint32_t* packed, uint8_t* out, uint32_t* sizes) {
     __shared__ uint16_t model[LE_NPROBS];
-    __shared__ uint32_t obuf[WAVE];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x)
-        le_item(load_item(items, i), arena, packed, model, obuf, out, sizes);
+        le_item(load_item(items, i), arena, packed, model, out, sizes);
 }
 #endif  // GC_BUILD_MAIN
