@@ -13,7 +13,10 @@ oracle.  The batched configurations (1 GiB Zstd L1+L2 in 16 MiB chunks; 640 MiB 
 4096 inputs, persistent device-queue form replayed from a hipGraph) are extra keys, each with its own
 roofline.
 
-    python bench.py [--steps K] [--warmup W] [--workload serve|zstd|lzma2|mixed] [--no-extras] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload serve|zstd|lzma2|mixed|lzma2_coder] [--no-extras] [--no-cpu]
+
+`--workload lzma2_coder` times the GPU LZMA2 coder (the 640 MiB of the LZMA2 configuration, streams byte-equal
+to the host coder's) and NVLZ end to end with the GPU and the host coder.
 
 Prints one JSON line; --gpus N > 1 exits with a message: this path runs on one GPU.
 """
@@ -637,6 +640,40 @@ def gpu_lzma2_coder(hip, ctx, data):
             "raw_sub_blocks": raw, "sub_blocks": nsub}
 
 
+def bench_lzma2_coder(a, hip, ctx):
+    """The GPU LZMA2 coder alone on configuration 3 (640 MiB, one launch over its 10240 sub-blocks, HIP
+    events), then NVLZ end to end from host memory (GPU match finding and coding, streams gathered and copied
+    back) beside the same container from the host coder on min(cores, 8) threads: byte-equal, decoded back."""
+    data, _ = W.config3()
+    r = gpu_lzma2_coder(hip, ctx, data)
+    n = int(data.size)
+    # per sub-block the coder reads its bytes and their packed matches (4 B per position) and writes the stream
+    alg = int(n * 5 + n / r["ratio"])
+    ms = r["kernel_ms"]
+    e2e = {}
+    blob = None
+    for key, fn in (("gpu_coder", lambda: gcompress.nvlz_compress_gpu(data, 128)),
+                    ("host_coder", lambda: gcompress.nvlz_compress_matches(data, 128, threads=min(threads_all(), 8)))):
+        t = time.perf_counter()
+        b = fn()
+        dt = time.perf_counter() - t
+        if blob is None:
+            blob = b
+            assert gcompress.decompress(b) == data.tobytes(), "NVLZ does not round-trip"
+        e2e[key] = {"s": round(dt, 3), "gbs": round(n / dt / 1e9, 3), "ratio": round(n / len(b), 4),
+                    "identical_to_gpu_coder": b == blob}
+    return {"metric": "LZMA2 coding input GB/s (GPU coder; streams byte-equal to the host coder's)", "value":
+            r["encode_input_gbs"], "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic (seeded disk-image-like mix)",
+            "config": {"workload": "LZMA2 coding of the 640 MiB disk-image-like mix from its GPU matches",
+                       "form": "one launch, one wavefront per 64 KiB sub-block"},
+            "roofline": {"bound": "hbm", "kernel": "gc_lzma2_encode_kernel", "achieved": round(alg / ms / 1e6, 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(alg / ms / 1e6 / HBM_PEAK_GBS, 5),
+                         "traffic": None, "algorithmic_bytes": alg},
+            "cpu_baseline": None, "detail": r, "end_to_end_nvlz": e2e, "host": host_cores()}
+
+
 def extras_all(a, hip, ctx):
     ex = {}
     if not a.no_lzma2_e2e:
@@ -666,7 +703,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed"], default="serve")
+    ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed", "lzma2_coder"], default="serve")
     ap.add_argument("--no-extras", action="store_true", help="serve: only the serving workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lzma2-e2e", action="store_true", help="serve: skip the LZMA2 end-to-end leg")
@@ -697,6 +734,8 @@ def main():
     if a.workload == "serve":
         extras = {} if a.no_extras else extras_all(a, hip, ctx)
         out = bench_serve(a, hip, ctx, extras)
+    elif a.workload == "lzma2_coder":
+        out = bench_lzma2_coder(a, hip, ctx)
     else:
         if a.workload == "zstd":
             data, inputs = W.config2()
