@@ -187,7 +187,7 @@ int launch_gather(gc_ctx* ctx, hipStream_t stream, const void* d_in, const uint3
 int launch_order(gc_ctx* ctx, hipStream_t stream, const uint32_t* cost, uint32_t n, uint32_t* order,
                  const gc_queue* q) {
     void* args[] = {&cost, &n, &order, &q};
-    return launch(stream, ctx->k.order_items, 1, 1024, args, "launch gc_order_items");
+    return launch(stream, ctx->k.order_items, 1, 256, args, "launch gc_order_items");
 }
 
 int launch_cand_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, gc_queue* q,

@@ -1067,10 +1067,14 @@ gc_zstd_info_sq_kernel(const uint8_t* arena, const gc_item* __restrict__ items, 
 
 // The parse's item order, costliest first (counting sort of cost >> 9 into
 // 129 buckets, descending): the sub-blocks that take longest start first and
-// the launch does not end on a few long ones.  One workgroup.
+// the launch does not end on a few long ones.  One workgroup of 256 threads
+// (a few microseconds of work): in the served form it is launched while the
+// other tick slots' kernels fill the CUs, and a 1024-thread workgroup waited
+// up to 12 ms for a CU with 16 free wavefront slots.
 #define ORDER_BUCKETS 129
+#define ORDER_THREADS 256
 // (q: the item count is the queue's parse count)
-extern "C" __global__ void __launch_bounds__(1024) gc_order_items(const uint32_t* cost, uint32_t n, uint32_t* order,
+extern "C" __global__ void __launch_bounds__(ORDER_THREADS) gc_order_items(const uint32_t* cost, uint32_t n, uint32_t* order,
                                                                   const gc_queue* q) {
     __shared__ uint32_t start[ORDER_BUCKETS];
     if (q) n = uniform(__atomic_load_n(&q->n[GC_Q_PARSE], __ATOMIC_RELAXED));
