@@ -52,6 +52,13 @@ namespace {
 const uint32_t kMaxSeq = GC_SUB_BLOCK / 3;
 const uint64_t kAlign = 256;
 const int kSlots = 3;
+// Under a backlog (requests waiting for admission, i.e. the HBM budget is
+// full, or more than kBacklogItems sub-blocks admitted and waiting) at most
+// kBacklogTicks ticks run at once: fewer, larger ticks leave fewer kernel
+// tails (saturation throughput), while under a light load every slot takes
+// work as soon as it is free (latency).
+const int kBacklogTicks = 2;
+const uint64_t kBacklogItems = 16384;
 
 uint64_t up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
@@ -201,6 +208,7 @@ struct gc_server {
     // not released)
     std::vector<Req*> copying;    // host inputs on their way in
     std::deque<Req*> ready;       // admitted, input on the device, in arrival order
+    uint64_t ready_items = 0;     // their sub-blocks
     Slot slot[kSlots];
     std::vector<Req*> fetching;
     std::deque<std::vector<Req*>> finished;  // ticks whose results still go back to the host
@@ -315,6 +323,7 @@ void gc_server::admit_locked() {
         if (r.d_in) {
             r.state = READY;
             ready.push_back(&r);
+            ready_items += r.nsub;
             continue;
         }
         r.state = COPYING;
@@ -340,6 +349,7 @@ bool gc_server::launch_tick(Slot& sl) {
             Req* r = ready.front();
             if (n_items + r->nsub > max_batch_items && !batch.empty()) break;
             ready.pop_front();
+            ready_items -= r->nsub;
             batch.push_back(r);
             n_items += r->nsub;
         }
@@ -583,6 +593,7 @@ void gc_server::loop() {
                 if (hipEventQuery(r->copied) == hipSuccess) {
                     r->state = READY;
                     ready.push_back(r);
+                    ready_items += r->nsub;
                     copying[j] = copying.back();
                     copying.pop_back();
                     progressed = true;
@@ -596,8 +607,21 @@ void gc_server::loop() {
                 tick_ended(sl);
                 progressed = true;
             }
-        for (auto& sl : slot)
-            if (!sl.busy) progressed |= launch_tick(sl);
+        {
+            int running = 0;
+            for (auto& sl : slot) running += sl.busy ? 1 : 0;
+            for (auto& sl : slot) {
+                if (sl.busy) continue;
+                if (running >= kBacklogTicks) {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (!pending.empty() || ready_items > kBacklogItems) break;
+                }
+                if (launch_tick(sl)) {
+                    progressed = true;
+                    running++;
+                }
+            }
+        }
         if (!fetching.empty() && hipEventQuery(fetch_done) == hipSuccess) {
             complete_fetch();
             progressed = true;
