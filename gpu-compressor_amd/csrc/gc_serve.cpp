@@ -784,6 +784,53 @@ int64_t gc_server_submit_dev(gc_server* s, const uint8_t* d_input, uint64_t n, u
     return submit(s, std::move(r));
 }
 
+int gc_server_submit_dev_many(gc_server* s, const uint8_t* const* d_inputs, const uint64_t* sizes,
+                              const uint32_t* algos, const uint32_t* levels, uint32_t count, int64_t* ids) {
+    if (!s || (count && (!d_inputs || !sizes || !algos || !levels || !ids)))
+        return fail(GC_EINVAL, "gc_server_submit_dev_many: null argument");
+    // checked and laid out first, then queued under one lock, in order
+    std::vector<std::unique_ptr<Req>> rs(count);
+    const double t = now_s();
+    for (uint32_t i = 0; i < count; i++) {
+        if ((!d_inputs[i] && sizes[i]) || algos[i] > GC_ALGO_LZMA2)
+            return fail(GC_EINVAL, "gc_server_submit_dev_many: bad request");
+        if ((uintptr_t)d_inputs[i] & 3u) return fail(GC_EINVAL, "gc_server_submit_dev_many: d_input must be 4-byte aligned");
+        auto r = std::make_unique<Req>();
+        r->d_in = d_inputs[i];
+        r->dev_result = true;
+        r->n = sizes[i];
+        r->algo = algos[i];
+        r->level = levels[i];
+        r->nsub = (uint32_t)((r->n + GC_SUB_BLOCK - 1) / GC_SUB_BLOCK);
+        layout(*r);
+        if (r->dev_len > s->budget) return fail(GC_EBUDGET, "request needs more device memory than the whole HBM budget");
+        if (r->nsub > s->max_batch_items) return fail(GC_EBUDGET, "request has more sub-blocks than a tick holds");
+        r->t_submit = t;
+        rs[i] = std::move(r);
+    }
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return fail(GC_EINVAL, "server stopped");
+        bool any_done = false;
+        for (uint32_t i = 0; i < count; i++) {
+            Req* r = rs[i].get();
+            r->id = s->next_id++;
+            ids[i] = (int64_t)r->id;
+            if (r->n == 0) {
+                r->state = DONE;
+                r->t_admit = r->t_start = r->t_done = r->t_submit;
+                any_done = true;
+            } else {
+                s->pending.push_back(r->id);
+            }
+            s->reqs[r->id] = std::move(rs[i]);
+        }
+        if (any_done) s->cv_done.notify_all();
+    }
+    s->cv_work.notify_one();
+    return GC_OK;
+}
+
 int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info) {
     if (!s) return fail(GC_EINVAL, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
@@ -865,6 +912,21 @@ int gc_server_release(gc_server* s, uint64_t id) {
     }
     s->reqs.erase(it);
     return GC_OK;
+}
+
+int gc_server_wait_release_many(gc_server* s, const uint64_t* ids, uint32_t count, gc_request_info* infos,
+                                int* status) {
+    if (!s || (count && (!ids || !infos || !status))) return fail(GC_EINVAL, "null argument");
+    int first_err = GC_OK;
+    for (uint32_t i = 0; i < count; i++) {
+        const int rc = gc_server_wait(s, ids[i], -1, &infos[i]);
+        status[i] = rc;
+        if (rc < 0 && first_err == GC_OK) first_err = rc;
+        if (rc == GC_EINVAL) continue;  // unknown id: nothing to release
+        const int rr = gc_server_release(s, ids[i]);
+        if (rr < 0 && first_err == GC_OK) first_err = rr;
+    }
+    return first_err;
 }
 
 int gc_server_get_stats(gc_server* s, gc_server_stats* out) {

@@ -110,6 +110,8 @@ SIGNATURES = [
     ("gc_server_submit", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
     ("gc_server_submit_into", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp, C.c_uint64, _vp]),
     ("gc_server_submit_dev", C.c_int64, [_vp, _vp, C.c_uint64, C.c_uint32, C.c_uint32]),
+    ("gc_server_submit_dev_many", C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    ("gc_server_wait_release_many", C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     ("gc_server_wait", C.c_int, [_vp, C.c_uint64, C.c_int, _vp]),
     ("gc_server_result_dev", C.c_int, [_vp, C.c_uint64, _vp, _vp]),
     ("gc_server_result_zstd", C.c_int, [_vp, C.c_uint64, _vp, C.c_uint64, _vp]),
@@ -637,6 +639,32 @@ class Server:
         if rid < 0:
             _check(int(rid), "gc_server_submit_dev")
         return rid
+
+    @staticmethod
+    def dev_batch(reqs):
+        """[(device pointer, n, algo, level)] as the arrays submit_dev_many takes."""
+        return (np.array([r[0] for r in reqs], np.uint64), np.array([r[1] for r in reqs], np.uint64),
+                np.array([r[2] for r in reqs], np.uint32), np.array([r[3] for r in reqs], np.uint32))
+
+    def submit_dev_many(self, batch) -> np.ndarray:
+        """Submit a dev_batch at once (one lock, in order); returns their ids."""
+        ptrs, sizes, algos, levels = batch
+        k = ptrs.size
+        ids = np.zeros(max(k, 1), np.int64)
+        _check(lib().gc_server_submit_dev_many(self.h, _ptr(ptrs), _ptr(sizes), _ptr(algos), _ptr(levels), k,
+                                               _ptr(ids)), "gc_server_submit_dev_many")
+        return ids[:k]
+
+    def wait_release_many(self, ids):
+        """Wait for each id in order and release it; returns (infos, status codes)."""
+        arr = np.ascontiguousarray(ids, np.uint64)
+        k = arr.size
+        infos = (RequestInfo * max(k, 1))()
+        status = np.zeros(max(k, 1), np.int32)
+        lib().gc_server_wait_release_many(self.h, _ptr(arr), k, infos, _ptr(status))
+        for rid in arr:
+            self._keep.pop(int(rid), None)
+        return infos, status[:k]
 
     def result_dev(self, rid: int, n_sub: int):
         """(device pointer of the results, Zstd sequence counts as an array or None)."""

@@ -455,6 +455,13 @@ int64_t gc_server_submit_into(gc_server* s, const uint8_t* data, uint64_t n, uin
  * bytes readable): it is read in place and its results stay on the device
  * (gc_server_result_dev) until gc_server_release. */
 int64_t gc_server_submit_dev(gc_server* s, const uint8_t* d_input, uint64_t n, uint32_t algo, uint32_t level);
+/* count gc_server_submit_dev requests at once (request i: d_inputs[i],
+ * sizes[i], algos[i], levels[i]), queued in order under one lock; ids[i]
+ * receives request i's id.  All are checked first: on an error none is
+ * submitted.  No reference counterpart (a serving client's batch of
+ * arrivals). */
+int gc_server_submit_dev_many(gc_server* s, const uint8_t* const* d_inputs, const uint64_t* sizes,
+                              const uint32_t* algos, const uint32_t* levels, uint32_t count, int64_t* ids);
 /* 0 done, 1 still running after timeout_ms (< 0 waits), < 0 failed. */
 int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* info);
 /* Results of a finished request: Zstd packed sequences (cap records) and per
@@ -468,6 +475,12 @@ int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed);
  * (Zstd, host memory, n_sub entries; NULL for LZMA2). */
 int gc_server_result_dev(gc_server* s, uint64_t id, void** d_results, const uint32_t** seq_counts);
 int gc_server_release(gc_server* s, uint64_t id);
+/* For each of ids in order: wait for it (no time limit), fill infos[i],
+ * status[i] = what gc_server_wait returned, then release it (its device
+ * results too).  Returns the first error, or GC_OK.  For a client that
+ * reaps many completions at once. */
+int gc_server_wait_release_many(gc_server* s, const uint64_t* ids, uint32_t count, gc_request_info* infos,
+                                int* status);
 int gc_server_get_stats(gc_server* s, gc_server_stats* out);
 /* Finishes every submitted request, then stops. */
 int gc_server_stop(gc_server* s);

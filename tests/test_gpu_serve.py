@@ -279,6 +279,40 @@ def test_device_resident_requests(gctx):
         gctx.free(d)
 
 
+def test_bulk_submit_and_reap(gctx):
+    """gc_server_submit_dev_many / gc_server_wait_release_many: the same results as one call per request
+    (equal to the oracle), every request released, and a batch with a bad request rejected whole."""
+    pool = gcompress.corpus(8 << 20, 92, [1, 1, 1, 1], 65536, False)
+    d = gctx.alloc(pool.size + 64)
+    gctx.htod(d, pool)
+    rng = np.random.default_rng(10)
+    reqs = [(0, 0, gcompress.ALGO_ZSTD, 1)]  # an empty request is done at once
+    for i in range(30):
+        n = int(np.exp(rng.uniform(np.log(1024), np.log(3 << 20))))
+        o = int(rng.integers(0, pool.size - n)) & ~3
+        reqs.append((o, n) + [(gcompress.ALGO_ZSTD, 1), (gcompress.ALGO_ZSTD, 2), (gcompress.ALGO_LZMA2, 1)][i % 3])
+    srv = gcompress.Server(gctx, 1 << 30)  # tight: admission waits for releases
+    try:
+        print("[serve] bulk submit", flush=True)
+        ids = srv.submit_dev_many(srv.dev_batch([(d + o, n, a, lv) for o, n, a, lv in reqs]))
+        assert len(ids) == len(reqs) and len(set(int(x) for x in ids)) == len(reqs)
+        for rid, (o, n, algo, level) in list(zip(ids, reqs))[:12]:
+            _check_dev(gctx, srv, int(rid), pool[o:o + n], algo, level)
+        infos, status = srv.wait_release_many(ids[12:])
+        assert list(status) == [0] * (len(reqs) - 12)
+        for info, (o, n, algo, level) in zip(infos, reqs[12:]):
+            assert info.size == n and info.t_submit <= info.t_admit <= info.t_start <= info.t_done
+        st = srv.stats()
+        assert st["hbm_in_use"] == 0 and st["pending"] == 0
+        bad = [(d, 4096, gcompress.ALGO_ZSTD, 1), (d + 2, 100, gcompress.ALGO_ZSTD, 1)]
+        with pytest.raises(gcompress.GcError, match="aligned"):
+            srv.submit_dev_many(srv.dev_batch(bad))
+        assert srv.stats()["pending"] == 0
+    finally:
+        srv.stop()
+        gctx.free(d)
+
+
 def test_ticks_overlap(gctx):
     """Continuous batching: a request that arrives while a long tick runs starts in the next tick slot
     beside it (after that tick started, before it ended), and its result is the same as alone."""
