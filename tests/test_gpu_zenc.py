@@ -97,7 +97,7 @@ def test_encode_block_dev_reference_abi(gctx, fmt, chunk):
     assert L.gc_zstd_encode_block_dev(gctx.h, 8, 8, fmt, 8, 8, 8, 8, 8, chunk, nsub, n, max_seq, chunk) != 0
 
 
-@pytest.mark.parametrize("n", [0, 7, SUB, 5 * SUB + 9])
+@pytest.mark.parametrize("n", [0, 7, SUB, 5 * SUB + 9, (1 << 30) + 3 * SUB + 5])  # the last: two pieces
 def test_nvzs_gpu_container(gctx, n):
     data = gcompress.corpus(n, 34, (1, 1, 1, 1), 4096) if n else np.zeros(0, np.uint8)
     print(f"[zenc nvzs] n={n}", flush=True)
