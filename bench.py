@@ -198,7 +198,7 @@ def served_digest(ctx, srv, rid, n, algo):
         out = np.empty(n, np.uint32)
         ctx.dtoh(out, p)
         return digest(out)
-    slots = np.empty((nsub, MAX_SEQ, 4), np.uint32)
+    slots = np.empty((nsub, gcompress.SERVE_ZSTD_SLOT, 4), np.uint32)
     ctx.dtoh(slots, p)
     return zstd_digest_from_slots(slots, counts)
 

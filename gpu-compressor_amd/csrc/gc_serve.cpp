@@ -50,6 +50,11 @@ using namespace gc;
 namespace {
 
 const uint32_t kMaxSeq = GC_SUB_BLOCK / 3;
+// A device-result Zstd request's sub-block region: its candidates (2 B per
+// position), then its info words (4 B), with its sequence slot
+// (GC_SERVE_ZSTD_SLOT records of 16 B) laid over both from the start.
+const uint64_t kOverlay = (uint64_t)GC_SERVE_ZSTD_SLOT * sizeof(gc_zstd_sequence);
+static_assert(kOverlay == 6ull * GC_SUB_BLOCK, "slot = candidates + info words of one sub-block");
 const uint64_t kAlign = 256;
 const int kSlots = 3;
 // Under a backlog (requests waiting for admission, i.e. the HBM budget is
@@ -131,6 +136,7 @@ struct Req {
     bool dev_result = false;  // results stay on the device (gc_server_submit_dev)
     uint64_t dev_off = 0, dev_len = 0;  // the request's region of the pool
     uint64_t in_off = 0, cand_off = 0, info_off = 0, out_off = 0;  // byte offsets in the region
+    bool overlay = false;  // the sequence slots lie over the candidates and info words (kOverlay per sub-block)
     uint64_t cnt_off = 0;  // first entry of the request's sequence counts in the counts zone
     uint32_t cnt_len = 0;
     void* dst = nullptr;      // caller's result buffer (gc_server_submit_into), else results kept here
@@ -147,16 +153,32 @@ struct Req {
     uint64_t fetch_total = 0;  // Zstd: sequences found
 };
 
-// Device bytes a request needs (every part 256-byte aligned).  For Zstd the
-// candidate and info regions are contiguous and together hold at least
-// 21845 x 16 bytes per sub-block: packed sequences for a pageable
-// destination are gathered there at the end.
+// Device bytes a request needs (every part 256-byte aligned).  For a Zstd
+// request whose results go to the host the candidate and info regions are
+// contiguous and together hold at least 21845 x 16 bytes per sub-block:
+// packed sequences for a pageable destination are gathered there at the
+// end.  A Zstd request whose results stay on the device has one region of
+// kOverlay bytes per sub-block holding its candidates, then its info words,
+// and, from its start, its sequence slot: the candidates are dead once the
+// info pass has read them, and the parse writes its records 64 at a time
+// behind the info words it still reads (after j sequences it has passed
+// position 3j, so record j + 63 ends at byte 16 (j + 64), below the first
+// info word it may still read, 2 * 65536 + 4 * (3 (j + 64) - 64): j + 64 <=
+// 21845 + 63 keeps it there), so a request holds 6 bytes per input byte
+// instead of 11.3 and the HBM budget admits larger ticks.
 void layout(Req& r) {
     const uint64_t ns = r.nsub, slots = ns * GC_SUB_BLOCK;
     uint64_t o = 0;
     if (!r.d_in) {  // a host input is copied here (with room past its end for the kernels' reads)
         r.in_off = o;
         o += up(r.n + 64, kAlign);
+    }
+    if (r.dev_result && r.algo == GC_ALGO_ZSTD) {
+        r.overlay = true;
+        r.cand_off = r.info_off = r.out_off = o;
+        o += ns * kOverlay;
+        r.dev_len = o;
+        return;
     }
     r.cand_off = o;
     o += up(slots * 2, kAlign);
@@ -378,7 +400,14 @@ bool gc_server::launch_tick(Slot& sl) {
                 it.lit_pos = 0;
                 it.n = (uint32_t)std::min<uint64_t>(GC_SUB_BLOCK, r->n - start);
                 it.out_span = GC_SUB_BLOCK;
-                if (r->algo == GC_ALGO_ZSTD) {
+                if (r->overlay) {
+                    const uint64_t reg = base + r->out_off + (uint64_t)s * kOverlay;
+                    it.cpos = reg / 2;
+                    it.opos = (reg + 2ull * GC_SUB_BLOCK) / 4;
+                    it.seq_pos = reg / sizeof(gc_zstd_sequence);
+                    it.count_pos = (uint32_t)(r->cnt_off + s);
+                    it.depth = zstd_depth_for_level(r->level);
+                } else if (r->algo == GC_ALGO_ZSTD) {
                     it.opos = (base + r->info_off) / 4 + start;
                     it.seq_pos = (base + r->out_off) / sizeof(gc_zstd_sequence) + (uint64_t)s * kMaxSeq;
                     it.count_pos = (uint32_t)(r->cnt_off + s);

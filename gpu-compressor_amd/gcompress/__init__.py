@@ -29,6 +29,7 @@ ZSTD_DEPTH_LAZY = 16         # constants.rs:6
 LZMA2_MAX_MATCHES = 8        # constants.rs:12
 LZMA2_DEPTH = 32             # constants.rs:13
 MAX_SEQ = SUB_BLOCK // 3     # compress_zstd_custom.rs:55
+SERVE_ZSTD_SLOT = 24576      # records per sub-block slot of a served device-result Zstd request
 
 SEQ_FORMAT_ZSTD = 0
 SEQ_FORMAT_REF = 1

@@ -469,10 +469,14 @@ int gc_server_wait(gc_server* s, uint64_t id, int timeout_ms, gc_request_info* i
 int gc_server_result_zstd(gc_server* s, uint64_t id, gc_zstd_sequence* seqs, uint64_t cap, uint32_t* seq_counts);
 int gc_server_result_lzma2(gc_server* s, uint64_t id, uint32_t* packed);
 /* Results of a finished gc_server_submit_dev request, valid until it is
- * released: *d_results is a device pointer, Zstd: n_sub * 21845
- * gc_zstd_sequence slots (sub-block k's at k * 21845, the first
- * seq_counts[k] used), LZMA2: one packed u32 per input byte; *seq_counts
- * (Zstd, host memory, n_sub entries; NULL for LZMA2). */
+ * released: *d_results is a device pointer, Zstd: n_sub slots of
+ * GC_SERVE_ZSTD_SLOT gc_zstd_sequence records (sub-block k's at k *
+ * GC_SERVE_ZSTD_SLOT, the first seq_counts[k] used, at most 21845), LZMA2:
+ * one packed u32 per input byte; *seq_counts (Zstd, host memory, n_sub
+ * entries; NULL for LZMA2).  (A slot is 6 bytes per position of its
+ * sub-block: while the request runs it holds the sub-block's candidates and
+ * info words, so the request takes 6 bytes of the HBM budget per input byte.) */
+#define GC_SERVE_ZSTD_SLOT 24576u
 int gc_server_result_dev(gc_server* s, uint64_t id, void** d_results, const uint32_t** seq_counts);
 int gc_server_release(gc_server* s, uint64_t id);
 /* For each of ids in order: wait for it (no time limit), fill infos[i],

@@ -226,7 +226,7 @@ def _dev_result(gctx, srv, rid, n, algo):
         out = np.empty(n, np.uint32)
         gctx.dtoh(out, p)
         return out, info
-    slots = np.empty((nsub, gcompress.MAX_SEQ, 4), np.uint32)
+    slots = np.empty((nsub, gcompress.SERVE_ZSTD_SLOT, 4), np.uint32)
     gctx.dtoh(slots, p)
     rows = np.concatenate([slots[k, : counts[k]] for k in range(nsub)] + [np.zeros((0, 4), np.uint32)])
     assert info.n_sequences == rows.shape[0]
