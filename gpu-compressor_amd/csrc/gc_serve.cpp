@@ -59,12 +59,12 @@ const uint64_t kAlign = 256;
 const int kSlots = 3;
 // Under a backlog (requests waiting for admission, i.e. the HBM budget is
 // full, or more than kBacklogItems sub-blocks admitted and waiting) at most
-// kBacklogTicks ticks run at once: each tick then holds as much as the
-// budget admits, and one large tick leaves fewer kernel tails than several
-// overlapping ones (saturation throughput: 1 tick 148 GB/s, 2 145, 3 141 on
-// the 6000-request mix), while under a light load every slot takes work as
-// soon as it is free (latency).
-const int kBacklogTicks = 1;
+// kBacklogTicks ticks run at once: each tick then holds a large share of
+// what the budget admits, and large ticks leave fewer kernel tails than
+// many overlapping ones (saturation throughput on the 6000-request mix: 2
+// ticks 150.6 GB/s, 1 tick 149.4, 3 ticks about 141), while under a light
+// load every slot takes work as soon as it is free (latency).
+const int kBacklogTicks = 2;
 const uint64_t kBacklogItems = 16384;
 
 uint64_t up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
