@@ -10,7 +10,7 @@ for p in (ROOT, os.path.join(ROOT, "gpu-compressor_amd"), os.path.join(ROOT, "te
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (run on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X")
     config.addinivalue_line("markers", "slow: long CPU test")
 
 

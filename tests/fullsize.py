@@ -15,7 +15,7 @@ MAX_SEQ = 65536 // 3
 
 
 def threads() -> int:
-    """Host threads for the oracle and the CPU encoders (the GPU box's CPU share, at most 16)."""
+    """Host threads for the oracle and the CPU encoders (the host's CPU share, at most 16)."""
     return max(1, min(len(os.sched_getaffinity(0)), 16))
 
 
