@@ -313,6 +313,28 @@ def test_bulk_submit_and_reap(gctx):
         gctx.free(d)
 
 
+def test_dense_sequences_device_results(gctx):
+    """Device-result Zstd requests whose sub-blocks hold about 10 000 sequences each (3-byte tokens in random
+    order): their records reach far into the info words they are laid over while the parse still reads
+    them, and every sequence equals the oracle's."""
+    rng = np.random.default_rng(17)
+    toks = rng.integers(0, 256, (16, 3), dtype=np.uint8)
+    data = np.ascontiguousarray(toks[rng.integers(0, 16, 3 * SUB // 3 + 900)].reshape(-1)[: 3 * SUB + 2500])
+    d = gctx.alloc(data.size + 64)
+    gctx.htod(d, data)
+    srv = gcompress.Server(gctx, 1 << 30)
+    try:
+        print("[serve] dense sequences", flush=True)
+        for level in (1, 2):
+            rid = srv.submit_dev(d, data.size, gcompress.ALGO_ZSTD, level)
+            _check_dev(gctx, srv, rid, data, gcompress.ALGO_ZSTD, level)
+        _, counts = oracle.zstd_match_find(data, depth=16)[:2]
+        assert counts[:3].min() > 9000  # a whole sub-block's records cover more than 140 KB of its 384 KB
+    finally:
+        srv.stop()
+        gctx.free(d)
+
+
 def test_ticks_overlap(gctx):
     """Continuous batching: a request that arrives while a long tick runs starts in the next tick slot
     beside it (after that tick started, before it ended), and its result is the same as alone."""
