@@ -59,12 +59,14 @@ const uint64_t kAlign = 256;
 const int kSlots = 3;
 // Under a backlog (requests waiting for admission, i.e. the HBM budget is
 // full, or more than kBacklogItems sub-blocks admitted and waiting) at most
-// kBacklogTicks ticks run at once: each tick then holds a large share of
-// what the budget admits, and large ticks leave fewer kernel tails than
-// many overlapping ones (saturation throughput on the 6000-request mix: 2
-// ticks 150.6 GB/s, 1 tick 149.4, 3 ticks about 141), while under a light
-// load every slot takes work as soon as it is free (latency).
-const int kBacklogTicks = 2;
+// kBacklogTicks ticks run at once: each tick then holds as much as the
+// budget admits, and large ticks leave fewer kernel tails than many
+// overlapping ones (saturation throughput on the 6000-request mix: 1 tick
+// 149.4-150.3 GB/s, 2 ticks 150.6, 3 ticks about 141); one tick at a time
+// also keeps each kernel alone on the GPU, so its measured duration is its
+// own.  Under a light load every slot takes work as soon as it is free
+// (latency).
+const int kBacklogTicks = 1;
 const uint64_t kBacklogItems = 16384;
 
 uint64_t up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
