@@ -390,9 +390,10 @@ int gc_lzma2_encode_dev(gc_ctx* ctx, const uint8_t* d_input, const uint32_t* d_p
     std::vector<gc_item> items;
     make_items(items, total_input_size, GC_SUB_BLOCK, (uint32_t)nsub, 0, 0, false);
     int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
+    if (!rc) rc = ctx->out_b.ensure(nsub * 8);  // the items' costs and their order
     if (rc) return rc;
-    return launch_lzma2_encode(ctx, ctx->stream, d_input, (const gc_item*)ctx->items.p, (uint32_t)nsub, d_packed,
-                               d_output, d_output_sizes);
+    return launch_lzma2_encode_ordered(ctx, ctx->stream, d_input, (const gc_item*)ctx->items.p, (uint32_t)nsub,
+                                       d_packed, d_output, d_output_sizes, (uint32_t*)ctx->out_b.p);
 }
 
 int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* out, size_t cap, uint64_t* sizes,
@@ -409,6 +410,7 @@ int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* o
     if (!rc) rc = ctx->out_e.ensure((size_t)nsub * GC_LZMA2_GPU_SLOT);  // one slot per sub-block
     if (!rc) rc = ctx->out_c.ensure(n + 4 * nsub + 16);            // the streams, back to back
     if (!rc) rc = ctx->out_f.ensure(nsub * 12 + 16);
+    if (!rc) rc = ctx->out_b.ensure(nsub * 8);  // the items' costs and their order
     if (rc) return rc;
     const uint8_t* d_in = (const uint8_t*)ctx->input.p;
     uint32_t* d_packed = (uint32_t*)ctx->out_a.p;
@@ -416,8 +418,8 @@ int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* o
     uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
     rc = run_lzma2(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, d_packed, nullptr, nullptr, 1, GC_LZMA2_DEPTH);
     if (!rc)  // the items run_lzma2 uploaded describe the same sub-blocks
-        rc = launch_lzma2_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, d_packed,
-                                 (uint8_t*)ctx->out_e.p, d_sizes);
+        rc = launch_lzma2_encode_ordered(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub,
+                                         d_packed, (uint8_t*)ctx->out_e.p, d_sizes, (uint32_t*)ctx->out_b.p);
     if (rc) return rc;
     std::vector<uint32_t> fs(nsub);
     hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);

@@ -154,11 +154,18 @@ int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
     return launch(stream, ctx->k.zstd_encode, grid, 256, args, "launch gc_zstd_encode_kernel");
 }
 
-int launch_lzma2_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                        const uint32_t* packed, uint8_t* out, uint32_t* sizes) {
-    void* args[] = {&arena, &items, &n_items, &packed, &out, &sizes};
-    // one wavefront and its 14.6 KB probability model per workgroup, one workgroup per sub-block:
-    // the dispatcher keeps ten per CU and hands out the next as one ends
+int launch_lzma2_encode_ordered(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items,
+                                uint32_t n_items, const uint32_t* packed, uint8_t* out, uint32_t* sizes,
+                                uint32_t* scratch) {
+    if (n_items == 0) return GC_OK;
+    uint32_t* cost = scratch;
+    uint32_t* order = scratch + n_items;
+    void* cargs[] = {&items, &n_items, &packed, &cost};
+    int rc = launch(stream, ctx->k.lzma2_cost, n_items < 4096 ? n_items : 4096, 256, cargs, "launch gc_lzma2_cost_kernel");
+    if (!rc) rc = launch_order(ctx, stream, cost, n_items, order, nullptr);
+    if (rc) return rc;
+    const uint32_t* ord = order;
+    void* args[] = {&arena, &items, &n_items, &packed, &out, &sizes, &ord};
     return launch(stream, ctx->k.lzma2_encode, n_items, 64, args, "launch gc_lzma2_encode_kernel");
 }
 
@@ -302,6 +309,7 @@ gc_ctx* gc_open(int device) {
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.zstd_encode, ctx->k.module, "gc_zstd_encode_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.gather_bytes, ctx->k.module, "gc_gather_bytes");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_encode, ctx->k.module, "gc_lzma2_encode_kernel");
+    if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_cost, ctx->k.module, "gc_lzma2_cost_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.make_items, ctx->k.module, "gc_make_items");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.cand_q, ctx->k.module, "gc_cand_q_kernel");
     if (e == hipSuccess) e = hipModuleGetFunction(&ctx->k.lzma2_len_q, ctx->k.module, "gc_lzma2_len_q_kernel");

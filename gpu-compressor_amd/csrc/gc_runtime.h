@@ -48,6 +48,7 @@ struct Kernels {
     hipFunction_t zstd_encode = nullptr;
     hipFunction_t gather_bytes = nullptr;
     hipFunction_t lzma2_encode = nullptr;
+    hipFunction_t lzma2_cost = nullptr;
     hipFunction_t make_items = nullptr;
     // persistent forms: items from a device work queue (gc_queue)
     hipFunction_t cand_q = nullptr;
@@ -137,9 +138,12 @@ int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, co
                        uint32_t max_seq);
 // LZMA2 chunks coded from packed matches: one 64-thread workgroup per item,
 // chunk of item count_pos s in out[s * GC_LZMA2_GPU_SLOT + GC_LZMA2_GPU_HEAD ...],
-// its length in sizes[s].
-int launch_lzma2_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
-                        const uint32_t* packed, uint8_t* out, uint32_t* sizes);
+// its length in sizes[s]; the items taken costliest first: a cost per item
+// (gc_lzma2_cost_kernel), then their order (gc_order_items), both in `scratch`
+// (8 bytes per item), then the coder.
+int launch_lzma2_encode_ordered(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items,
+                                uint32_t n_items, const uint32_t* packed, uint8_t* out, uint32_t* sizes,
+                                uint32_t* scratch);
 int launch_gather_bytes(gc_ctx* ctx, hipStream_t stream, const uint8_t* in, const uint32_t* sizes,
                         const uint64_t* offsets, uint8_t* out, uint32_t slot, uint32_t nsub);
 int launch_copy_u32(gc_ctx* ctx, hipStream_t stream, const uint32_t* src, uint32_t* dst, uint64_t n);

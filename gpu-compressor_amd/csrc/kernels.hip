@@ -3320,11 +3320,34 @@ __device__ void le_item(const gc_item& it, const uint8_t* arena, const uint32_t*
     }
 }
 
+// order (optional): the items in the order to code them, costliest first
+// (gc_order_items over gc_lzma2_cost_kernel's counts).  A sub-block's walk
+// is long and serial, and one full of literals takes about three times the
+// average: started last, such walks would set the end of the launch.
 extern "C" __global__ void __launch_bounds__(WAVE)
 gc_lzma2_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items,
-                       const uint32_t* packed, uint8_t* out, uint32_t* sizes) {
+                       const uint32_t* packed, uint8_t* out, uint32_t* sizes, const uint32_t* order) {
     __shared__ uint16_t model[LE_NPROBS];
     for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x)
-        le_item(load_item(items, i), arena, packed, model, out, sizes);
+        le_item(load_item(items, order ? uniform(order[i]) : i), arena, packed, model, out, sizes);
+}
+
+// The coder's cost of each item, estimated as its positions without a packed
+// match of 2 bytes or more (where the walk codes literals one at a time).
+// One 256-thread workgroup per item.
+extern "C" __global__ void __launch_bounds__(256)
+gc_lzma2_cost_kernel(const gc_item* __restrict__ items, uint32_t n_items, const uint32_t* packed, uint32_t* cost) {
+    __shared__ uint32_t part[256 / WAVE];
+    for (uint32_t i = blockIdx.x; i < n_items; i += gridDim.x) {
+        const gc_item it = load_item(items, i);
+        const uint32_t* pk = packed + it.opos;
+        uint32_t c = 0;
+        for (uint32_t p = threadIdx.x; p < it.n; p += 256) c += (pk[p] & 0xFFFFu) < 2 ? 1u : 0u;
+        c = wave_sum(c);
+        if (lane_id() == 0) part[threadIdx.x / WAVE] = c;
+        __syncthreads();
+        if (threadIdx.x == 0) cost[i] = part[0] + part[1] + part[2] + part[3];
+        __syncthreads();
+    }
 }
 #endif  // GC_BUILD_MAIN
