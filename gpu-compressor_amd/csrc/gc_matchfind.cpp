@@ -290,10 +290,11 @@ int gc_zstd_encode_block_dev(gc_ctx* ctx, const uint8_t* d_input, const void* d_
     std::vector<gc_item> items;
     make_items(items, total_input_size, chunk_size, num_chunks, max_sequences_per_chunk, 0, true);
     int rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
+    if (!rc) rc = ctx->out_b.ensure((size_t)num_chunks * 4);  // the frames' order
     if (rc) return rc;
     return launch_zstd_encode(ctx, ctx->stream, d_input, (const gc_item*)ctx->items.p, num_chunks, d_sequences,
                               seq_format, d_literals, d_seq_counts, d_lit_counts, d_output, d_output_sizes,
-                              max_frame_size, max_sequences_per_chunk);
+                              max_frame_size, max_sequences_per_chunk, (uint32_t*)ctx->out_b.p);
 }
 
 int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out, size_t cap,
@@ -312,17 +313,19 @@ int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t lev
     if (!rc) rc = ctx->out_c.ensure((size_t)nsub * slot);  // literals, then the packed frames
     if (!rc) rc = ctx->out_d.ensure(nsub * 4);
     if (!rc) rc = ctx->out_e.ensure((size_t)nsub * slot);
-    if (!rc) rc = ctx->out_f.ensure(nsub * 12 + 16);
+    if (!rc) rc = ctx->out_f.ensure(nsub * 16 + 32);  // sizes, offsets, the frames' order
     if (rc) return rc;
     const uint8_t* d_in = (const uint8_t*)ctx->input.p;
     uint32_t* d_sizes = (uint32_t*)ctx->out_f.p;
     uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
+    uint32_t* d_order = (uint32_t*)(d_offs + nsub);
     rc = run_zstd(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p, GC_SEQ_FORMAT_ZSTD, (uint8_t*)ctx->out_c.p,
                   (uint32_t*)ctx->out_b.p, (uint32_t*)ctx->out_d.p, zstd_depth_for_level(level), max_seq);
     if (!rc)  // the items run_zstd uploaded describe the same sub-blocks
         rc = launch_zstd_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, ctx->out_a.p,
                                 GC_SEQ_FORMAT_ZSTD, (const uint8_t*)ctx->out_c.p, (const uint32_t*)ctx->out_b.p,
-                                (const uint32_t*)ctx->out_d.p, (uint8_t*)ctx->out_e.p, d_sizes, slot, max_seq);
+                                (const uint32_t*)ctx->out_d.p, (uint8_t*)ctx->out_e.p, d_sizes, slot, max_seq,
+                                d_order);
     if (rc) return rc;
     std::vector<uint32_t> fs(nsub);
     hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);

@@ -143,9 +143,14 @@ int launch_zstd_parse_seg(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena,
 int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                        const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
                        const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
-                       uint32_t max_seq) {
-    void* args[] = {&arena, &items,      &n_items, &seqs,  &seq_fmt,   &lits,
-                    &seq_counts, &lit_counts, &out, &sizes, &max_frame, &max_seq};
+                       uint32_t max_seq, uint32_t* order) {
+    // the items with the most sequences first (item i's count is seq_counts[i])
+    if (order && n_items) {
+        if (int rc = launch_order(ctx, stream, seq_counts, n_items, order, nullptr)) return rc;
+    }
+    const uint32_t* ord = order;
+    void* args[] = {&arena, &items,      &n_items, &seqs,  &seq_fmt,   &lits,     &seq_counts,
+                    &lit_counts, &out, &sizes, &max_frame, &max_seq, &ord};
     // four wavefronts (items) per workgroup; each workgroup first tabulates the FSE tables in LDS
     uint32_t grid = (n_items + 3) / 4;
     const uint32_t cap = (uint32_t)ctx->cu_count * 8;

@@ -135,7 +135,7 @@ int launch_zstd_parse_q(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, c
 int launch_zstd_encode(gc_ctx* ctx, hipStream_t stream, const uint8_t* arena, const gc_item* items, uint32_t n_items,
                        const void* seqs, uint32_t seq_fmt, const uint8_t* lits, const uint32_t* seq_counts,
                        const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
-                       uint32_t max_seq);
+                       uint32_t max_seq, uint32_t* order);
 // LZMA2 chunks coded from packed matches: one 64-thread workgroup per item,
 // chunk of item count_pos s in out[s * GC_LZMA2_GPU_SLOT + GC_LZMA2_GPU_HEAD ...],
 // its length in sizes[s]; the items taken costliest first: a cost per item

@@ -2739,14 +2739,16 @@ extern "C" __global__ void __launch_bounds__(ZE_THREADS) __attribute__((amdgpu_n
 gc_zstd_encode_kernel(const uint8_t* arena, const gc_item* __restrict__ items, uint32_t n_items, const void* seqs,
                       uint32_t seq_fmt, const uint8_t* literals, const uint32_t* seq_counts,
                       const uint32_t* lit_counts, uint8_t* out, uint32_t* sizes, uint32_t max_frame,
-                      uint32_t max_seq) {
+                      uint32_t max_seq, const uint32_t* order) {
     __shared__ ZeShared sh;
     __shared__ ZeWave ws[ZE_WAVES];
     // the predefined tables, built with the first wavefront's workspace as scratch
     ze_build(sh, ws[0].u.f.nc, ws[0].u.f.cell);
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    // (order, optional: the items with the most sequences first, so that the
+    // longest frames do not start last)
     for (uint32_t i = blockIdx.x * ZE_WAVES + wave; i < n_items; i += gridDim.x * ZE_WAVES) {
-        const gc_item it = load_item(items, i);
+        const gc_item it = load_item(items, order ? uniform(order[i]) : i);
         if (seq_fmt == GC_SEQ_REF)
             ze_frame<GC_SEQ_REF>(sh, ws[wave], it, arena, seqs, literals, seq_counts, lit_counts, out, sizes,
                                  max_frame, max_seq);
