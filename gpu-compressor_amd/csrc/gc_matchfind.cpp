@@ -68,21 +68,30 @@ int run_zstd(gc_ctx* ctx, const uint8_t* d_in, uint64_t total, uint32_t chunk, u
              uint32_t fmt, uint8_t* d_lit, uint32_t* d_sc, uint32_t* d_lc, uint32_t depth, uint32_t max_seq) {
     std::vector<gc_item> items;
     make_items(items, total, chunk, nsub, max_seq, depth, true);
+    // the info words, then per sub-block a cost word and the parse order
+    const size_t info_bytes = (size_t)nsub * pitch(chunk) * 4;
     int rc = ctx->cand.ensure((size_t)nsub * pitch(chunk) * 2 + 256);
-    if (!rc) rc = ctx->info.ensure((size_t)nsub * pitch(chunk) * 4 + 256);
+    if (!rc) rc = ctx->info.ensure(info_bytes + (size_t)nsub * 8 + 256);
     if (!rc) rc = upload(ctx, ctx->items, items.data(), items.size() * sizeof(gc_item));
     if (rc) return rc;
     const gc_item* d_items = (const gc_item*)ctx->items.p;
+    const bool seg = nsub <= ctx->seg_max_sub;  // too few sub-blocks to fill the GPU with one walk each
+    uint32_t* cost = seg ? nullptr : (uint32_t*)((uint8_t*)ctx->info.p + info_bytes);
+    uint32_t* order = seg ? nullptr : cost + nsub;
     rc = launch_cand(ctx, ctx->stream, d_in, d_items, nsub, nsub, (uint16_t*)ctx->cand.p);
     if (!rc)
         rc = launch_zstd_info(ctx, ctx->stream, d_in, d_items, nsub, nsub, (const uint16_t*)ctx->cand.p,
-                              (uint32_t*)ctx->info.p);
+                              (uint32_t*)ctx->info.p, 8, cost);
     if (rc) return rc;
-    if (nsub <= ctx->seg_max_sub)  // too few sub-blocks to fill the GPU with one walk each
+    if (seg)
         return launch_zstd_parse_seg(ctx, ctx->stream, d_in, d_items, nsub, (const uint32_t*)ctx->info.p, d_seq, fmt,
                                      d_lit, d_sc, d_lc, max_seq, depth);
+    // the sub-blocks with the most walk steps start first, so that the
+    // longest walks do not set the launch's end
+    rc = launch_order(ctx, ctx->stream, cost, nsub, order);
+    if (rc) return rc;
     return launch_zstd_parse(ctx, ctx->stream, d_in, d_items, nsub, (uint32_t)div_up(nsub, 4),
-                             (const uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq);
+                             (const uint32_t*)ctx->info.p, d_seq, fmt, d_lit, d_sc, d_lc, max_seq, order);
 }
 
 // Device copy of a host buffer, padded so that aligned reads near the end
