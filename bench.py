@@ -13,10 +13,12 @@ oracle.  The batched configurations (1 GiB Zstd L1+L2 in 16 MiB chunks; 640 MiB 
 4096 inputs, persistent device-queue form replayed from a hipGraph) are extra keys, each with its own
 roofline.
 
-    python bench.py [--steps K] [--warmup W] [--workload serve|zstd|lzma2|mixed|lzma2_coder] [--no-extras] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload serve|zstd|lzma2|mixed|lzma2_coder|containers]
+                    [--no-extras] [--no-cpu]
 
 `--workload lzma2_coder` times the GPU LZMA2 coder (the 640 MiB of the LZMA2 configuration, streams byte-equal
-to the host coder's) and NVLZ end to end with the GPU and the host coder.
+to the host coder's) and NVLZ end to end with the GPU and the host coder.  `--workload containers` times the
+NVZS and NVLZ containers written on the GPU from host memory, in a process that runs nothing else.
 
 Prints one JSON line; --gpus N > 1 exits with a message: this path runs on one GPU.
 """
@@ -562,6 +564,7 @@ def lzma2_end_to_end(a):
             ("gpu_matches_range_coder", lambda: gcompress.nvlz_compress_matches(data, 128, threads=pool_n)),
             ("gpu_matches_gpu_range_coder", lambda: gcompress.nvlz_compress_gpu(data, 128)),
             ("level0_liblzma", lambda: gcompress.nvlz_compress_l0(data, 128, 0, threads=pool_n)))
+    gpu_blob = None
     for key, fn in legs:
         print(f"[bench] LZMA2 end to end: {key}", file=sys.stderr, flush=True)
         t = time.perf_counter()
@@ -571,7 +574,30 @@ def lzma2_end_to_end(a):
         assert same, f"{key}: NVLZ does not round-trip"
         out[key] = {"s": round(dt, 2), "gbs": round(data.size / dt / 1e9, 4), "ratio": round(data.size / len(blob), 4),
                     "size_pct": round(100.0 * len(blob) / data.size, 2), "round_trip": same}
+        if key == "gpu_matches_gpu_range_coder":
+            gpu_blob = blob
+    out["gpu_matches_gpu_range_coder_reused_buffer"] = reused_buffer_leg(
+        lambda buf: gcompress.nvlz_compress_gpu(data, 128, out=buf),
+        gcompress.lib().gc_container_bound(data.size, gcompress.ALGO_LZMA2), data.size, gpu_blob)
     return out
+
+
+def reused_buffer_leg(call, cap, n, want):
+    """A container call through the C ABI into an output buffer the caller keeps across calls, as a host
+    program holding its own buffer makes it (the first call untimed): no fresh pages and no copy into a
+    Python bytes object, which the bytes-returning legs include.  Three timed calls: the median and the
+    fastest.  Checked equal to `want`."""
+    buf = np.ones(cap, np.uint8)
+    call(buf)
+    ts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        m = call(buf)
+        ts.append(time.perf_counter() - t)
+    same = buf[:m].tobytes() == want
+    assert same, "the container written into the caller's buffer differs"
+    dt = float(np.median(ts))
+    return {"s": round(dt, 3), "gbs": round(n / dt / 1e9, 3), "s_min": round(min(ts), 3), "identical": same}
 
 
 def gpu_frames(hip, ctx, data, inputs, sample_bytes=256 * MiB):
@@ -618,10 +644,20 @@ def gpu_frames(hip, ctx, data, inputs, sample_bytes=256 * MiB):
     rows = np.concatenate([seqs[b, :counts[b]] for b in range(k)] + [np.zeros((1, 4), np.uint32)])
     _, zs = gcompress.zstd_encode_sequences(x, rows, counts, threads=min(threads_all(), 16))
     kms = float(np.median(ms))
+    # the NVZS container of the same bytes from host memory (match finding and frames on the GPU, the
+    # frames copied back): returned as bytes, and written into a buffer the caller keeps
+    xs = np.ascontiguousarray(data[o:o + n])
+    t = time.perf_counter()
+    blob = gcompress.nvzs_compress_gpu(xs, 1)
+    dt = time.perf_counter() - t
+    assert gcompress.decompress(blob) == xs.tobytes(), "NVZS does not round-trip"
+    nvzs = {"bytes_returned": {"s": round(dt, 3), "gbs": round(n / dt / 1e9, 3), "ratio": round(n / len(blob), 4)},
+            "reused_buffer": reused_buffer_leg(lambda buf: gcompress.nvzs_compress_gpu(xs, 1, out=buf),
+                                               gcompress.lib().gc_nvzs_gpu_bound(n), n, blob)}
     return {"sample_bytes": int(n), "kernel": "gc_zstd_encode_kernel", "kernel_ms": round(kms, 3),
             "encode_input_gbs": round(n / (kms / 1e3) / 1e9, 2), "ratio": round(n / int(sizes.sum()), 4),
             "ratio_first_16mib": round(x.size / int(sizes[:k].sum()), 4),
-            "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4)}
+            "libzstd_ratio_first_16mib": round(x.size / sum(zs), 4), "nvzs_end_to_end": nvzs}
 
 
 def gpu_lzma2_coder(hip, ctx, data):
@@ -680,6 +716,9 @@ def bench_lzma2_coder(a, hip, ctx):
             assert gcompress.decompress(b) == data.tobytes(), "NVLZ does not round-trip"
         e2e[key] = {"s": round(dt, 3), "gbs": round(n / dt / 1e9, 3), "ratio": round(n / len(b), 4),
                     "identical_to_gpu_coder": b == blob}
+    e2e["gpu_coder_reused_buffer"] = reused_buffer_leg(lambda buf: gcompress.nvlz_compress_gpu(data, 128, out=buf),
+                                                       gcompress.lib().gc_container_bound(n, gcompress.ALGO_LZMA2),
+                                                       n, blob)
     return {"metric": "LZMA2 coding input GB/s (GPU coder; streams byte-equal to the host coder's)", "value":
             r["encode_input_gbs"], "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": ms,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
@@ -690,6 +729,39 @@ def bench_lzma2_coder(a, hip, ctx):
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(alg / ms / 1e6 / HBM_PEAK_GBS, 5),
                          "traffic": None, "algorithmic_bytes": alg},
             "cpu_baseline": None, "detail": r, "end_to_end_nvlz": e2e, "host": host_cores()}
+
+
+def bench_containers(a, hip, ctx):
+    """The GPU containers end to end from host memory, in a process that has run nothing else: NVZS of the
+    first 256 MiB of the configuration-2 mix (Zstd L1, match finding and frames on the GPU) and NVLZ of the
+    640 MiB of configuration 3 (match finding and range coding on the GPU).  Each leg returned as bytes (a
+    fresh output buffer and a copy into a Python bytes object per call) and written into a buffer the caller
+    keeps; decoded back and compared with the input."""
+    z2, _ = W.config2()
+    xs = np.ascontiguousarray(z2[:256 * MiB])
+    z3, _ = W.config3()
+    out = {}
+    for name, x, call, cap in (
+            ("nvzs_256MiB", xs, lambda x, **k: gcompress.nvzs_compress_gpu(x, 1, **k),
+             gcompress.lib().gc_nvzs_gpu_bound(xs.size)),
+            ("nvlz_640MiB", z3, lambda x, **k: gcompress.nvlz_compress_gpu(x, 128, **k),
+             gcompress.lib().gc_container_bound(z3.size, gcompress.ALGO_LZMA2))):
+        call(x)  # first call: the library's device buffers are allocated
+        t = time.perf_counter()
+        blob = call(x)
+        dt = time.perf_counter() - t
+        assert gcompress.decompress(blob) == x.tobytes(), f"{name}: does not round-trip"
+        out[name] = {"input_bytes": int(x.size), "ratio": round(x.size / len(blob), 4),
+                     "bytes_returned": {"s": round(dt, 3), "gbs": round(x.size / dt / 1e9, 3)},
+                     "reused_buffer": reused_buffer_leg(lambda buf: call(x, out=buf), cap, x.size, blob)}
+    top = out["nvlz_640MiB"]["reused_buffer"]
+    return {"metric": "container input GB/s from host memory (GPU match finding and coding)",
+            "value": top["gbs"], "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 1, "ms_per_step": top["s"] * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+            "data": "synthetic (seeded mixes)",
+            "config": {"workload": "NVLZ of the 640 MiB disk-image-like mix (value) and NVZS of 256 MiB of the "
+                                   "four-kind mix, inputs and outputs in host memory"},
+            "detail": out, "host": host_cores()}
 
 
 def extras_all(a, hip, ctx):
@@ -721,7 +793,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed", "lzma2_coder"], default="serve")
+    ap.add_argument("--workload", choices=["serve", "zstd", "lzma2", "mixed", "lzma2_coder", "containers"], default="serve")
     ap.add_argument("--no-extras", action="store_true", help="serve: only the serving workload")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lzma2-e2e", action="store_true", help="serve: skip the LZMA2 end-to-end leg")
@@ -754,6 +826,8 @@ def main():
         out = bench_serve(a, hip, ctx, extras)
     elif a.workload == "lzma2_coder":
         out = bench_lzma2_coder(a, hip, ctx)
+    elif a.workload == "containers":
+        out = bench_containers(a, hip, ctx)
     else:
         if a.workload == "zstd":
             data, inputs = W.config2()

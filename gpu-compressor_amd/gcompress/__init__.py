@@ -475,15 +475,23 @@ def zstd_frames_gpu(data, level: int = 1, device_id: int = 0):
     return frames, [int(z) for z in sizes[:nsub]]
 
 
-def nvzs_compress_gpu(data, level: int = 1, device_id: int = 0) -> bytes:
-    """NVZS container whose frames are coded on the GPU."""
+def _out_buffer(out, cap, name):
+    if out.dtype != np.uint8 or not out.flags.c_contiguous or out.size < cap:
+        raise ValueError(f"{name}: out must be a contiguous uint8 array of at least {cap} bytes")
+    return out
+
+
+def nvzs_compress_gpu(data, level: int = 1, device_id: int = 0, out=None):
+    """NVZS container whose frames are coded on the GPU -> bytes; with `out` (a uint8 array of at least
+    gc_nvzs_gpu_bound bytes, kept by the caller across calls) the container is written there and its
+    length returned, with no copy of it in Python."""
     src = _as_u8(data)
     cap = lib().gc_nvzs_gpu_bound(src.size)
-    out = np.empty(cap, np.uint8)
+    buf = np.empty(cap, np.uint8) if out is None else _out_buffer(out, cap, "nvzs_compress_gpu")
     n = C.c_size_t()
-    _check(lib().gc_nvzs_compress_gpu(context(device_id).h, _ptr(src), src.size, level, _ptr(out), cap, C.byref(n)),
-           "gc_nvzs_compress_gpu")
-    return out[: n.value].tobytes()
+    _check(lib().gc_nvzs_compress_gpu(context(device_id).h, _ptr(src), src.size, level, _ptr(buf), buf.size,
+                                      C.byref(n)), "gc_nvzs_compress_gpu")
+    return n.value if out is not None else buf[: n.value].tobytes()
 
 
 def lzma2_streams_gpu(data, device_id: int = 0):
@@ -503,15 +511,17 @@ def lzma2_streams_gpu(data, device_id: int = 0):
     return streams, [int(z) for z in sizes[:nsub]]
 
 
-def nvlz_compress_gpu(data, dict_mb: int = 128, device_id: int = 0) -> bytes:
-    """NVLZ container whose sub-blocks are coded on the GPU from the GPU's matches."""
+def nvlz_compress_gpu(data, dict_mb: int = 128, device_id: int = 0, out=None):
+    """NVLZ container whose sub-blocks are coded on the GPU from the GPU's matches -> bytes; with `out` (a
+    uint8 array of at least gc_container_bound bytes, kept by the caller across calls) the container is
+    written there and its length returned, with no copy of it in Python."""
     src = _as_u8(data)
     cap = lib().gc_container_bound(src.size, ALGO_LZMA2)
-    out = np.empty(cap, np.uint8)
+    buf = np.empty(cap, np.uint8) if out is None else _out_buffer(out, cap, "nvlz_compress_gpu")
     n = C.c_size_t()
-    _check(lib().gc_nvlz_compress_gpu(context(device_id).h, _ptr(src), src.size, dict_mb, _ptr(out), cap,
+    _check(lib().gc_nvlz_compress_gpu(context(device_id).h, _ptr(src), src.size, dict_mb, _ptr(buf), buf.size,
                                       C.byref(n)), "gc_nvlz_compress_gpu")
-    return out[: n.value].tobytes()
+    return n.value if out is not None else buf[: n.value].tobytes()
 
 
 def nvzs_compress(data, level: int = 1, device_id: int = 0, threads: int = 1) -> bytes:

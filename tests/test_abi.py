@@ -36,3 +36,14 @@ def test_constants_match_header():
     txt = open(os.path.join(ROOT, "include", "gcompress.h")).read()
     m = re.search(r"#define GC_SEG_DEFAULT_MAX_SUB (\d+)u", txt)
     assert m and int(m.group(1)) == gcompress.SEG_DEFAULT_MAX_SUB
+
+
+def test_container_output_buffer_checked():
+    """An output buffer below the container bound is refused before any device work."""
+    import numpy as np
+    import pytest
+    x = np.zeros(1000, np.uint8)
+    with pytest.raises(ValueError):
+        gcompress.nvzs_compress_gpu(x, 1, out=np.zeros(10, np.uint8))
+    with pytest.raises(ValueError):
+        gcompress.nvlz_compress_gpu(x, 128, out=np.zeros(gcompress.lib().gc_container_bound(1000, 1), np.uint32))

@@ -98,3 +98,19 @@ def test_containers_through_the_streaming_reader(tmp_path, algo):
     src.write_bytes(blob)
     assert gcompress.decompress_file(src, dst, 4) == data.size
     assert dst.read_bytes() == data.tobytes()
+
+
+@pytest.mark.parametrize("n", [1, 3 * 65536 + 17])
+def test_gpu_containers_into_caller_buffer(n):
+    """The GPU containers written into an output buffer the caller keeps equal the returned bytes, on two
+    calls in a row into the same buffer (the second over a shorter input)."""
+    x = datagen.mix(n, seed=n)
+    buf_z = np.full(gcompress.lib().gc_nvzs_gpu_bound(n), 0xAB, np.uint8)
+    buf_l = np.full(gcompress.lib().gc_container_bound(n, gcompress.ALGO_LZMA2), 0xAB, np.uint8)
+    for y in (x, x[: max(1, n // 2)]):
+        m = gcompress.nvzs_compress_gpu(y, 1, out=buf_z)
+        assert buf_z[:m].tobytes() == gcompress.nvzs_compress_gpu(y, 1)
+        m = gcompress.nvlz_compress_gpu(y, 128, out=buf_l)
+        blob = gcompress.nvlz_compress_gpu(y, 128)
+        assert buf_l[:m].tobytes() == blob
+        assert gcompress.decompress(blob) == y.tobytes()
