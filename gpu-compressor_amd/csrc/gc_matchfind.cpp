@@ -103,6 +103,66 @@ int stage_input(gc_ctx* ctx, const uint8_t* chunk, size_t n) {
     return e == hipSuccess ? GC_OK : hip_fail(e, "hipMemcpyAsync(input)");
 }
 
+// Frames on the GPU (one per 64 KiB sub-block, compress_zstd_custom.rs:47-203 as the reference's
+// unused zstd_encode_block would code them), in three steps on ctx->stream.  The buffers shared by
+// the steps of one call: out_a sequences, out_b / out_d sequence and literal counts, out_e one frame
+// slot per sub-block, out_f the frame sizes, their offsets and the frames' order.
+const uint32_t kFrameMaxSeq = GC_SUB_BLOCK / 3;  // compress_zstd_custom.rs:55
+const uint32_t kFrameSlot = GC_SUB_BLOCK + 16;
+
+int frame_buffers(gc_ctx* ctx, uint64_t nsub) {
+    int rc = ctx->out_a.ensure((size_t)nsub * kFrameMaxSeq * sizeof(gc_zstd_sequence));
+    if (!rc) rc = ctx->out_b.ensure(nsub * 4);
+    if (!rc) rc = ctx->out_d.ensure(nsub * 4);
+    if (!rc) rc = ctx->out_e.ensure((size_t)nsub * kFrameSlot);
+    if (!rc) rc = ctx->out_f.ensure(nsub * 16 + 32);
+    return rc;
+}
+
+uint32_t* frame_sizes(gc_ctx* ctx) { return (uint32_t*)ctx->out_f.p; }
+uint64_t* frame_offsets(gc_ctx* ctx, uint64_t nsub) {
+    return (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
+}
+
+// Step 1: match finding with literals (into d_lp, nsub x kFrameSlot bytes) and frame coding of the n
+// bytes at d_in.
+int frames_launch(gc_ctx* ctx, const uint8_t* d_in, size_t n, uint32_t level, uint8_t* d_lp) {
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    int rc = run_zstd(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p, GC_SEQ_FORMAT_ZSTD, d_lp,
+                      (uint32_t*)ctx->out_b.p, (uint32_t*)ctx->out_d.p, zstd_depth_for_level(level), kFrameMaxSeq);
+    if (rc) return rc;
+    // the items run_zstd uploaded describe the same sub-blocks
+    return launch_zstd_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, ctx->out_a.p,
+                              GC_SEQ_FORMAT_ZSTD, d_lp, (const uint32_t*)ctx->out_b.p, (const uint32_t*)ctx->out_d.p,
+                              (uint8_t*)ctx->out_e.p, frame_sizes(ctx), kFrameSlot, kFrameMaxSeq,
+                              (uint32_t*)(frame_offsets(ctx, nsub) + nsub));
+}
+
+// Step 2: waits for step 1, reads the frame sizes (into sizes[]) and packs the frames back to back into
+// d_lp (the literals are no longer needed); *total = their bytes, at most cap.
+int frames_pack(gc_ctx* ctx, uint64_t nsub, uint8_t* d_lp, uint64_t* sizes, size_t cap, uint64_t* total) {
+    std::vector<uint32_t> fs(nsub);
+    hipError_t e = hipMemcpyAsync(fs.data(), frame_sizes(ctx), nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "frame sizes");
+    std::vector<uint64_t> offs(nsub);
+    uint64_t t = 0;
+    for (uint64_t s = 0; s < nsub; s++) {
+        offs[s] = t;
+        sizes[s] = fs[s];
+        t += fs[s];
+    }
+    if (t > cap) return fail(GC_ESIZE, "output capacity below the frames' total size");
+    uint64_t* d_offs = frame_offsets(ctx, nsub);
+    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "frame offsets");
+    int rc = launch_gather_bytes(ctx, ctx->stream, (const uint8_t*)ctx->out_e.p, frame_sizes(ctx), d_offs, d_lp,
+                                 kFrameSlot, (uint32_t)nsub);
+    // (offs is pageable: the copy has read it before hipMemcpyAsync returned)
+    *total = t;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -311,51 +371,17 @@ int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t lev
     if (!ctx || (!chunk && n) || !out || !sizes || !out_len) return fail(GC_EINVAL, "null argument");
     *out_len = 0;
     if (n == 0) return GC_OK;
-    const uint32_t max_seq = GC_SUB_BLOCK / 3;  // compress_zstd_custom.rs:55
-    const uint32_t slot = GC_SUB_BLOCK + 16;
     const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     int rc = stage_input(ctx, chunk, n);
-    if (!rc) rc = ctx->out_a.ensure((size_t)nsub * max_seq * sizeof(gc_zstd_sequence));
-    if (!rc) rc = ctx->out_b.ensure(nsub * 4);
-    if (!rc) rc = ctx->out_c.ensure((size_t)nsub * slot);  // literals, then the packed frames
-    if (!rc) rc = ctx->out_d.ensure(nsub * 4);
-    if (!rc) rc = ctx->out_e.ensure((size_t)nsub * slot);
-    if (!rc) rc = ctx->out_f.ensure(nsub * 16 + 32);  // sizes, offsets, the frames' order
-    if (rc) return rc;
-    const uint8_t* d_in = (const uint8_t*)ctx->input.p;
-    uint32_t* d_sizes = (uint32_t*)ctx->out_f.p;
-    uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
-    uint32_t* d_order = (uint32_t*)(d_offs + nsub);
-    rc = run_zstd(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, ctx->out_a.p, GC_SEQ_FORMAT_ZSTD, (uint8_t*)ctx->out_c.p,
-                  (uint32_t*)ctx->out_b.p, (uint32_t*)ctx->out_d.p, zstd_depth_for_level(level), max_seq);
-    if (!rc)  // the items run_zstd uploaded describe the same sub-blocks
-        rc = launch_zstd_encode(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub, ctx->out_a.p,
-                                GC_SEQ_FORMAT_ZSTD, (const uint8_t*)ctx->out_c.p, (const uint32_t*)ctx->out_b.p,
-                                (const uint32_t*)ctx->out_d.p, (uint8_t*)ctx->out_e.p, d_sizes, slot, max_seq,
-                                d_order);
-    if (rc) return rc;
-    std::vector<uint32_t> fs(nsub);
-    hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: frame sizes");
-    std::vector<uint64_t> offs(nsub);
+    if (!rc) rc = frame_buffers(ctx, nsub);
+    if (!rc) rc = ctx->out_c.ensure((size_t)nsub * kFrameSlot);  // literals, then the packed frames
+    if (!rc) rc = frames_launch(ctx, (const uint8_t*)ctx->input.p, n, level, (uint8_t*)ctx->out_c.p);
     uint64_t t = 0;
-    for (uint64_t s = 0; s < nsub; s++) {
-        offs[s] = t;
-        sizes[s] = fs[s];
-        t += fs[s];
-    }
-    if (t > cap) return fail(GC_ESIZE, "output capacity below the frames' total size");
-    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: offsets");
-    // the packed frames go where the literals were (the input is no larger than the literal slots)
-    uint8_t* d_packed = (uint8_t*)ctx->out_c.p;
-    rc = launch_gather_bytes(ctx, ctx->stream, (const uint8_t*)ctx->out_e.p, d_sizes, d_offs, d_packed, slot,
-                             (uint32_t)nsub);
+    if (!rc) rc = frames_pack(ctx, nsub, (uint8_t*)ctx->out_c.p, sizes, cap, &t);
     if (rc) return rc;
-    e = hipMemcpyAsync(out, d_packed, t, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e = hipMemcpyAsync(out, ctx->out_c.p, t, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: frames");
     *out_len = (size_t)t;
@@ -377,17 +403,74 @@ int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t lev
     put64(out + 20, nsub);
     std::vector<uint64_t> sizes(nsub, 0);
     size_t o = head;
-    // pieces of up to 1 GiB (16384 sub-blocks; 5.7 GB of sequence slots on the device): each piece is
-    // one pass of match finding and frame coding with a handful of synchronisations, so small pieces
-    // would spend their time in copies and waits (16 MiB pieces: 3.1 GB/s end to end)
-    const size_t piece = (size_t)1 << 30;
-    for (size_t c0 = 0; c0 < n; c0 += piece) {
-        const size_t len = n - c0 < piece ? n - c0 : piece;
-        size_t w = 0;
-        int rc = gc_zstd_frames_gpu(ctx, src + c0, len, level, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
-        if (rc) return rc;
-        o += w;
+    if (n == 0) {
+        *out_len = o;
+        return GC_OK;
     }
+    // Pieces of 128 MiB, two in flight: while the GPU codes piece k on ctx->stream, this thread copies
+    // piece k + 1 in and piece k - 1's frames out on stream2 (copies from and to pageable memory hold
+    // the calling thread until they are done; the kernels run meanwhile).  A slot j = k & 1 holds a
+    // piece's input and its literals, then its packed frames, until they are copied out; the other
+    // buffers serve one piece at a time in stream order.
+    const size_t piece = (size_t)128 << 20;
+    const uint64_t psub = div_up(n < piece ? n : piece, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    gc::DevBuf* in[2] = {&ctx->input, &ctx->input_b};
+    gc::DevBuf* lp[2] = {&ctx->out_c, &ctx->out_c_b};
+    int rc = frame_buffers(ctx, psub);
+    for (int j = 0; j < 2 && !rc; j++) {
+        rc = in[j]->ensure(psub * GC_SUB_BLOCK + 64);
+        if (!rc) rc = lp[j]->ensure((size_t)psub * kFrameSlot);
+    }
+    if (rc) return rc;
+    hipEvent_t packed = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&packed, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    // a piece's frames packed in its slot j (the pack's end recorded as `packed` on ctx->stream, before
+    // the next piece's kernels), out to the container
+    auto copy_out = [&](int j, uint64_t t) -> int {
+        hipError_t e2 = hipStreamWaitEvent(ctx->stream2, packed, 0);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(out + o, lp[j]->p, t, hipMemcpyDeviceToHost, ctx->stream2);
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(ctx->stream2);
+        if (e2 != hipSuccess) return hip_fail(e2, "gc_nvzs_compress_gpu: frames");
+        o += t;
+        return GC_OK;
+    };
+    size_t prev0 = 0, prev_len = 0;  // piece k - 1
+    uint64_t t_prev = 0;
+    int k = 0;
+    for (size_t c0 = 0; c0 < n && !rc; c0 += piece, k++) {
+        const int j = k & 1;
+        const size_t len = n - c0 < piece ? n - c0 : piece;
+        e = hipMemcpyAsync(in[j]->p, src + c0, len, hipMemcpyHostToDevice, ctx->stream2);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream2);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "gc_nvzs_compress_gpu: input");
+            break;
+        }
+        if (k > 0) {  // piece k - 1 coded: its frames packed before piece k's kernels take the shared buffers
+            const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
+            rc = frames_pack(ctx, ps, (uint8_t*)lp[j ^ 1]->p, sizes.data() + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
+            if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+            if (rc) break;
+        }
+        rc = frames_launch(ctx, (const uint8_t*)in[j]->p, len, level, (uint8_t*)lp[j]->p);
+        if (!rc && k > 0) rc = copy_out(j ^ 1, t_prev);
+        prev0 = c0;
+        prev_len = len;
+    }
+    if (!rc) {  // the last piece
+        const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
+        rc = frames_pack(ctx, ps, (uint8_t*)lp[(k - 1) & 1]->p, sizes.data() + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
+        if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+        if (!rc) rc = copy_out((k - 1) & 1, t_prev);
+    }
+    // nothing of this call stays in flight when it returns, on success or not
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipEventDestroy(packed);
+    if (rc) return rc;
     for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
     *out_len = o;
     return GC_OK;

@@ -360,6 +360,8 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_d.release();
     ctx->out_e.release();
     ctx->out_f.release();
+    ctx->input_b.release();
+    ctx->out_c_b.release();
     ctx->seg.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
     if (ctx->k.module_parse) (void)hipModuleUnload(ctx->k.module_parse);

@@ -78,6 +78,7 @@ struct gc_ctx {
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
     gc::DevBuf out_a, out_b, out_c, out_d, out_e, out_f;
+    gc::DevBuf input_b, out_c_b;  // the second slot of input and out_c (the NVZS pipeline)
     gc::DevBuf seg;        // record buffers of the segmented parse
     // launches of at most this many sub-blocks parse with the segmented
     // kernel (0: never); larger ones fill the GPU with one walk per wavefront

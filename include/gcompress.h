@@ -286,7 +286,9 @@ int gc_zstd_encode_block_dev(gc_ctx* ctx, const uint8_t* d_input, const void* d_
  * (libzstd's entropy coding on the CPU): faster, larger. */
 int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out, size_t cap,
                        uint64_t* sizes, size_t* out_len);
-/* NVZS container with GPU-coded frames (decoded by gc_decompress). */
+/* NVZS container with GPU-coded frames (decoded by gc_decompress).  The input
+ * is coded in pieces of 128 MiB, one piece's copies to and from host memory
+ * overlapping the next piece's kernels (the context's second stream). */
 size_t gc_nvzs_gpu_bound(size_t n);
 int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
                          size_t* out_len);
