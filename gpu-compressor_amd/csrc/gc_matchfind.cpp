@@ -163,6 +163,83 @@ int frames_pack(gc_ctx* ctx, uint64_t nsub, uint8_t* d_lp, uint64_t* sizes, size
     return rc;
 }
 
+// LZMA2 streams on the GPU from a host buffer, in two steps on one stream; the buffers of one pass (the
+// NVLZ pipeline runs two passes at once, one set each): the input, the items, the candidates, the packed
+// matches, one output slot per sub-block, the stream sizes and offsets, the coder's costs and order, and
+// the streams back to back.
+struct LzSet {
+    hipStream_t stream;
+    gc::DevBuf *in, *items, *cand, *packed, *slots, *meta, *scratch, *streams;
+};
+
+LzSet lz_set(gc_ctx* ctx, int j) {
+    if (j == 0)
+        return {ctx->stream, &ctx->input, &ctx->items, &ctx->cand, &ctx->out_a,
+                &ctx->out_e, &ctx->out_f, &ctx->out_b, &ctx->out_c};
+    return {ctx->stream2, &ctx->input_b, &ctx->items_b, &ctx->cand_b, &ctx->out_a_b,
+            &ctx->out_e_b, &ctx->out_f_b, &ctx->out_b_b, &ctx->out_c_b};
+}
+
+// Step 1: copy the n bytes at src in (a copy from pageable memory holds the calling thread until it is
+// done), find the matches and launch the coder.
+int lz_launch(gc_ctx* ctx, const LzSet& s, const uint8_t* src, size_t n) {
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    int rc = s.in->ensure(n + 64);
+    if (!rc) rc = s.cand->ensure((size_t)nsub * pitch(GC_SUB_BLOCK) * 2 + 256);
+    if (!rc) rc = s.packed->ensure(n * 4);
+    if (!rc) rc = s.slots->ensure((size_t)nsub * GC_LZMA2_GPU_SLOT);
+    if (!rc) rc = s.meta->ensure(nsub * 12 + 16);
+    if (!rc) rc = s.scratch->ensure(nsub * 8);
+    if (!rc) rc = s.streams->ensure(n + 4 * nsub + 16);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(s.in->p, src, n, hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(input)");
+    std::vector<gc_item> items;
+    make_items(items, n, GC_SUB_BLOCK, (uint32_t)nsub, 0, GC_LZMA2_DEPTH, false);
+    rc = upload_on(ctx, s.stream, *s.items, items.data(), items.size() * sizeof(gc_item));
+    if (rc) return rc;
+    const gc_item* d_items = (const gc_item*)s.items->p;
+    const uint8_t* d_in = (const uint8_t*)s.in->p;
+    rc = launch_cand(ctx, s.stream, d_in, d_items, (uint32_t)nsub, (uint32_t)nsub, (uint16_t*)s.cand->p);
+    if (!rc)
+        rc = launch_lzma2_len(ctx, s.stream, d_in, d_items, (uint32_t)nsub, (uint32_t)nsub,
+                              (const uint16_t*)s.cand->p, (uint32_t*)s.packed->p, nullptr, nullptr, 1);
+    if (!rc)
+        rc = launch_lzma2_encode_ordered(ctx, s.stream, d_in, d_items, (uint32_t)nsub, (const uint32_t*)s.packed->p,
+                                         (uint8_t*)s.slots->p, (uint32_t*)s.meta->p, (uint32_t*)s.scratch->p);
+    return rc;
+}
+
+// Step 2: wait for the coder, gather the streams back to back and copy them to out (at most cap bytes);
+// sizes[] (one per sub-block) and *len set.
+int lz_finish(gc_ctx* ctx, const LzSet& s, size_t n, uint8_t* out, size_t cap, uint64_t* sizes, size_t* len) {
+    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
+    uint32_t* d_sizes = (uint32_t*)s.meta->p;
+    uint64_t* d_offs = (uint64_t*)((uint8_t*)s.meta->p + ((nsub * 4 + 15) & ~(uint64_t)15));
+    std::vector<uint32_t> fs(nsub);
+    hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+    if (e != hipSuccess) return hip_fail(e, "LZMA2 stream sizes");
+    std::vector<uint64_t> offs(nsub);
+    uint64_t t = 0;
+    for (uint64_t k = 0; k < nsub; k++) {
+        offs[k] = t;
+        sizes[k] = fs[k];
+        t += fs[k];
+    }
+    if (t > cap) return fail(GC_ESIZE, "output capacity below the streams' total size");
+    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, s.stream);
+    if (e != hipSuccess) return hip_fail(e, "LZMA2 stream offsets");
+    int rc = launch_gather_bytes(ctx, s.stream, (const uint8_t*)s.slots->p + GC_LZMA2_GPU_HEAD, d_sizes, d_offs,
+                                 (uint8_t*)s.streams->p, GC_LZMA2_GPU_SLOT, (uint32_t)nsub);
+    if (rc) return rc;
+    e = hipMemcpyAsync(out, s.streams->p, t, hipMemcpyDeviceToHost, s.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
+    if (e != hipSuccess) return hip_fail(e, "LZMA2 streams");
+    *len = (size_t)t;
+    return GC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -500,45 +577,11 @@ int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* o
     if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    int rc = stage_input(ctx, chunk, n);
-    if (!rc) rc = ctx->out_a.ensure(n * 4);                        // packed matches
-    if (!rc) rc = ctx->out_e.ensure((size_t)nsub * GC_LZMA2_GPU_SLOT);  // one slot per sub-block
-    if (!rc) rc = ctx->out_c.ensure(n + 4 * nsub + 16);            // the streams, back to back
-    if (!rc) rc = ctx->out_f.ensure(nsub * 12 + 16);
-    if (!rc) rc = ctx->out_b.ensure(nsub * 8);  // the items' costs and their order
-    if (rc) return rc;
-    const uint8_t* d_in = (const uint8_t*)ctx->input.p;
-    uint32_t* d_packed = (uint32_t*)ctx->out_a.p;
-    uint32_t* d_sizes = (uint32_t*)ctx->out_f.p;
-    uint64_t* d_offs = (uint64_t*)((uint8_t*)ctx->out_f.p + ((nsub * 4 + 15) & ~(uint64_t)15));
-    rc = run_lzma2(ctx, d_in, n, GC_SUB_BLOCK, (uint32_t)nsub, d_packed, nullptr, nullptr, 1, GC_LZMA2_DEPTH);
-    if (!rc)  // the items run_lzma2 uploaded describe the same sub-blocks
-        rc = launch_lzma2_encode_ordered(ctx, ctx->stream, d_in, (const gc_item*)ctx->items.p, (uint32_t)nsub,
-                                         d_packed, (uint8_t*)ctx->out_e.p, d_sizes, (uint32_t*)ctx->out_b.p);
-    if (rc) return rc;
-    std::vector<uint32_t> fs(nsub);
-    hipError_t e = hipMemcpyAsync(fs.data(), d_sizes, nsub * 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: stream sizes");
-    std::vector<uint64_t> offs(nsub);
-    uint64_t t = 0;
-    for (uint64_t s = 0; s < nsub; s++) {
-        offs[s] = t;
-        sizes[s] = fs[s];
-        t += fs[s];
-    }
-    if (t > cap) return fail(GC_ESIZE, "output capacity below the streams' total size");
-    e = hipMemcpyAsync(d_offs, offs.data(), nsub * 8, hipMemcpyHostToDevice, ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: offsets");
-    uint8_t* d_out = (uint8_t*)ctx->out_c.p;
-    rc = launch_gather_bytes(ctx, ctx->stream, (const uint8_t*)ctx->out_e.p + GC_LZMA2_GPU_HEAD, d_sizes, d_offs,
-                             d_out, GC_LZMA2_GPU_SLOT, (uint32_t)nsub);
-    if (rc) return rc;
-    e = hipMemcpyAsync(out, d_out, t, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_lzma2_streams_gpu: streams");
-    *out_len = (size_t)t;
-    return GC_OK;
+    const LzSet s = lz_set(ctx, 0);
+    int rc = lz_launch(ctx, s, chunk, n);
+    if (!rc) rc = lz_finish(ctx, s, n, out, cap, sizes, out_len);
+    (void)hipStreamSynchronize(ctx->stream);
+    return rc;
 }
 
 int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
@@ -556,20 +599,38 @@ int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dic
     put64(out + 20, nsub);
     // Matches never cross a 64 KiB sub-block, so the dictionary-size chunks of
     // pipeline_lzma2.rs:40 change no byte of the output; the whole input goes
-    // through in pieces of up to 2 GiB instead (one coder launch per piece:
-    // a sub-block's stream is a long serial walk, so a launch needs many
-    // sub-blocks to keep the GPU busy).
+    // through in pieces of up to 2 GiB instead.  A sub-block's stream is a long
+    // serial walk, so a coder launch needs thousands of sub-blocks to keep the
+    // GPU busy: each piece is coded in two halves, one launch each on the
+    // context's two streams.  The second half's copy in overlaps the first
+    // half's coding, and the first half's streams are copied out while the
+    // second half's coder runs.
     (void)dict_mb;
     const size_t piece = (size_t)2 << 30;
     std::vector<uint64_t> sizes(nsub, 0);
     size_t o = head;
-    for (size_t c0 = 0; c0 < n; c0 += piece) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    const LzSet set[2] = {lz_set(ctx, 0), lz_set(ctx, 1)};
+    int rc = GC_OK;
+    for (size_t c0 = 0; c0 < n && !rc; c0 += piece) {
         const size_t len = n - c0 < piece ? n - c0 : piece;
-        size_t w = 0;
-        int rc = gc_lzma2_streams_gpu(ctx, src + c0, len, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
-        if (rc) return rc;
-        o += w;
+        const size_t half = len > GC_SUB_BLOCK ? (size_t)div_up(len / 2, GC_SUB_BLOCK) * GC_SUB_BLOCK : len;
+        const size_t part[2] = {half, len - half};
+        for (int j = 0; j < 2 && !rc; j++)
+            if (part[j]) rc = lz_launch(ctx, set[j], src + c0 + (j ? half : 0), part[j]);
+        for (int j = 0; j < 2 && !rc; j++) {
+            if (!part[j]) continue;
+            size_t w = 0;
+            rc = lz_finish(ctx, set[j], part[j], out + o, cap - o, sizes.data() + (c0 + (j ? half : 0)) / GC_SUB_BLOCK,
+                           &w);
+            o += w;
+        }
     }
+    // nothing of this call stays in flight when it returns, on success or not
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
+    if (rc) return rc;
     for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
     *out_len = o;
     return GC_OK;

@@ -74,14 +74,19 @@ void DevBuf::release() {
     size = 0;
 }
 
-int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes) {
+int upload_on(gc_ctx* ctx, hipStream_t stream, DevBuf& buf, const void* src, size_t bytes) {
+    (void)ctx;
     int rc = buf.ensure(bytes);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(items)");
-    e = hipStreamSynchronize(ctx->stream);  // src is pageable host memory owned by the caller
+    e = hipStreamSynchronize(stream);  // src is pageable host memory owned by the caller
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     return GC_OK;
+}
+
+int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes) {
+    return upload_on(ctx, ctx->stream, buf, src, bytes);
 }
 
 static int launch(hipStream_t stream, hipFunction_t f, uint32_t grid, uint32_t block, void** args, const char* name) {
@@ -362,6 +367,12 @@ void gc_close(gc_ctx* ctx) {
     ctx->out_f.release();
     ctx->input_b.release();
     ctx->out_c_b.release();
+    ctx->items_b.release();
+    ctx->cand_b.release();
+    ctx->out_a_b.release();
+    ctx->out_b_b.release();
+    ctx->out_e_b.release();
+    ctx->out_f_b.release();
     ctx->seg.release();
     if (ctx->k.module) (void)hipModuleUnload(ctx->k.module);
     if (ctx->k.module_parse) (void)hipModuleUnload(ctx->k.module_parse);

@@ -78,7 +78,8 @@ struct gc_ctx {
     gc::DevBuf info;       // zstd per-position candidate words
     gc::DevBuf input;      // staging for host-pointer entry points
     gc::DevBuf out_a, out_b, out_c, out_d, out_e, out_f;
-    gc::DevBuf input_b, out_c_b;  // the second slot of input and out_c (the NVZS pipeline)
+    gc::DevBuf input_b, out_c_b;  // the second slot of input and out_c (the NVZS / NVLZ pipelines)
+    gc::DevBuf items_b, cand_b, out_a_b, out_b_b, out_e_b, out_f_b;  // the second set (NVLZ pipeline)
     gc::DevBuf seg;        // record buffers of the segmented parse
     // launches of at most this many sub-blocks parse with the segmented
     // kernel (0: never); larger ones fill the GPU with one walk per wavefront
@@ -155,6 +156,7 @@ int launch_make_items(gc_ctx* ctx, hipStream_t stream, gc_item* items, uint64_t 
 // looked up once per device when a context opens.
 int noalloc_attach(int device);
 int upload(gc_ctx* ctx, DevBuf& buf, const void* src, size_t bytes);
+int upload_on(gc_ctx* ctx, hipStream_t stream, DevBuf& buf, const void* src, size_t bytes);
 
 uint32_t zstd_depth_for_level(uint32_t level);
 

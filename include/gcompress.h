@@ -263,7 +263,9 @@ int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* o
 /* NVLZ container of GPU-coded streams, byte-equal to gc_nvlz_compress_matches'
  * for the same input (dict_mb is accepted for that interface and changes
  * nothing: no match crosses a sub-block); cap >= gc_container_bound(n,
- * GC_ALGO_LZMA2).  Decoded by gc_decompress. */
+ * GC_ALGO_LZMA2).  Decoded by gc_decompress.  Coded in two halves, one
+ * launch each on the context's two streams, so that copies to and from host
+ * memory overlap the coding. */
 int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
                          size_t* out_len);
 
