@@ -240,6 +240,79 @@ int lz_finish(gc_ctx* ctx, const LzSet& s, size_t n, uint8_t* out, size_t cap, u
     return GC_OK;
 }
 
+// Frames of the n bytes at src (host memory), packed back to back into out (at most cap bytes),
+// sizes[] one per sub-block, *len their total.
+int frames_pipelined(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
+                     uint64_t* sizes, size_t* len) {
+    size_t o = 0;
+    // Pieces of 128 MiB, two in flight: while the GPU codes piece k on ctx->stream, this thread copies
+    // piece k + 1 in and piece k - 1's frames out on stream2 (copies from and to pageable memory hold
+    // the calling thread until they are done; the kernels run meanwhile).  A slot j = k & 1 holds a
+    // piece's input and its literals, then its packed frames, until they are copied out; the other
+    // buffers serve one piece at a time in stream order.
+    const size_t piece = (size_t)128 << 20;
+    const uint64_t psub = div_up(n < piece ? n : piece, GC_SUB_BLOCK);
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    gc::DevBuf* in[2] = {&ctx->input, &ctx->input_b};
+    gc::DevBuf* lp[2] = {&ctx->out_c, &ctx->out_c_b};
+    int rc = frame_buffers(ctx, psub);
+    for (int j = 0; j < 2 && !rc; j++) {
+        rc = in[j]->ensure(psub * GC_SUB_BLOCK + 64);
+        if (!rc) rc = lp[j]->ensure((size_t)psub * kFrameSlot);
+    }
+    if (rc) return rc;
+    hipEvent_t packed = nullptr;
+    hipError_t e = hipEventCreateWithFlags(&packed, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    // a piece's frames packed in its slot j (the pack's end recorded as `packed` on ctx->stream, before
+    // the next piece's kernels), out to the container
+    auto copy_out = [&](int j, uint64_t t) -> int {
+        hipError_t e2 = hipStreamWaitEvent(ctx->stream2, packed, 0);
+        if (e2 == hipSuccess) e2 = hipMemcpyAsync(out + o, lp[j]->p, t, hipMemcpyDeviceToHost, ctx->stream2);
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(ctx->stream2);
+        if (e2 != hipSuccess) return hip_fail(e2, "GPU frames: frames out");
+        o += t;
+        return GC_OK;
+    };
+    size_t prev0 = 0, prev_len = 0;  // piece k - 1
+    uint64_t t_prev = 0;
+    int k = 0;
+    for (size_t c0 = 0; c0 < n && !rc; c0 += piece, k++) {
+        const int j = k & 1;
+        const size_t len = n - c0 < piece ? n - c0 : piece;
+        e = hipMemcpyAsync(in[j]->p, src + c0, len, hipMemcpyHostToDevice, ctx->stream2);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream2);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "GPU frames: input");
+            break;
+        }
+        if (k > 0) {  // piece k - 1 coded: its frames packed before piece k's kernels take the shared buffers
+            const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
+            rc = frames_pack(ctx, ps, (uint8_t*)lp[j ^ 1]->p, sizes + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
+            if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+            if (rc) break;
+        }
+        rc = frames_launch(ctx, (const uint8_t*)in[j]->p, len, level, (uint8_t*)lp[j]->p);
+        if (!rc && k > 0) rc = copy_out(j ^ 1, t_prev);
+        prev0 = c0;
+        prev_len = len;
+    }
+    if (!rc) {  // the last piece
+        const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
+        rc = frames_pack(ctx, ps, (uint8_t*)lp[(k - 1) & 1]->p, sizes + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
+        if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
+        if (!rc) rc = copy_out((k - 1) & 1, t_prev);
+    }
+    // nothing of this call stays in flight when it returns, on success or not
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
+    (void)hipEventDestroy(packed);
+    if (rc) return rc;
+    *len = o;
+    return GC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -448,21 +521,7 @@ int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t lev
     if (!ctx || (!chunk && n) || !out || !sizes || !out_len) return fail(GC_EINVAL, "null argument");
     *out_len = 0;
     if (n == 0) return GC_OK;
-    const uint64_t nsub = div_up(n, GC_SUB_BLOCK);
-    std::lock_guard<std::mutex> g(ctx->mu);
-    (void)hipSetDevice(ctx->device);
-    int rc = stage_input(ctx, chunk, n);
-    if (!rc) rc = frame_buffers(ctx, nsub);
-    if (!rc) rc = ctx->out_c.ensure((size_t)nsub * kFrameSlot);  // literals, then the packed frames
-    if (!rc) rc = frames_launch(ctx, (const uint8_t*)ctx->input.p, n, level, (uint8_t*)ctx->out_c.p);
-    uint64_t t = 0;
-    if (!rc) rc = frames_pack(ctx, nsub, (uint8_t*)ctx->out_c.p, sizes, cap, &t);
-    if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(out, ctx->out_c.p, t, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "gc_zstd_frames_gpu: frames");
-    *out_len = (size_t)t;
-    return GC_OK;
+    return frames_pipelined(ctx, chunk, n, level, out, cap, sizes, out_len);
 }
 
 int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
@@ -479,77 +538,13 @@ int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t lev
     put64(out + 12, GC_SUB_BLOCK);
     put64(out + 20, nsub);
     std::vector<uint64_t> sizes(nsub, 0);
-    size_t o = head;
-    if (n == 0) {
-        *out_len = o;
-        return GC_OK;
+    size_t w = 0;
+    if (n) {
+        int rc = frames_pipelined(ctx, src, n, level, out + head, cap - head, sizes.data(), &w);
+        if (rc) return rc;
     }
-    // Pieces of 128 MiB, two in flight: while the GPU codes piece k on ctx->stream, this thread copies
-    // piece k + 1 in and piece k - 1's frames out on stream2 (copies from and to pageable memory hold
-    // the calling thread until they are done; the kernels run meanwhile).  A slot j = k & 1 holds a
-    // piece's input and its literals, then its packed frames, until they are copied out; the other
-    // buffers serve one piece at a time in stream order.
-    const size_t piece = (size_t)128 << 20;
-    const uint64_t psub = div_up(n < piece ? n : piece, GC_SUB_BLOCK);
-    std::lock_guard<std::mutex> g(ctx->mu);
-    (void)hipSetDevice(ctx->device);
-    gc::DevBuf* in[2] = {&ctx->input, &ctx->input_b};
-    gc::DevBuf* lp[2] = {&ctx->out_c, &ctx->out_c_b};
-    int rc = frame_buffers(ctx, psub);
-    for (int j = 0; j < 2 && !rc; j++) {
-        rc = in[j]->ensure(psub * GC_SUB_BLOCK + 64);
-        if (!rc) rc = lp[j]->ensure((size_t)psub * kFrameSlot);
-    }
-    if (rc) return rc;
-    hipEvent_t packed = nullptr;
-    hipError_t e = hipEventCreateWithFlags(&packed, hipEventDisableTiming);
-    if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
-    // a piece's frames packed in its slot j (the pack's end recorded as `packed` on ctx->stream, before
-    // the next piece's kernels), out to the container
-    auto copy_out = [&](int j, uint64_t t) -> int {
-        hipError_t e2 = hipStreamWaitEvent(ctx->stream2, packed, 0);
-        if (e2 == hipSuccess) e2 = hipMemcpyAsync(out + o, lp[j]->p, t, hipMemcpyDeviceToHost, ctx->stream2);
-        if (e2 == hipSuccess) e2 = hipStreamSynchronize(ctx->stream2);
-        if (e2 != hipSuccess) return hip_fail(e2, "gc_nvzs_compress_gpu: frames");
-        o += t;
-        return GC_OK;
-    };
-    size_t prev0 = 0, prev_len = 0;  // piece k - 1
-    uint64_t t_prev = 0;
-    int k = 0;
-    for (size_t c0 = 0; c0 < n && !rc; c0 += piece, k++) {
-        const int j = k & 1;
-        const size_t len = n - c0 < piece ? n - c0 : piece;
-        e = hipMemcpyAsync(in[j]->p, src + c0, len, hipMemcpyHostToDevice, ctx->stream2);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream2);
-        if (e != hipSuccess) {
-            rc = hip_fail(e, "gc_nvzs_compress_gpu: input");
-            break;
-        }
-        if (k > 0) {  // piece k - 1 coded: its frames packed before piece k's kernels take the shared buffers
-            const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
-            rc = frames_pack(ctx, ps, (uint8_t*)lp[j ^ 1]->p, sizes.data() + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
-            if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
-            if (rc) break;
-        }
-        rc = frames_launch(ctx, (const uint8_t*)in[j]->p, len, level, (uint8_t*)lp[j]->p);
-        if (!rc && k > 0) rc = copy_out(j ^ 1, t_prev);
-        prev0 = c0;
-        prev_len = len;
-    }
-    if (!rc) {  // the last piece
-        const uint64_t ps = div_up(prev_len, GC_SUB_BLOCK);
-        rc = frames_pack(ctx, ps, (uint8_t*)lp[(k - 1) & 1]->p, sizes.data() + prev0 / GC_SUB_BLOCK, cap - o, &t_prev);
-        if (!rc && (e = hipEventRecord(packed, ctx->stream)) != hipSuccess) rc = hip_fail(e, "hipEventRecord");
-        if (!rc) rc = copy_out((k - 1) & 1, t_prev);
-    }
-    // nothing of this call stays in flight when it returns, on success or not
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream2);
-    (void)hipEventDestroy(packed);
-    if (rc) return rc;
     for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
-    *out_len = o;
+    *out_len = head + w;
     return GC_OK;
 }
 

@@ -285,12 +285,13 @@ int gc_zstd_encode_block_dev(gc_ctx* ctx, const uint8_t* d_input, const void* d_
 /* Host chunk -> match finding and frame coding on the GPU: one frame per
  * 64 KiB sub-block, packed in order into out, sizes[s]; cap >= n + 16 per
  * sub-block suffices.  The GPU counterpart of gc_compress_chunk_zstd_custom
- * (libzstd's entropy coding on the CPU): faster, larger. */
+ * (libzstd's entropy coding on the CPU): faster, larger.  Coded in pieces of
+ * 128 MiB, one piece's copies to and from host memory overlapping the next
+ * piece's kernels (the context's second stream). */
 int gc_zstd_frames_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint32_t level, uint8_t* out, size_t cap,
                        uint64_t* sizes, size_t* out_len);
-/* NVZS container with GPU-coded frames (decoded by gc_decompress).  The input
- * is coded in pieces of 128 MiB, one piece's copies to and from host memory
- * overlapping the next piece's kernels (the context's second stream). */
+/* NVZS container with GPU-coded frames (decoded by gc_decompress), coded as
+ * gc_zstd_frames_gpu codes them. */
 size_t gc_nvzs_gpu_bound(size_t n);
 int gc_nvzs_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, uint8_t* out, size_t cap,
                          size_t* out_len);
