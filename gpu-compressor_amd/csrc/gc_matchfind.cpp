@@ -313,6 +313,31 @@ int frames_pipelined(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t level, 
     return GC_OK;
 }
 
+// The streams of one piece of at most 2 GiB at src, coded in two halves, one coder launch each on the
+// context's two streams: the second half's copy in overlaps the first half's coding, and the first
+// half's streams are copied out while the second half's coder runs.  The caller holds ctx->mu.
+int lz_two_halves(gc_ctx* ctx, const uint8_t* src, size_t len, uint8_t* out, size_t cap, uint64_t* sizes,
+                  size_t* written) {
+    const LzSet set[2] = {lz_set(ctx, 0), lz_set(ctx, 1)};
+    const size_t half = len > GC_SUB_BLOCK ? (size_t)div_up(len / 2, GC_SUB_BLOCK) * GC_SUB_BLOCK : len;
+    const size_t part[2] = {half, len - half};
+    size_t o = 0;
+    int rc = GC_OK;
+    for (int j = 0; j < 2 && !rc; j++)
+        if (part[j]) rc = lz_launch(ctx, set[j], src + (j ? half : 0), part[j]);
+    for (int j = 0; j < 2 && !rc; j++) {
+        if (!part[j]) continue;
+        size_t w = 0;
+        rc = lz_finish(ctx, set[j], part[j], out + o, cap - o, sizes + (j ? half : 0) / GC_SUB_BLOCK, &w);
+        o += w;
+    }
+    // nothing stays in flight when this returns, on success or not
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream2);
+    *written = o;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -572,11 +597,7 @@ int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* o
     if (nsub > 0xFFFFFFFFull) return fail(GC_EINVAL, "input too large");
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    const LzSet s = lz_set(ctx, 0);
-    int rc = lz_launch(ctx, s, chunk, n);
-    if (!rc) rc = lz_finish(ctx, s, n, out, cap, sizes, out_len);
-    (void)hipStreamSynchronize(ctx->stream);
-    return rc;
+    return lz_two_halves(ctx, chunk, n, out, cap, sizes, out_len);
 }
 
 int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
@@ -594,38 +615,21 @@ int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dic
     put64(out + 20, nsub);
     // Matches never cross a 64 KiB sub-block, so the dictionary-size chunks of
     // pipeline_lzma2.rs:40 change no byte of the output; the whole input goes
-    // through in pieces of up to 2 GiB instead.  A sub-block's stream is a long
-    // serial walk, so a coder launch needs thousands of sub-blocks to keep the
-    // GPU busy: each piece is coded in two halves, one launch each on the
-    // context's two streams.  The second half's copy in overlaps the first
-    // half's coding, and the first half's streams are copied out while the
-    // second half's coder runs.
+    // through in pieces of up to 2 GiB instead, each as gc_lzma2_streams_gpu
+    // codes a chunk (two halves on two streams).
     (void)dict_mb;
     const size_t piece = (size_t)2 << 30;
     std::vector<uint64_t> sizes(nsub, 0);
     size_t o = head;
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
-    const LzSet set[2] = {lz_set(ctx, 0), lz_set(ctx, 1)};
-    int rc = GC_OK;
-    for (size_t c0 = 0; c0 < n && !rc; c0 += piece) {
+    for (size_t c0 = 0; c0 < n; c0 += piece) {
         const size_t len = n - c0 < piece ? n - c0 : piece;
-        const size_t half = len > GC_SUB_BLOCK ? (size_t)div_up(len / 2, GC_SUB_BLOCK) * GC_SUB_BLOCK : len;
-        const size_t part[2] = {half, len - half};
-        for (int j = 0; j < 2 && !rc; j++)
-            if (part[j]) rc = lz_launch(ctx, set[j], src + c0 + (j ? half : 0), part[j]);
-        for (int j = 0; j < 2 && !rc; j++) {
-            if (!part[j]) continue;
-            size_t w = 0;
-            rc = lz_finish(ctx, set[j], part[j], out + o, cap - o, sizes.data() + (c0 + (j ? half : 0)) / GC_SUB_BLOCK,
-                           &w);
-            o += w;
-        }
+        size_t w = 0;
+        int rc = lz_two_halves(ctx, src + c0, len, out + o, cap - o, sizes.data() + c0 / GC_SUB_BLOCK, &w);
+        if (rc) return rc;
+        o += w;
     }
-    // nothing of this call stays in flight when it returns, on success or not
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream2);
-    if (rc) return rc;
     for (uint64_t s = 0; s < nsub; s++) put64(out + 28 + 8 * s, sizes[s]);
     *out_len = o;
     return GC_OK;

@@ -257,15 +257,16 @@ int gc_lzma2_encode_dev(gc_ctx* ctx, const uint8_t* d_input, const uint32_t* d_p
 /* Host chunk -> match finding and LZMA2 coding on the GPU: one stream per
  * 64 KiB sub-block, packed in order into out, sizes[s]; cap >= n + 4 per
  * sub-block suffices.  The GPU counterpart of gc_cpu_encode_from_matches on
- * gc_lzma2_find_matches_packed's matches. */
+ * gc_lzma2_find_matches_packed's matches.  Coded in two halves, one launch
+ * each on the context's two streams, so that copies to and from host memory
+ * overlap the coding. */
 int gc_lzma2_streams_gpu(gc_ctx* ctx, const uint8_t* chunk, size_t n, uint8_t* out, size_t cap, uint64_t* sizes,
                          size_t* out_len);
 /* NVLZ container of GPU-coded streams, byte-equal to gc_nvlz_compress_matches'
  * for the same input (dict_mb is accepted for that interface and changes
  * nothing: no match crosses a sub-block); cap >= gc_container_bound(n,
- * GC_ALGO_LZMA2).  Decoded by gc_decompress.  Coded in two halves, one
- * launch each on the context's two streams, so that copies to and from host
- * memory overlap the coding. */
+ * GC_ALGO_LZMA2).  Decoded by gc_decompress.  Coded in pieces of up to 2 GiB,
+ * each as gc_lzma2_streams_gpu codes a chunk. */
 int gc_nvlz_compress_gpu(gc_ctx* ctx, const uint8_t* src, size_t n, uint32_t dict_mb, uint8_t* out, size_t cap,
                          size_t* out_len);
 
