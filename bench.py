@@ -1,7 +1,7 @@
 """Match-find throughput and latency on one MI355X: the served and batched forms against the plain
 per-chunk GPU form, at identical results.
 
-Default (the driver's line): the serving workload.  Requests arrive open-loop (Poisson at 500, 2000 and
+Default (no --workload): the serving workload.  Requests arrive open-loop (Poisson at 500, 2000 and
 8000 req/s, then all at once for the saturation passes that `value` is measured on), sizes log-uniform
 1 KiB-64 MiB, content from the four-kind mix (a 1 GiB pool), 50% Zstd L1 / 25% Zstd L2 / 25% LZMA2 L1,
 under a 64 GiB HBM budget.  Inputs are resident in HBM before the timed region (each request is a window
